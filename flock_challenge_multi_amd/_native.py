@@ -1,0 +1,105 @@
+"""ctypes binding of the gfx950 device library `libflock_gpu.so` (C ABI in
+include/flock_gpu.h).
+
+The library is built in-tree by `flock_challenge_multi_amd/build.py`.  Loading
+fails loudly when it is missing: there is no CPU fallback in the product path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflock_gpu.so")
+
+FLK_OK = 0
+FLK_E_SHAPE = 1
+FLK_E_DEVICE = 2
+FLK_E_DISABLED = 3
+FLK_E_NOMEM = 4
+FLK_HASH_SHA256 = 0
+FLK_HASH_BLAKE3 = 1
+
+
+class F128(ctypes.Structure):
+    """flk_f128: `lo` (x^0..x^63) then `hi` (x^64..x^127)."""
+
+    _fields_ = [("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64)]
+
+    @classmethod
+    def from_int(cls, v):
+        return cls(v & (2**64 - 1), (v >> 64) & (2**64 - 1))
+
+    def to_int(self):
+        return self.lo | (self.hi << 64)
+
+
+class FlockGpuError(RuntimeError):
+    def __init__(self, code, func, msg):
+        super().__init__(f"{func} failed with status {code}: {msg}")
+        self.code = code
+
+
+_P = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+_I = ctypes.c_int
+_F = F128
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "flk_device_available": (_I, []),
+    "flk_is_disabled": (_I, []),
+    "flk_undisable": (None, []),
+    "flk_set_enabled": (None, [_I]),
+    "flk_last_error": (ctypes.c_char_p, []),
+    "flk_version": (ctypes.c_char_p, []),
+    "flk_f128_mul": (_F, [_F, _F]),
+    "flk_ntt_twiddles": (_I, [_SZ, _P]),
+    "flk_ntt_forward_interleaved": (_I, [_SZ, _P, _SZ, _SZ, _SZ, _P]),
+    "flk_ntt_rs_encode_interleaved": (_I, [_SZ, _P, _SZ, _P, _SZ, _SZ, _P]),
+    "flk_merkle_hash_leaves": (_I, [_P, _SZ, _SZ, _P, _P]),
+    "flk_merkle_hash_pairs_level": (_I, [_P, _SZ, _P, _P]),
+    "flk_merkle_tree": (_I, [_P, _SZ, _SZ, _I, _P, _P]),
+    "flk_commit": (_I, [_P, _SZ, _SZ, _SZ, _I, _P, _P, _P, _P]),
+    "flk_build_eq": (_I, [_P, _SZ, _P, _P]),
+    "flk_sc_round_msg": (_I, [_P, _P, _SZ, _P, _P]),
+    "flk_sc_round_msg_and_eval": (_I, [_P, _P, _SZ, _P, _P]),
+    "flk_sc_fold_and_msg": (_I, [_P, _P, _SZ, _F, _P, _P, _P, _P]),
+    "flk_sc_glue": (_I, [_P, _P, _SZ, _F, _P]),
+    "flk_partial_eval_lsb": (_I, [_P, _SZ, _P, _SZ, _P, _P]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libflock_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python flock_challenge_multi_amd/build.py` "
+            "(the device path has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status, func):
+    if status != FLK_OK:
+        msg = load().flk_last_error()
+        raise FlockGpuError(status, func, msg.decode() if msg else "")
+    return status
+
+
+def f128_mul(a, b):
+    """Host scalar GF(2^128) product of two ints (device library's host path)."""
+    return load().flk_f128_mul(F128.from_int(a), F128.from_int(b)).to_int()
+
+
+def call(name, *args):
+    """Call an int-returning entry point and raise on a non-OK status."""
+    return check(getattr(load(), name)(*args), name)

@@ -1,0 +1,205 @@
+// GF(2^128) arithmetic for gfx950 (GHASH polynomial x^128 + x^7 + x^2 + x + 1,
+// polynomial basis, CF/field/gf2_128.rs:10-27).
+//
+// An element lives in four 32-bit words (uint4: x = bits 0..31 ... w = bits
+// 96..127), which is exactly the reference's in-memory layout (lo u64 LE then
+// hi u64 LE), so a 16-byte global load is one element.
+//
+// CDNA4 has no carry-less multiply instruction.  Products use a 4-bit comb:
+// a 16-entry table T[k] = k * t (reduced, 256 B) of one operand lives in LDS;
+// the other operand is scanned nibble by nibble, XOR-ing T[nibble] at word
+// offset w into a 256-bit accumulator that is shifted by 4 bits between the
+// eight nibble positions.  The accumulator is congruent to the product mod p,
+// so sums of products are XOR-accumulated unreduced and reduced once
+// (the deferred-reduction identity of CF/field/gf2_128.rs:46-53).
+//
+// All routines are plain integer VALU work; nothing here touches MFMA.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace flk {
+
+struct U256 {
+  uint32_t w[8];
+};
+
+__device__ __forceinline__ uint4 f_zero() { return make_uint4(0u, 0u, 0u, 0u); }
+__device__ __forceinline__ uint4 f_add(uint4 a, uint4 b) {
+  return make_uint4(a.x ^ b.x, a.y ^ b.y, a.z ^ b.z, a.w ^ b.w);
+}
+__device__ __forceinline__ bool f_is_zero(uint4 a) { return (a.x | a.y | a.z | a.w) == 0u; }
+
+// (hi:lo) >> s for 0 < s < 32, i.e. hi << (32-s) | lo >> s.
+__device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t s) {
+  return __builtin_amdgcn_alignbit(hi, lo, s);
+}
+
+// x * a (CF/field/gf2_128.rs:174-184).
+__device__ __forceinline__ uint4 f_mul_x(uint4 a) {
+  uint32_t carry = a.w >> 31;
+  uint4 o;
+  o.w = funnel(a.w, a.z, 31);
+  o.z = funnel(a.z, a.y, 31);
+  o.y = funnel(a.y, a.x, 31);
+  o.x = (a.x << 1) ^ (carry ? 0x87u : 0u);
+  return o;
+}
+
+__device__ __forceinline__ void u256_zero(U256& a) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a.w[i] = 0u;
+}
+__device__ __forceinline__ void u256_xor(U256& a, const U256& b) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a.w[i] ^= b.w[i];
+}
+
+// Fold bits 128..255 into 0..127 mod p: U*x^128 = U ^ U<<1 ^ U<<2 ^ U<<7, with
+// the 7 bits that leave position 127 folded once more
+// (same identity as ghash_reduce, CF/field/gf2_128.rs:242-262).
+__device__ __forceinline__ uint4 f_reduce(const U256& r) {
+  const uint32_t u0 = r.w[4], u1 = r.w[5], u2 = r.w[6], u3 = r.w[7];
+  uint4 t;
+  t.x = u0 ^ (u0 << 1) ^ (u0 << 2) ^ (u0 << 7);
+  t.y = u1 ^ funnel(u1, u0, 31) ^ funnel(u1, u0, 30) ^ funnel(u1, u0, 25);
+  t.z = u2 ^ funnel(u2, u1, 31) ^ funnel(u2, u1, 30) ^ funnel(u2, u1, 25);
+  t.w = u3 ^ funnel(u3, u2, 31) ^ funnel(u3, u2, 30) ^ funnel(u3, u2, 25);
+  const uint32_t ov = (u3 >> 31) ^ (u3 >> 30) ^ (u3 >> 25);
+  const uint32_t corr = ov ^ (ov << 1) ^ (ov << 2) ^ (ov << 7);
+  return make_uint4(r.w[0] ^ t.x ^ corr, r.w[1] ^ t.y, r.w[2] ^ t.z, r.w[3] ^ t.w);
+}
+
+// Entry k (0..15) of the comb table of t: k * t = XOR of t*x^j over set bits j.
+__device__ __forceinline__ uint4 comb_entry(uint4 t, uint32_t k) {
+  uint4 acc = f_zero();
+  uint4 p = t;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if ((k >> j) & 1u) acc = f_add(acc, p);
+    if (j < 3) p = f_mul_x(p);
+  }
+  return acc;
+}
+
+// All 16 entries of k * t into registers (used by threads that own a table).
+__device__ __forceinline__ void comb_entries(uint4 t, uint4 e[16]) {
+  uint4 b0 = t, b1 = f_mul_x(b0), b2 = f_mul_x(b1), b3 = f_mul_x(b2);
+  e[0] = f_zero();
+  e[1] = b0;
+  e[2] = b1;
+  e[3] = f_add(b1, b0);
+  e[4] = b2;
+  e[5] = f_add(b2, b0);
+  e[6] = f_add(b2, b1);
+  e[7] = f_add(e[6], b0);
+  e[8] = b3;
+#pragma unroll
+  for (int k = 9; k < 16; ++k) e[k] = f_add(b3, e[k - 8]);
+}
+
+// Unreduced comb product v * t, where tab[k * stride] = k * t (LDS).
+// The result is congruent to v * t mod p; XOR several and call f_reduce once.
+template <int STRIDE>
+__device__ __forceinline__ void comb_mul_acc(uint4 v, const uint4* __restrict__ tab, U256& acc) {
+  const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+  U256 a;
+  u256_zero(a);
+  // Not unrolled over j: keeps only four table loads in flight per lane, which
+  // bounds register pressure when several products are inlined together.
+#pragma unroll 1
+  for (int j = 7; j >= 0; --j) {
+    if (j != 7) {
+#pragma unroll
+      for (int i = 7; i >= 1; --i) a.w[i] = funnel(a.w[i], a.w[i - 1], 28);
+      a.w[0] <<= 4;
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const uint32_t nib = (vw[w] >> (4 * j)) & 15u;
+      const uint4 e = tab[nib * STRIDE];
+      a.w[w] ^= e.x;
+      a.w[w + 1] ^= e.y;
+      a.w[w + 2] ^= e.z;
+      a.w[w + 3] ^= e.w;
+    }
+  }
+  u256_xor(acc, a);
+}
+
+// Same with a run-time stride (per-thread tables laid out entry-major).
+__device__ __forceinline__ void comb_mul_acc_rt(uint4 v, const uint4* __restrict__ tab, uint32_t stride,
+                                                U256& acc) {
+  const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+  U256 a;
+  u256_zero(a);
+  // Not unrolled over j: keeps only four table loads in flight per lane, which
+  // bounds register pressure when several products are inlined together.
+#pragma unroll 1
+  for (int j = 7; j >= 0; --j) {
+    if (j != 7) {
+#pragma unroll
+      for (int i = 7; i >= 1; --i) a.w[i] = funnel(a.w[i], a.w[i - 1], 28);
+      a.w[0] <<= 4;
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const uint32_t nib = (vw[w] >> (4 * j)) & 15u;
+      const uint4 e = tab[nib * stride];
+      a.w[w] ^= e.x;
+      a.w[w + 1] ^= e.y;
+      a.w[w + 2] ^= e.z;
+      a.w[w + 3] ^= e.w;
+    }
+  }
+  u256_xor(acc, a);
+}
+
+template <int STRIDE>
+__device__ __forceinline__ uint4 comb_mul(uint4 v, const uint4* __restrict__ tab) {
+  U256 a;
+  u256_zero(a);
+  comb_mul_acc<STRIDE>(v, tab, a);
+  return f_reduce(a);
+}
+
+__device__ __forceinline__ uint4 comb_mul_rt(uint4 v, const uint4* __restrict__ tab, uint32_t stride) {
+  U256 a;
+  u256_zero(a);
+  comb_mul_acc_rt(v, tab, stride, a);
+  return f_reduce(a);
+}
+
+// Per-thread private table: entry k of thread `tid` at tab_base[k * nthreads + tid]
+// (entry-major, so a wave's 64 lanes hit 64 consecutive 16-byte slots).
+__device__ __forceinline__ void table_store_private(uint4* tab_base, uint32_t nthreads, uint32_t tid, uint4 t) {
+  uint4 e[16];
+  comb_entries(t, e);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) tab_base[k * nthreads + tid] = e[k];
+}
+
+// Register-only product for scalar set-up work (slow path, a few calls per
+// kernel): shift-and-add over the 128 bits of b.
+__device__ __forceinline__ uint4 f_mul_slow(uint4 a, uint4 b) {
+  U256 acc;
+  u256_zero(acc);
+  const uint32_t bw[4] = {b.x, b.y, b.z, b.w};
+  uint32_t s[5] = {a.x, a.y, a.z, a.w, 0u};
+  for (int bit = 0; bit < 128; ++bit) {
+    const uint32_t word = bit >> 5, off = bit & 31;
+    if ((bw[word] >> off) & 1u) {
+      // a << bit lands at words word..word+4 shifted by off
+      uint32_t sh[5];
+      sh[0] = s[0] << off;
+      for (int i = 1; i < 5; ++i) sh[i] = off ? ((s[i] << off) | (s[i - 1] >> (32 - off))) : s[i];
+      for (int i = 0; i < 5; ++i)
+        if (word + i < 8) acc.w[word + i] ^= sh[i];
+    }
+  }
+  return f_reduce(acc);
+}
+
+__device__ __forceinline__ uint4 ld_f128(const uint4* p) { return *p; }
+
+}  // namespace flk

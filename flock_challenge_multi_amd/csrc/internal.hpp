@@ -1,0 +1,43 @@
+// Shared host-side plumbing of the device library: status codes, the
+// process-wide failure latch, the per-device context (cached twiddle tables)
+// and launch checks.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/flock_gpu.h"
+
+namespace flk {
+
+// Record a message for flk_last_error() and return `code`.
+int set_error(int code, const char* fmt, ...);
+// A runtime failure: latch the library disabled (CF/gpu.rs:455-466) and record.
+int device_failure(hipError_t e, const char* where);
+// FLK_OK unless the library is disabled (latch, API or FLOCK_NO_GPU).
+int check_enabled();
+// Device copy of the twiddle table of AdditiveNttF128::standard(dim), cached per
+// (device, dim).  Returns nullptr (and records the error) on failure.
+const uint4* twiddles_device(size_t dim, int* status);
+// Host copy of the same table (computed once per dim).
+const flk_f128* twiddles_host(size_t dim);
+
+inline bool is_pow2(size_t n) { return n && !(n & (n - 1)); }
+inline int log2_exact(size_t n) {
+  if (!is_pow2(n)) return -1;
+  int l = 0;
+  while ((size_t(1) << l) < n) ++l;
+  return l;
+}
+
+#define FLK_LAUNCH_CHECK(where)                              \
+  do {                                                       \
+    hipError_t e_ = hipGetLastError();                       \
+    if (e_ != hipSuccess) return ::flk::device_failure(e_, where); \
+  } while (0)
+
+// Host-side GF(2^128) product (used for set-up constants only).
+flk_f128 host_mul(flk_f128 a, flk_f128 b);
+flk_f128 host_inv(flk_f128 a);
+
+}  // namespace flk

@@ -1,0 +1,204 @@
+// Interleaved additive NTT / Reed-Solomon encode over GF(2^128) for gfx950.
+//
+// Semantics: AdditiveNttF128::forward_transform_interleaved_from_layer and
+// rs_encode_interleaved (CF/ntt/additive_ntt_f128.rs:2728-2830; scalar form
+// :4766-4795).  Layout data[pos * num_ntts + lane]; layer l pairs positions
+// 2^(log_d-l-1) apart inside blocks of 2^(log_d-l); u += v*twiddle(l,b); v += u.
+//
+// Schedule.  The layers are processed in passes of K consecutive layers.  A
+// thread owns one lane of one "set": the 2^K positions that the K layers of the
+// pass connect (they differ only in the K position bits the pass consumes), so a
+// pass reads and writes every element once and all K layers run in registers.
+// Consecutive threads are consecutive lanes of the same positions, so every
+// access is a contiguous 16-byte-per-lane stream.
+//
+// Twiddle products use the comb multiply of gf128.hpp.  All lanes of one set
+// share the twiddles, so the G = min(num_ntts, 64) lanes of a set build the
+// 16-entry tables of the twiddles of the current layer together in LDS.
+//
+// The RS encode reads the message directly in its first pass: after the
+// log_inv_rate copy-only layers of the zero-padded transform, position p holds
+// message position p mod 2^(log_d - log_inv_rate) (replicate_message_fill,
+// :7455-7476), so the replicated codeword is never materialised.
+#include <hip/hip_runtime.h>
+
+#include "gf128.hpp"
+#include "internal.hpp"
+
+namespace flk {
+
+constexpr int NTT_WG = 256;
+
+template <int K>
+__global__ void __launch_bounds__(NTT_WG) ntt_pass_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                         const uint4* __restrict__ tw, uint32_t log_d, uint32_t L,
+                                                         uint32_t log_lanes, uint64_t src_pos_mask,
+                                                         uint64_t n_threads, uint32_t log_group) {
+  extern __shared__ uint4 lds[];
+  const uint32_t G = 1u << log_group;            // lanes sharing one set's twiddles (<= 64)
+  const uint32_t groups_per_wave = 64u >> log_group;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t wave = tid >> 6, lane_w = tid & 63u;
+  const uint32_t grp = lane_w >> log_group, lane_g = lane_w & (G - 1u);
+  uint4* my_tabs = lds + ((size_t)(wave * groups_per_wave + grp) << (K - 1 + 4));
+
+  const uint64_t g = (uint64_t)blockIdx.x * NTT_WG + tid;
+  const bool active = g < n_threads;
+  const uint64_t c = g & ((1ull << log_lanes) - 1ull);
+  const uint64_t s = g >> log_lanes;
+  const uint32_t lo_bits = log_d - L - K;
+  const uint64_t lo = s & ((1ull << lo_bits) - 1ull);
+  const uint64_t prefix = s >> lo_bits;
+
+  uint4 x[1 << K];
+#pragma unroll
+  for (int j = 0; j < (1 << K); ++j) {
+    const uint64_t p = (prefix << (log_d - L)) | ((uint64_t)j << lo_bits) | lo;
+    x[j] = active ? src[((p & src_pos_mask) << log_lanes) | c] : f_zero();
+  }
+
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    // Tables of the 2^i twiddles of layer L+i met by this set.
+    const uint32_t n_entries = 16u << i;
+    const uint64_t tw_base = ((1ull << (L + i)) - 1ull) + (prefix << i);
+    for (uint32_t e = lane_g; e < n_entries; e += G) {
+      const uint32_t sb = e >> 4, k = e & 15u;
+      const uint4 t = active ? tw[tw_base + sb] : f_zero();
+      my_tabs[e] = comb_entry(t, k);
+    }
+    __syncthreads();
+    constexpr int dummy = 0;
+    (void)dummy;
+#pragma unroll
+    for (int sb = 0; sb < (1 << i); ++sb) {
+      const uint4* tab = my_tabs + (sb << 4);
+#pragma unroll
+      for (int jj = 0; jj < (1 << (K - 1 - i)); ++jj) {
+        const int j0 = (sb << (K - i)) + jj;
+        const int j1 = j0 + (1 << (K - 1 - i));
+        const uint4 prod = comb_mul<1>(x[j1], tab);
+        x[j0] = f_add(x[j0], prod);
+        x[j1] = f_add(x[j1], x[j0]);
+      }
+    }
+    __syncthreads();
+  }
+
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < (1 << K); ++j) {
+      const uint64_t p = (prefix << (log_d - L)) | ((uint64_t)j << lo_bits) | lo;
+      dst[(p << log_lanes) | c] = x[j];
+    }
+  }
+}
+
+// Plain replication (used only when no layer is left after the copy layers).
+__global__ void replicate_kernel(const uint4* __restrict__ msg, uint4* __restrict__ cw, uint64_t n_msg, uint64_t n_cw) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_cw) cw[i] = msg[i & (n_msg - 1)];
+}
+
+constexpr int NTT_KMAX = 4;
+
+template <int K>
+static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t log_d, uint32_t L, uint32_t log_lanes,
+                       uint64_t src_pos_mask, uint32_t log_group, hipStream_t st) {
+  const uint64_t n_threads = (1ull << (log_d - K)) << log_lanes;
+  const uint64_t blocks = (n_threads + NTT_WG - 1) / NTT_WG;
+  const size_t lds = (size_t)(NTT_WG / 64) * (64u >> log_group) * ((size_t)16 << (K - 1)) * sizeof(uint4);
+  hipLaunchKernelGGL(ntt_pass_kernel<K>, dim3((unsigned)blocks), dim3(NTT_WG), lds, st, src, dst, tw, log_d, L,
+                     log_lanes, src_pos_mask, n_threads, log_group);
+  FLK_LAUNCH_CHECK("ntt_pass_kernel");
+  return FLK_OK;
+}
+
+// Run layers [start, log_d) reading the first pass from `src` (positions masked
+// by src_pos_mask) and writing all passes to `dst`.
+static int run_layers(const uint4* src, uint4* dst, const uint4* tw, uint32_t log_d, uint32_t start,
+                      uint32_t log_lanes, uint64_t src_pos_mask, hipStream_t st) {
+  const uint32_t log_group = log_lanes < 6 ? log_lanes : 6;
+  // LDS per wave is (64/G) * 2^(K-1) tables of 256 B; keep it <= 16 KiB.
+  int kmax = (int)log_group + 1;
+  if (kmax > NTT_KMAX) kmax = NTT_KMAX;
+  uint32_t L = start;
+  const uint4* s = src;
+  uint64_t mask = src_pos_mask;
+  while (L < log_d) {
+    int K = (int)(log_d - L) < kmax ? (int)(log_d - L) : kmax;
+    int rc;
+    switch (K) {
+      case 1: rc = launch_pass<1>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
+      case 2: rc = launch_pass<2>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
+      case 3: rc = launch_pass<3>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
+      default: rc = launch_pass<4>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
+    }
+    if (rc) return rc;
+    L += (uint32_t)K;
+    s = dst;
+    mask = ~0ull;
+  }
+  return FLK_OK;
+}
+
+int ntt_rs_encode_device(size_t dim, const uint4* msg, size_t n_msg, uint4* cw, size_t n_cw, size_t num_ntts,
+                         hipStream_t st) {
+  int rc = check_enabled();
+  if (rc) return rc;
+  if (!msg || !cw) return set_error(FLK_E_SHAPE, "rs_encode: null pointer");
+  if (!is_pow2(num_ntts) || n_msg == 0 || n_msg % num_ntts || n_cw % n_msg)
+    return set_error(FLK_E_SHAPE, "rs_encode: bad shape n_msg=%zu n_cw=%zu num_ntts=%zu", n_msg, n_cw, num_ntts);
+  const size_t inv_rate = n_cw / n_msg;
+  const int log_inv_rate = log2_exact(inv_rate);
+  const int log_d = log2_exact(n_cw / num_ntts);
+  const int log_lanes = log2_exact(num_ntts);
+  if (log_inv_rate < 1 || log_d < 0 || log_inv_rate > log_d || (size_t)log_d > dim)
+    return set_error(FLK_E_SHAPE, "rs_encode: bad rate/domain (inv_rate=%zu log_d=%d dim=%zu)", inv_rate, log_d, dim);
+  if (dim > 24) return set_error(FLK_E_SHAPE, "rs_encode: dim %zu exceeds the device twiddle cache (24)", dim);
+  if ((const void*)msg < (const void*)(cw + n_cw) && (const void*)cw < (const void*)(msg + n_msg))
+    return set_error(FLK_E_SHAPE, "rs_encode: msg and codeword overlap");
+  const uint4* tw = twiddles_device(dim, &rc);
+  if (rc) return rc;
+  const uint64_t msg_pos_mask = (1ull << (log_d - log_inv_rate)) - 1ull;
+  if (log_inv_rate == log_d) {
+    const uint64_t blocks = (n_cw + 255) / 256;
+    hipLaunchKernelGGL(replicate_kernel, dim3((unsigned)blocks), dim3(256), 0, st, msg, cw, (uint64_t)n_msg,
+                       (uint64_t)n_cw);
+    FLK_LAUNCH_CHECK("replicate_kernel");
+    return FLK_OK;
+  }
+  return run_layers(msg, cw, tw, (uint32_t)log_d, (uint32_t)log_inv_rate, (uint32_t)log_lanes, msg_pos_mask, st);
+}
+
+}  // namespace flk
+
+extern "C" {
+
+int flk_ntt_rs_encode_interleaved(size_t dim, const flk_f128* msg, size_t n_msg, flk_f128* codeword, size_t n_cw,
+                                  size_t num_ntts, void* stream) {
+  return flk::ntt_rs_encode_device(dim, reinterpret_cast<const uint4*>(msg), n_msg, reinterpret_cast<uint4*>(codeword),
+                                   n_cw, num_ntts, static_cast<hipStream_t>(stream));
+}
+
+int flk_ntt_forward_interleaved(size_t dim, flk_f128* data, size_t n_total, size_t num_ntts, size_t start_layer,
+                                void* stream) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!data) return flk::set_error(FLK_E_SHAPE, "forward_interleaved: null pointer");
+  if (!flk::is_pow2(num_ntts) || n_total % num_ntts)
+    return flk::set_error(FLK_E_SHAPE, "forward_interleaved: bad shape n_total=%zu num_ntts=%zu", n_total, num_ntts);
+  const int log_d = flk::log2_exact(n_total / num_ntts);
+  if (log_d < 0 || (size_t)log_d > dim || start_layer > (size_t)log_d)
+    return flk::set_error(FLK_E_SHAPE, "forward_interleaved: bad domain log_d=%d dim=%zu start=%zu", log_d, dim,
+                          start_layer);
+  if (dim > 24) return flk::set_error(FLK_E_SHAPE, "forward_interleaved: dim %zu exceeds 24", dim);
+  if ((size_t)log_d == start_layer) return FLK_OK;
+  const uint4* tw = flk::twiddles_device(dim, &rc);
+  if (rc) return rc;
+  uint4* d = reinterpret_cast<uint4*>(data);
+  return flk::run_layers(d, d, tw, (uint32_t)log_d, (uint32_t)start_layer, (uint32_t)flk::log2_exact(num_ntts), ~0ull,
+                         static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,204 @@
+// Host runtime of the device library: status/latch handling (CF/gpu.rs:13-21,
+// 455-516), the per-device twiddle cache (the device counterpart of
+// cached_standard_twiddles, CF/ntt/additive_ntt_f128.rs:146-178) and set-up
+// arithmetic.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace flk {
+
+static std::atomic<int> g_latched{0};
+static std::atomic<int> g_enabled{1};
+static thread_local char g_err[512];
+
+int set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int device_failure(hipError_t e, const char* where) {
+  g_latched.store(1);
+  return set_error(FLK_E_DEVICE, "%s: %s", where, hipGetErrorString(e));
+}
+
+static bool env_disabled() {
+  const char* v = getenv("FLOCK_NO_GPU");
+  return v && v[0] && strcmp(v, "0") != 0;
+}
+
+int check_enabled() {
+  if (g_latched.load()) return set_error(FLK_E_DISABLED, "device path latched off after a runtime failure");
+  if (!g_enabled.load()) return set_error(FLK_E_DISABLED, "device path disabled by flk_set_enabled(0)");
+  if (env_disabled()) return set_error(FLK_E_DISABLED, "device path disabled by FLOCK_NO_GPU");
+  return FLK_OK;
+}
+
+// ---- host GF(2^128) for set-up constants -------------------------------
+static inline void clmul64(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
+  uint64_t l = 0, h = 0;
+  for (int i = 0; i < 64; ++i)
+    if ((a >> i) & 1) {
+      l ^= b << i;
+      if (i) h ^= b >> (64 - i);
+    }
+  lo = l;
+  hi = h;
+}
+
+flk_f128 host_mul(flk_f128 a, flk_f128 b) {
+  uint64_t r[4], x0, x1;
+  clmul64(a.lo, b.lo, r[0], r[1]);
+  clmul64(a.hi, b.hi, r[2], r[3]);
+  clmul64(a.lo ^ a.hi, b.lo ^ b.hi, x0, x1);  // Karatsuba middle term
+  x0 ^= r[0] ^ r[2];
+  x1 ^= r[1] ^ r[3];
+  r[1] ^= x0;
+  r[2] ^= x1;
+  // fold r2:r3 * x^128 = U ^ U<<1 ^ U<<2 ^ U<<7
+  uint64_t u0 = r[2], u1 = r[3];
+  uint64_t t0 = u0 ^ (u0 << 1) ^ (u0 << 2) ^ (u0 << 7);
+  uint64_t t1 = u1 ^ ((u1 << 1) | (u0 >> 63)) ^ ((u1 << 2) | (u0 >> 62)) ^ ((u1 << 7) | (u0 >> 57));
+  uint64_t ov = (u1 >> 63) ^ (u1 >> 62) ^ (u1 >> 57);
+  uint64_t corr = ov ^ (ov << 1) ^ (ov << 2) ^ (ov << 7);
+  return flk_f128{r[0] ^ t0 ^ corr, r[1] ^ t1};
+}
+
+flk_f128 host_inv(flk_f128 a) {
+  // a^(2^128 - 2): exponent bits 1..127 set.
+  flk_f128 result{1, 0}, base = a;
+  for (int i = 0; i < 128; ++i) {
+    if (i >= 1) result = host_mul(result, base);
+    base = host_mul(base, base);
+  }
+  return result;
+}
+
+// ---- twiddle tables ---------------------------------------------------------
+// Row i of the normalised subspace evaluations: evals[i][k] = Ŵ_i(x^{i+k})
+// (CF/ntt/additive_ntt_f128.rs:54-84); twiddle(l, b) = sum_j bit_j(b) evals[dim-l-1][1+j]
+// (:2684-2693), materialised breadth-first with layer l at offset 2^l - 1 by the
+// half-doubling construction of :108-130.
+static std::vector<flk_f128> compute_twiddles(size_t dim) {
+  std::vector<std::vector<flk_f128>> ev(dim);
+  for (size_t k = 0; k < dim; ++k) ev[0].push_back(flk_f128{k < 64 ? (1ull << k) : 0ull, k >= 64 ? (1ull << (k - 64)) : 0ull});
+  for (size_t i = 1; i < dim; ++i) {
+    const auto& prev = ev[i - 1];
+    for (size_t k = 1; k < prev.size(); ++k) {
+      flk_f128 s{prev[k].lo ^ prev[0].lo, prev[k].hi ^ prev[0].hi};
+      ev[i].push_back(host_mul(prev[k], s));
+    }
+  }
+  for (auto& row : ev) {
+    flk_f128 inv = host_inv(row[0]);
+    for (auto& v : row) v = host_mul(v, inv);
+  }
+  std::vector<flk_f128> tw;
+  tw.reserve((size_t(1) << dim) - 1);
+  for (size_t layer = 0; layer < dim; ++layer) {
+    size_t start = tw.size();
+    const auto& row = ev[dim - layer - 1];
+    tw.push_back(flk_f128{0, 0});
+    for (size_t bit = 0; bit + 1 < row.size(); ++bit) {
+      size_t half = size_t(1) << bit;
+      for (size_t b = 0; b < half; ++b) {
+        flk_f128 v = tw[start + b];
+        tw.push_back(flk_f128{v.lo ^ row[1 + bit].lo, v.hi ^ row[1 + bit].hi});
+      }
+    }
+  }
+  return tw;
+}
+
+static std::mutex g_tw_mu;
+static std::map<size_t, std::vector<flk_f128>> g_tw_host;
+static std::map<std::pair<int, size_t>, uint4*> g_tw_dev;
+
+const flk_f128* twiddles_host(size_t dim) {
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto it = g_tw_host.find(dim);
+  if (it == g_tw_host.end()) it = g_tw_host.emplace(dim, compute_twiddles(dim)).first;
+  return it->second.data();
+}
+
+const uint4* twiddles_device(size_t dim, int* status) {
+  *status = FLK_OK;
+  if (dim == 0) return nullptr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    *status = device_failure(e, "hipGetDevice");
+    return nullptr;
+  }
+  const flk_f128* host = twiddles_host(dim);
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto key = std::make_pair(dev, dim);
+  auto it = g_tw_dev.find(key);
+  if (it != g_tw_dev.end()) return it->second;
+  size_t bytes = ((size_t(1) << dim) - 1) * sizeof(flk_f128);
+  uint4* d = nullptr;
+  e = hipMalloc(&d, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *status = set_error(FLK_E_NOMEM, "hipMalloc(%zu) for twiddles: %s", bytes, hipGetErrorString(e));
+    return nullptr;
+  }
+  e = hipMemcpy(d, host, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    *status = device_failure(e, "hipMemcpy twiddles");
+    return nullptr;
+  }
+  g_tw_dev.emplace(key, d);
+  return d;
+}
+
+}  // namespace flk
+
+extern "C" {
+
+int flk_device_available(void) {
+  if (flk::check_enabled() != FLK_OK) return 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n > 0;
+}
+
+int flk_is_disabled(void) { return flk::check_enabled() != FLK_OK; }
+
+void flk_undisable(void) { flk::g_latched.store(0); }
+
+void flk_set_enabled(int enabled) { flk::g_enabled.store(enabled ? 1 : 0); }
+
+const char* flk_last_error(void) { return flk::g_err; }
+
+const char* flk_version(void) { return "flock-mi355x 0.1 (gfx950)"; }
+
+flk_f128 flk_f128_mul(flk_f128 a, flk_f128 b) { return flk::host_mul(a, b); }
+
+int flk_ntt_twiddles(size_t dim, flk_f128* out) {
+  if (dim > 30) return flk::set_error(FLK_E_SHAPE, "flk_ntt_twiddles: dim %zu too large", dim);
+  if (!out && dim) return flk::set_error(FLK_E_SHAPE, "flk_ntt_twiddles: null output");
+  if (dim == 0) return FLK_OK;
+  const flk_f128* t = flk::twiddles_host(dim);
+  memcpy(out, t, ((size_t(1) << dim) - 1) * sizeof(flk_f128));
+  return FLK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,369 @@
+// Ligerito sumcheck reductions, eq tables and multilinear folds for gfx950
+// (CF/pcs/ligerito.rs:1162-1175,4958-5075,5709-6058,9213-9260;
+//  CF/lincheck.rs:2080-2121).
+//
+// Every reduction is an XOR sum, so partial sums are combined in any order:
+// each lane XOR-accumulates unreduced 256-bit comb products, reduces once,
+// the wave folds with cross-lane shuffles and one lane per work-group XORs the
+// result into the output with atomicXor.  The result is bit-identical to the
+// sequential sum (CF/pcs.rs:96-107: schedule width cannot change wire bytes).
+//
+// Data x data products (f.b) use a per-lane comb table of one operand in LDS,
+// laid out entry-major so a wave's 64 lanes touch 64 consecutive 16-byte slots.
+// Products by a challenge (r, alpha, eq factors) use one work-group-shared table.
+#include <hip/hip_runtime.h>
+
+#include "gf128.hpp"
+#include "internal.hpp"
+
+namespace flk {
+
+constexpr int SC_WG = 128;
+
+__device__ __forceinline__ uint4 shfl_xor4(uint4 v, int m) {
+  return make_uint4(__shfl_xor(v.x, m), __shfl_xor(v.y, m), __shfl_xor(v.z, m), __shfl_xor(v.w, m));
+}
+
+// XOR-reduce `v` over the work-group and XOR it into *out (4 atomics).
+__device__ __forceinline__ void wg_xor_reduce_to(uint4 v, uint4* red_lds, uint4* out) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = f_add(v, shfl_xor4(v, m));
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  if (lane == 0) red_lds[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint4 acc = red_lds[0];
+    for (uint32_t w = 1; w < (blockDim.x + 63u) / 64u; ++w) acc = f_add(acc, red_lds[w]);
+    uint32_t* o = reinterpret_cast<uint32_t*>(out);
+    atomicXor(o + 0, acc.x);
+    atomicXor(o + 1, acc.y);
+    atomicXor(o + 2, acc.z);
+    atomicXor(o + 3, acc.w);
+  }
+}
+
+__global__ void zero_kernel(uint4* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Build the shared table of a uniform multiplier t (threads 0..15 write).
+__device__ __forceinline__ void shared_table(uint4* tab, uint4 t) {
+  if (threadIdx.x < 16) tab[threadIdx.x] = comb_entry(t, threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// round_msg_lsb / round_msg_and_eval_lsb
+// ---------------------------------------------------------------------------
+template <bool EVAL>
+__global__ void __launch_bounds__(SC_WG) round_msg_kernel(const uint4* __restrict__ f, const uint4* __restrict__ b,
+                                                         uint64_t n_pairs, uint4* __restrict__ out) {
+  __shared__ uint4 tabs[16 * SC_WG];
+  __shared__ uint4 red[3][SC_WG / 64];
+  const uint32_t tid = threadIdx.x;
+  U256 a0, a2, a1;
+  u256_zero(a0);
+  u256_zero(a2);
+  u256_zero(a1);
+  uint4* my = tabs + tid;
+  for (uint64_t j = (uint64_t)blockIdx.x * SC_WG + tid; j < n_pairs; j += (uint64_t)gridDim.x * SC_WG) {
+    const uint4 f0 = f[2 * j], f1 = f[2 * j + 1], b0 = b[2 * j], b1 = b[2 * j + 1];
+    table_store_private(tabs, SC_WG, tid, f0);
+    comb_mul_acc_rt(b0, my, SC_WG, a0);
+    table_store_private(tabs, SC_WG, tid, f_add(f0, f1));
+    comb_mul_acc_rt(f_add(b0, b1), my, SC_WG, a2);
+    if (EVAL) {
+      table_store_private(tabs, SC_WG, tid, f1);
+      comb_mul_acc_rt(b1, my, SC_WG, a1);
+    }
+  }
+  const uint4 u0 = f_reduce(a0), u2 = f_reduce(a2);
+  wg_xor_reduce_to(u0, red[0], out + 0);
+  wg_xor_reduce_to(u2, red[1], out + 1);
+  if (EVAL) wg_xor_reduce_to(f_add(u0, f_reduce(a1)), red[2], out + 2);
+}
+
+static unsigned reduce_grid(uint64_t items) {
+  uint64_t g = (items + SC_WG - 1) / SC_WG;
+  if (g > 4096) g = 4096;
+  if (g == 0) g = 1;
+  return (unsigned)g;
+}
+
+static int launch_zero(uint4* p, uint32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, p, n);
+  FLK_LAUNCH_CHECK("zero_kernel");
+  return FLK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fold_and_msg_lsb: one lane owns one output pair (4 inputs of f and b).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(SC_WG) fold_and_msg_kernel(const uint4* __restrict__ f, const uint4* __restrict__ b,
+                                                            uint64_t n_quads, uint4 r, uint4* __restrict__ f_out,
+                                                            uint4* __restrict__ b_out, uint4* __restrict__ out) {
+  __shared__ uint4 tabs[16 * SC_WG];
+  __shared__ uint4 rtab[16];
+  __shared__ uint4 red[2][SC_WG / 64];
+  const uint32_t tid = threadIdx.x;
+  shared_table(rtab, r);
+  __syncthreads();
+  U256 a0, a2;
+  u256_zero(a0);
+  u256_zero(a2);
+  uint4* my = tabs + tid;
+  for (uint64_t q = (uint64_t)blockIdx.x * SC_WG + tid; q < n_quads; q += (uint64_t)gridDim.x * SC_WG) {
+    const uint4* fp = f + 4 * q;
+    const uint4* bp = b + 4 * q;
+    const uint4 fa = fp[0], fb = fp[1], fc = fp[2], fd = fp[3];
+    const uint4 ba = bp[0], bb = bp[1], bc = bp[2], bd = bp[3];
+    const uint4 nf0 = f_add(fa, comb_mul<1>(f_add(fa, fb), rtab));
+    const uint4 nf1 = f_add(fc, comb_mul<1>(f_add(fc, fd), rtab));
+    const uint4 nb0 = f_add(ba, comb_mul<1>(f_add(ba, bb), rtab));
+    const uint4 nb1 = f_add(bc, comb_mul<1>(f_add(bc, bd), rtab));
+    f_out[2 * q] = nf0;
+    f_out[2 * q + 1] = nf1;
+    b_out[2 * q] = nb0;
+    b_out[2 * q + 1] = nb1;
+    table_store_private(tabs, SC_WG, tid, nf0);
+    comb_mul_acc_rt(nb0, my, SC_WG, a0);
+    table_store_private(tabs, SC_WG, tid, f_add(nf0, nf1));
+    comb_mul_acc_rt(f_add(nb0, nb1), my, SC_WG, a2);
+  }
+  wg_xor_reduce_to(f_reduce(a0), red[0], out + 0);
+  wg_xor_reduce_to(f_reduce(a2), red[1], out + 1);
+}
+
+// ---------------------------------------------------------------------------
+// glue: b += alpha * b_new
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) glue_kernel(uint4* __restrict__ b, const uint4* __restrict__ bn, uint64_t n,
+                                                  uint4 alpha) {
+  __shared__ uint4 atab[16];
+  shared_table(atab, alpha);
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+    b[i] = f_add(b[i], comb_mul<1>(bn[i], atab));
+}
+
+// ---------------------------------------------------------------------------
+// eq table: out[i] = prod_j (bit_j(i) ? r_j : 1 + r_j)  (LSB first)
+// ---------------------------------------------------------------------------
+struct EqFactors {
+  uint4 e[16];  // e[k] = prod over the pass's t coordinates, selected by the bits of k
+};
+
+// First 2^s entries (s <= 8) by one work-group: thread i multiplies its s factors.
+__global__ void eq_seed_kernel(uint4* __restrict__ out, EqFactors one_plus, EqFactors rr, uint32_t s) {
+  const uint32_t i = threadIdx.x;
+  if (i >= (1u << s)) return;
+  uint4 acc = make_uint4(1u, 0u, 0u, 0u);
+  for (uint32_t j = 0; j < s; ++j) {
+    const uint4 fac = ((i >> j) & 1u) ? rr.e[j] : one_plus.e[j];
+    acc = j == 0 ? fac : f_mul_slow(acc, fac);
+  }
+  out[i] = acc;
+}
+
+// out[i + k*len] = out[i] * E[k] for k < 2^t, i < len (in place; reads [0,len)).
+__global__ void __launch_bounds__(256) eq_expand_kernel(uint4* __restrict__ out, uint64_t len, uint32_t t, EqFactors E) {
+  __shared__ uint4 tabs[16 * 16];
+  for (uint32_t e = threadIdx.x; e < (16u << t); e += 256) tabs[e] = comb_entry(E.e[e >> 4], e & 15u);
+  __syncthreads();
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= len) return;
+  const uint4 v = out[i];
+  for (uint32_t k = 0; k < (1u << t); ++k) out[i + k * len] = comb_mul<1>(v, tabs + 16 * k);
+}
+
+// ---------------------------------------------------------------------------
+// partial_eval_lsb: T folds per pass, out[i] = e0 + r (e0 + e1) each fold.
+// ---------------------------------------------------------------------------
+template <int T>
+__global__ void __launch_bounds__(256) fold_pass_kernel(const uint4* __restrict__ in, uint4* __restrict__ out,
+                                                       uint64_t n_out, EqFactors rs) {
+  __shared__ uint4 tabs[16 * T];
+  for (uint32_t e = threadIdx.x; e < 16u * T; e += 256) tabs[e] = comb_entry(rs.e[e >> 4], e & 15u);
+  __syncthreads();
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_out) return;
+  uint4 x[1 << T];
+#pragma unroll
+  for (int j = 0; j < (1 << T); ++j) x[j] = in[(i << T) + j];
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+#pragma unroll
+    for (int j = 0; j < (1 << (T - 1 - t)); ++j) {
+      const uint4 e0 = x[2 * j], e1 = x[2 * j + 1];
+      x[j] = f_add(e0, comb_mul<1>(f_add(e0, e1), tabs + 16 * t));
+    }
+  }
+  out[i] = x[0];
+}
+
+static inline uint4 to_u4(flk_f128 a) {
+  return make_uint4((uint32_t)a.lo, (uint32_t)(a.lo >> 32), (uint32_t)a.hi, (uint32_t)(a.hi >> 32));
+}
+
+}  // namespace flk
+
+using flk::SC_WG;
+
+extern "C" {
+
+int flk_sc_round_msg(const flk_f128* f, const flk_f128* b, size_t n, flk_f128* out2, void* stream) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!f || !b || !out2) return flk::set_error(FLK_E_SHAPE, "round_msg: null pointer");
+  if (!flk::is_pow2(n) || n < 2) return flk::set_error(FLK_E_SHAPE, "round_msg: n=%zu must be a power of two >= 2", n);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  uint4* o = reinterpret_cast<uint4*>(out2);
+  if ((rc = flk::launch_zero(o, 2, st))) return rc;
+  const uint64_t pairs = n / 2;
+  hipLaunchKernelGGL(flk::round_msg_kernel<false>, dim3(flk::reduce_grid(pairs)), dim3(SC_WG), 0, st,
+                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, o);
+  FLK_LAUNCH_CHECK("round_msg_kernel");
+  return FLK_OK;
+}
+
+int flk_sc_round_msg_and_eval(const flk_f128* f, const flk_f128* b, size_t n, flk_f128* out3, void* stream) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!f || !b || !out3) return flk::set_error(FLK_E_SHAPE, "round_msg_and_eval: null pointer");
+  if (!flk::is_pow2(n) || n < 2)
+    return flk::set_error(FLK_E_SHAPE, "round_msg_and_eval: n=%zu must be a power of two >= 2", n);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  uint4* o = reinterpret_cast<uint4*>(out3);
+  if ((rc = flk::launch_zero(o, 3, st))) return rc;
+  const uint64_t pairs = n / 2;
+  hipLaunchKernelGGL(flk::round_msg_kernel<true>, dim3(flk::reduce_grid(pairs)), dim3(SC_WG), 0, st,
+                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, o);
+  FLK_LAUNCH_CHECK("round_msg_kernel<eval>");
+  return FLK_OK;
+}
+
+int flk_sc_fold_and_msg(const flk_f128* f, const flk_f128* b, size_t n, flk_f128 r, flk_f128* f_out, flk_f128* b_out,
+                        flk_f128* out2, void* stream) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!f || !b || !f_out || !b_out || !out2) return flk::set_error(FLK_E_SHAPE, "fold_and_msg: null pointer");
+  if (!flk::is_pow2(n) || n < 4) return flk::set_error(FLK_E_SHAPE, "fold_and_msg: n=%zu must be a power of two >= 4", n);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  uint4* o = reinterpret_cast<uint4*>(out2);
+  if ((rc = flk::launch_zero(o, 2, st))) return rc;
+  const uint64_t quads = n / 4;
+  hipLaunchKernelGGL(flk::fold_and_msg_kernel, dim3(flk::reduce_grid(quads)), dim3(SC_WG), 0, st,
+                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), quads, flk::to_u4(r),
+                     reinterpret_cast<uint4*>(f_out), reinterpret_cast<uint4*>(b_out), o);
+  FLK_LAUNCH_CHECK("fold_and_msg_kernel");
+  return FLK_OK;
+}
+
+int flk_sc_glue(flk_f128* b, const flk_f128* b_new, size_t n, flk_f128 alpha, void* stream) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!b || !b_new) return flk::set_error(FLK_E_SHAPE, "glue: null pointer");
+  if (n == 0) return FLK_OK;
+  uint64_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(flk::glue_kernel, dim3((unsigned)g), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     reinterpret_cast<uint4*>(b), reinterpret_cast<const uint4*>(b_new), (uint64_t)n, flk::to_u4(alpha));
+  FLK_LAUNCH_CHECK("glue_kernel");
+  return FLK_OK;
+}
+
+int flk_build_eq(const flk_f128* point, size_t d, flk_f128* out, void* stream) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!out || (d && !point)) return flk::set_error(FLK_E_SHAPE, "build_eq: null pointer");
+  if (d > 34) return flk::set_error(FLK_E_SHAPE, "build_eq: d=%zu too large", d);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  uint4* o = reinterpret_cast<uint4*>(out);
+  const size_t s = d < 8 ? d : 8;
+  flk::EqFactors onep{}, rr{};
+  for (size_t j = 0; j < s; ++j) {
+    rr.e[j] = flk::to_u4(point[j]);
+    onep.e[j] = flk::to_u4(flk_f128{point[j].lo ^ 1ull, point[j].hi});
+  }
+  if (d == 0) {
+    // single entry 1
+    hipLaunchKernelGGL(flk::eq_seed_kernel, dim3(1), dim3(64), 0, st, o, onep, rr, 0u);
+    FLK_LAUNCH_CHECK("eq_seed_kernel");
+    return FLK_OK;
+  }
+  hipLaunchKernelGGL(flk::eq_seed_kernel, dim3(1), dim3(256), 0, st, o, onep, rr, (uint32_t)s);
+  FLK_LAUNCH_CHECK("eq_seed_kernel");
+  size_t done = s;
+  while (done < d) {
+    const size_t t = (d - done) < 4 ? (d - done) : 4;
+    flk::EqFactors E{};
+    for (size_t k = 0; k < (size_t(1) << t); ++k) {
+      flk_f128 acc{1, 0};
+      for (size_t bit = 0; bit < t; ++bit) {
+        flk_f128 r = point[done + bit];
+        flk_f128 fac = ((k >> bit) & 1) ? r : flk_f128{r.lo ^ 1ull, r.hi};
+        acc = flk::host_mul(acc, fac);
+      }
+      E.e[k] = flk::to_u4(acc);
+    }
+    const uint64_t len = uint64_t(1) << done;
+    hipLaunchKernelGGL(flk::eq_expand_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, o, len, (uint32_t)t,
+                       E);
+    FLK_LAUNCH_CHECK("eq_expand_kernel");
+    done += t;
+  }
+  return FLK_OK;
+}
+
+int flk_partial_eval_lsb(const flk_f128* evals, size_t n, const flk_f128* rs, size_t k, flk_f128* out, void* stream) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!evals || !out || (k && !rs)) return flk::set_error(FLK_E_SHAPE, "partial_eval: null pointer");
+  if (!flk::is_pow2(n) || (size_t(1) << k) > n)
+    return flk::set_error(FLK_E_SHAPE, "partial_eval: n=%zu k=%zu", n, k);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (k == 0) {
+    hipError_t e = hipMemcpyAsync(out, evals, n * sizeof(flk_f128), hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return flk::device_failure(e, "partial_eval copy");
+    return FLK_OK;
+  }
+  const uint4* src = reinterpret_cast<const uint4*>(evals);
+  const int n_passes = (int)((k + 3) / 4);
+  // Passes never fold in place: intermediates ping-pong between two scratch
+  // regions (n/16 and n/256 elements) and the last pass writes `out`.
+  uint4* tmp = nullptr;
+  if (n_passes > 1) {
+    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&tmp), ((n >> 4) + (n >> 8) + 1) * sizeof(uint4), st);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return flk::set_error(FLK_E_NOMEM, "partial_eval: scratch allocation failed");
+    }
+  }
+  uint4* scratch[2] = {tmp, tmp ? tmp + (n >> 4) : nullptr};
+  size_t cur = n, done = 0;
+  for (int pass = 0; pass < n_passes; ++pass) {
+    const size_t t = (k - done) < 4 ? (k - done) : 4;
+    flk::EqFactors R{};
+    for (size_t j = 0; j < t; ++j) R.e[j] = flk::to_u4(rs[done + j]);
+    const uint64_t n_out = cur >> t;
+    uint4* dst = (pass == n_passes - 1) ? reinterpret_cast<uint4*>(out) : scratch[pass & 1];
+    const unsigned g = (unsigned)((n_out + 255) / 256);
+    switch (t) {
+      case 1: hipLaunchKernelGGL(flk::fold_pass_kernel<1>, dim3(g), dim3(256), 0, st, src, dst, n_out, R); break;
+      case 2: hipLaunchKernelGGL(flk::fold_pass_kernel<2>, dim3(g), dim3(256), 0, st, src, dst, n_out, R); break;
+      case 3: hipLaunchKernelGGL(flk::fold_pass_kernel<3>, dim3(g), dim3(256), 0, st, src, dst, n_out, R); break;
+      default: hipLaunchKernelGGL(flk::fold_pass_kernel<4>, dim3(g), dim3(256), 0, st, src, dst, n_out, R); break;
+    }
+    FLK_LAUNCH_CHECK("fold_pass_kernel");
+    src = dst;
+    cur = n_out;
+    done += t;
+  }
+  if (tmp) {
+    hipError_t e = hipFreeAsync(tmp, st);
+    if (e != hipSuccess) return flk::device_failure(e, "partial_eval free");
+  }
+  return FLK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,320 @@
+"""Python mirror of the reference's hot-path interface over the device library.
+
+Names, argument meaning and error behaviour follow the reference entry points
+(`CF/` = crates/flock-core/src/):
+
+  AdditiveNttF128.standard / twiddle / rs_encode_interleaved /
+      forward_transform_interleaved        CF/ntt/additive_ntt_f128.rs:2661-2830
+  merkle_tree / hash_leaves / hash_pairs_level  CF/merkle.rs:498,683,753
+  PcsParams / commit                       CF/pcs/commit.rs:34-140,187
+  ligero_commit                            CF/pcs/ligerito.rs:4750
+  build_eq_table                           CF/lincheck.rs:2080
+  partial_eval_lsb                         CF/pcs/ligerito.rs:1162
+  SumcheckProver (new/fold/introduce_new/glue)  CF/pcs/ligerito.rs:8893-9260
+
+Buffers are torch tensors on the GPU: F128 arrays are int64 tensors of shape
+(n, 2) holding (lo, hi); hashes are uint8 tensors of shape (n, 32).  Work is
+enqueued on torch's current stream.  Shape errors raise ValueError before any
+launch; device failures raise FlockGpuError (and latch the library off, as the
+reference's GPU facade does, CF/gpu.rs:455-466).
+"""
+from dataclasses import dataclass
+
+import torch
+
+from . import _native as nat
+from ._native import F128, FlockGpuError  # noqa: F401  (re-exported)
+
+
+def _stream():
+    return ctypes_void(torch.cuda.current_stream().cuda_stream)
+
+
+def ctypes_void(v):
+    import ctypes
+    return ctypes.c_void_p(v)
+
+
+def _ptr(t):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _check_f128(t, name):
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.int64 and t.dim() == 2
+            and t.shape[1] == 2 and t.is_contiguous()):
+        raise ValueError(f"{name} must be a contiguous CUDA int64 tensor of shape (n, 2)")
+    return t.shape[0]
+
+
+def f128_empty(n, device="cuda"):
+    return torch.empty((n, 2), dtype=torch.int64, device=device)
+
+
+def hashes_empty(n, device="cuda"):
+    return torch.empty((n, 32), dtype=torch.uint8, device=device)
+
+
+def _pow2(n):
+    return n > 0 and (n & (n - 1)) == 0
+
+
+def _log2(n):
+    return n.bit_length() - 1
+
+
+class AdditiveNttF128:
+    """AdditiveNttF128::standard(dim): basis {1, x, ..., x^(dim-1)}."""
+
+    def __init__(self, dim):
+        if not 0 <= dim <= 24:
+            raise ValueError("standard NTT: dim must be in 0..24 on the device")
+        self.dim = dim
+        self._tw = None
+
+    @classmethod
+    def standard(cls, dim):
+        return cls(dim)
+
+    def log_domain_size(self):
+        return self.dim
+
+    def twiddles(self):
+        """Breadth-first twiddle table (2^dim - 1 values, layer l at 2^l - 1) as ints."""
+        if self._tw is None:
+            n = (1 << self.dim) - 1
+            buf = (F128 * max(n, 1))()
+            nat.call("flk_ntt_twiddles", self.dim, buf)
+            self._tw = [buf[i].to_int() for i in range(n)]
+        return self._tw
+
+    def twiddle(self, layer, block):
+        if not (layer < self.dim and block < (1 << layer)):
+            raise ValueError("twiddle: (layer, block) out of range")
+        return self.twiddles()[(1 << layer) - 1 + block]
+
+    def forward_transform_interleaved(self, data, num_ntts, start_layer=0):
+        n = _check_f128(data, "data")
+        if not _pow2(num_ntts) or n % num_ntts or not _pow2(n // num_ntts):
+            raise ValueError("forward_transform_interleaved: bad shape")
+        if _log2(n // num_ntts) > self.dim or start_layer > _log2(n // num_ntts):
+            raise ValueError("forward_transform_interleaved: domain exceeds the NTT")
+        nat.call("flk_ntt_forward_interleaved", self.dim, _ptr(data), n, num_ntts, start_layer, _stream())
+
+    def rs_encode_interleaved(self, msg, codeword, num_ntts):
+        n_msg = _check_f128(msg, "msg")
+        n_cw = _check_f128(codeword, "codeword")
+        if not _pow2(num_ntts) or n_msg == 0 or n_msg % num_ntts or n_cw % n_msg:
+            raise ValueError("rs_encode_interleaved: bad shape")
+        inv_rate = n_cw // n_msg
+        if not _pow2(inv_rate) or inv_rate < 2:
+            raise ValueError("rs_encode_interleaved: inverse rate must be a power of two >= 2")
+        if _log2(n_cw // num_ntts) > self.dim:
+            raise ValueError("rs_encode_interleaved: domain exceeds the NTT")
+        nat.call("flk_ntt_rs_encode_interleaved", self.dim, _ptr(msg), n_msg, _ptr(codeword), n_cw, num_ntts,
+                 _stream())
+
+
+def hash_leaves(data, leaf_size, out):
+    """merkle::hash_leaves (BLAKE3): out[i] = non-root CV of leaf i."""
+    n = out.shape[0]
+    if data.numel() * data.element_size() != n * leaf_size:
+        raise ValueError("hash_leaves: data length != n_leaves * leaf_size")
+    nat.call("flk_merkle_hash_leaves", _ptr(data), leaf_size, n, _ptr(out), _stream())
+
+
+def hash_pairs_level(children, out):
+    n_children = children.shape[0]
+    if out.shape[0] * 2 != n_children:
+        raise ValueError("hash_pairs_level: output must have half as many nodes")
+    nat.call("flk_merkle_hash_pairs_level", _ptr(children), n_children, _ptr(out), _stream())
+
+
+def merkle_tree(data, num_leaves, kind=nat.FLK_HASH_BLAKE3, tree=None):
+    """merkle::merkle_tree: flat 2n-1 tree (leaves first, root last)."""
+    nbytes = data.numel() * data.element_size()
+    if not _pow2(num_leaves) or nbytes % num_leaves:
+        raise ValueError("merkle_tree: num_leaves must be a power of two dividing the data length")
+    if tree is None:
+        tree = hashes_empty(2 * num_leaves - 1, data.device)
+    nat.call("flk_merkle_tree", _ptr(data), nbytes, num_leaves, kind, _ptr(tree), _stream())
+    return tree
+
+
+@dataclass
+class PcsParams:
+    """CF/pcs/commit.rs:34-95 (LOG_PACKING = 7)."""
+
+    m: int
+    log_inv_rate: int = 1
+    log_batch_size: int = 6
+    merkle_hash: int = nat.FLK_HASH_BLAKE3
+
+    def log_msg_len(self):
+        return self.m - 7
+
+    def log_dim(self):
+        return self.log_msg_len() - self.log_batch_size
+
+    def k_code(self):
+        return self.log_dim() + self.log_inv_rate
+
+    def n_positions(self):
+        return 1 << self.k_code()
+
+    def num_ntts(self):
+        return 1 << self.log_batch_size
+
+    def codeword_len_f128(self):
+        return self.n_positions() * self.num_ntts()
+
+    def n_leaves(self):
+        return self.n_positions()
+
+    def leaf_size_bytes(self):
+        return 16 << self.log_batch_size
+
+
+@dataclass
+class ProverData:
+    codeword: torch.Tensor
+    merkle_tree: torch.Tensor
+
+    def root(self):
+        return self.merkle_tree[-1]
+
+
+def ligero_commit(poly, log_msg_cols, log_num_interleaved, log_inv_rate, kind=nat.FLK_HASH_BLAKE3,
+                  codeword=None, tree=None):
+    """ligerito::ligero_commit: RS-encode the interleaved matrix, Merkle-hash its rows."""
+    n = _check_f128(poly, "poly")
+    if n != 1 << (log_msg_cols + log_num_interleaved):
+        raise ValueError("ligero_commit: poly length != 2^(log_msg_cols + log_num_interleaved)")
+    n_pos = 1 << (log_msg_cols + log_inv_rate)
+    if codeword is None:
+        codeword = f128_empty(n_pos << log_num_interleaved, poly.device)
+    if tree is None:
+        tree = hashes_empty(2 * n_pos - 1, poly.device)
+    nat.call("flk_commit", _ptr(poly), log_msg_cols, log_num_interleaved, log_inv_rate, kind, _ptr(codeword),
+             _ptr(tree), None, _stream())
+    return ProverData(codeword, tree)
+
+
+def commit(z_packed, params):
+    """pcs::commit: returns (root tensor, ProverData)."""
+    n = _check_f128(z_packed, "z_packed")
+    if n != 1 << params.log_msg_len():
+        raise ValueError("commit: z_packed length must be 2^(m-7)")
+    if params.log_inv_rate < 1:
+        raise ValueError("log_inv_rate must be >= 1 for a non-trivial RS code")
+    pd = ligero_commit(z_packed, params.log_dim(), params.log_batch_size, params.log_inv_rate, params.merkle_hash)
+    return pd.root(), pd
+
+
+def build_eq_table(point, out=None, device="cuda"):
+    """lincheck::build_eq_table: LSB-first eq table of `point` (list of ints)."""
+    d = len(point)
+    if out is None:
+        out = f128_empty(1 << d, device)
+    arr = (F128 * max(d, 1))(*[F128.from_int(v) for v in point])
+    nat.call("flk_build_eq", arr, d, _ptr(out), _stream())
+    return out
+
+
+def partial_eval_lsb(evals, rs, out=None):
+    n = _check_f128(evals, "evals")
+    k = len(rs)
+    if not _pow2(n) or (1 << k) > n:
+        raise ValueError("partial_eval_lsb: need 2^k <= len(evals), power of two")
+    if out is None:
+        out = f128_empty(n >> k, evals.device)
+    arr = (F128 * max(k, 1))(*[F128.from_int(v) for v in rs])
+    nat.call("flk_partial_eval_lsb", _ptr(evals), n, arr, k, _ptr(out), _stream())
+    return out
+
+
+def round_msg(f, b, out=None):
+    n = _check_f128(f, "f")
+    if _check_f128(b, "b") != n or not _pow2(n) or n < 2:
+        raise ValueError("round_msg: f and b must have equal power-of-two length >= 2")
+    if out is None:
+        out = f128_empty(2, f.device)
+    nat.call("flk_sc_round_msg", _ptr(f), _ptr(b), n, _ptr(out), _stream())
+    return out
+
+
+def round_msg_and_eval(f, b, out=None):
+    n = _check_f128(f, "f")
+    if _check_f128(b, "b") != n or not _pow2(n) or n < 2:
+        raise ValueError("round_msg_and_eval: f and b must have equal power-of-two length >= 2")
+    if out is None:
+        out = f128_empty(3, f.device)
+    nat.call("flk_sc_round_msg_and_eval", _ptr(f), _ptr(b), n, _ptr(out), _stream())
+    return out
+
+
+def fold_and_msg(f, b, r, f_out=None, b_out=None, out=None):
+    n = _check_f128(f, "f")
+    if _check_f128(b, "b") != n or not _pow2(n) or n < 4:
+        raise ValueError("fold_and_msg: f and b must have equal power-of-two length >= 4")
+    f_out = f128_empty(n // 2, f.device) if f_out is None else f_out
+    b_out = f128_empty(n // 2, f.device) if b_out is None else b_out
+    out = f128_empty(2, f.device) if out is None else out
+    nat.call("flk_sc_fold_and_msg", _ptr(f), _ptr(b), n, F128.from_int(r), _ptr(f_out), _ptr(b_out), _ptr(out),
+             _stream())
+    return f_out, b_out, out
+
+
+def glue(b, b_new, alpha):
+    n = _check_f128(b, "b")
+    if _check_f128(b_new, "b_new") != n:
+        raise ValueError("glue: length mismatch")
+    nat.call("flk_sc_glue", _ptr(b), _ptr(b_new), n, F128.from_int(alpha), _stream())
+
+
+class SumcheckProver:
+    """ligerito::SumcheckProver over device-resident f and combined basis
+    (CF/pcs/ligerito.rs:8893-9260): new -> first message; fold(r) -> message of
+    the folded pair; introduce_new(b_new, h_new) -> message of (f, b_new);
+    glue(alpha) -> b += alpha b_new, T += alpha h_new.  Messages are (u_0, u_2)
+    device tensors; the running target T is kept as an int."""
+
+    def __init__(self, f, b1, h1):
+        if _check_f128(f, "f") != _check_f128(b1, "b1"):
+            raise ValueError("SumcheckProver: f and b1 lengths differ")
+        self.f = f
+        self.combined_basis = b1
+        self.t_r = h1
+        self.transcript = []
+        self.pending_glue = None
+
+    @classmethod
+    def new(cls, f, b1, h1):
+        inst = cls(f, b1, h1)
+        msg = round_msg(inst.f, inst.combined_basis)
+        inst.transcript.append(msg)
+        return inst, msg
+
+    def fold(self, r):
+        if self.pending_glue is not None:
+            raise RuntimeError("fold before ordinary glue")
+        nf, nb, msg = fold_and_msg(self.f, self.combined_basis, r)
+        self.f, self.combined_basis = nf, nb
+        self.transcript.append(msg)
+        return msg
+
+    def introduce_new(self, b_new, h_new):
+        if _check_f128(b_new, "b_new") != self.f.shape[0]:
+            raise ValueError("introduce_new: length mismatch")
+        msg = round_msg(self.f, b_new)
+        self.transcript.append(msg)
+        self.pending_glue = (b_new, h_new)
+        return msg
+
+    def glue(self, alpha):
+        if self.pending_glue is None:
+            raise RuntimeError("glue without introduce_new")
+        b_new, h_new = self.pending_glue
+        self.pending_glue = None
+        glue(self.combined_basis, b_new, alpha)
+        self.t_r ^= nat.f128_mul(alpha, h_new)

@@ -1,0 +1,157 @@
+/*
+ * flock_gpu.h — C ABI of the MI355X (gfx950) implementation of the Flock
+ * prover's data-parallel hot path.
+ *
+ * Every entry point mirrors a reference entry point (cited per declaration,
+ * `CF/` = crates/flock-core/src/).  Rust slices become (pointer, length);
+ * `&mut [T]` outputs become caller-owned output pointers.  Unless a
+ * declaration says "host", every pointer is a DEVICE pointer (HBM-resident
+ * buffers, e.g. from hipMalloc or a torch CUDA tensor) and the work is
+ * enqueued on `stream` (a hipStream_t; NULL = the default stream) without a
+ * host synchronisation.  Nothing retains a pointer after return.
+ *
+ * Element layout: flk_f128 is 16 bytes, `lo` (x^0..x^63) then `hi`
+ * (x^64..x^127), little-endian — identical to the reference F128
+ * (CF/field/gf2_128.rs:22-27).  Hashes are 32 raw bytes.
+ *
+ * Status codes follow the reference's device-offload contract
+ * (CF/gpu.rs:13-21,455-466,1219-1221): FLK_E_SHAPE rejects a shape without
+ * latching anything; FLK_E_DEVICE is a runtime failure and latches the
+ * library disabled until flk_undisable(); FLK_E_DISABLED means the latch,
+ * flk_set_enabled(0) or the FLOCK_NO_GPU environment variable is active.
+ * There is no CPU fallback inside this library: callers decide.
+ */
+#ifndef FLOCK_GPU_H
+#define FLOCK_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    uint64_t lo;
+    uint64_t hi;
+} flk_f128;
+
+enum {
+    FLK_OK = 0,
+    FLK_E_SHAPE = 1,    /* invalid or unsupported shape; nothing latched */
+    FLK_E_DEVICE = 2,   /* HIP runtime failure; library latched disabled */
+    FLK_E_DISABLED = 3, /* disabled by latch, API or FLOCK_NO_GPU */
+    FLK_E_NOMEM = 4     /* device allocation failed; nothing latched */
+};
+
+/* Merkle hash selector, values of the reference HashKind (CF/hash.rs:24-30). */
+enum { FLK_HASH_SHA256 = 0, FLK_HASH_BLAKE3 = 1 };
+
+/* ------------------------------------------------------------------------ */
+/* Device facade — replaces crate::gpu's metal_available / is_disabled /    */
+/* undisable / set_enabled (CF/gpu.rs:67-125).                              */
+/* ------------------------------------------------------------------------ */
+
+/* 1 when a HIP device is visible and the library is not disabled. */
+int flk_device_available(void);
+/* 1 when the failure latch, flk_set_enabled(0) or FLOCK_NO_GPU disables the library. */
+int flk_is_disabled(void);
+/* Clear the failure latch (CF/gpu.rs undisable). */
+void flk_undisable(void);
+/* Enable/disable the library (CF/gpu.rs merkle::set_enabled). */
+void flk_set_enabled(int enabled);
+/* Message for the last non-OK status returned on this thread (host string). */
+const char *flk_last_error(void);
+/* Library version string (host). */
+const char *flk_version(void);
+/* Host scalar product in GF(2^128) (F128 Mul, CF/field/gf2_128.rs:140-171), for
+ * the host-side bookkeeping of running claims. */
+flk_f128 flk_f128_mul(flk_f128 a, flk_f128 b);
+
+/* ------------------------------------------------------------------------ */
+/* Additive NTT over GF(2^128), LCH novel basis, neighbours-last            */
+/* (CF/ntt/additive_ntt_f128.rs).  `dim` is AdditiveNttF128::standard(dim)'s */
+/* log domain size; a transform of 2^log_d positions needs log_d <= dim.    */
+/* ------------------------------------------------------------------------ */
+
+/* AdditiveNttF128::standard(dim) twiddle table into HOST memory `out`
+ * ((2^dim - 1) entries, layer l at offset 2^l - 1; :54-130, :2661-2693). */
+int flk_ntt_twiddles(size_t dim, flk_f128 *out);
+
+/* AdditiveNttF128::forward_transform_interleaved_from_layer (:2728-2773):
+ * in place on data[pos * num_ntts + lane], n_total = 2^log_d * num_ntts. */
+int flk_ntt_forward_interleaved(size_t dim, flk_f128 *data, size_t n_total, size_t num_ntts,
+                                size_t start_layer, void *stream);
+
+/* AdditiveNttF128::rs_encode_interleaved (:2785-2830): codeword = forward
+ * transform of [msg, 0, ...]; n_cw / n_msg is the inverse rate (a power of
+ * two >= 2).  msg and codeword must not overlap. */
+int flk_ntt_rs_encode_interleaved(size_t dim, const flk_f128 *msg, size_t n_msg, flk_f128 *codeword,
+                                  size_t n_cw, size_t num_ntts, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* BLAKE3 Merkle tree (CF/merkle.rs).  Leaf = non-root chaining value of the */
+/* leaf bytes; parent = PARENT-flagged merge (:192-203).  Flat 2n-1 layout,  */
+/* leaves first, root at index 2n-2 (:15-22).  Leaf sizes: powers of two     */
+/* from 16 to 1024 bytes.                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* merkle::hash_leaves (:498): out[i] = leaf CV of data[i*leaf_size ..]. */
+int flk_merkle_hash_leaves(const uint8_t *data, size_t leaf_size, size_t n_leaves, uint8_t *out,
+                           void *stream);
+/* merkle::hash_pairs_level (:683): parents[i] = parent(children[2i], children[2i+1]). */
+int flk_merkle_hash_pairs_level(const uint8_t *children, size_t n_children, uint8_t *parents,
+                                void *stream);
+/* merkle::merkle_tree (:753): tree_out holds 2*num_leaves - 1 hashes. */
+int flk_merkle_tree(const uint8_t *data, size_t data_len, size_t num_leaves, int hash_kind,
+                    uint8_t *tree_out, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* PCS commit = RS encode + Merkle over codeword rows.                       */
+/* ------------------------------------------------------------------------ */
+
+/* ligerito::ligero_commit (CF/pcs/ligerito.rs:4750-4868) and, with
+ * log_num_interleaved = log_batch_size, pcs::commit (CF/pcs/commit.rs:187-227):
+ * poly has 2^(log_msg_cols + log_num_interleaved) elements laid out
+ * poly[pos * 2^log_num_interleaved + lane]; the codeword has 2^log_inv_rate
+ * times as many; tree_out has 2 * 2^(log_msg_cols + log_inv_rate) - 1 hashes and
+ * its last entry is the root.  root_out (HOST, may be NULL) receives the root
+ * after the stream has been synchronised. */
+int flk_commit(const flk_f128 *poly, size_t log_msg_cols, size_t log_num_interleaved,
+               size_t log_inv_rate, int hash_kind, flk_f128 *codeword_out, uint8_t *tree_out,
+               uint8_t *root_out, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* eq tables and Ligerito sumcheck reductions (CF/pcs/ligerito.rs).          */
+/* Message outputs are device arrays of F128: out[0] = u_0, out[1] = u_2.    */
+/* ------------------------------------------------------------------------ */
+
+/* lincheck::build_eq_table (CF/lincheck.rs:2080-2121): out[i] = prod_j eq(point[j], bit_j(i)),
+ * LSB first.  `point` is a HOST array of d elements; out has 2^d elements. */
+int flk_build_eq(const flk_f128 *point, size_t d, flk_f128 *out, void *stream);
+
+/* round_msg_lsb (:4958-5019): u_0 = sum f[2j] b[2j], u_2 = sum (f[2j]+f[2j+1])(b[2j]+b[2j+1]). */
+int flk_sc_round_msg(const flk_f128 *f, const flk_f128 *b, size_t n, flk_f128 *out2, void *stream);
+
+/* round_msg_and_eval_lsb (:5020-5075): out3 = {u_0, u_2, sum f.b}. */
+int flk_sc_round_msg_and_eval(const flk_f128 *f, const flk_f128 *b, size_t n, flk_f128 *out3,
+                              void *stream);
+
+/* fold_and_msg_lsb (:5709-6058) / SumcheckProver::fold (:9012-9070):
+ * f_out[j] = f[2j] + r (f[2j] + f[2j+1]), same for b, then the round message of
+ * (f_out, b_out) into out2.  n >= 4. */
+int flk_sc_fold_and_msg(const flk_f128 *f, const flk_f128 *b, size_t n, flk_f128 r, flk_f128 *f_out,
+                        flk_f128 *b_out, flk_f128 *out2, void *stream);
+
+/* SumcheckProver::glue (:9213-9260): b[i] += alpha * b_new[i]. */
+int flk_sc_glue(flk_f128 *b, const flk_f128 *b_new, size_t n, flk_f128 alpha, void *stream);
+
+/* partial_eval_lsb (:1162-1175): fold `evals` (n = 2^k' elements) at rs[0..k]
+ * (HOST array), LSB first; out has n >> k elements.  `evals` is not modified. */
+int flk_partial_eval_lsb(const flk_f128 *evals, size_t n, const flk_f128 *rs, size_t k, flk_f128 *out,
+                         void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLOCK_GPU_H */

@@ -1,0 +1,105 @@
+/*
+ * CPU oracle for the Flock prover hot path — TEST INFRASTRUCTURE ONLY.
+ *
+ * This library restates, in plain C, the reference prover's algorithms for the
+ * GF(2^128) arithmetic, the additive NTT / Reed-Solomon encode, the BLAKE3
+ * Merkle tree, the PCS commit and the sumcheck-family reductions.  It is the
+ * checker the GPU path is compared against and the CPU baseline that the
+ * benchmark times.  Only the test suite, the smoke check and the benchmark's
+ * CPU-baseline leg may load it; the product library never links or calls it.
+ *
+ * Reference files are cited per function as `CF/...` = crates/flock-core/src/.
+ *
+ * Element layout (CF/field/gf2_128.rs:22-27): an F128 is 16 bytes, `lo`
+ * (coefficients x^0..x^63) then `hi` (x^64..x^127), both little-endian u64.
+ */
+#ifndef FLOCK_ORACLE_H
+#define FLOCK_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { uint64_t lo, hi; } orc_f128;
+
+/* ---- GF(2^128), GHASH polynomial x^128 + x^7 + x^2 + x + 1 ------------- */
+orc_f128 orc_f128_mul(orc_f128 a, orc_f128 b);
+orc_f128 orc_f128_square(orc_f128 a);
+orc_f128 orc_f128_inv(orc_f128 a);
+orc_f128 orc_f128_mul_by_x(orc_f128 a);
+void orc_clmul64(uint64_t a, uint64_t b, uint64_t *lo, uint64_t *hi);
+void orc_clmul64_bitwise(uint64_t a, uint64_t b, uint64_t *lo, uint64_t *hi);
+void orc_f128_mul_unreduced(orc_f128 a, orc_f128 b, uint64_t out[4]);
+orc_f128 orc_ghash_reduce(const uint64_t r[4]);
+void orc_f128_mul_many(const orc_f128 *a, const orc_f128 *b, orc_f128 *out, size_t n);
+
+/* ---- GF(2^8), AES polynomial, and the phi8 embedding ------------------- */
+uint8_t orc_f8_mul(uint8_t a, uint8_t b);
+uint8_t orc_f8_inv(uint8_t a);
+/* phi8 table derived from the image of x (0x02) by the homomorphism. */
+void orc_phi8_table(orc_f128 out[256]);
+
+/* ---- SplitMix64 (reference test RNG, CF/ntt.rs:175-187) ----------------- */
+uint64_t orc_splitmix64_next(uint64_t *state);
+void orc_fill_f128(uint64_t seed, orc_f128 *out, size_t n);
+
+/* ---- Additive NTT over F128, LCH novel basis (CF/ntt/additive_ntt_f128.rs) */
+/* Twiddle table of AdditiveNttF128::standard(dim): (2^dim - 1) entries,
+ * layer l at offset 2^l - 1. */
+int orc_ntt_twiddles(size_t dim, orc_f128 *out);
+int orc_ntt_forward_interleaved(size_t dim, orc_f128 *data, size_t n_total, size_t num_ntts,
+                                size_t start_layer);
+int orc_ntt_inverse(size_t dim, orc_f128 *data, size_t n);
+int orc_ntt_rs_encode_interleaved(size_t dim, const orc_f128 *msg, size_t n_msg,
+                                  orc_f128 *codeword, size_t n_cw, size_t num_ntts);
+/* Normalised subspace polynomial Ŵ_i evaluated at an arbitrary point (brute
+ * force product over the span); used by tests to evaluate the novel basis. */
+orc_f128 orc_subspace_poly_hat(size_t i, orc_f128 z);
+
+/* ---- BLAKE3 (specification restatement) -------------------------------- */
+void orc_blake3_compress(const uint32_t cv[8], const uint8_t block[64], uint64_t counter,
+                         uint32_t block_len, uint32_t flags, uint32_t out16[16]);
+void orc_blake3_hash(const uint8_t *data, size_t len, uint8_t out[32]);
+/* Non-root chaining value of `data` (Hasher::update + finalize_non_root). */
+void orc_blake3_leaf_cv(const uint8_t *data, size_t len, uint8_t out[32]);
+/* PARENT-flagged non-root merge of two chaining values. */
+void orc_blake3_parent_cv(const uint8_t left[32], const uint8_t right[32], uint8_t out[32]);
+/* Root-flagged parent (used by tests to pin the parent path to BLAKE3 vectors). */
+void orc_blake3_parent_root(const uint8_t left[32], const uint8_t right[32], uint8_t out[32]);
+
+/* ---- Merkle tree, flat 2n-1 layout (CF/merkle.rs:15-22, 753-815) -------- */
+int orc_merkle_tree(const uint8_t *data, size_t data_len, size_t num_leaves, uint8_t *tree_out);
+int orc_merkle_proof(const uint8_t *tree, size_t num_leaves, size_t index, uint8_t *path_out);
+int orc_merkle_verify(const uint8_t root[32], const uint8_t leaf_hash[32], size_t index,
+                      const uint8_t *path, size_t path_len);
+
+/* ---- PCS commit = RS encode + Merkle (CF/pcs/commit.rs:187-227,1838-1877) */
+int orc_commit(const orc_f128 *msg, size_t n_msg, size_t num_ntts, size_t log_inv_rate,
+               orc_f128 *codeword_out, uint8_t *tree_out, uint8_t root_out[32]);
+
+/* ---- eq tables and multilinear folds ------------------------------------ */
+void orc_build_eq(const orc_f128 *point, size_t d, orc_f128 *out);
+void orc_partial_eval_lsb(const orc_f128 *evals, size_t n, const orc_f128 *rs, size_t k,
+                          orc_f128 *out);
+
+/* ---- Ligerito sumcheck (CF/pcs/ligerito.rs:4958-5019, 5709-6058, 8893-9280) */
+void orc_sc_round_msg(const orc_f128 *f, const orc_f128 *b, size_t n, orc_f128 out[2]);
+void orc_sc_fold_and_msg(const orc_f128 *f, const orc_f128 *b, size_t n, orc_f128 r,
+                         orc_f128 *f_out, orc_f128 *b_out, orc_f128 out[2]);
+void orc_sc_glue(orc_f128 *b, const orc_f128 *b_new, size_t n, orc_f128 alpha);
+void orc_sc_round_msg_and_eval(const orc_f128 *f, const orc_f128 *b, size_t n, orc_f128 out[3]);
+
+/* ---- Ligerito basis induction (CF/pcs/ligerito.rs:1186-1300,1436-1548) -- */
+void orc_eval_sk_at_vks(size_t log_n, orc_f128 *out /* log_n + 1 */);
+int orc_induce_sumcheck_poly(size_t log_msg_cols, const orc_f128 *rows, size_t n_queries,
+                             size_t row_len, const orc_f128 *v_challenges, size_t n_v,
+                             const uint64_t *queries, const orc_f128 *alpha, size_t n_alpha,
+                             orc_f128 *basis_out, orc_f128 *sum_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

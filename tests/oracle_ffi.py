@@ -1,0 +1,247 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is test infrastructure: only tests, the smoke check and the
+benchmark's CPU-baseline leg load it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+F128_DT = np.dtype([("lo", "<u8"), ("hi", "<u8")])
+
+
+class COrcF128(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64)]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def ints_to_f128(vals):
+    a = np.zeros(len(vals), dtype=F128_DT)
+    for i, v in enumerate(vals):
+        a[i]["lo"] = v & (2**64 - 1)
+        a[i]["hi"] = v >> 64
+    return a
+
+
+def f128_to_ints(a):
+    return [int(x["lo"]) | (int(x["hi"]) << 64) for x in a]
+
+
+class Oracle:
+    def __init__(self):
+        if not os.path.exists(LIB):
+            subprocess.run(["make", "-C", ORACLE_DIR], check=True, capture_output=True)
+        L = ctypes.CDLL(LIB)
+        V, P, SZ, U64 = ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64
+        F = COrcF128
+        sig = {
+            "orc_f128_mul": (F, [F, F]),
+            "orc_f128_square": (F, [F]),
+            "orc_f128_inv": (F, [F]),
+            "orc_f128_mul_by_x": (F, [F]),
+            "orc_clmul64": (None, [U64, U64, P, P]),
+            "orc_clmul64_bitwise": (None, [U64, U64, P, P]),
+            "orc_f128_mul_many": (None, [P, P, P, SZ]),
+            "orc_f8_mul": (ctypes.c_uint8, [ctypes.c_uint8, ctypes.c_uint8]),
+            "orc_f8_inv": (ctypes.c_uint8, [ctypes.c_uint8]),
+            "orc_phi8_table": (None, [P]),
+            "orc_fill_f128": (None, [U64, P, SZ]),
+            "orc_ntt_twiddles": (ctypes.c_int, [SZ, P]),
+            "orc_ntt_forward_interleaved": (ctypes.c_int, [SZ, P, SZ, SZ, SZ]),
+            "orc_ntt_inverse": (ctypes.c_int, [SZ, P, SZ]),
+            "orc_ntt_rs_encode_interleaved": (ctypes.c_int, [SZ, P, SZ, P, SZ, SZ]),
+            "orc_subspace_poly_hat": (F, [SZ, F]),
+            "orc_blake3_hash": (None, [P, SZ, P]),
+            "orc_blake3_leaf_cv": (None, [P, SZ, P]),
+            "orc_blake3_parent_cv": (None, [P, P, P]),
+            "orc_blake3_parent_root": (None, [P, P, P]),
+            "orc_merkle_tree": (ctypes.c_int, [P, SZ, SZ, P]),
+            "orc_merkle_proof": (ctypes.c_int, [P, SZ, SZ, P]),
+            "orc_merkle_verify": (ctypes.c_int, [P, P, SZ, P, SZ]),
+            "orc_commit": (ctypes.c_int, [P, SZ, SZ, SZ, P, P, P]),
+            "orc_build_eq": (None, [P, SZ, P]),
+            "orc_partial_eval_lsb": (None, [P, SZ, P, SZ, P]),
+            "orc_sc_round_msg": (None, [P, P, SZ, P]),
+            "orc_sc_round_msg_and_eval": (None, [P, P, SZ, P]),
+            "orc_sc_fold_and_msg": (None, [P, P, SZ, F, P, P, P]),
+            "orc_sc_glue": (None, [P, P, SZ, F]),
+            "orc_eval_sk_at_vks": (None, [SZ, P]),
+            "orc_induce_sumcheck_poly": (ctypes.c_int, [SZ, P, SZ, SZ, P, SZ, P, P, SZ, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        self.L = L
+
+    # ---- scalars (ints) ----
+    @staticmethod
+    def _F(v):
+        return COrcF128(v & (2**64 - 1), v >> 64)
+
+    def mul(self, a, b):
+        r = self.L.orc_f128_mul(self._F(a), self._F(b))
+        return r.lo | (r.hi << 64)
+
+    def inv(self, a):
+        r = self.L.orc_f128_inv(self._F(a))
+        return r.lo | (r.hi << 64)
+
+    def mul_by_x(self, a):
+        r = self.L.orc_f128_mul_by_x(self._F(a))
+        return r.lo | (r.hi << 64)
+
+    def clmul64(self, a, b, bitwise=False):
+        lo, hi = ctypes.c_uint64(), ctypes.c_uint64()
+        fn = self.L.orc_clmul64_bitwise if bitwise else self.L.orc_clmul64
+        fn(a, b, ctypes.byref(lo), ctypes.byref(hi))
+        return lo.value | (hi.value << 64)
+
+    def subspace_poly_hat(self, i, z):
+        r = self.L.orc_subspace_poly_hat(i, self._F(z))
+        return r.lo | (r.hi << 64)
+
+    def f8_mul(self, a, b):
+        return self.L.orc_f8_mul(a, b)
+
+    def phi8_table(self):
+        t = np.zeros(256, dtype=F128_DT)
+        self.L.orc_phi8_table(_p(t))
+        return t
+
+    # ---- arrays (numpy F128_DT) ----
+    def fill(self, seed, n):
+        a = np.zeros(n, dtype=F128_DT)
+        self.L.orc_fill_f128(seed, _p(a), n)
+        return a
+
+    def mul_many(self, a, b):
+        out = np.zeros(len(a), dtype=F128_DT)
+        self.L.orc_f128_mul_many(_p(a), _p(b), _p(out), len(a))
+        return out
+
+    def twiddles(self, dim):
+        t = np.zeros(max((1 << dim) - 1, 1), dtype=F128_DT)
+        assert self.L.orc_ntt_twiddles(dim, _p(t)) == 0
+        return t[: (1 << dim) - 1]
+
+    def forward_interleaved(self, dim, data, num_ntts, start_layer=0):
+        d = data.copy()
+        assert self.L.orc_ntt_forward_interleaved(dim, _p(d), len(d), num_ntts, start_layer) == 0
+        return d
+
+    def inverse(self, dim, data):
+        d = data.copy()
+        assert self.L.orc_ntt_inverse(dim, _p(d), len(d)) == 0
+        return d
+
+    def rs_encode(self, dim, msg, n_cw, num_ntts):
+        cw = np.zeros(n_cw, dtype=F128_DT)
+        assert self.L.orc_ntt_rs_encode_interleaved(dim, _p(msg), len(msg), _p(cw), n_cw, num_ntts) == 0
+        return cw
+
+    def blake3(self, data):
+        b = np.frombuffer(bytes(data), dtype=np.uint8).copy() if len(data) else np.zeros(1, np.uint8)
+        out = np.zeros(32, np.uint8)
+        self.L.orc_blake3_hash(_p(b), len(data), _p(out))
+        return out.tobytes()
+
+    def leaf_cv(self, data):
+        b = np.frombuffer(bytes(data), dtype=np.uint8).copy() if len(data) else np.zeros(1, np.uint8)
+        out = np.zeros(32, np.uint8)
+        self.L.orc_blake3_leaf_cv(_p(b), len(data), _p(out))
+        return out.tobytes()
+
+    def parent_cv(self, l, r, root=False):
+        a = np.frombuffer(l, np.uint8).copy()
+        b = np.frombuffer(r, np.uint8).copy()
+        out = np.zeros(32, np.uint8)
+        (self.L.orc_blake3_parent_root if root else self.L.orc_blake3_parent_cv)(_p(a), _p(b), _p(out))
+        return out.tobytes()
+
+    def merkle_tree(self, data_bytes, num_leaves):
+        d = np.frombuffer(data_bytes, np.uint8).copy() if isinstance(data_bytes, (bytes, bytearray)) else data_bytes.view(np.uint8).reshape(-1).copy()
+        tree = np.zeros((2 * num_leaves - 1, 32), np.uint8)
+        assert self.L.orc_merkle_tree(_p(d), d.size, num_leaves, _p(tree)) == 0
+        return tree
+
+    def merkle_proof(self, tree, num_leaves, index):
+        path = np.zeros((max(num_leaves.bit_length() - 1, 1), 32), np.uint8)
+        k = self.L.orc_merkle_proof(_p(tree), num_leaves, index, _p(path))
+        return path[:k]
+
+    def merkle_verify(self, root, leaf_hash, index, path):
+        r = np.frombuffer(bytes(root), np.uint8).copy()
+        lh = np.frombuffer(bytes(leaf_hash), np.uint8).copy()
+        p = np.ascontiguousarray(path) if len(path) else np.zeros((1, 32), np.uint8)
+        return bool(self.L.orc_merkle_verify(_p(r), _p(lh), index, _p(p), len(path)))
+
+    def commit(self, msg, num_ntts, log_inv_rate):
+        n_cw = len(msg) << log_inv_rate
+        n_pos = n_cw // num_ntts
+        cw = np.zeros(n_cw, dtype=F128_DT)
+        tree = np.zeros((2 * n_pos - 1, 32), np.uint8)
+        root = np.zeros(32, np.uint8)
+        assert self.L.orc_commit(_p(msg), len(msg), num_ntts, log_inv_rate, _p(cw), _p(tree), _p(root)) == 0
+        return cw, tree, root.tobytes()
+
+    def build_eq(self, point):
+        p = ints_to_f128(point) if not isinstance(point, np.ndarray) else point
+        out = np.zeros(1 << len(p), dtype=F128_DT)
+        self.L.orc_build_eq(_p(p) if len(p) else None, len(p), _p(out))
+        return out
+
+    def partial_eval_lsb(self, evals, rs):
+        r = ints_to_f128(rs)
+        out = np.zeros(len(evals) >> len(rs), dtype=F128_DT)
+        self.L.orc_partial_eval_lsb(_p(evals), len(evals), _p(r) if len(rs) else None, len(rs), _p(out))
+        return out
+
+    def round_msg(self, f, b):
+        out = np.zeros(2, dtype=F128_DT)
+        self.L.orc_sc_round_msg(_p(f), _p(b), len(f), _p(out))
+        return out
+
+    def round_msg_and_eval(self, f, b):
+        out = np.zeros(3, dtype=F128_DT)
+        self.L.orc_sc_round_msg_and_eval(_p(f), _p(b), len(f), _p(out))
+        return out
+
+    def fold_and_msg(self, f, b, r):
+        h = len(f) // 2
+        fo = np.zeros(h, dtype=F128_DT)
+        bo = np.zeros(h, dtype=F128_DT)
+        out = np.zeros(2, dtype=F128_DT)
+        self.L.orc_sc_fold_and_msg(_p(f), _p(b), len(f), self._F(r), _p(fo), _p(bo), _p(out))
+        return fo, bo, out
+
+    def glue(self, b, b_new, alpha):
+        bb = b.copy()
+        self.L.orc_sc_glue(_p(bb), _p(b_new), len(b), self._F(alpha))
+        return bb
+
+    def eval_sk_at_vks(self, log_n):
+        out = np.zeros(log_n + 1, dtype=F128_DT)
+        self.L.orc_eval_sk_at_vks(log_n, _p(out))
+        return out
+
+    def induce(self, log_msg_cols, rows, v_challenges, queries, alpha):
+        nq, row_len = rows.shape
+        v = ints_to_f128(v_challenges)
+        a = ints_to_f128(alpha)
+        q = np.asarray(queries, dtype=np.uint64)
+        basis = np.zeros(1 << log_msg_cols, dtype=F128_DT)
+        s = np.zeros(1, dtype=F128_DT)
+        rr = np.ascontiguousarray(rows)
+        assert self.L.orc_induce_sumcheck_poly(log_msg_cols, _p(rr), nq, row_len, _p(v) if len(v) else None, len(v),
+                                               _p(q), _p(a) if len(a) else None, len(a), _p(basis), _p(s)) == 0
+        return basis, f128_to_ints(s)[0]

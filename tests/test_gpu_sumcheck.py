@@ -1,0 +1,108 @@
+"""GPU parity: eq tables, LSB folds and the Ligerito sumcheck reductions on
+gfx950 against the oracle, bit-exact, through the C ABI."""
+import random
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import pyfield as pf
+from flock_challenge_multi_amd import hotpath as hp
+from gpu_util import from_dev, same, to_dev
+from oracle_ffi import f128_to_ints
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("d", [0, 1, 3, 8, 9, 12, 17, 20])
+def test_build_eq_matches_oracle(orc, dev, d):
+    _log(f"build_eq d={d}")
+    rng = random.Random(d)
+    pt = [rng.getrandbits(128) for _ in range(d)]
+    got = hp.build_eq_table(pt)
+    torch.cuda.synchronize()
+    assert same(from_dev(got), orc.build_eq(pt))
+
+
+@pytest.mark.parametrize("logn", [1, 2, 5, 10, 16, 21])
+def test_round_msg_matches_oracle(orc, dev, logn):
+    _log(f"round_msg logn={logn}")
+    f, b = orc.fill(100 + logn, 1 << logn), orc.fill(200 + logn, 1 << logn)
+    got = hp.round_msg(to_dev(f), to_dev(b))
+    got3 = hp.round_msg_and_eval(to_dev(f), to_dev(b))
+    torch.cuda.synchronize()
+    assert same(from_dev(got), orc.round_msg(f, b))
+    assert same(from_dev(got3), orc.round_msg_and_eval(f, b))
+
+
+@pytest.mark.parametrize("logn", [2, 3, 6, 12, 18, 22])
+def test_fold_and_msg_matches_oracle(orc, dev, logn):
+    _log(f"fold_and_msg logn={logn}")
+    rng = random.Random(logn)
+    r = rng.getrandbits(128)
+    f, b = orc.fill(300 + logn, 1 << logn), orc.fill(400 + logn, 1 << logn)
+    fo, bo, msg = orc.fold_and_msg(f, b, r)
+    gf, gb, gm = hp.fold_and_msg(to_dev(f), to_dev(b), r)
+    torch.cuda.synchronize()
+    assert same(from_dev(gf), fo) and same(from_dev(gb), bo) and same(from_dev(gm), msg)
+
+
+@pytest.mark.parametrize("n", [1, 7, 256, 100003])
+def test_glue_matches_oracle(orc, dev, n):
+    b, bn = orc.fill(5 + n, n), orc.fill(6 + n, n)
+    alpha = random.Random(n).getrandbits(128)
+    bt = to_dev(b)
+    hp.glue(bt, to_dev(bn), alpha)
+    torch.cuda.synchronize()
+    assert same(from_dev(bt), orc.glue(b, bn, alpha))
+
+
+@pytest.mark.parametrize("logn,k", [(1, 1), (4, 2), (8, 8), (12, 5), (16, 9), (20, 13)])
+def test_partial_eval_matches_oracle(orc, dev, logn, k):
+    _log(f"partial_eval logn={logn} k={k}")
+    rng = random.Random(logn * 31 + k)
+    ev = orc.fill(logn + k, 1 << logn)
+    rs = [rng.getrandbits(128) for _ in range(k)]
+    got = hp.partial_eval_lsb(to_dev(ev), rs)
+    torch.cuda.synchronize()
+    assert same(from_dev(got), orc.partial_eval_lsb(ev, rs))
+
+
+def test_sumcheck_prover_flow_matches_oracle(orc, dev):
+    """new -> fold x3 -> introduce_new -> glue -> fold, as the Ligerito prover
+    sequences it (CF/pcs/ligerito.rs:8906-9260), against the oracle step by step."""
+    rng = random.Random(9)
+    n = 1 << 14
+    f, b1 = orc.fill(1, n), orc.fill(2, n)
+    h1 = 0
+    for x, y in zip(f128_to_ints(f), f128_to_ints(b1)):
+        h1 ^= pf.mul(x, y)
+    prover, msg = hp.SumcheckProver.new(to_dev(f), to_dev(b1), h1)
+    assert same(from_dev(msg), orc.round_msg(f, b1))
+    cf, cb = f, b1
+    for _ in range(3):
+        r = rng.getrandbits(128)
+        msg = prover.fold(r)
+        cf, cb, want = orc.fold_and_msg(cf, cb, r)
+        assert same(from_dev(msg), want)
+    b_new = orc.fill(3, len(cf))
+    h_new = rng.getrandbits(128)
+    msg = prover.introduce_new(to_dev(b_new), h_new)
+    assert same(from_dev(msg), orc.round_msg(cf, b_new))
+    alpha = rng.getrandbits(128)
+    prover.glue(alpha)
+    cb = orc.glue(cb, b_new, alpha)
+    assert same(from_dev(prover.combined_basis), cb)
+    assert prover.t_r == h1 ^ pf.mul(alpha, h_new)
+    r = rng.getrandbits(128)
+    msg = prover.fold(r)
+    cf, cb, want = orc.fold_and_msg(cf, cb, r)
+    torch.cuda.synchronize()
+    assert same(from_dev(msg), want)
+    assert same(from_dev(prover.f), cf) and same(from_dev(prover.combined_basis), cb)
