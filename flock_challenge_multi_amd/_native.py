@@ -65,6 +65,7 @@ SIGNATURES = {
     "flk_sc_fold_and_msg": (_I, [_P, _P, _SZ, _F, _P, _P, _P, _P]),
     "flk_sc_glue": (_I, [_P, _P, _SZ, _F, _P]),
     "flk_partial_eval_lsb": (_I, [_P, _SZ, _P, _SZ, _P, _P]),
+    "flk_induce_sumcheck_poly": (_I, [_SZ, _P, _SZ, _P, _SZ, _P, _P, _SZ, _P, _P, _P]),
 }
 
 _lib = None

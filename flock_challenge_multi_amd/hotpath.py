@@ -272,6 +272,27 @@ def glue(b, b_new, alpha):
     nat.call("flk_sc_glue", _ptr(b), _ptr(b_new), n, F128.from_int(alpha), _stream())
 
 
+def induce_sumcheck_poly(log_msg_cols, opened_rows, v_challenges, queries, alpha, basis=None, enforced_sum=None):
+    """ligerito::induce_sumcheck_poly: (basis_poly, enforced_sum) on the device.
+
+    opened_rows: (n_queries * 2^len(v), 2) int64 tensor, row-major; queries: CUDA
+    int64 tensor of query indices; v_challenges, alpha: lists of ints."""
+    import ctypes
+    nq = int(queries.numel())
+    n_v = len(v_challenges)
+    if _check_f128(opened_rows, "opened_rows") != nq << n_v:
+        raise ValueError("induce_sumcheck_poly: rows must be n_queries x 2^len(v_challenges)")
+    if nq > (1 << len(alpha)):
+        raise ValueError("induce_sumcheck_poly: alpha too short for the query count")
+    basis = f128_empty(1 << log_msg_cols, opened_rows.device) if basis is None else basis
+    enforced_sum = f128_empty(1, opened_rows.device) if enforced_sum is None else enforced_sum
+    v = (F128 * max(n_v, 1))(*[F128.from_int(x) for x in v_challenges])
+    a = (F128 * max(len(alpha), 1))(*[F128.from_int(x) for x in alpha])
+    nat.call("flk_induce_sumcheck_poly", log_msg_cols, _ptr(opened_rows), nq, v, n_v, _ptr(queries), a, len(alpha),
+             _ptr(basis), _ptr(enforced_sum), _stream())
+    return basis, enforced_sum
+
+
 class SumcheckProver:
     """ligerito::SumcheckProver over device-resident f and combined basis
     (CF/pcs/ligerito.rs:8893-9260): new -> first message; fold(r) -> message of

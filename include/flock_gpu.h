@@ -151,6 +151,17 @@ int flk_sc_glue(flk_f128 *b, const flk_f128 *b_new, size_t n, flk_f128 alpha, vo
 int flk_partial_eval_lsb(const flk_f128 *evals, size_t n, const flk_f128 *rs, size_t k, flk_f128 *out,
                          void *stream);
 
+/* induce_sumcheck_poly (CF/pcs/ligerito.rs:1436-1548): basis_out (2^log_msg_cols
+ * elements) = sum_q alpha_q * X(x_q) with X_j(x) = prod_{k in bits(j)} s_k(x)/s_k(v_k)
+ * the LCH basis at x_q = queries[q]; sum_out (1 element) = sum_q alpha_q <rows[q], eq(v)>.
+ * rows: device n_queries x 2^n_v matrix (opened codeword rows); queries: device
+ * u64[n_queries]; v_challenges (n_v) and alpha (n_alpha, alpha_q = build_eq_table(alpha)[q])
+ * are HOST arrays. */
+int flk_induce_sumcheck_poly(size_t log_msg_cols, const flk_f128 *rows, size_t n_queries,
+                             const flk_f128 *v_challenges, size_t n_v, const uint64_t *queries,
+                             const flk_f128 *alpha, size_t n_alpha, flk_f128 *basis_out,
+                             flk_f128 *sum_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

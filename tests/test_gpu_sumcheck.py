@@ -106,3 +106,20 @@ def test_sumcheck_prover_flow_matches_oracle(orc, dev):
     torch.cuda.synchronize()
     assert same(from_dev(msg), want)
     assert same(from_dev(prover.f), cf) and same(from_dev(prover.combined_basis), cb)
+
+
+@pytest.mark.parametrize("log_cols,n_v,nq,n_alpha", [(0, 3, 1, 1), (4, 3, 5, 3), (9, 3, 40, 6), (13, 6, 71, 7),
+                                                   (19, 6, 218, 8)])
+def test_induce_sumcheck_poly_matches_oracle(orc, dev, log_cols, n_v, nq, n_alpha):
+    _log(f"induce cols=2^{log_cols} n_v={n_v} nq={nq}")
+    rng = random.Random(log_cols * 100 + nq)
+    rows = orc.fill(7 + log_cols, nq << n_v).reshape(nq, 1 << n_v)
+    v = [rng.getrandbits(128) for _ in range(n_v)]
+    alpha = [rng.getrandbits(128) for _ in range(n_alpha)]
+    queries = [rng.randrange(1 << (log_cols + 1)) for _ in range(nq)]
+    want_basis, want_sum = orc.induce(log_cols, rows, v, queries, alpha)
+    qt = torch.tensor(queries, dtype=torch.int64, device="cuda")
+    basis, s = hp.induce_sumcheck_poly(log_cols, to_dev(rows.reshape(-1)), v, qt, alpha)
+    torch.cuda.synchronize()
+    assert same(from_dev(basis), want_basis)
+    assert f128_to_ints(from_dev(s))[0] == want_sum
