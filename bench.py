@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-m32", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (counter calibration)")
     return ap.parse_args()
 
 
@@ -124,6 +126,17 @@ def main():
     enc_bytes = 16 * (n_msg + n_cw)  # algorithmic: message read once + codeword written once
     mer_bytes = 16 * n_cw + 32 * (2 * n_pos - 1)
     enc_gbs = enc_bytes / (enc_ms / 1e3) / 1e9
+
+    # ---- optional calibration copy: a plain kernel of known traffic ------------
+    if args.calib_copy_mib:
+        nb = args.calib_copy_mib << 20
+        src = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        dst = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        src.fill_(1)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # ---- verification of the timed output against the oracle (untimed) ------
     sys.path.insert(0, os.path.join(ROOT, "tests"))

@@ -170,6 +170,43 @@ __device__ __forceinline__ uint4 comb_mul_rt(uint4 v, const uint4* __restrict__ 
   return f_reduce(a);
 }
 
+// Unreduced byte-comb product v * t, where tab[b] = b * t for b in 0..255 (LDS,
+// 4 KiB).  Sixteen lookups (one per byte of v) and three 8-bit shifts of the
+// accumulator, versus 32 lookups and seven shifts for the 4-bit comb.
+__device__ __forceinline__ void comb8_mul_acc(uint4 v, const uint4* __restrict__ tab, U256& acc) {
+  const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+  U256 a;
+  u256_zero(a);
+#pragma unroll 1
+  for (int j = 3; j >= 0; --j) {
+    if (j != 3) {
+#pragma unroll
+      for (int i = 7; i >= 1; --i) a.w[i] = funnel(a.w[i], a.w[i - 1], 24);
+      a.w[0] <<= 8;
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const uint32_t byte = (vw[w] >> (8 * j)) & 255u;
+      const uint4 e = tab[byte];
+      a.w[w] ^= e.x;
+      a.w[w + 1] ^= e.y;
+      a.w[w + 2] ^= e.z;
+      a.w[w + 3] ^= e.w;
+    }
+  }
+  u256_xor(acc, a);
+}
+
+__device__ __forceinline__ uint4 comb8_mul(uint4 v, const uint4* __restrict__ tab) {
+  U256 a;
+  u256_zero(a);
+  comb8_mul_acc(v, tab, a);
+  return f_reduce(a);
+}
+
+// t * x^4 (four doublings).
+__device__ __forceinline__ uint4 f_mul_x4(uint4 a) { return f_mul_x(f_mul_x(f_mul_x(f_mul_x(a)))); }
+
 // Per-thread private table: entry k of thread `tid` at tab_base[k * nthreads + tid]
 // (entry-major, so a wave's 64 lanes hit 64 consecutive 16-byte slots).
 __device__ __forceinline__ void table_store_private(uint4* tab_base, uint32_t nthreads, uint32_t tid, uint4 t) {

@@ -22,6 +22,8 @@
 // :7455-7476), so the replicated codeword is never materialised.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "gf128.hpp"
 #include "internal.hpp"
 
@@ -94,6 +96,80 @@ __global__ void __launch_bounds__(NTT_WG) ntt_pass_kernel(const uint4* __restric
   }
 }
 
+// Passes whose work-group covers sets that all share the block prefix (every
+// pass except the deepest ones): the 2^K - 1 twiddles of the pass are common to
+// the whole work-group, so their 256-entry byte-comb tables (4 KiB each) are
+// built once per work-group at kernel start, and every product is a 16-lookup
+// byte comb.  Table build: the 16-entry nibble tables of t and t*x^4 first, then
+// byte entry b = T_lo[b & 15] ^ T_hi[b >> 4].
+template <int K>
+__global__ void __launch_bounds__(NTT_WG) ntt_pass_shared_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                                const uint4* __restrict__ tw, uint32_t log_d,
+                                                                uint32_t L, uint32_t log_lanes, uint64_t src_pos_mask,
+                                                                uint64_t n_threads) {
+  constexpr int NTW = (1 << K) - 1;
+  __shared__ uint4 tabs[NTW * 256];
+  __shared__ uint4 nib[NTW * 32];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t g = (uint64_t)blockIdx.x * NTT_WG + tid;
+  const bool active = g < n_threads;
+  const uint64_t c = g & ((1ull << log_lanes) - 1ull);
+  const uint64_t s = g >> log_lanes;
+  const uint32_t lo_bits = log_d - L - K;
+  const uint64_t lo = s & ((1ull << lo_bits) - 1ull);
+  // Every set of this work-group has the same prefix; take it from thread 0's set.
+  const uint64_t g0 = (uint64_t)blockIdx.x * NTT_WG;
+  const uint64_t prefix = (g0 >> log_lanes) >> lo_bits;
+
+  // Loads first so their latency overlaps the table build.
+  uint4 x[1 << K];
+#pragma unroll
+  for (int j = 0; j < (1 << K); ++j) {
+    const uint64_t p = (prefix << (log_d - L)) | ((uint64_t)j << lo_bits) | lo;
+    x[j] = active ? src[((p & src_pos_mask) << log_lanes) | c] : f_zero();
+  }
+
+  // Twiddle q (0 .. 2^K - 2) is layer L+i, sub-block sb with q = 2^i - 1 + sb.
+  for (uint32_t e = tid; e < (uint32_t)NTW * 32u; e += NTT_WG) {
+    const uint32_t q = e >> 5, half = (e >> 4) & 1u, k = e & 15u;
+    const uint32_t i = 31u - __builtin_clz(q + 1u);
+    const uint32_t sb = q + 1u - (1u << i);
+    uint4 t = tw[((1ull << (L + i)) - 1ull) + (prefix << i) + sb];
+    if (half) t = f_mul_x4(t);
+    nib[e] = comb_entry(t, k);
+  }
+  __syncthreads();
+  for (uint32_t e = tid; e < (uint32_t)NTW * 256u; e += NTT_WG) {
+    const uint32_t q = e >> 8, b = e & 255u;
+    tabs[e] = f_add(nib[q * 32 + (b & 15u)], nib[q * 32 + 16 + (b >> 4)]);
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+#pragma unroll
+    for (int sb = 0; sb < (1 << i); ++sb) {
+      const uint4* tab = tabs + (((1 << i) - 1 + sb) << 8);
+#pragma unroll
+      for (int jj = 0; jj < (1 << (K - 1 - i)); ++jj) {
+        const int j0 = (sb << (K - i)) + jj;
+        const int j1 = j0 + (1 << (K - 1 - i));
+        const uint4 prod = comb8_mul(x[j1], tab);
+        x[j0] = f_add(x[j0], prod);
+        x[j1] = f_add(x[j1], x[j0]);
+      }
+    }
+  }
+
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < (1 << K); ++j) {
+      const uint64_t p = (prefix << (log_d - L)) | ((uint64_t)j << lo_bits) | lo;
+      dst[(p << log_lanes) | c] = x[j];
+    }
+  }
+}
+
 // Plain replication (used only when no layer is left after the copy layers).
 __global__ void replicate_kernel(const uint4* __restrict__ msg, uint4* __restrict__ cw, uint64_t n_msg, uint64_t n_cw) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -101,12 +177,45 @@ __global__ void replicate_kernel(const uint4* __restrict__ msg, uint4* __restric
 }
 
 constexpr int NTT_KMAX = 4;
+// Default layers per pass: 2^3 keeps ~100 VGPRs (4 waves/SIMD) and beat 2^4 on MI355X.
+constexpr int NTT_KDEFAULT = 3;
+
+// Layers per pass (the radix-2^K of a pass).  FLOCK_NTT_RADIX_LOG (1..4)
+// overrides the default for schedule sweeps; read once per process.
+static int ntt_radix_cap() {
+  static const int cap = [] {
+    const char* v = getenv("FLOCK_NTT_RADIX_LOG");
+    int k = v ? atoi(v) : NTT_KDEFAULT;
+    return (k >= 1 && k <= NTT_KMAX) ? k : NTT_KMAX;
+  }();
+  return cap;
+}
+
+// FLOCK_NTT_SHARED_TABLES=0 keeps every pass on the per-group nibble-comb kernel.
+static bool ntt_shared_tables_enabled() {
+  static const bool on = [] {
+    const char* v = getenv("FLOCK_NTT_SHARED_TABLES");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
 
 template <int K>
 static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t log_d, uint32_t L, uint32_t log_lanes,
                        uint64_t src_pos_mask, uint32_t log_group, hipStream_t st) {
   const uint64_t n_threads = (1ull << (log_d - K)) << log_lanes;
   const uint64_t blocks = (n_threads + NTT_WG - 1) / NTT_WG;
+  // Sets per work-group = NTT_WG / num_ntts; they share the block prefix when the
+  // low position bits of the pass cover them.
+  const uint32_t lo_bits = log_d - L - K;
+  const int log_sets_per_wg = 8 - (int)log_lanes;
+  if (ntt_shared_tables_enabled() && n_threads >= (uint64_t)NTT_WG &&
+      (log_sets_per_wg <= 0 || (int)lo_bits >= log_sets_per_wg)) {
+    hipLaunchKernelGGL(ntt_pass_shared_kernel<K>, dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw, log_d, L,
+                       log_lanes, src_pos_mask, n_threads);
+    FLK_LAUNCH_CHECK("ntt_pass_shared_kernel");
+    return FLK_OK;
+  }
   const size_t lds = (size_t)(NTT_WG / 64) * (64u >> log_group) * ((size_t)16 << (K - 1)) * sizeof(uint4);
   hipLaunchKernelGGL(ntt_pass_kernel<K>, dim3((unsigned)blocks), dim3(NTT_WG), lds, st, src, dst, tw, log_d, L,
                      log_lanes, src_pos_mask, n_threads, log_group);
@@ -121,7 +230,8 @@ static int run_layers(const uint4* src, uint4* dst, const uint4* tw, uint32_t lo
   const uint32_t log_group = log_lanes < 6 ? log_lanes : 6;
   // LDS per wave is (64/G) * 2^(K-1) tables of 256 B; keep it <= 16 KiB.
   int kmax = (int)log_group + 1;
-  if (kmax > NTT_KMAX) kmax = NTT_KMAX;
+  const int kcap = ntt_radix_cap();
+  if (kmax > kcap) kmax = kcap;
   uint32_t L = start;
   const uint4* s = src;
   uint64_t mask = src_pos_mask;
