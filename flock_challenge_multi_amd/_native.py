@@ -66,6 +66,14 @@ SIGNATURES = {
     "flk_sc_glue": (_I, [_P, _P, _SZ, _F, _P]),
     "flk_partial_eval_lsb": (_I, [_P, _SZ, _P, _SZ, _P, _P]),
     "flk_induce_sumcheck_poly": (_I, [_SZ, _P, _SZ, _P, _SZ, _P, _P, _SZ, _P, _P, _P]),
+    "flk_lc_partial_fold": (_I, [_P, _SZ, _SZ, _P, _P, _P]),
+    "flk_lc_round_msg_msb": (_I, [_P, _P, _SZ, _P, _P]),
+    "flk_lc_bind_top": (_I, [_P, _SZ, _F, _P]),
+    "flk_rs_fold_1b_rows": (_I, [_P, _P, _SZ, _P, _P]),
+    "flk_rs_fold_b128_elems": (_I, [_P, _P, _SZ, _P, _P]),
+    "flk_zc_lagrange_weights": (_I, [_SZ, _F, _P]),
+    "flk_zc_fold_at_z": (_I, [_P, _SZ, _SZ, _P, _P, _P]),
+    "flk_zc_round_pair": (_I, [_P, _P, _SZ, _P, _P, _P]),
 }
 
 _lib = None

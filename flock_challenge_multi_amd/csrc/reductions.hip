@@ -1,0 +1,444 @@
+// Lincheck, ring-switch and zerocheck evaluation reductions for gfx950.
+//
+//   flk_lc_partial_fold     partial_fold_packed_z(_fast)  CF/lincheck.rs:2233-2420
+//   flk_lc_bind_top         sumcheck_bind_top_in_place    CF/lincheck.rs:4968-4990
+//   flk_lc_round_msg_msb    product-sumcheck round eval   CF/lincheck.rs:5010-5030
+//   flk_rs_fold_1b_rows     fold_1b_rows                  CF/pcs/ring_switch.rs:1594-1642
+//   flk_rs_fold_b128_elems  fold_b128_elems               CF/pcs/ring_switch.rs:1952-2087
+//   flk_zc_fold_at_z        fold_at_z                     CF/zerocheck/multilinear.rs:575-597
+//   flk_zc_round_pair       round_pair (wire form)        CF/zerocheck/multilinear.rs:616-640,
+//                                                         CF/zerocheck.rs:1748-1781
+//
+// These are XOR reductions of F2-linear maps over bit-packed witness data.  The
+// sums of known values selected by bits ("sum eq[i] over the set bits") use
+// subset-sum tables in LDS, the GPU form of the reference's method-of-four-
+// Russians tables: a 16-entry table per nibble of selector bits (conflict-free,
+// 256 B) or a 256-entry table per byte where the table is shared by a whole
+// work-group.  Products of two data values use the per-lane comb of gf128.hpp.
+#include <hip/hip_runtime.h>
+
+#include "gf128.hpp"
+#include "internal.hpp"
+
+namespace flk {
+
+__device__ __forceinline__ uint4 shfl_xor4_r(uint4 v, int m) {
+  return make_uint4(__shfl_xor(v.x, m), __shfl_xor(v.y, m), __shfl_xor(v.z, m), __shfl_xor(v.w, m));
+}
+
+__device__ __forceinline__ void atomic_xor4(uint4* out, uint4 v) {
+  uint32_t* o = reinterpret_cast<uint32_t*>(out);
+  if (v.x) atomicXor(o + 0, v.x);
+  if (v.y) atomicXor(o + 1, v.y);
+  if (v.z) atomicXor(o + 2, v.z);
+  if (v.w) atomicXor(o + 3, v.w);
+}
+
+// XOR-reduce over the work-group (blockDim multiple of 64) into *out.
+__device__ __forceinline__ void wg_reduce_xor(uint4 v, uint4* red, uint4* out) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = f_add(v, shfl_xor4_r(v, m));
+  if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint4 acc = red[0];
+    for (uint32_t w = 1; w < blockDim.x / 64u; ++w) acc = f_add(acc, red[w]);
+    atomic_xor4(out, acc);
+  }
+}
+
+__global__ void zero_n_kernel(uint4* p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+static int zero_n(uint4* p, uint64_t n, hipStream_t st) {
+  if (!n) return FLK_OK;
+  hipLaunchKernelGGL(zero_n_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+  FLK_LAUNCH_CHECK("zero_n_kernel");
+  return FLK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Lincheck partial fold.  out[i] = sum over stripes s and bits r of byte
+// z[s*k + i] of eq_outer[8s + r].  A lane owns 4 consecutive i (one u32 load per
+// stripe); per stripe the 8 eq values become two 16-entry nibble tables.
+// grid = (k / 1024, stripe splits); out must be zeroed.
+// ---------------------------------------------------------------------------
+constexpr int LC_WG = 256;
+
+__global__ void __launch_bounds__(LC_WG) lc_partial_fold_kernel(const uint32_t* __restrict__ z, uint64_t k,
+                                                               uint64_t n_stripes, uint64_t stripes_per_split,
+                                                               const uint4* __restrict__ eq, uint4* __restrict__ out) {
+  __shared__ uint4 tab[2][32];
+  const uint64_t i4 = ((uint64_t)blockIdx.x * LC_WG + threadIdx.x);  // index of the 4-byte group
+  const bool active = i4 * 4 < k;
+  const uint64_t s0 = (uint64_t)blockIdx.y * stripes_per_split;
+  uint64_t s1 = s0 + stripes_per_split;
+  if (s1 > n_stripes) s1 = n_stripes;
+  uint4 acc[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) acc[b] = f_zero();
+  for (uint64_t s = s0; s < s1; ++s) {
+    uint4* t = tab[s & 1];
+    if (threadIdx.x < 32) {
+      const uint32_t half = threadIdx.x >> 4, kk = threadIdx.x & 15u;
+      uint4 v = f_zero();
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if ((kk >> r) & 1u) v = f_add(v, eq[8 * s + 4 * half + r]);
+      t[threadIdx.x] = v;
+    }
+    __syncthreads();
+    if (active) {
+      const uint32_t w = z[(s * k) / 4 + i4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const uint32_t byte = (w >> (8 * b)) & 255u;
+        acc[b] = f_add(acc[b], f_add(t[byte & 15u], t[16 + (byte >> 4)]));
+      }
+    }
+  }
+  if (active) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) atomic_xor4(out + i4 * 4 + b, acc[b]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Lincheck product sumcheck, top variable: message and binding.
+// ---------------------------------------------------------------------------
+constexpr int RD_WG = 128;
+
+__global__ void __launch_bounds__(RD_WG) lc_round_msg_msb_kernel(const uint4* __restrict__ comb,
+                                                                const uint4* __restrict__ z, uint64_t half,
+                                                                uint4* __restrict__ out) {
+  __shared__ uint4 tabs[16 * RD_WG];
+  __shared__ uint4 red[2][RD_WG / 64];
+  U256 e1, einf;
+  u256_zero(e1);
+  u256_zero(einf);
+  uint4* my = tabs + threadIdx.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * RD_WG + threadIdx.x; i < half; i += (uint64_t)gridDim.x * RD_WG) {
+    const uint4 lo = comb[i], hi = comb[i + half], zl = z[i], zh = z[i + half];
+    table_store_private(tabs, RD_WG, threadIdx.x, hi);
+    comb_mul_acc_rt(zh, my, RD_WG, e1);
+    table_store_private(tabs, RD_WG, threadIdx.x, f_add(hi, lo));
+    comb_mul_acc_rt(f_add(zh, zl), my, RD_WG, einf);
+  }
+  wg_reduce_xor(f_reduce(e1), red[0], out + 0);
+  wg_reduce_xor(f_reduce(einf), red[1], out + 1);
+}
+
+__global__ void __launch_bounds__(256) lc_bind_top_kernel(uint4* __restrict__ v, uint64_t half, uint4 r) {
+  __shared__ uint4 rt[16];
+  if (threadIdx.x < 16) rt[threadIdx.x] = comb_entry(r, threadIdx.x);
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < half; i += (uint64_t)gridDim.x * 256) {
+    const uint4 lo = v[i], hi = v[i + half];
+    v[i] = f_add(lo, comb_mul<1>(f_add(hi, lo), rt));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ring switch fold_1b_rows: s_hat_v[r] = sum_i bit_r(z_i) eq_i, r < 128.
+// A tile of 256 elements is staged in LDS; every group of 4 elements gets a
+// 16-entry subset-sum table of its eq values; lane (r, half) turns bit r of the
+// 4 elements of each group into a nibble and looks it up.  out must be zeroed.
+// ---------------------------------------------------------------------------
+constexpr int F1_WG = 256;
+
+__global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_kernel(const uint4* __restrict__ zp,
+                                                               const uint4* __restrict__ eq, uint64_t n,
+                                                               uint4* __restrict__ out) {
+  __shared__ uint32_t zt[F1_WG * 4];
+  __shared__ uint4 tabs[(F1_WG / 4) * 16];
+  __shared__ uint4 red[128];
+  const uint32_t t = threadIdx.x;
+  const uint32_t r = t & 127u, halfsel = t >> 7;
+  uint4 acc = f_zero();
+  const uint64_t n_tiles = (n + F1_WG - 1) / F1_WG;
+  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const uint64_t i = tile * F1_WG + t;
+    uint4 zv = f_zero(), ev = f_zero();
+    if (i < n) {
+      zv = zp[i];
+      ev = eq[i];
+    }
+    zt[t * 4 + 0] = zv.x;
+    zt[t * 4 + 1] = zv.y;
+    zt[t * 4 + 2] = zv.z;
+    zt[t * 4 + 3] = zv.w;
+    // Table of group g = t/4 holds sums over subsets of its 4 eq values; this lane
+    // writes the 4 entries whose bits 2,3 equal q = t%4.  Gather the group's eq
+    // values from the 4 lanes of the group with cross-lane shuffles.
+    const int base_lane = (int)(t & ~3u) & 63;
+    uint4 e0, e1, e2, e3;
+    e0 = make_uint4(__shfl(ev.x, base_lane + 0), __shfl(ev.y, base_lane + 0), __shfl(ev.z, base_lane + 0),
+                    __shfl(ev.w, base_lane + 0));
+    e1 = make_uint4(__shfl(ev.x, base_lane + 1), __shfl(ev.y, base_lane + 1), __shfl(ev.z, base_lane + 1),
+                    __shfl(ev.w, base_lane + 1));
+    e2 = make_uint4(__shfl(ev.x, base_lane + 2), __shfl(ev.y, base_lane + 2), __shfl(ev.z, base_lane + 2),
+                    __shfl(ev.w, base_lane + 2));
+    e3 = make_uint4(__shfl(ev.x, base_lane + 3), __shfl(ev.y, base_lane + 3), __shfl(ev.z, base_lane + 3),
+                    __shfl(ev.w, base_lane + 3));
+    const uint32_t q = t & 3u;
+    uint4 base = f_zero();
+    if (q & 1u) base = f_add(base, e2);
+    if (q & 2u) base = f_add(base, e3);
+    uint4* tg = tabs + (t >> 2) * 16 + 4 * q;
+    tg[0] = base;
+    tg[1] = f_add(base, e0);
+    tg[2] = f_add(base, e1);
+    tg[3] = f_add(f_add(base, e0), e1);
+    __syncthreads();
+    const uint32_t word = r >> 5, bit = r & 31u;
+#pragma unroll 4
+    for (uint32_t g = halfsel * 32; g < halfsel * 32 + 32; ++g) {
+      const uint32_t nibv = ((zt[(4 * g + 0) * 4 + word] >> bit) & 1u) | (((zt[(4 * g + 1) * 4 + word] >> bit) & 1u) << 1) |
+                            (((zt[(4 * g + 2) * 4 + word] >> bit) & 1u) << 2) |
+                            (((zt[(4 * g + 3) * 4 + word] >> bit) & 1u) << 3);
+      acc = f_add(acc, tabs[g * 16 + nibv]);
+    }
+    __syncthreads();
+  }
+  if (halfsel == 1) red[r] = acc;
+  __syncthreads();
+  if (halfsel == 0) atomic_xor4(out + r, f_add(acc, red[r]));
+}
+
+// ---------------------------------------------------------------------------
+// Ring switch fold_b128_elems: out_i = sum_b bit_b(s_i) eq_r[b].  The 16 byte
+// tables (table[p][v] = sum of eq_r[8p + bit] over the bits of v, 64 KiB) are
+// built once per work-group; each element is 16 lookups.
+// ---------------------------------------------------------------------------
+constexpr int FB_WG = 256;
+
+__global__ void __launch_bounds__(FB_WG) rs_fold_b128_kernel(const uint4* __restrict__ sfx, const uint4* __restrict__ eqr,
+                                                            uint64_t n, uint4* __restrict__ out) {
+  __shared__ uint4 tables[16 * 256];  // byte tables, 64 KiB (gfx950 has 160 KiB of LDS per CU)
+  __shared__ uint4 nibt[32 * 16];     // nibble tables
+  for (uint32_t e = threadIdx.x; e < 32u * 16u; e += FB_WG) {
+    const uint32_t np = e >> 4, k = e & 15u;  // nibble position np covers bits 4np..4np+3
+    uint4 v = f_zero();
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if ((k >> j) & 1u) v = f_add(v, eqr[4 * np + j]);
+    nibt[e] = v;
+  }
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < 16u * 256u; e += FB_WG) {
+    const uint32_t p = e >> 8, v = e & 255u;
+    tables[e] = f_add(nibt[(2 * p) * 16 + (v & 15u)], nibt[(2 * p + 1) * 16 + (v >> 4)]);
+  }
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * FB_WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * FB_WG) {
+    const uint4 s = sfx[i];
+    const uint32_t w[4] = {s.x, s.y, s.z, s.w};
+    uint4 acc = f_zero();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc = f_add(acc, tables[(4 * q + b) * 256 + ((w[q] >> (8 * b)) & 255u)]);
+    out[i] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Zerocheck fold at z: out[x] = sum_{s < 2^k_skip} bit(x*2^k_skip + s) * w[s].
+// Byte tables of the weights (2^k_skip / 8 tables of 256 entries) per work-group.
+// ---------------------------------------------------------------------------
+constexpr int FZ_WG = 256;
+
+__global__ void __launch_bounds__(FZ_WG) zc_fold_at_z_kernel(const uint8_t* __restrict__ bits, uint32_t row_bytes,
+                                                            uint64_t n_rows, const uint4* __restrict__ w,
+                                                            uint4* __restrict__ out) {
+  __shared__ uint4 tables[8 * 256];  // row_bytes (<= 8) byte tables
+  __shared__ uint4 nibt[8 * 32];
+  for (uint32_t e = threadIdx.x; e < row_bytes * 32u; e += FZ_WG) {
+    const uint32_t np = e >> 4, k = e & 15u;
+    uint4 v = f_zero();
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if ((k >> j) & 1u) v = f_add(v, w[4 * np + j]);
+    nibt[e] = v;
+  }
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < row_bytes * 256u; e += FZ_WG) {
+    const uint32_t p = e >> 8, v = e & 255u;
+    tables[e] = f_add(nibt[(2 * p) * 16 + (v & 15u)], nibt[(2 * p + 1) * 16 + (v >> 4)]);
+  }
+  __syncthreads();
+  for (uint64_t x = (uint64_t)blockIdx.x * FZ_WG + threadIdx.x; x < n_rows; x += (uint64_t)gridDim.x * FZ_WG) {
+    const uint8_t* row = bits + x * row_bytes;
+    uint4 acc = f_zero();
+    if (row_bytes == 8) {
+      const uint2 v = *reinterpret_cast<const uint2*>(row);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc = f_add(acc, tables[b * 256 + ((v.x >> (8 * b)) & 255u)]);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc = f_add(acc, tables[(4 + b) * 256 + ((v.y >> (8 * b)) & 255u)]);
+    } else {
+      for (uint32_t b = 0; b < row_bytes; ++b) acc = f_add(acc, tables[b * 256 + row[b]]);
+    }
+    out[x] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Zerocheck multilinear round (wire form): G(1) = sum_x eq[x] a1 b1,
+// G(inf) = sum_x eq[x] (a0+a1)(b0+b1), pairs (2x, 2x+1).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(RD_WG) zc_round_pair_kernel(const uint4* __restrict__ a, const uint4* __restrict__ b,
+                                                             const uint4* __restrict__ eq, uint64_t half,
+                                                             uint4* __restrict__ out) {
+  __shared__ uint4 tabs[16 * RD_WG];
+  __shared__ uint4 red[2][RD_WG / 64];
+  U256 g1, ginf;
+  u256_zero(g1);
+  u256_zero(ginf);
+  uint4* my = tabs + threadIdx.x;
+  for (uint64_t x = (uint64_t)blockIdx.x * RD_WG + threadIdx.x; x < half; x += (uint64_t)gridDim.x * RD_WG) {
+    const uint4 a0 = a[2 * x], a1 = a[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1], e = eq[x];
+    table_store_private(tabs, RD_WG, threadIdx.x, e);
+    const uint4 ea1 = comb_mul_rt(a1, my, RD_WG);
+    const uint4 eas = comb_mul_rt(f_add(a0, a1), my, RD_WG);
+    table_store_private(tabs, RD_WG, threadIdx.x, ea1);
+    comb_mul_acc_rt(b1, my, RD_WG, g1);
+    table_store_private(tabs, RD_WG, threadIdx.x, eas);
+    comb_mul_acc_rt(f_add(b0, b1), my, RD_WG, ginf);
+  }
+  wg_reduce_xor(f_reduce(g1), red[0], out + 0);
+  wg_reduce_xor(f_reduce(ginf), red[1], out + 1);
+}
+
+static unsigned grid_for(uint64_t items, int wg, unsigned cap) {
+  uint64_t g = (items + wg - 1) / wg;
+  if (g > cap) g = cap;
+  return (unsigned)(g ? g : 1);
+}
+
+static inline uint4 u4(flk_f128 a) {
+  return make_uint4((uint32_t)a.lo, (uint32_t)(a.lo >> 32), (uint32_t)a.hi, (uint32_t)(a.hi >> 32));
+}
+
+}  // namespace flk
+
+using namespace flk;
+
+extern "C" {
+
+int flk_lc_partial_fold(const uint8_t* z_packed, size_t m, size_t k_log, const flk_f128* eq_outer, flk_f128* out,
+                        void* stream) {
+  int rc = check_enabled();
+  if (rc) return rc;
+  if (!z_packed || !eq_outer || !out) return set_error(FLK_E_SHAPE, "lc_partial_fold: null pointer");
+  if (k_log < 2 || k_log > m || m - k_log < 3 || m > 40)
+    return set_error(FLK_E_SHAPE, "lc_partial_fold: need 2 <= k_log and m - k_log >= 3 (m=%zu k_log=%zu)", m, k_log);
+  if ((uintptr_t)z_packed % 4) return set_error(FLK_E_SHAPE, "lc_partial_fold: z_packed must be 4-byte aligned");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const uint64_t k = uint64_t(1) << k_log;
+  const uint64_t n_stripes = (uint64_t(1) << (m - k_log)) / 8;
+  uint4* o = reinterpret_cast<uint4*>(out);
+  if ((rc = zero_n(o, k, st))) return rc;
+  const uint64_t gx = (k / 4 + LC_WG - 1) / LC_WG;
+  uint64_t splits = (2048 + gx - 1) / gx;
+  if (splits > n_stripes) splits = n_stripes;
+  if (splits > 65535) splits = 65535;
+  const uint64_t per = (n_stripes + splits - 1) / splits;
+  splits = (n_stripes + per - 1) / per;
+  hipLaunchKernelGGL(lc_partial_fold_kernel, dim3((unsigned)gx, (unsigned)splits), dim3(LC_WG), 0, st,
+                     reinterpret_cast<const uint32_t*>(z_packed), k, n_stripes, per,
+                     reinterpret_cast<const uint4*>(eq_outer), o);
+  FLK_LAUNCH_CHECK("lc_partial_fold_kernel");
+  return FLK_OK;
+}
+
+int flk_lc_round_msg_msb(const flk_f128* comb, const flk_f128* z, size_t n, flk_f128* out2, void* stream) {
+  int rc = check_enabled();
+  if (rc) return rc;
+  if (!comb || !z || !out2) return set_error(FLK_E_SHAPE, "lc_round_msg_msb: null pointer");
+  if (!is_pow2(n) || n < 2) return set_error(FLK_E_SHAPE, "lc_round_msg_msb: n=%zu must be a power of two >= 2", n);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  uint4* o = reinterpret_cast<uint4*>(out2);
+  if ((rc = zero_n(o, 2, st))) return rc;
+  hipLaunchKernelGGL(lc_round_msg_msb_kernel, dim3(grid_for(n / 2, RD_WG, 2048)), dim3(RD_WG), 0, st,
+                     reinterpret_cast<const uint4*>(comb), reinterpret_cast<const uint4*>(z), (uint64_t)(n / 2), o);
+  FLK_LAUNCH_CHECK("lc_round_msg_msb_kernel");
+  return FLK_OK;
+}
+
+int flk_lc_bind_top(flk_f128* v, size_t n, flk_f128 r, void* stream) {
+  int rc = check_enabled();
+  if (rc) return rc;
+  if (!v) return set_error(FLK_E_SHAPE, "lc_bind_top: null pointer");
+  if (!is_pow2(n) || n < 2) return set_error(FLK_E_SHAPE, "lc_bind_top: n=%zu must be a power of two >= 2", n);
+  hipLaunchKernelGGL(lc_bind_top_kernel, dim3(grid_for(n / 2, 256, 65536)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), reinterpret_cast<uint4*>(v), (uint64_t)(n / 2), u4(r));
+  FLK_LAUNCH_CHECK("lc_bind_top_kernel");
+  return FLK_OK;
+}
+
+int flk_rs_fold_1b_rows(const flk_f128* packed_witness, const flk_f128* suffix_tensor, size_t n, flk_f128* out128,
+                        void* stream) {
+  int rc = check_enabled();
+  if (rc) return rc;
+  if (!packed_witness || !suffix_tensor || !out128) return set_error(FLK_E_SHAPE, "rs_fold_1b_rows: null pointer");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  uint4* o = reinterpret_cast<uint4*>(out128);
+  if ((rc = zero_n(o, 128, st))) return rc;
+  if (n == 0) return FLK_OK;
+  hipLaunchKernelGGL(rs_fold_1b_rows_kernel, dim3(grid_for(n, F1_WG, 2048)), dim3(F1_WG), 0, st,
+                     reinterpret_cast<const uint4*>(packed_witness), reinterpret_cast<const uint4*>(suffix_tensor),
+                     (uint64_t)n, o);
+  FLK_LAUNCH_CHECK("rs_fold_1b_rows_kernel");
+  return FLK_OK;
+}
+
+int flk_rs_fold_b128_elems(const flk_f128* suffix_tensor, const flk_f128* eq_r_dprime, size_t n, flk_f128* out,
+                           void* stream) {
+  int rc = check_enabled();
+  if (rc) return rc;
+  if (!suffix_tensor || !eq_r_dprime || !out) return set_error(FLK_E_SHAPE, "rs_fold_b128_elems: null pointer");
+  if (n == 0) return FLK_OK;
+  hipLaunchKernelGGL(rs_fold_b128_kernel, dim3(grid_for(n, FB_WG, 1024)), dim3(FB_WG), 0,
+                     static_cast<hipStream_t>(stream), reinterpret_cast<const uint4*>(suffix_tensor),
+                     reinterpret_cast<const uint4*>(eq_r_dprime), (uint64_t)n, reinterpret_cast<uint4*>(out));
+  FLK_LAUNCH_CHECK("rs_fold_b128_kernel");
+  return FLK_OK;
+}
+
+int flk_zc_fold_at_z(const uint8_t* bits, size_t m, size_t k_skip, const flk_f128* weights, flk_f128* out,
+                     void* stream) {
+  int rc = check_enabled();
+  if (rc) return rc;
+  if (!bits || !weights || !out) return set_error(FLK_E_SHAPE, "zc_fold_at_z: null pointer");
+  if (k_skip < 3 || k_skip > 6 || k_skip > m || m > 40)
+    return set_error(FLK_E_SHAPE, "zc_fold_at_z: k_skip=%zu unsupported (3..6, <= m=%zu)", k_skip, m);
+  if (k_skip == 6 && (uintptr_t)bits % 8) return set_error(FLK_E_SHAPE, "zc_fold_at_z: bits must be 8-byte aligned");
+  const uint32_t row_bytes = (uint32_t)((size_t(1) << k_skip) / 8);
+  const uint64_t n_rows = uint64_t(1) << (m - k_skip);
+  hipLaunchKernelGGL(zc_fold_at_z_kernel, dim3(grid_for(n_rows, FZ_WG, 2048)), dim3(FZ_WG), 0,
+                     static_cast<hipStream_t>(stream), bits, row_bytes, n_rows, reinterpret_cast<const uint4*>(weights),
+                     reinterpret_cast<uint4*>(out));
+  FLK_LAUNCH_CHECK("zc_fold_at_z_kernel");
+  return FLK_OK;
+}
+
+int flk_zc_round_pair(const flk_f128* a, const flk_f128* b, size_t n, const flk_f128* eq_rest, flk_f128* out2,
+                      void* stream) {
+  int rc = check_enabled();
+  if (rc) return rc;
+  if (!a || !b || !eq_rest || !out2) return set_error(FLK_E_SHAPE, "zc_round_pair: null pointer");
+  if (!is_pow2(n) || n < 2) return set_error(FLK_E_SHAPE, "zc_round_pair: n=%zu must be a power of two >= 2", n);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  uint4* o = reinterpret_cast<uint4*>(out2);
+  if ((rc = zero_n(o, 2, st))) return rc;
+  hipLaunchKernelGGL(zc_round_pair_kernel, dim3(grid_for(n / 2, RD_WG, 2048)), dim3(RD_WG), 0, st,
+                     reinterpret_cast<const uint4*>(a), reinterpret_cast<const uint4*>(b),
+                     reinterpret_cast<const uint4*>(eq_rest), (uint64_t)(n / 2), o);
+  FLK_LAUNCH_CHECK("zc_round_pair_kernel");
+  return FLK_OK;
+}
+
+}  // extern "C"

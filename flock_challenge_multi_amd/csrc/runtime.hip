@@ -192,6 +192,35 @@ const char* flk_version(void) { return "flock-mi355x 0.1 (gfx950)"; }
 
 flk_f128 flk_f128_mul(flk_f128 a, flk_f128 b) { return flk::host_mul(a, b); }
 
+int flk_zc_lagrange_weights(size_t k_skip, flk_f128 z, flk_f128* out) {
+  // L_i(z) = prod_{j != i} (z + phi8(j)) / (phi8(i) + phi8(j)) on the nodes
+  // phi8(0 .. 2^k_skip - 1) (CF/zerocheck/multilinear.rs:285-330).  phi8 is the
+  // embedding GF(2^8) -> GF(2^128) fixed by the image of x (0x02).
+  if (k_skip > 8 || !out) return flk::set_error(FLK_E_SHAPE, "zc_lagrange_weights: k_skip=%zu", k_skip);
+  const flk_f128 g{0x6b8330483c2e9849ull, 0x0dcb364640a222feull};
+  flk_f128 pw[8];
+  pw[0] = flk_f128{1, 0};
+  for (int i = 1; i < 8; ++i) pw[i] = flk::host_mul(pw[i - 1], g);
+  const size_t ell = size_t(1) << k_skip;
+  std::vector<flk_f128> phi(ell);
+  for (size_t v = 0; v < ell; ++v) {
+    flk_f128 acc{0, 0};
+    for (int i = 0; i < 8; ++i)
+      if ((v >> i) & 1) acc = flk_f128{acc.lo ^ pw[i].lo, acc.hi ^ pw[i].hi};
+    phi[v] = acc;
+  }
+  for (size_t i = 0; i < ell; ++i) {
+    flk_f128 num{1, 0}, den{1, 0};
+    for (size_t j = 0; j < ell; ++j) {
+      if (j == i) continue;
+      num = flk::host_mul(num, flk_f128{z.lo ^ phi[j].lo, z.hi ^ phi[j].hi});
+      den = flk::host_mul(den, flk_f128{phi[i].lo ^ phi[j].lo, phi[i].hi ^ phi[j].hi});
+    }
+    out[i] = flk::host_mul(num, flk::host_inv(den));
+  }
+  return FLK_OK;
+}
+
 int flk_ntt_twiddles(size_t dim, flk_f128* out) {
   if (dim > 30) return flk::set_error(FLK_E_SHAPE, "flk_ntt_twiddles: dim %zu too large", dim);
   if (!out && dim) return flk::set_error(FLK_E_SHAPE, "flk_ntt_twiddles: null output");

@@ -293,6 +293,85 @@ def induce_sumcheck_poly(log_msg_cols, opened_rows, v_challenges, queries, alpha
     return basis, enforced_sum
 
 
+# ---- lincheck (CF/lincheck.rs) -------------------------------------------------
+def partial_fold_packed_z(z_packed, m, k_log, eq_outer, out=None):
+    """lincheck::partial_fold_packed_z: z_packed is a CUDA uint8 tensor in the stripe layout."""
+    if z_packed.numel() != (1 << m) // 8:
+        raise ValueError("partial_fold_packed_z: z_packed must hold 2^m bits")
+    if _check_f128(eq_outer, "eq_outer") != 1 << (m - k_log):
+        raise ValueError("partial_fold_packed_z: eq_outer must have 2^(m-k_log) entries")
+    out = f128_empty(1 << k_log, z_packed.device) if out is None else out
+    nat.call("flk_lc_partial_fold", _ptr(z_packed), m, k_log, _ptr(eq_outer), _ptr(out), _stream())
+    return out
+
+
+def lincheck_round_msg(comb, z, out=None):
+    """(q(1), q(inf)) of the product sumcheck over the top variable."""
+    n = _check_f128(comb, "comb")
+    if _check_f128(z, "z") != n or not _pow2(n) or n < 2:
+        raise ValueError("lincheck_round_msg: equal power-of-two lengths >= 2 required")
+    out = f128_empty(2, comb.device) if out is None else out
+    nat.call("flk_lc_round_msg_msb", _ptr(comb), _ptr(z), n, _ptr(out), _stream())
+    return out
+
+
+def sumcheck_bind_top_in_place(v, r):
+    """Binds the top variable in place; returns the bound half as a view."""
+    n = _check_f128(v, "v")
+    if not _pow2(n) or n < 2:
+        raise ValueError("sumcheck_bind_top_in_place: power-of-two length >= 2 required")
+    nat.call("flk_lc_bind_top", _ptr(v), n, F128.from_int(r), _stream())
+    return v[: n // 2]
+
+
+# ---- ring switching (CF/pcs/ring_switch.rs) -------------------------------------
+def fold_1b_rows(packed_witness, suffix_tensor, out=None):
+    n = _check_f128(packed_witness, "packed_witness")
+    if _check_f128(suffix_tensor, "suffix_tensor") != n:
+        raise ValueError("fold_1b_rows: length mismatch")
+    out = f128_empty(128, packed_witness.device) if out is None else out
+    nat.call("flk_rs_fold_1b_rows", _ptr(packed_witness), _ptr(suffix_tensor), n, _ptr(out), _stream())
+    return out
+
+
+def fold_b128_elems(suffix_tensor, eq_r_dprime, out=None):
+    n = _check_f128(suffix_tensor, "suffix_tensor")
+    if _check_f128(eq_r_dprime, "eq_r_dprime") != 128:
+        raise ValueError("fold_b128_elems: eq_r_dprime must have 128 entries")
+    out = f128_empty(n, suffix_tensor.device) if out is None else out
+    nat.call("flk_rs_fold_b128_elems", _ptr(suffix_tensor), _ptr(eq_r_dprime), n, _ptr(out), _stream())
+    return out
+
+
+# ---- zerocheck (CF/zerocheck) ----------------------------------------------------
+def lagrange_weights(k_skip, z):
+    """lagrange_weights_naive on the phi8 nodes (host), as a list of ints."""
+    buf = (F128 * (1 << k_skip))()
+    nat.call("flk_zc_lagrange_weights", k_skip, F128.from_int(z), buf)
+    return [buf[i].to_int() for i in range(1 << k_skip)]
+
+
+def fold_at_z(bits, m, k_skip, weights, out=None):
+    """fold_at_z_naive: bits is a CUDA uint8 tensor of 2^m witness bits; weights a (2^k, 2) tensor."""
+    if bits.numel() != (1 << m) // 8:
+        raise ValueError("fold_at_z: bits must hold 2^m bits")
+    if _check_f128(weights, "weights") != 1 << k_skip:
+        raise ValueError("fold_at_z: weights must have 2^k_skip entries")
+    out = f128_empty(1 << (m - k_skip), bits.device) if out is None else out
+    nat.call("flk_zc_fold_at_z", _ptr(bits), m, k_skip, _ptr(weights), _ptr(out), _stream())
+    return out
+
+
+def zerocheck_round_pair(a_mlv, b_mlv, eq_rest, out=None):
+    """(G(1), G(inf)) of a multilinear zerocheck round (wire form)."""
+    n = _check_f128(a_mlv, "a_mlv")
+    if _check_f128(b_mlv, "b_mlv") != n or _check_f128(eq_rest, "eq_rest") != n // 2:
+        raise ValueError("zerocheck_round_pair: shapes")
+    out = f128_empty(2, a_mlv.device) if out is None else out
+    nat.call("flk_zc_round_pair", _ptr(a_mlv), _ptr(b_mlv), n, _ptr(eq_rest), _ptr(out), _stream())
+    return out
+
+
 class SumcheckProver:
     """ligerito::SumcheckProver over device-resident f and combined basis
     (CF/pcs/ligerito.rs:8893-9260): new -> first message; fold(r) -> message of

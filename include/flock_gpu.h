@@ -162,6 +162,56 @@ int flk_induce_sumcheck_poly(size_t log_msg_cols, const flk_f128 *rows, size_t n
                              const flk_f128 *alpha, size_t n_alpha, flk_f128 *basis_out,
                              flk_f128 *sum_out, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Lincheck reductions (CF/lincheck.rs).                                     */
+/* ------------------------------------------------------------------------ */
+
+/* partial_fold_packed_z / _fast (:2233-2420): z_packed is the lincheck stripe
+ * layout of 2^m witness bits (byte (s, i) at z_packed[s * 2^k_log + i], bit r =
+ * z[i, 8s + r]); eq_outer has 2^(m - k_log) elements; out[i] = sum over set bits
+ * of eq_outer[8s + r], 2^k_log elements.  Needs k_log >= 2, m - k_log >= 3. */
+int flk_lc_partial_fold(const uint8_t *z_packed, size_t m, size_t k_log, const flk_f128 *eq_outer,
+                        flk_f128 *out, void *stream);
+/* Product-sumcheck round over the top variable (:5010-5030): out2 = {q(1), q(inf)} =
+ * {sum comb[i+h] z[i+h], sum (comb[i]+comb[i+h])(z[i]+z[i+h])}, h = n/2. */
+int flk_lc_round_msg_msb(const flk_f128 *comb, const flk_f128 *z, size_t n, flk_f128 *out2,
+                         void *stream);
+/* sumcheck_bind_top_in_place (:4968-4990): v[i] += r (v[i+h] + v[i]) for i < h = n/2,
+ * in place (the caller then treats v as length h). */
+int flk_lc_bind_top(flk_f128 *v, size_t n, flk_f128 r, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Ring switching (CF/pcs/ring_switch.rs).                                  */
+/* ------------------------------------------------------------------------ */
+
+/* fold_1b_rows (:1594-1642): out128[r] = sum_i bit_r(packed_witness[i]) * suffix_tensor[i],
+ * r < 128 (polynomial-basis bit r of the element). */
+int flk_rs_fold_1b_rows(const flk_f128 *packed_witness, const flk_f128 *suffix_tensor, size_t n,
+                        flk_f128 *out128, void *stream);
+/* fold_b128_elems (:1952-2087): out[i] = sum_b bit_b(suffix_tensor[i]) * eq_r_dprime[b];
+ * eq_r_dprime is a device array of 128 elements. */
+int flk_rs_fold_b128_elems(const flk_f128 *suffix_tensor, const flk_f128 *eq_r_dprime, size_t n,
+                           flk_f128 *out, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Zerocheck (CF/zerocheck.rs, CF/zerocheck/multilinear.rs).                */
+/* ------------------------------------------------------------------------ */
+
+/* lagrange_weights_naive (multilinear.rs:285-330) into HOST memory: out[i] = L_i(z)
+ * for the nodes phi8(0 .. 2^k_skip - 1); k_skip <= 8. */
+int flk_zc_lagrange_weights(size_t k_skip, flk_f128 z, flk_f128 *out);
+/* fold_at_z (multilinear.rs:575-597): bits holds 2^m witness bits LSB-first, bit
+ * x * 2^k_skip + s = a(s, x); weights (device, 2^k_skip) are the Lagrange weights
+ * L_s(z); out[x] = sum_s a(s, x) L_s(z), 2^(m - k_skip) elements.  3 <= k_skip <= 6. */
+int flk_zc_fold_at_z(const uint8_t *bits, size_t m, size_t k_skip, const flk_f128 *weights,
+                     flk_f128 *out, void *stream);
+/* Multilinear round message in wire form (zerocheck.rs:1748-1781; the naive
+ * round_pair_naive, multilinear.rs:616-640, sends r0 * G(1)):
+ * out2 = {G(1), G(inf)} = {sum_x eq_rest[x] a[2x+1] b[2x+1],
+ *                          sum_x eq_rest[x] (a[2x]+a[2x+1]) (b[2x]+b[2x+1])}, eq_rest n/2. */
+int flk_zc_round_pair(const flk_f128 *a, const flk_f128 *b, size_t n, const flk_f128 *eq_rest,
+                      flk_f128 *out2, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

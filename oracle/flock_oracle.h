@@ -99,6 +99,16 @@ int orc_induce_sumcheck_poly(size_t log_msg_cols, const orc_f128 *rows, size_t n
                              const uint64_t *queries, const orc_f128 *alpha, size_t n_alpha,
                              orc_f128 *basis_out, orc_f128 *sum_out);
 
+/* ---- lincheck / ring switch / zerocheck reductions (reductions.c) ------- */
+int orc_lc_partial_fold(const uint8_t *z, size_t m, size_t k_log, const orc_f128 *eq_outer, orc_f128 *out);
+void orc_lc_bind_top(orc_f128 *v, size_t n, orc_f128 r);
+void orc_lc_round_msg_msb(const orc_f128 *comb, const orc_f128 *z, size_t n, orc_f128 out[2]);
+void orc_rs_fold_1b_rows(const orc_f128 *z, const orc_f128 *eq, size_t n, orc_f128 out[128]);
+void orc_rs_fold_b128_elems(const orc_f128 *suffix, const orc_f128 eq_r[128], size_t n, orc_f128 *out);
+int orc_zc_lagrange_weights(size_t k_skip, orc_f128 z, orc_f128 *out);
+int orc_zc_fold_at_z(const uint8_t *bits, size_t m, size_t k_skip, const orc_f128 *weights, orc_f128 *out);
+void orc_zc_round_pair(const orc_f128 *a, const orc_f128 *b, size_t n, const orc_f128 *eq_rest, orc_f128 out[2]);
+
 #ifdef __cplusplus
 }
 #endif

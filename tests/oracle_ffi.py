@@ -76,6 +76,14 @@ class Oracle:
             "orc_sc_glue": (None, [P, P, SZ, F]),
             "orc_eval_sk_at_vks": (None, [SZ, P]),
             "orc_induce_sumcheck_poly": (ctypes.c_int, [SZ, P, SZ, SZ, P, SZ, P, P, SZ, P, P]),
+            "orc_lc_partial_fold": (ctypes.c_int, [P, SZ, SZ, P, P]),
+            "orc_lc_bind_top": (None, [P, SZ, F]),
+            "orc_lc_round_msg_msb": (None, [P, P, SZ, P]),
+            "orc_rs_fold_1b_rows": (None, [P, P, SZ, P]),
+            "orc_rs_fold_b128_elems": (None, [P, P, SZ, P]),
+            "orc_zc_lagrange_weights": (ctypes.c_int, [SZ, F, P]),
+            "orc_zc_fold_at_z": (ctypes.c_int, [P, SZ, SZ, P, P]),
+            "orc_zc_round_pair": (None, [P, P, SZ, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -245,3 +253,46 @@ class Oracle:
         assert self.L.orc_induce_sumcheck_poly(log_msg_cols, _p(rr), nq, row_len, _p(v) if len(v) else None, len(v),
                                                _p(q), _p(a) if len(a) else None, len(a), _p(basis), _p(s)) == 0
         return basis, f128_to_ints(s)[0]
+
+    # ---- lincheck / ring switch / zerocheck ----
+    def lc_partial_fold(self, z_bytes, m, k_log, eq_outer):
+        z = np.ascontiguousarray(z_bytes, dtype=np.uint8)
+        out = np.zeros(1 << k_log, dtype=F128_DT)
+        assert self.L.orc_lc_partial_fold(_p(z), m, k_log, _p(eq_outer), _p(out)) == 0
+        return out
+
+    def lc_bind_top(self, v, r):
+        vv = v.copy()
+        self.L.orc_lc_bind_top(_p(vv), len(vv), self._F(r))
+        return vv[: len(vv) // 2]
+
+    def lc_round_msg_msb(self, comb, z):
+        out = np.zeros(2, dtype=F128_DT)
+        self.L.orc_lc_round_msg_msb(_p(comb), _p(z), len(comb), _p(out))
+        return out
+
+    def rs_fold_1b_rows(self, z, eq):
+        out = np.zeros(128, dtype=F128_DT)
+        self.L.orc_rs_fold_1b_rows(_p(z), _p(eq), len(z), _p(out))
+        return out
+
+    def rs_fold_b128_elems(self, suffix, eq_r):
+        out = np.zeros(len(suffix), dtype=F128_DT)
+        self.L.orc_rs_fold_b128_elems(_p(suffix), _p(eq_r), len(suffix), _p(out))
+        return out
+
+    def zc_lagrange_weights(self, k_skip, z):
+        out = np.zeros(1 << k_skip, dtype=F128_DT)
+        assert self.L.orc_zc_lagrange_weights(k_skip, self._F(z), _p(out)) == 0
+        return out
+
+    def zc_fold_at_z(self, bits_bytes, m, k_skip, weights):
+        b = np.ascontiguousarray(bits_bytes, dtype=np.uint8)
+        out = np.zeros(1 << (m - k_skip), dtype=F128_DT)
+        assert self.L.orc_zc_fold_at_z(_p(b), m, k_skip, _p(weights), _p(out)) == 0
+        return out
+
+    def zc_round_pair(self, a, b, eq_rest):
+        out = np.zeros(2, dtype=F128_DT)
+        self.L.orc_zc_round_pair(_p(a), _p(b), len(a), _p(eq_rest), _p(out))
+        return out

@@ -1,0 +1,141 @@
+"""GPU parity: lincheck partial fold and product-sumcheck rounds, ring-switch
+folds and the zerocheck fold-at-z / multilinear rounds on gfx950 against the
+oracle, bit-exact, through the C ABI.  The largest cases use the m = 26
+(2^12-compression) shapes of the reference configuration."""
+import random
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import pyfield as pf
+from flock_challenge_multi_amd import hotpath as hp
+from gpu_util import from_dev, same, to_dev
+from oracle_ffi import f128_to_ints, ints_to_f128
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rand_bytes(seed, n):
+    return np.random.default_rng(seed).integers(0, 256, n, dtype=np.uint8)
+
+
+@pytest.mark.parametrize("m,k_log", [(8, 3), (12, 4), (17, 14), (20, 14), (26, 14)])
+def test_lc_partial_fold(orc, dev, m, k_log):
+    _log(f"lc_partial_fold m={m} k_log={k_log}")
+    rng = random.Random(m)
+    zb = _rand_bytes(m, (1 << m) // 8)
+    eq = orc.build_eq([rng.getrandbits(128) for _ in range(m - k_log)])
+    want = orc.lc_partial_fold(zb, m, k_log, eq)
+    got = hp.partial_fold_packed_z(torch.from_numpy(zb).cuda(), m, k_log, to_dev(eq))
+    torch.cuda.synchronize()
+    assert same(from_dev(got), want)
+
+
+@pytest.mark.parametrize("logn", [1, 3, 9, 14, 18])
+def test_lc_round_and_bind(orc, dev, logn):
+    rng = random.Random(logn)
+    comb, z = orc.fill(10 + logn, 1 << logn), orc.fill(20 + logn, 1 << logn)
+    got = hp.lincheck_round_msg(to_dev(comb), to_dev(z))
+    r = rng.getrandbits(128)
+    t = to_dev(comb)
+    bound = hp.sumcheck_bind_top_in_place(t, r)
+    torch.cuda.synchronize()
+    assert same(from_dev(got), orc.lc_round_msg_msb(comb, z))
+    assert same(from_dev(bound), orc.lc_bind_top(comb, r))
+
+
+def test_lincheck_product_sumcheck_8_rounds(orc, dev):
+    """The 8-round top-variable product sumcheck over 2^14 tables, round by round."""
+    rng = random.Random(5)
+    comb, z = orc.fill(1, 1 << 14), orc.fill(2, 1 << 14)
+    tc, tz = to_dev(comb), to_dev(z)
+    for rnd in range(8):
+        msg = hp.lincheck_round_msg(tc, tz)
+        want = orc.lc_round_msg_msb(comb, z)
+        assert same(from_dev(msg), want), rnd
+        r = rng.getrandbits(128)
+        tc = hp.sumcheck_bind_top_in_place(tc, r).contiguous()
+        tz = hp.sumcheck_bind_top_in_place(tz, r).contiguous()
+        comb, z = orc.lc_bind_top(comb, r), orc.lc_bind_top(z, r)
+    torch.cuda.synchronize()
+    assert same(from_dev(tc), comb) and same(from_dev(tz), z)
+
+
+@pytest.mark.parametrize("n", [1, 5, 256, 1000, 1 << 16, 1 << 19])
+def test_rs_fold_1b_rows(orc, dev, n):
+    _log(f"fold_1b_rows n={n}")
+    z, eq = orc.fill(30 + n, n), orc.fill(40 + n, n)
+    got = hp.fold_1b_rows(to_dev(z), to_dev(eq))
+    torch.cuda.synchronize()
+    assert same(from_dev(got), orc.rs_fold_1b_rows(z, eq))
+
+
+@pytest.mark.parametrize("n", [1, 7, 4096, 1 << 19])
+def test_rs_fold_b128_elems(orc, dev, n):
+    _log(f"fold_b128_elems n={n}")
+    sfx, eqr = orc.fill(50 + n, n), orc.fill(60 + n, 128)
+    got = hp.fold_b128_elems(to_dev(sfx), to_dev(eqr))
+    torch.cuda.synchronize()
+    assert same(from_dev(got), orc.rs_fold_b128_elems(sfx, eqr))
+
+
+def test_lagrange_weights_host(orc, dev):
+    rng = random.Random(3)
+    for k in (3, 6):
+        z = rng.getrandbits(128)
+        assert hp.lagrange_weights(k, z) == f128_to_ints(orc.zc_lagrange_weights(k, z))
+
+
+@pytest.mark.parametrize("m,k_skip", [(6, 6), (9, 3), (12, 6), (20, 6), (26, 6), (14, 5)])
+def test_zc_fold_at_z(orc, dev, m, k_skip):
+    _log(f"fold_at_z m={m} k_skip={k_skip}")
+    rng = random.Random(m * 10 + k_skip)
+    bits = _rand_bytes(m + 100, (1 << m) // 8)
+    w = orc.zc_lagrange_weights(k_skip, rng.getrandbits(128))
+    got = hp.fold_at_z(torch.from_numpy(bits).cuda(), m, k_skip, to_dev(w))
+    torch.cuda.synchronize()
+    assert same(from_dev(got), orc.zc_fold_at_z(bits, m, k_skip, w))
+
+
+@pytest.mark.parametrize("logn", [1, 4, 12, 20])
+def test_zc_round_pair(orc, dev, logn):
+    _log(f"zc_round_pair logn={logn}")
+    rng = random.Random(logn)
+    a, b = orc.fill(70 + logn, 1 << logn), orc.fill(80 + logn, 1 << logn)
+    eq_rest = orc.build_eq([rng.getrandbits(128) for _ in range(logn - 1)])
+    got = hp.zerocheck_round_pair(to_dev(a), to_dev(b), to_dev(eq_rest))
+    torch.cuda.synchronize()
+    assert same(from_dev(got), orc.zc_round_pair(a, b, eq_rest))
+
+
+def test_zerocheck_multilinear_rounds_chain(orc, dev):
+    """fold at z, then round message + LSB bind per round with the eq table of the
+    remaining challenges, as the prover chains them (CF/zerocheck.rs:1748-1781)."""
+    rng = random.Random(17)
+    m, k_skip = 16, 6
+    nv = m - k_skip
+    abits, bbits = _rand_bytes(1, (1 << m) // 8), _rand_bytes(2, (1 << m) // 8)
+    z = rng.getrandbits(128)
+    w = orc.zc_lagrange_weights(k_skip, z)
+    a = orc.zc_fold_at_z(abits, m, k_skip, w)
+    b = orc.zc_fold_at_z(bbits, m, k_skip, w)
+    ta = hp.fold_at_z(torch.from_numpy(abits).cuda(), m, k_skip, to_dev(w))
+    tb = hp.fold_at_z(torch.from_numpy(bbits).cuda(), m, k_skip, to_dev(w))
+    r = [rng.getrandbits(128) for _ in range(nv)]
+    for i in range(nv):
+        eq_rest_pt = r[i + 1:]
+        eq_rest = orc.build_eq(eq_rest_pt)
+        msg = hp.zerocheck_round_pair(ta, tb, hp.build_eq_table(eq_rest_pt))
+        assert same(from_dev(msg), orc.zc_round_pair(a, b, eq_rest)), i
+        rho = rng.getrandbits(128)
+        ta, tb = hp.partial_eval_lsb(ta, [rho]), hp.partial_eval_lsb(tb, [rho])
+        a, b = orc.partial_eval_lsb(a, [rho]), orc.partial_eval_lsb(b, [rho])
+    torch.cuda.synchronize()
+    assert same(from_dev(ta), a) and same(from_dev(tb), b)
