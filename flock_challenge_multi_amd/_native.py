@@ -74,6 +74,7 @@ SIGNATURES = {
     "flk_zc_lagrange_weights": (_I, [_SZ, _F, _P]),
     "flk_zc_fold_at_z": (_I, [_P, _SZ, _SZ, _P, _P, _P]),
     "flk_zc_round_pair": (_I, [_P, _P, _SZ, _P, _P, _P]),
+    "flk_zc_round1": (_I, [_P, _P, _P, _SZ, _P, _P, _P, _P]),
 }
 
 _lib = None

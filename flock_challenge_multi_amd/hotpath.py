@@ -372,6 +372,20 @@ def zerocheck_round_pair(a_mlv, b_mlv, eq_rest, out=None):
     return out
 
 
+def zerocheck_round1(a_bits, b_bits, c_bits, m, r_outer):
+    """Round-1 univariate-skip message (P_AB, P_C on Lambda, 64 each) for the
+    protocol's fixed inner challenges and r_outer (list of m - 13 ints)."""
+    for t in (a_bits, b_bits, c_bits):
+        if t.numel() != (1 << m) // 8:
+            raise ValueError("zerocheck_round1: bit vectors must hold 2^m bits")
+    if len(r_outer) != m - 13:
+        raise ValueError("zerocheck_round1: r_outer must have m - 13 entries")
+    out = f128_empty(128, a_bits.device)
+    r = (F128 * max(len(r_outer), 1))(*[F128.from_int(v) for v in r_outer])
+    nat.call("flk_zc_round1", _ptr(a_bits), _ptr(b_bits), _ptr(c_bits), m, r, _ptr(out), _ptr(out[64:]), _stream())
+    return out[:64], out[64:]
+
+
 class SumcheckProver:
     """ligerito::SumcheckProver over device-resident f and combined basis
     (CF/pcs/ligerito.rs:8893-9260): new -> first message; fold(r) -> message of

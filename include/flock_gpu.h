@@ -212,6 +212,16 @@ int flk_zc_fold_at_z(const uint8_t *bits, size_t m, size_t k_skip, const flk_f12
 int flk_zc_round_pair(const flk_f128 *a, const flk_f128 *b, size_t n, const flk_f128 *eq_rest,
                       flk_f128 *out2, void *stream);
 
+/* Round 1 of the zerocheck (univariate skip over k_skip = 6): round1_naive
+ * (univariate_skip.rs:90-152) for the protocol's challenge layout
+ * (zerocheck.rs:1680-1692): r[6..9) = phi8(0xF7, 0x53, 0xB5), r[9..13) =
+ * x^(2^i) / (1 + x^(2^i)) (univariate_skip_optimized.rs:136-232), r[13..m) =
+ * r_outer (HOST array of m - 13 elements).  a_bits, b_bits, c_bits hold 2^m bits
+ * each (8-byte aligned).  Writes P_AB on Lambda to out_ab[0..64) and P_C to
+ * out_c[0..64); out_c must equal out_ab + 64.  13 <= m <= 40. */
+int flk_zc_round1(const uint8_t *a_bits, const uint8_t *b_bits, const uint8_t *c_bits, size_t m,
+                  const flk_f128 *r_outer, flk_f128 *out_ab, flk_f128 *out_c, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,13 @@ int orc_zc_lagrange_weights(size_t k_skip, orc_f128 z, orc_f128 *out);
 int orc_zc_fold_at_z(const uint8_t *bits, size_t m, size_t k_skip, const orc_f128 *weights, orc_f128 *out);
 void orc_zc_round_pair(const orc_f128 *a, const orc_f128 *b, size_t n, const orc_f128 *eq_rest, orc_f128 out[2]);
 
+/* ---- GF(2^8) additive NTT and zerocheck round 1 (zerocheck_round1.c) ---- */
+int orc_gf8_twiddles(size_t k, uint8_t beta, uint8_t *tw);
+int orc_gf8_forward(size_t k, uint8_t beta, uint8_t *v);
+int orc_gf8_inverse(size_t k, uint8_t beta, uint8_t *v);
+int orc_zc_round1(const uint8_t *a, const uint8_t *b, const uint8_t *c, size_t m, size_t k_skip,
+                  const orc_f128 *r, orc_f128 *p_ab, orc_f128 *p_c);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,9 @@ class Oracle:
             "orc_zc_lagrange_weights": (ctypes.c_int, [SZ, F, P]),
             "orc_zc_fold_at_z": (ctypes.c_int, [P, SZ, SZ, P, P]),
             "orc_zc_round_pair": (None, [P, P, SZ, P, P]),
+            "orc_gf8_forward": (ctypes.c_int, [SZ, ctypes.c_uint8, P]),
+            "orc_gf8_inverse": (ctypes.c_int, [SZ, ctypes.c_uint8, P]),
+            "orc_zc_round1": (ctypes.c_int, [P, P, P, SZ, SZ, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -296,3 +299,21 @@ class Oracle:
         out = np.zeros(2, dtype=F128_DT)
         self.L.orc_zc_round_pair(_p(a), _p(b), len(a), _p(eq_rest), _p(out))
         return out
+
+    def gf8_forward(self, k, beta, v):
+        a = np.array(v, dtype=np.uint8)
+        assert self.L.orc_gf8_forward(k, beta, _p(a)) == 0
+        return [int(x) for x in a]
+
+    def gf8_inverse(self, k, beta, v):
+        a = np.array(v, dtype=np.uint8)
+        assert self.L.orc_gf8_inverse(k, beta, _p(a)) == 0
+        return [int(x) for x in a]
+
+    def zc_round1(self, a, b, c, m, k_skip, r):
+        rr = ints_to_f128(r)
+        pab = np.zeros(1 << k_skip, dtype=F128_DT)
+        pc = np.zeros(1 << k_skip, dtype=F128_DT)
+        args = [np.ascontiguousarray(x, dtype=np.uint8) for x in (a, b, c)]
+        assert self.L.orc_zc_round1(_p(args[0]), _p(args[1]), _p(args[2]), m, k_skip, _p(rr), _p(pab), _p(pc)) == 0
+        return pab, pc

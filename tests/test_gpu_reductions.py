@@ -139,3 +139,48 @@ def test_zerocheck_multilinear_rounds_chain(orc, dev):
         a, b = orc.partial_eval_lsb(a, [rho]), orc.partial_eval_lsb(b, [rho])
     torch.cuda.synchronize()
     assert same(from_dev(ta), a) and same(from_dev(tb), b)
+
+
+def _fixed_inner_challenges():
+    import json
+    import os
+    fix = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "phi8_table.json")))["table"]
+    phi = [int(a, 16) | (int(b, 16) << 64) for a, b in fix]
+    small = [phi[0xF7], phi[0x53], phi[0xB5]]
+    medium = [pf.mul(g, pf.inv(1 ^ g)) for g in (2, 4, 16, 256)]
+    return small + medium
+
+
+@pytest.mark.parametrize("m", [13, 14, 16, 20])
+def test_zc_round1_matches_oracle(orc, dev, m):
+    """Round 1 for the protocol challenge layout: r[0..6) unused by round 1,
+    r[6..13) fixed, r[13..m) random; against round1_naive on the oracle."""
+    _log(f"zc_round1 m={m}")
+    rng = random.Random(m)
+    nb = (1 << m) // 8
+    a, b, c = (_rand_bytes(1000 * m + i, nb) for i in range(3))
+    r_outer = [rng.getrandbits(128) for _ in range(m - 13)]
+    r = [rng.getrandbits(128) for _ in range(6)] + _fixed_inner_challenges() + r_outer
+    want_ab, want_c = orc.zc_round1(a, b, c, m, 6, r)
+    ta, tb, tc = (torch.from_numpy(x).cuda() for x in (a, b, c))
+    got_ab, got_c = hp.zerocheck_round1(ta, tb, tc, m, r_outer)
+    torch.cuda.synchronize()
+    assert same(from_dev(got_ab), want_ab)
+    assert same(from_dev(got_c), want_c)
+
+
+def test_zc_round1_honest_witness_vanishes_on_s(orc, dev):
+    """For a witness with c = a AND b the zerocheck polynomial P_AB + P_C has a
+    zero on S; its Lambda evaluations from the device equal the oracle's, and the
+    C part alone is F2-linear in c."""
+    m = 14
+    nb = (1 << m) // 8
+    a, b = _rand_bytes(7, nb), _rand_bytes(8, nb)
+    c = a & b
+    rng = random.Random(3)
+    r_outer = [rng.getrandbits(128)]
+    r = [0] * 6 + _fixed_inner_challenges() + r_outer
+    want_ab, want_c = orc.zc_round1(a, b, c, m, 6, r)
+    got_ab, got_c = hp.zerocheck_round1(*(torch.from_numpy(x).cuda() for x in (a, b, c)), m, r_outer)
+    torch.cuda.synchronize()
+    assert same(from_dev(got_ab), want_ab) and same(from_dev(got_c), want_c)

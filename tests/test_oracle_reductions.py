@@ -166,3 +166,59 @@ def test_zc_round_pair_verifier_identity(orc):
     for x in range(n // 2):
         want ^= pf.mul(eqr[x], pf.mul(ab[x], bb[x]))
     assert g_rho == want
+
+
+def _interp_eval(xs, ys, z):
+    """Lagrange interpolation over GF(2^8) through (xs, ys), evaluated at z."""
+    acc = 0
+    for i, (xi, yi) in enumerate(zip(xs, ys)):
+        num, den = 1, 1
+        for j, xj in enumerate(xs):
+            if j != i:
+                num = pf.mul8(num, z ^ xj)
+                den = pf.mul8(den, xi ^ xj)
+        # den^-1 = den^254
+        inv = 1
+        for _ in range(254):
+            inv = pf.mul8(inv, den)
+        acc ^= pf.mul8(yi, pf.mul8(num, inv))
+    return acc
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_gf8_lde_is_polynomial_extension(orc, k):
+    """inverse NTT on S = {0..2^k-1} then forward NTT on Lambda = {2^k..} evaluates the
+    unique degree < 2^k interpolant of the S values at the Lambda points
+    (CF/zerocheck/univariate_skip.rs:1-20)."""
+    rng = random.Random(k)
+    ell = 1 << k
+    vals = [rng.randrange(256) for _ in range(ell)]
+    coeff = orc.gf8_inverse(k, 0, vals)
+    assert orc.gf8_forward(k, 0, coeff) == vals  # round trip on S
+    lam = orc.gf8_forward(k, ell, coeff)
+    xs = list(range(ell))
+    assert lam == [_interp_eval(xs, vals, ell + i) for i in range(ell)]
+
+
+def test_zc_round1_closed_form(orc):
+    """round1_naive against the defining sum with the LDE by interpolation."""
+    rng = random.Random(21)
+    m, k = 6, 3
+    ell = 1 << k
+    nb = (1 << m) // 8
+    a, b, c = (np.frombuffer(rng.randbytes(nb), np.uint8).copy() for _ in range(3))
+    r = [rng.getrandbits(128) for _ in range(m)]
+    pab, pc = orc.zc_round1(a, b, c, m, k, r)
+    phi = [int(e["lo"]) | (int(e["hi"]) << 64) for e in orc.phi8_table()]
+    want_ab, want_c = [0] * ell, [0] * ell
+    xs = list(range(ell))
+    for x in range(1 << (m - k)):
+        bits = lambda arr: [int((arr[(x * ell + s) >> 3] >> ((x * ell + s) & 7)) & 1) for s in range(ell)]
+        av, bv, cv = bits(a), bits(b), bits(c)
+        e = pf.eq_entry(r[k:], x)
+        for i in range(ell):
+            al, bl, cl = (_interp_eval(xs, v, ell + i) for v in (av, bv, cv))
+            want_ab[i] ^= pf.mul(e, phi[pf.mul8(al, bl)])
+            want_c[i] ^= pf.mul(e, phi[cl])
+    assert f128_to_ints(pab) == want_ab
+    assert f128_to_ints(pc) == want_c
