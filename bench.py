@@ -35,6 +35,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-m32", action="store_true")
+    ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel table at m = 32 sizes")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -48,6 +49,120 @@ def pctl(xs, q):
     lo = int(k)
     hi = min(lo + 1, len(xs) - 1)
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def _time(torch, st, fn, reps=5):
+    """Average device time (ms) of fn() over `reps` launches, HIP events on `st`."""
+    fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for s, e in ev:
+        s.record(st)
+        fn()
+        e.record(st)
+    torch.cuda.synchronize()
+    return sum(s.elapsed_time(e) for s, e in ev) / reps
+
+
+def kernel_table(torch, nat, hp, sptr, st):
+    """Every hot-path kernel at the sizes of the 2^18-compression proof (m = 32).
+
+    Bytes are ALGORITHMIC (inputs read once + outputs written once); GB/s =
+    bytes / device time; frac = GB/s / 8000.  These kernels are integer-VALU
+    bound, so the HBM fraction is a lower bound on how busy the chip is.
+    """
+    import ctypes
+    import random
+    P = ctypes.c_void_p
+    rng = random.Random(7)
+    m = 32
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1234)
+
+    def rnd_f128(n):
+        return torch.randint(-(2**63), 2**63 - 1, (n, 2), dtype=torch.int64, device="cuda", generator=g)
+
+    def rnd_bytes(n):
+        return torch.randint(0, 256, (n,), dtype=torch.uint8, device="cuda", generator=g)
+
+    def F(v):
+        return nat.F128.from_int(v)
+
+    res = {}
+
+    def add(name, ms, nbytes, extra=None):
+        gbs = nbytes / (ms / 1e3) / 1e9
+        res[name] = {"ms": ms, "bytes": nbytes, "GB_s": gbs, "frac": gbs / HBM_PEAK_GBS}
+        if extra:
+            res[name].update(extra)
+
+    N = 1 << 25  # packed witness / Ligerito L0 basis length at m = 32
+    f, b = rnd_f128(N), rnd_f128(N)
+    fo, bo, out = hp.f128_empty(N // 2), hp.f128_empty(N // 2), hp.f128_empty(4)
+    r = rng.getrandbits(128)
+    add("a6_round_msg_2^25", _time(torch, st, lambda: nat.call("flk_sc_round_msg", P(f.data_ptr()), P(b.data_ptr()), N,
+                                                                P(out.data_ptr()), sptr)), 32 * N)
+    add("a6_fold_and_msg_2^25", _time(torch, st, lambda: nat.call(
+        "flk_sc_fold_and_msg", P(f.data_ptr()), P(b.data_ptr()), N, F(r), P(fo.data_ptr()), P(bo.data_ptr()),
+        P(out.data_ptr()), sptr)), 48 * N)
+    add("a6_glue_2^25", _time(torch, st, lambda: nat.call("flk_sc_glue", P(fo.data_ptr()), P(bo.data_ptr()), N // 2,
+                                                           F(r), sptr)), 48 * (N // 2))
+    pt = (nat.F128 * 25)(*[F(rng.getrandbits(128)) for _ in range(25)])
+    add("eq_table_2^25", _time(torch, st, lambda: nat.call("flk_build_eq", pt, 25, P(f.data_ptr()), sptr)), 16 * N)
+    # a11 ring switch
+    o128 = hp.f128_empty(128)
+    add("a11_fold_1b_rows_2^25", _time(torch, st, lambda: nat.call(
+        "flk_rs_fold_1b_rows", P(f.data_ptr()), P(b.data_ptr()), N, P(o128.data_ptr()), sptr)), 32 * N + 2048)
+    add("a11_fold_b128_elems_2^25", _time(torch, st, lambda: nat.call(
+        "flk_rs_fold_b128_elems", P(b.data_ptr()), P(o128.data_ptr()), N, P(f.data_ptr()), sptr)), 32 * N + 2048)
+    del f, b, fo, bo
+    # a8 / a9 zerocheck, a10 lincheck on 2^32-bit witness columns
+    nb = (1 << m) // 8
+    A, B, C = rnd_bytes(nb), rnd_bytes(nb), rnd_bytes(nb)
+    r_outer = (nat.F128 * (m - 13))(*[F(rng.getrandbits(128)) for _ in range(m - 13)])
+    o = hp.f128_empty(128)
+    add("a8_zc_round1_m32", _time(torch, st, lambda: nat.call(
+        "flk_zc_round1", P(A.data_ptr()), P(B.data_ptr()), P(C.data_ptr()), m, r_outer, P(o.data_ptr()),
+        P(o[64:].data_ptr()), sptr), reps=2), 3 * nb)
+    w = rnd_f128(64)
+    amlv = hp.f128_empty(1 << (m - 6))
+    add("a9_fold_at_z_m32", _time(torch, st, lambda: nat.call(
+        "flk_zc_fold_at_z", P(A.data_ptr()), m, 6, P(w.data_ptr()), P(amlv.data_ptr()), sptr)),
+        nb + 16 * (1 << (m - 6)))
+    eq_outer = rnd_f128(1 << (m - 14))
+    zlc = hp.f128_empty(1 << 14)
+    add("a10_lc_partial_fold_m32", _time(torch, st, lambda: nat.call(
+        "flk_lc_partial_fold", P(B.data_ptr()), m, 14, P(eq_outer.data_ptr()), P(zlc.data_ptr()), sptr)),
+        nb + 16 * (1 << (m - 14)) + 16 * (1 << 14))
+    del A, B, C
+    bmlv = rnd_f128(1 << (m - 6))
+    eqr = rnd_f128(1 << (m - 7))
+    add("a9_zc_round_pair_2^26", _time(torch, st, lambda: nat.call(
+        "flk_zc_round_pair", P(amlv.data_ptr()), P(bmlv.data_ptr()), 1 << (m - 6), P(eqr.data_ptr()),
+        P(o.data_ptr()), sptr)), 16 * (2 * (1 << (m - 6)) + (1 << (m - 7))))
+    del amlv, bmlv, eqr
+    # a7 basis induction at the m = 32 L1 shape (2^19 columns, 218 queries, 64-wide rows)
+    rows = rnd_f128(218 * 64)
+    qs = torch.randint(0, 1 << 20, (218,), dtype=torch.int64, device="cuda", generator=g)
+    vv = (nat.F128 * 6)(*[F(rng.getrandbits(128)) for _ in range(6)])
+    al = (nat.F128 * 8)(*[F(rng.getrandbits(128)) for _ in range(8)])
+    basis, sm = hp.f128_empty(1 << 19), hp.f128_empty(1)
+    ms = _time(torch, st, lambda: nat.call("flk_induce_sumcheck_poly", 19, P(rows.data_ptr()), 218, vv, 6,
+                                            P(qs.data_ptr()), al, 8, P(basis.data_ptr()), P(sm.data_ptr()), sptr), reps=3)
+    add("a7_induce_2^19x218", ms, 16 * (1 << 19) + 16 * 218 * 64,
+        {"gf128_products_per_s": 218 * (1 << 19) / (ms / 1e3)})
+    # a3 + a4 at m = 32 (the commit breakdown)
+    n_msg, n_cw, n_pos = 1 << 25, 1 << 26, 1 << 20
+    msg, cw, tree = rnd_f128(n_msg), hp.f128_empty(n_cw), hp.hashes_empty(2 * n_pos - 1)
+    ms = _time(torch, st, lambda: nat.call("flk_ntt_rs_encode_interleaved", 20, P(msg.data_ptr()), n_msg,
+                                            P(cw.data_ptr()), n_cw, 64, sptr), reps=3)
+    add("a3_rs_encode_m32", ms, 16 * (n_msg + n_cw), {"gf128_products_per_s": 603979776 / (ms / 1e3)})
+    ms = _time(torch, st, lambda: nat.call("flk_merkle_tree", P(cw.data_ptr()), n_cw * 16, n_pos, nat.FLK_HASH_BLAKE3,
+                                            P(tree.data_ptr()), sptr), reps=3)
+    add("a4_merkle_m32", ms, 16 * n_cw + 32 * (2 * n_pos - 1), {"blake3_compressions_per_s": (17 * n_pos - 1) / (ms / 1e3)})
+    del msg, cw, tree
+    torch.cuda.synchronize()
+    return res
 
 
 def main():
@@ -188,6 +303,8 @@ def main():
                "committed_compressions_per_s": (1 << 18) / (sum(l32) / len(l32) / 1e3)}
         del msg32, cw32, tree32
 
+    kernels = None if args.no_kernels else kernel_table(torch, nat, hp, sptr, st)
+
     line = {
         "metric": "verified BLAKE3 compressions/s (2^18 batch, 1 GPU) + p50/p99 proof latency",
         "value": value,
@@ -220,6 +337,7 @@ def main():
         "merkle_roofline": {"achieved": mer_bytes / (mer_ms / 1e3) / 1e9, "unit": "GB/s",
                             "compressions_per_s": (n_pos * 17 - 1) / (mer_ms / 1e3)},
         "m32_l0_commit": m32,
+        "kernels_m32": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

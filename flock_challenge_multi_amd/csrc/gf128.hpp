@@ -204,6 +204,54 @@ __device__ __forceinline__ uint4 comb8_mul(uint4 v, const uint4* __restrict__ ta
   return f_reduce(a);
 }
 
+// Unreduced product v * t with eight 16-entry tables tab[j*16 + k] = k * t * x^(4j)
+// (j = nibble position inside a 32-bit word).  Every lookup lands in a 256-byte
+// table, i.e. one 16-byte slot per bank group, so the 32 lookups are free of LDS
+// bank conflicts; no accumulator shifts are needed (word w of v places its
+// entries at word offset w).
+__device__ __forceinline__ void combj_mul_acc(uint4 v, const uint4* __restrict__ tab, U256& acc) {
+  const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+  U256 a;
+  u256_zero(a);
+  // Two nibble positions per iteration: eight loads in flight per lane.
+#pragma unroll 2
+  for (int j = 0; j < 8; ++j) {
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const uint32_t nib = (vw[w] >> (4 * j)) & 15u;
+      const uint4 e = tab[j * 16 + nib];
+      a.w[w] ^= e.x;
+      a.w[w + 1] ^= e.y;
+      a.w[w + 2] ^= e.z;
+      a.w[w + 3] ^= e.w;
+    }
+  }
+  u256_xor(acc, a);
+}
+
+__device__ __forceinline__ uint4 combj_mul(uint4 v, const uint4* __restrict__ tab) {
+  U256 a;
+  u256_zero(a);
+  combj_mul_acc(v, tab, a);
+  return f_reduce(a);
+}
+
+// t * x^s for 0 <= s < 32: shift by s, fold the bits above 127 once
+// (U x^128 = U ^ U<<1 ^ U<<2 ^ U<<7 for the <= 31-bit overflow U).
+__device__ __forceinline__ uint4 f_mul_xs(uint4 t, uint32_t s) {
+  if (s == 0) return t;
+  const uint32_t o = t.w >> (32 - s);  // bits that leave position 127
+  uint4 r;
+  r.w = funnel(t.w, t.z, 32 - s);
+  r.z = funnel(t.z, t.y, 32 - s);
+  r.y = funnel(t.y, t.x, 32 - s);
+  r.x = t.x << s;
+  // o * (x^7 + x^2 + x + 1): up to 38 bits, words 0 and 1
+  r.x ^= o ^ (o << 1) ^ (o << 2) ^ (o << 7);
+  r.y ^= (o >> 31) ^ (o >> 30) ^ (o >> 25);
+  return r;
+}
+
 // t * x^4 (four doublings).
 __device__ __forceinline__ uint4 f_mul_x4(uint4 a) { return f_mul_x(f_mul_x(f_mul_x(f_mul_x(a)))); }
 

@@ -102,13 +102,14 @@ __global__ void __launch_bounds__(NTT_WG) ntt_pass_kernel(const uint4* __restric
 // built once per work-group at kernel start, and every product is a 16-lookup
 // byte comb.  Table build: the 16-entry nibble tables of t and t*x^4 first, then
 // byte entry b = T_lo[b & 15] ^ T_hi[b >> 4].
-template <int K>
+template <int K, int MODE>  // MODE 0: byte comb (256-entry tables); 1: eight nibble tables per twiddle
 __global__ void __launch_bounds__(NTT_WG) ntt_pass_shared_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                                 const uint4* __restrict__ tw, uint32_t log_d,
                                                                 uint32_t L, uint32_t log_lanes, uint64_t src_pos_mask,
                                                                 uint64_t n_threads) {
   constexpr int NTW = (1 << K) - 1;
-  __shared__ uint4 tabs[NTW * 256];
+  constexpr int TAB = MODE == 0 ? 256 : 128;  // entries per twiddle
+  __shared__ uint4 tabs[NTW * TAB];
   __shared__ uint4 nib[NTW * 32];
   const uint32_t tid = threadIdx.x;
   const uint64_t g = (uint64_t)blockIdx.x * NTT_WG + tid;
@@ -130,18 +131,37 @@ __global__ void __launch_bounds__(NTT_WG) ntt_pass_shared_kernel(const uint4* __
   }
 
   // Twiddle q (0 .. 2^K - 2) is layer L+i, sub-block sb with q = 2^i - 1 + sb.
-  for (uint32_t e = tid; e < (uint32_t)NTW * 32u; e += NTT_WG) {
-    const uint32_t q = e >> 5, half = (e >> 4) & 1u, k = e & 15u;
-    const uint32_t i = 31u - __builtin_clz(q + 1u);
-    const uint32_t sb = q + 1u - (1u << i);
-    uint4 t = tw[((1ull << (L + i)) - 1ull) + (prefix << i) + sb];
-    if (half) t = f_mul_x4(t);
-    nib[e] = comb_entry(t, k);
-  }
-  __syncthreads();
-  for (uint32_t e = tid; e < (uint32_t)NTW * 256u; e += NTT_WG) {
-    const uint32_t q = e >> 8, b = e & 255u;
-    tabs[e] = f_add(nib[q * 32 + (b & 15u)], nib[q * 32 + 16 + (b >> 4)]);
+  if (MODE == 0) {
+    for (uint32_t e = tid; e < (uint32_t)NTW * 32u; e += NTT_WG) {
+      const uint32_t q = e >> 5, half = (e >> 4) & 1u, k = e & 15u;
+      const uint32_t i = 31u - __builtin_clz(q + 1u);
+      const uint32_t sb = q + 1u - (1u << i);
+      uint4 t = tw[((1ull << (L + i)) - 1ull) + (prefix << i) + sb];
+      if (half) t = f_mul_x4(t);
+      nib[e] = comb_entry(t, k);
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < (uint32_t)NTW * 256u; e += NTT_WG) {
+      const uint32_t q = e >> 8, b = e & 255u;
+      tabs[e] = f_add(nib[q * 32 + (b & 15u)], nib[q * 32 + 16 + (b >> 4)]);
+    }
+  } else {
+    // basis values t * x^s, s = 0..31, then entry (j, k) = XOR of t*x^(4j+b) over bits b of k
+    for (uint32_t e = tid; e < (uint32_t)NTW * 32u; e += NTT_WG) {
+      const uint32_t q = e >> 5, sh = e & 31u;
+      const uint32_t i = 31u - __builtin_clz(q + 1u);
+      const uint32_t sb = q + 1u - (1u << i);
+      nib[e] = f_mul_xs(tw[((1ull << (L + i)) - 1ull) + (prefix << i) + sb], sh);
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < (uint32_t)NTW * 128u; e += NTT_WG) {
+      const uint32_t q = e >> 7, j = (e >> 4) & 7u, k = e & 15u;
+      uint4 v = f_zero();
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if ((k >> b) & 1u) v = f_add(v, nib[q * 32 + 4 * j + b]);
+      tabs[e] = v;
+    }
   }
   __syncthreads();
 
@@ -149,12 +169,12 @@ __global__ void __launch_bounds__(NTT_WG) ntt_pass_shared_kernel(const uint4* __
   for (int i = 0; i < K; ++i) {
 #pragma unroll
     for (int sb = 0; sb < (1 << i); ++sb) {
-      const uint4* tab = tabs + (((1 << i) - 1 + sb) << 8);
+      const uint4* tab = tabs + ((1 << i) - 1 + sb) * TAB;
 #pragma unroll
       for (int jj = 0; jj < (1 << (K - 1 - i)); ++jj) {
         const int j0 = (sb << (K - i)) + jj;
         const int j1 = j0 + (1 << (K - 1 - i));
-        const uint4 prod = comb8_mul(x[j1], tab);
+        const uint4 prod = MODE == 0 ? comb8_mul(x[j1], tab) : combj_mul(x[j1], tab);
         x[j0] = f_add(x[j0], prod);
         x[j1] = f_add(x[j1], x[j0]);
       }
@@ -200,6 +220,17 @@ static bool ntt_shared_tables_enabled() {
   return on;
 }
 
+// Product kernel of the shared-table passes: 0 = byte comb (default: fewer VALU
+// ops, measured faster despite LDS bank conflicts on its 4 KiB tables), 1 = eight
+// conflict-free nibble tables (FLOCK_NTT_COMB=1, for schedule sweeps).
+static int ntt_comb_mode() {
+  static const int mode = [] {
+    const char* v = getenv("FLOCK_NTT_COMB");
+    return (v && v[0] == '1') ? 1 : 0;
+  }();
+  return mode;
+}
+
 template <int K>
 static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t log_d, uint32_t L, uint32_t log_lanes,
                        uint64_t src_pos_mask, uint32_t log_group, hipStream_t st) {
@@ -211,8 +242,12 @@ static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t l
   const int log_sets_per_wg = 8 - (int)log_lanes;
   if (ntt_shared_tables_enabled() && n_threads >= (uint64_t)NTT_WG &&
       (log_sets_per_wg <= 0 || (int)lo_bits >= log_sets_per_wg)) {
-    hipLaunchKernelGGL(ntt_pass_shared_kernel<K>, dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw, log_d, L,
-                       log_lanes, src_pos_mask, n_threads);
+    if (ntt_comb_mode() == 0)
+      hipLaunchKernelGGL((ntt_pass_shared_kernel<K, 0>), dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw,
+                         log_d, L, log_lanes, src_pos_mask, n_threads);
+    else
+      hipLaunchKernelGGL((ntt_pass_shared_kernel<K, 1>), dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw,
+                         log_d, L, log_lanes, src_pos_mask, n_threads);
     FLK_LAUNCH_CHECK("ntt_pass_shared_kernel");
     return FLK_OK;
   }
@@ -223,31 +258,81 @@ static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t l
   return FLK_OK;
 }
 
+// Pass schedule for layers [start, log_d): a list of layer counts (each 1..4).
+// The deepest pass (whose blocks are too small to share twiddles across a
+// work-group) takes up to 3 layers; the layers above it go in passes of 3 with
+// the remainder as a leading 1- or 2-layer pass (measured on MI355X: a leading
+// 4-layer pass at 2 waves/SIMD is slower than the extra HBM round trip).
+// FLOCK_NTT_SCHEDULE="4,3,3,3,3" overrides it (must sum to the layer count).
+static int ntt_schedule(uint32_t n, int kmax, int* ks) {
+  int cnt = 0;
+  const char* env = getenv("FLOCK_NTT_SCHEDULE");
+  if (env && env[0]) {
+    uint32_t sum = 0;
+    for (const char* p = env; *p;) {
+      const int k = atoi(p);
+      if (k < 1 || k > NTT_KMAX || cnt >= 64) return -1;
+      ks[cnt++] = k;
+      sum += (uint32_t)k;
+      while (*p && *p != ',') ++p;
+      if (*p == ',') ++p;
+    }
+    if (sum == n) return cnt;
+    cnt = 0;  // does not match this transform: fall back to the default
+  }
+  if (n == 0) return 0;
+  const uint32_t deep = n < (uint32_t)kmax ? n : (uint32_t)kmax;
+  uint32_t rest = n - deep;
+  if (kmax >= 3 && rest % 3 != 0) {
+    ks[cnt++] = (int)(rest % 3);
+    rest -= rest % 3;
+  }
+  while (rest > 0) {
+    const uint32_t k = rest < (uint32_t)kmax ? rest : (uint32_t)kmax;
+    ks[cnt++] = (int)k;
+    rest -= k;
+  }
+  ks[cnt++] = (int)deep;
+  return cnt;
+}
+
 // Run layers [start, log_d) reading the first pass from `src` (positions masked
 // by src_pos_mask) and writing all passes to `dst`.
 static int run_layers(const uint4* src, uint4* dst, const uint4* tw, uint32_t log_d, uint32_t start,
                       uint32_t log_lanes, uint64_t src_pos_mask, hipStream_t st) {
   const uint32_t log_group = log_lanes < 6 ? log_lanes : 6;
-  // LDS per wave is (64/G) * 2^(K-1) tables of 256 B; keep it <= 16 KiB.
-  int kmax = (int)log_group + 1;
-  const int kcap = ntt_radix_cap();
-  if (kmax > kcap) kmax = kcap;
+  // The per-group kernel holds (64/G) * 2^(K-1) tables of 256 B per wave: K <= log2(G) + 1.
+  const int kgroup = (int)log_group + 1;
+  int kmax = ntt_radix_cap();
+  int ks[64];
+  const int n_pass = ntt_schedule(log_d - start, kmax, ks);
+  if (n_pass < 0) return set_error(FLK_E_SHAPE, "FLOCK_NTT_SCHEDULE is malformed");
   uint32_t L = start;
   const uint4* s = src;
   uint64_t mask = src_pos_mask;
-  while (L < log_d) {
-    int K = (int)(log_d - L) < kmax ? (int)(log_d - L) : kmax;
-    int rc;
-    switch (K) {
-      case 1: rc = launch_pass<1>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
-      case 2: rc = launch_pass<2>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
-      case 3: rc = launch_pass<3>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
-      default: rc = launch_pass<4>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
+  for (int pi = 0; pi < n_pass; ++pi) {
+    int K = ks[pi];
+    // split a pass the per-group kernel cannot hold into smaller ones
+    const uint32_t lo_bits = log_d - L - (uint32_t)K;
+    const int log_sets_per_wg = 8 - (int)log_lanes;
+    const bool shared = ntt_shared_tables_enabled() && ((log_d - (uint32_t)K + log_lanes) >= 8) &&
+                        (log_sets_per_wg <= 0 || (int)lo_bits >= log_sets_per_wg);
+    int todo = K;
+    while (todo > 0) {
+      const int k = shared ? todo : (todo < kgroup ? todo : kgroup);
+      int rc;
+      switch (k) {
+        case 1: rc = launch_pass<1>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
+        case 2: rc = launch_pass<2>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
+        case 3: rc = launch_pass<3>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
+        default: rc = launch_pass<4>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
+      }
+      if (rc) return rc;
+      L += (uint32_t)k;
+      todo -= k;
+      s = dst;
+      mask = ~0ull;
     }
-    if (rc) return rc;
-    L += (uint32_t)K;
-    s = dst;
-    mask = ~0ull;
   }
   return FLK_OK;
 }
