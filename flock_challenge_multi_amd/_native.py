@@ -52,6 +52,11 @@ SIGNATURES = {
     "flk_last_error": (ctypes.c_char_p, []),
     "flk_version": (ctypes.c_char_p, []),
     "flk_f128_mul": (_F, [_F, _F]),
+    "flk_alloc": (_I, [_SZ, ctypes.POINTER(ctypes.c_void_p)]),
+    "flk_free": (_I, [_P]),
+    "flk_copy_to_device": (_I, [_P, _P, _SZ, _P]),
+    "flk_copy_to_host": (_I, [_P, _P, _SZ, _P]),
+    "flk_synchronize": (_I, [_P]),
     "flk_ntt_twiddles": (_I, [_SZ, _P]),
     "flk_ntt_forward_interleaved": (_I, [_SZ, _P, _SZ, _SZ, _SZ, _P]),
     "flk_ntt_rs_encode_interleaved": (_I, [_SZ, _P, _SZ, _P, _SZ, _SZ, _P]),

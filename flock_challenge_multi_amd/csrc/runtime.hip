@@ -192,6 +192,54 @@ const char* flk_version(void) { return "flock-mi355x 0.1 (gfx950)"; }
 
 flk_f128 flk_f128_mul(flk_f128 a, flk_f128 b) { return flk::host_mul(a, b); }
 
+int flk_alloc(size_t bytes, void** out) {
+  if (!out) return flk::set_error(FLK_E_SHAPE, "flk_alloc: null output");
+  *out = nullptr;
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (bytes == 0) return FLK_OK;
+  hipError_t e = hipMalloc(out, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return flk::set_error(FLK_E_NOMEM, "flk_alloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  return FLK_OK;
+}
+
+int flk_free(void* p) {
+  if (!p) return FLK_OK;
+  hipError_t e = hipFree(p);
+  if (e != hipSuccess) return flk::device_failure(e, "flk_free");
+  return FLK_OK;
+}
+
+int flk_copy_to_device(void* dst, const void* src, size_t bytes, void* stream) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!bytes) return FLK_OK;
+  if (!dst || !src) return flk::set_error(FLK_E_SHAPE, "flk_copy_to_device: null pointer");
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return flk::device_failure(e, "flk_copy_to_device");
+  return FLK_OK;
+}
+
+int flk_copy_to_host(void* dst, const void* src, size_t bytes, void* stream) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!bytes) return FLK_OK;
+  if (!dst || !src) return flk::set_error(FLK_E_SHAPE, "flk_copy_to_host: null pointer");
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return flk::device_failure(e, "flk_copy_to_host");
+  return FLK_OK;
+}
+
+int flk_synchronize(void* stream) {
+  hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return flk::device_failure(e, "flk_synchronize");
+  return FLK_OK;
+}
+
 int flk_zc_lagrange_weights(size_t k_skip, flk_f128 z, flk_f128* out) {
   // L_i(z) = prod_{j != i} (z + phi8(j)) / (phi8(i) + phi8(j)) on the nodes
   // phi8(0 .. 2^k_skip - 1) (CF/zerocheck/multilinear.rs:285-330).  phi8 is the

@@ -69,6 +69,24 @@ const char *flk_version(void);
 flk_f128 flk_f128_mul(flk_f128 a, flk_f128 b);
 
 /* ------------------------------------------------------------------------ */
+/* Device memory for callers without their own HIP binding (a Rust caller   */
+/* holding Vec<F128> host buffers).  No reference counterpart: the Metal    */
+/* facade wraps host memory zero-copy (CF/gpu.rs:1207-1213), which discrete */
+/* HBM cannot; these are the explicit staging calls that replace it.        */
+/* ------------------------------------------------------------------------ */
+
+/* hipMalloc of `bytes` into *out (NULL for 0 bytes). */
+int flk_alloc(size_t bytes, void **out);
+/* Free a pointer from flk_alloc (NULL is a no-op). */
+int flk_free(void *p);
+/* Enqueue a host -> device copy on `stream` (src is HOST memory). */
+int flk_copy_to_device(void *dst, const void *src, size_t bytes, void *stream);
+/* Enqueue a device -> host copy on `stream` (dst is HOST memory). */
+int flk_copy_to_host(void *dst, const void *src, size_t bytes, void *stream);
+/* Wait for all work enqueued on `stream`. */
+int flk_synchronize(void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Additive NTT over GF(2^128), LCH novel basis, neighbours-last            */
 /* (CF/ntt/additive_ntt_f128.rs).  `dim` is AdditiveNttF128::standard(dim)'s */
 /* log domain size; a transform of 2^log_d positions needs log_d <= dim.    */

@@ -140,3 +140,27 @@ def test_shape_refusals_do_not_latch(dev):
                              ctypes.c_void_p(t.data_ptr()), None)
     assert st == nat.FLK_E_SHAPE
     assert lib.flk_is_disabled() == 0
+
+
+def test_commit_from_host_buffers_via_c_abi(orc, dev):
+    """The host-buffer route a Rust caller takes: flk_alloc + flk_copy_to_device,
+    flk_commit, flk_copy_to_host of the codeword, flk_synchronize, flk_free."""
+    lib = nat.load()
+    z = orc.fill(77, 1 << 13)
+    cw_want, tree_want, root_want = orc.commit(z, 8, 2)
+    n_cw, n_pos = (1 << 13) << 2, (1 << 12)  # 8 lanes x 2^10 cols at rate 1/4 -> 2^12 positions
+    ptrs = [ctypes.c_void_p() for _ in range(3)]
+    for p, nbytes in zip(ptrs, (16 << 13, 16 * n_cw, 32 * (2 * n_pos - 1))):
+        nat.call("flk_alloc", nbytes, ctypes.byref(p))
+    try:
+        nat.call("flk_copy_to_device", ptrs[0], z.ctypes.data_as(ctypes.c_void_p), 16 << 13, None)
+        root = (ctypes.c_uint8 * 32)()
+        nat.call("flk_commit", ptrs[0], 10, 3, 2, nat.FLK_HASH_BLAKE3, ptrs[1], ptrs[2], root, None)
+        cw = np.zeros_like(cw_want)
+        nat.call("flk_copy_to_host", cw.ctypes.data_as(ctypes.c_void_p), ptrs[1], 16 * n_cw, None)
+        nat.call("flk_synchronize", None)
+        assert bytes(root) == root_want
+        assert same(cw, cw_want)
+    finally:
+        for p in ptrs:
+            nat.call("flk_free", p)
