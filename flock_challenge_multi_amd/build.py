@@ -54,5 +54,20 @@ def build(verbose=False):
     return OUT
 
 
+def build_host_mirror_test():
+    """Compile tests/cpp/host_mirror_test (the C++ host mirror's parity program,
+    linked against this library and the oracle).  Needs oracle/liboracle.so."""
+    root = os.path.dirname(HERE)
+    exe = os.path.join(root, "tests", "cpp", "host_mirror_test")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "include"), "-I", os.path.join(root, "oracle"),
+           os.path.join(root, "tests", "cpp", "host_mirror_test.cpp"), "-o", exe,
+           "-L", HERE, "-lflock_gpu", "-L", os.path.join(root, "oracle"), "-loracle",
+           "-Wl,-rpath,$ORIGIN/../../flock_challenge_multi_amd", "-Wl,-rpath,$ORIGIN/../../oracle"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host mirror test build failed:\n{r.stderr}")
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

@@ -3,6 +3,7 @@
 // cached_standard_twiddles, CF/ntt/additive_ntt_f128.rs:146-178) and set-up
 // arithmetic.
 #include <hip/hip_runtime.h>
+#include <immintrin.h>
 
 #include <atomic>
 #include <cstdarg>
@@ -48,7 +49,16 @@ int check_enabled() {
 }
 
 // ---- host GF(2^128) for set-up constants -------------------------------
-static inline void clmul64(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
+// 64x64 carry-less product: PCLMULQDQ when the host CPU has it (checked once),
+// otherwise the bit-serial loop.
+__attribute__((target("pclmul,sse4.1"))) static inline void clmul64_pclmul(uint64_t a, uint64_t b, uint64_t& lo,
+                                                                          uint64_t& hi) {
+  const __m128i p = _mm_clmulepi64_si128(_mm_set_epi64x(0, (long long)a), _mm_set_epi64x(0, (long long)b), 0x00);
+  lo = (uint64_t)_mm_cvtsi128_si64(p);
+  hi = (uint64_t)_mm_extract_epi64(p, 1);
+}
+
+static inline void clmul64_portable(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
   uint64_t l = 0, h = 0;
   for (int i = 0; i < 64; ++i)
     if ((a >> i) & 1) {
@@ -57,6 +67,15 @@ static inline void clmul64(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
     }
   lo = l;
   hi = h;
+}
+
+static const bool g_has_pclmul = __builtin_cpu_supports("pclmul");
+
+static inline void clmul64(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
+  if (g_has_pclmul)
+    clmul64_pclmul(a, b, lo, hi);
+  else
+    clmul64_portable(a, b, lo, hi);
 }
 
 flk_f128 host_mul(flk_f128 a, flk_f128 b) {

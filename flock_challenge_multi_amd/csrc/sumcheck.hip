@@ -153,16 +153,23 @@ struct EqFactors {
   uint4 e[16];  // e[k] = prod over the pass's t coordinates, selected by the bits of k
 };
 
-// First 2^s entries (s <= 8) by one work-group: thread i multiplies its s factors.
-__global__ void eq_seed_kernel(uint4* __restrict__ out, EqFactors one_plus, EqFactors rr, uint32_t s) {
+// First 2^s entries (s <= 8) by one work-group: thread i multiplies its s factors
+// with a private comb table of its running product.
+__global__ void __launch_bounds__(256) eq_seed_kernel(uint4* __restrict__ out, EqFactors one_plus, EqFactors rr,
+                                                      uint32_t s) {
+  __shared__ uint4 tabs[16 * 256];
   const uint32_t i = threadIdx.x;
-  if (i >= (1u << s)) return;
   uint4 acc = make_uint4(1u, 0u, 0u, 0u);
   for (uint32_t j = 0; j < s; ++j) {
     const uint4 fac = ((i >> j) & 1u) ? rr.e[j] : one_plus.e[j];
-    acc = j == 0 ? fac : f_mul_slow(acc, fac);
+    if (j == 0) {
+      acc = fac;
+    } else {
+      table_store_private(tabs, blockDim.x, i, acc);
+      acc = comb_mul_rt(fac, tabs + i, blockDim.x);
+    }
   }
-  out[i] = acc;
+  if (i < (1u << s)) out[i] = acc;
 }
 
 // out[i + k*len] = out[i] * E[k] for k < 2^t, i < len (in place; reads [0,len)).

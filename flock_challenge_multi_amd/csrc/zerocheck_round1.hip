@@ -361,7 +361,9 @@ extern "C" int flk_zc_round1(const uint8_t* a_bits, const uint8_t* b_bits, const
   uint4* out = reinterpret_cast<uint4*>(out_ab);
   hipLaunchKernelGGL(zero128_kernel, dim3(1), dim3(128), 0, st, out);
   FLK_LAUNCH_CHECK("zero128_kernel");
-  uint64_t grid = n_outer < 512 ? n_outer : 512;
+  // LDS (~53 KiB per work-group) allows 3 work-groups per CU: 768 fill the chip;
+  // 2048 leave room for uneven block counts per work-group.
+  uint64_t grid = n_outer < 2048 ? n_outer : 2048;
   hipLaunchKernelGGL(zc_round1_kernel, dim3((unsigned)grid), dim3(R1_WG), 0, st,
                      reinterpret_cast<const uint64_t*>(a_bits), reinterpret_cast<const uint64_t*>(b_bits),
                      reinterpret_cast<const uint64_t*>(c_bits), (uint64_t)n_outer, eq, T, out);

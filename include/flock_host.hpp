@@ -1,0 +1,317 @@
+// flock_host.hpp — C++ mirror of the reference's Rust hot-path interface over the
+// C ABI of include/flock_gpu.h.  Names and argument meaning follow the Rust
+// entry points (`CF/` = crates/flock-core/src/); device-resident data lives in
+// DeviceVec<T> (RAII over flk_alloc/flk_free).  A non-OK status throws GpuError
+// carrying the library message; shape errors are raised before any launch.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "flock_gpu.h"
+
+namespace flock {
+
+using F128 = flk_f128;  // #[repr(C, align(16))] { lo: u64, hi: u64 } (CF/field/gf2_128.rs:22-27)
+using Hash = std::array<uint8_t, 32>;
+
+struct GpuError : std::runtime_error {
+  int code;
+  GpuError(int c, const std::string& where) : std::runtime_error(where + ": " + flk_last_error()), code(c) {}
+};
+
+inline void check(int rc, const char* where) {
+  if (rc != FLK_OK) throw GpuError(rc, where);
+}
+
+inline F128 operator+(F128 a, F128 b) { return F128{a.lo ^ b.lo, a.hi ^ b.hi}; }
+inline F128 operator*(F128 a, F128 b) { return flk_f128_mul(a, b); }
+inline bool operator==(F128 a, F128 b) { return a.lo == b.lo && a.hi == b.hi; }
+
+// Device buffer of n elements of T.
+template <class T>
+class DeviceVec {
+ public:
+  DeviceVec() = default;
+  explicit DeviceVec(size_t n) : n_(n) {
+    void* p = nullptr;
+    check(flk_alloc(n * sizeof(T), &p), "flk_alloc");
+    p_ = static_cast<T*>(p);
+  }
+  explicit DeviceVec(const std::vector<T>& host, void* stream = nullptr) : DeviceVec(host.size()) {
+    upload(host, stream);
+  }
+  ~DeviceVec() {
+    if (p_) flk_free(p_);
+  }
+  DeviceVec(DeviceVec&& o) noexcept : p_(o.p_), n_(o.n_) {
+    o.p_ = nullptr;
+    o.n_ = 0;
+  }
+  DeviceVec& operator=(DeviceVec&& o) noexcept {
+    std::swap(p_, o.p_);
+    std::swap(n_, o.n_);
+    return *this;
+  }
+  DeviceVec(const DeviceVec&) = delete;
+  DeviceVec& operator=(const DeviceVec&) = delete;
+
+  T* data() { return p_; }
+  const T* data() const { return p_; }
+  size_t len() const { return n_; }
+
+  void upload(const std::vector<T>& host, void* stream = nullptr) {
+    if (host.size() != n_) throw std::invalid_argument("DeviceVec::upload: length mismatch");
+    check(flk_copy_to_device(p_, host.data(), n_ * sizeof(T), stream), "flk_copy_to_device");
+  }
+  std::vector<T> to_host(void* stream = nullptr) const {
+    std::vector<T> h(n_);
+    check(flk_copy_to_host(h.data(), p_, n_ * sizeof(T), stream), "flk_copy_to_host");
+    check(flk_synchronize(stream), "flk_synchronize");
+    return h;
+  }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+inline bool is_pow2(size_t n) { return n && !(n & (n - 1)); }
+inline size_t log2_exact(size_t n) {
+  if (!is_pow2(n)) throw std::invalid_argument("length must be a power of two");
+  size_t l = 0;
+  while ((size_t(1) << l) < n) ++l;
+  return l;
+}
+
+// ---------------------------------------------------------------------------
+// crate::ntt::AdditiveNttF128 (CF/ntt/additive_ntt_f128.rs:2661-2830)
+// ---------------------------------------------------------------------------
+namespace ntt {
+class AdditiveNttF128 {
+ public:
+  static AdditiveNttF128 standard(size_t dim) { return AdditiveNttF128(dim); }
+  size_t log_domain_size() const { return dim_; }
+  F128 twiddle(size_t layer, size_t block) const {
+    if (layer >= dim_ || block >= (size_t(1) << layer)) throw std::out_of_range("twiddle");
+    if (tw_.empty()) {
+      tw_.resize((size_t(1) << dim_) - 1);
+      check(flk_ntt_twiddles(dim_, tw_.data()), "flk_ntt_twiddles");
+    }
+    return tw_[(size_t(1) << layer) - 1 + block];
+  }
+  void rs_encode_interleaved(const DeviceVec<F128>& msg, DeviceVec<F128>& codeword, size_t num_ntts,
+                             void* stream = nullptr) const {
+    check(flk_ntt_rs_encode_interleaved(dim_, msg.data(), msg.len(), codeword.data(), codeword.len(), num_ntts, stream),
+          "rs_encode_interleaved");
+  }
+  void forward_transform_interleaved(DeviceVec<F128>& data, size_t num_ntts, size_t start_layer = 0,
+                                     void* stream = nullptr) const {
+    check(flk_ntt_forward_interleaved(dim_, data.data(), data.len(), num_ntts, start_layer, stream),
+          "forward_transform_interleaved");
+  }
+
+ private:
+  explicit AdditiveNttF128(size_t dim) : dim_(dim) {
+    if (dim > 24) throw std::invalid_argument("standard NTT: dim <= 24 on the device");
+  }
+  size_t dim_;
+  mutable std::vector<F128> tw_;
+};
+}  // namespace ntt
+
+// ---------------------------------------------------------------------------
+// crate::merkle (CF/merkle.rs:498,683,753) — BLAKE3 trees
+// ---------------------------------------------------------------------------
+namespace merkle {
+enum class HashKind { Sha256 = FLK_HASH_SHA256, Blake3 = FLK_HASH_BLAKE3 };
+
+inline DeviceVec<Hash> merkle_tree(const uint8_t* data, size_t data_len, size_t num_leaves, HashKind kind,
+                                   void* stream = nullptr) {
+  DeviceVec<Hash> tree(2 * num_leaves - 1);
+  check(flk_merkle_tree(data, data_len, num_leaves, static_cast<int>(kind), tree.data()->data(), stream),
+        "merkle_tree");
+  return tree;
+}
+}  // namespace merkle
+
+// ---------------------------------------------------------------------------
+// crate::pcs (CF/pcs/commit.rs:34-227) and ligerito::ligero_commit (:4750)
+// ---------------------------------------------------------------------------
+namespace pcs {
+constexpr size_t LOG_PACKING = 7;
+
+struct PcsParams {
+  size_t m;
+  size_t log_inv_rate = 1;
+  size_t log_batch_size = 6;
+  merkle::HashKind merkle_hash = merkle::HashKind::Blake3;
+  size_t log_msg_len() const { return m - LOG_PACKING; }
+  size_t log_dim() const { return log_msg_len() - log_batch_size; }
+  size_t k_code() const { return log_dim() + log_inv_rate; }
+  size_t n_positions() const { return size_t(1) << k_code(); }
+  size_t num_ntts() const { return size_t(1) << log_batch_size; }
+  size_t codeword_len_f128() const { return n_positions() * num_ntts(); }
+  size_t n_leaves() const { return n_positions(); }
+};
+
+struct Commitment {
+  Hash root;
+  PcsParams params;
+};
+
+struct ProverData {
+  DeviceVec<F128> codeword;
+  DeviceVec<Hash> merkle_tree;
+};
+
+inline ProverData ligero_commit(const DeviceVec<F128>& poly, size_t log_msg_cols, size_t log_num_interleaved,
+                                size_t log_inv_rate, merkle::HashKind kind, Hash* root_out = nullptr,
+                                void* stream = nullptr) {
+  if (poly.len() != size_t(1) << (log_msg_cols + log_num_interleaved))
+    throw std::invalid_argument("ligero_commit: poly length");
+  const size_t n_pos = size_t(1) << (log_msg_cols + log_inv_rate);
+  ProverData pd{DeviceVec<F128>(n_pos << log_num_interleaved), DeviceVec<Hash>(2 * n_pos - 1)};
+  check(flk_commit(poly.data(), log_msg_cols, log_num_interleaved, log_inv_rate, static_cast<int>(kind),
+                   pd.codeword.data(), pd.merkle_tree.data()->data(), root_out ? root_out->data() : nullptr, stream),
+        "commit");
+  return pd;
+}
+
+inline std::pair<Commitment, ProverData> commit(const DeviceVec<F128>& z_packed, const PcsParams& params,
+                                                void* stream = nullptr) {
+  if (z_packed.len() != size_t(1) << params.log_msg_len()) throw std::invalid_argument("commit: z_packed length");
+  if (params.log_inv_rate < 1) throw std::invalid_argument("log_inv_rate must be >= 1 for a non-trivial RS code");
+  Commitment c{{}, params};
+  ProverData pd = ligero_commit(z_packed, params.log_dim(), params.log_batch_size, params.log_inv_rate,
+                                params.merkle_hash, &c.root, stream);
+  return {c, std::move(pd)};
+}
+}  // namespace pcs
+
+// ---------------------------------------------------------------------------
+// ligerito::SumcheckProver (CF/pcs/ligerito.rs:8893-9280)
+// ---------------------------------------------------------------------------
+namespace ligerito {
+struct SumcheckMessage {
+  F128 u_0, u_2;
+};
+
+class SumcheckProver {
+ public:
+  static std::pair<SumcheckProver, SumcheckMessage> new_(DeviceVec<F128> f, DeviceVec<F128> b1, F128 h1,
+                                                         void* stream = nullptr) {
+    if (f.len() != b1.len()) throw std::invalid_argument("SumcheckProver::new: lengths differ");
+    SumcheckProver p(std::move(f), std::move(b1), h1, stream);
+    SumcheckMessage msg = p.message_of(p.f_, p.basis_);
+    p.transcript_.push_back(msg);
+    return {std::move(p), msg};
+  }
+
+  SumcheckMessage fold(F128 r) {
+    if (pending_) throw std::logic_error("fold before ordinary glue");
+    const size_t n = f_.len();
+    DeviceVec<F128> nf(n / 2), nb(n / 2);
+    check(flk_sc_fold_and_msg(f_.data(), basis_.data(), n, r, nf.data(), nb.data(), msg_.data(), stream_),
+          "fold_and_msg");
+    f_ = std::move(nf);
+    basis_ = std::move(nb);
+    SumcheckMessage m = read_msg();
+    transcript_.push_back(m);
+    return m;
+  }
+
+  SumcheckMessage introduce_new(DeviceVec<F128> b_new, F128 h_new) {
+    if (b_new.len() != f_.len()) throw std::invalid_argument("introduce_new: length");
+    SumcheckMessage m = message_of(f_, b_new);
+    transcript_.push_back(m);
+    b_new_ = std::move(b_new);
+    h_new_ = h_new;
+    pending_ = true;
+    return m;
+  }
+
+  void glue(F128 alpha) {
+    if (!pending_) throw std::logic_error("glue without introduce_new");
+    check(flk_sc_glue(basis_.data(), b_new_.data(), basis_.len(), alpha, stream_), "glue");
+    t_r_ = t_r_ + alpha * h_new_;
+    pending_ = false;
+  }
+
+  const DeviceVec<F128>& f() const { return f_; }
+  const DeviceVec<F128>& combined_basis() const { return basis_; }
+  F128 target() const { return t_r_; }
+  const std::vector<SumcheckMessage>& transcript() const { return transcript_; }
+
+ private:
+  SumcheckProver(DeviceVec<F128> f, DeviceVec<F128> b, F128 h, void* stream)
+      : f_(std::move(f)), basis_(std::move(b)), msg_(2), t_r_(h), stream_(stream) {}
+  SumcheckMessage message_of(const DeviceVec<F128>& f, const DeviceVec<F128>& b) {
+    check(flk_sc_round_msg(f.data(), b.data(), f.len(), msg_.data(), stream_), "round_msg");
+    return read_msg();
+  }
+  SumcheckMessage read_msg() {
+    std::vector<F128> h = msg_.to_host(stream_);
+    return SumcheckMessage{h[0], h[1]};
+  }
+  DeviceVec<F128> f_, basis_, msg_, b_new_;
+  F128 t_r_, h_new_{0, 0};
+  bool pending_ = false;
+  void* stream_;
+  std::vector<SumcheckMessage> transcript_;
+};
+}  // namespace ligerito
+
+// ---------------------------------------------------------------------------
+// lincheck / ring_switch / zerocheck reductions
+// ---------------------------------------------------------------------------
+namespace lincheck {
+inline DeviceVec<F128> partial_fold_packed_z(const DeviceVec<uint8_t>& z_packed, size_t m, size_t k_log,
+                                             const DeviceVec<F128>& eq_outer, void* stream = nullptr) {
+  DeviceVec<F128> out(size_t(1) << k_log);
+  check(flk_lc_partial_fold(z_packed.data(), m, k_log, eq_outer.data(), out.data(), stream), "partial_fold");
+  return out;
+}
+}  // namespace lincheck
+
+namespace ring_switch {
+inline DeviceVec<F128> fold_1b_rows(const DeviceVec<F128>& packed_witness, const DeviceVec<F128>& suffix_tensor,
+                                    void* stream = nullptr) {
+  DeviceVec<F128> out(128);
+  check(flk_rs_fold_1b_rows(packed_witness.data(), suffix_tensor.data(), packed_witness.len(), out.data(), stream),
+        "fold_1b_rows");
+  return out;
+}
+inline DeviceVec<F128> fold_b128_elems(const DeviceVec<F128>& suffix_tensor, const DeviceVec<F128>& eq_r_dprime,
+                                       void* stream = nullptr) {
+  if (eq_r_dprime.len() != 128) throw std::invalid_argument("fold_b128_elems: eq_r_dprime must have 128 entries");
+  DeviceVec<F128> out(suffix_tensor.len());
+  check(flk_rs_fold_b128_elems(suffix_tensor.data(), eq_r_dprime.data(), suffix_tensor.len(), out.data(), stream),
+        "fold_b128_elems");
+  return out;
+}
+}  // namespace ring_switch
+
+namespace zerocheck {
+inline std::vector<F128> lagrange_weights(size_t k_skip, F128 z) {
+  std::vector<F128> w(size_t(1) << k_skip);
+  check(flk_zc_lagrange_weights(k_skip, z, w.data()), "lagrange_weights");
+  return w;
+}
+inline std::pair<std::vector<F128>, std::vector<F128>> round1(const DeviceVec<uint8_t>& a, const DeviceVec<uint8_t>& b,
+                                                              const DeviceVec<uint8_t>& c, size_t m,
+                                                              const std::vector<F128>& r_outer,
+                                                              void* stream = nullptr) {
+  DeviceVec<F128> out(128);
+  check(flk_zc_round1(a.data(), b.data(), c.data(), m, r_outer.data(), out.data(), out.data() + 64, stream), "round1");
+  std::vector<F128> h = out.to_host(stream);
+  return {std::vector<F128>(h.begin(), h.begin() + 64), std::vector<F128>(h.begin() + 64, h.end())};
+}
+}  // namespace zerocheck
+
+}  // namespace flock

@@ -1,0 +1,108 @@
+// Parity of the C++ host mirror (include/flock_host.hpp) on the GPU against the
+// oracle, written the way the reference's own tests read
+// (CF/pcs/commit.rs:1838-1877 commit_matches_full_ntt_oracle; the SumcheckProver
+// fold/introduce/glue sequence of CF/pcs/ligerito.rs:8906-9275).  Exit code 0 = pass.
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "flock_host.hpp"
+#include "flock_oracle.h"
+
+using flock::F128;
+using flock::DeviceVec;
+using flock::operator+;
+using flock::operator*;
+using flock::operator==;
+
+static std::vector<F128> rand_f128(uint64_t seed, size_t n) {
+  std::vector<F128> v(n);
+  orc_fill_f128(seed, reinterpret_cast<orc_f128*>(v.data()), n);
+  return v;
+}
+static bool same(const std::vector<F128>& a, const std::vector<F128>& b) {
+  if (a.size() != b.size()) return false;
+  for (size_t i = 0; i < a.size(); ++i)
+    if (!(a[i] == b[i])) return false;
+  return true;
+}
+#define EXPECT(cond, what)                      \
+  do {                                          \
+    if (!(cond)) {                              \
+      std::fprintf(stderr, "FAIL: %s\n", what); \
+      return 1;                                 \
+    }                                           \
+  } while (0)
+
+static orc_f128 O(F128 a) { return orc_f128{a.lo, a.hi}; }
+
+int main() {
+  if (!flk_device_available()) {
+    std::fprintf(stderr, "no device: %s\n", flk_last_error());
+    return 2;
+  }
+  // commit_matches_full_ntt_oracle at (m, r, log_batch) = (14, 2, 3) and (18, 1, 6)
+  const size_t shapes[2][3] = {{14, 2, 3}, {18, 1, 6}};
+  for (auto& sh : shapes) {
+    flock::pcs::PcsParams params{sh[0], sh[1], sh[2]};
+    std::vector<F128> z = rand_f128(0xFEED + sh[0], size_t(1) << params.log_msg_len());
+    DeviceVec<F128> zd(z);
+    auto [commitment, pd] = flock::pcs::commit(zd, params);
+    std::vector<F128> cw(params.codeword_len_f128());
+    std::vector<uint8_t> tree(32 * (2 * params.n_leaves() - 1));
+    uint8_t root[32];
+    EXPECT(orc_commit(reinterpret_cast<const orc_f128*>(z.data()), z.size(), params.num_ntts(), params.log_inv_rate,
+                      reinterpret_cast<orc_f128*>(cw.data()), tree.data(), root) == 0,
+           "oracle commit");
+    EXPECT(same(pd.codeword.to_host(), cw), "codeword == zero-padded full NTT");
+    EXPECT(std::memcmp(commitment.root.data(), root, 32) == 0, "root == merkle_tree(codeword)");
+    auto ntt = flock::ntt::AdditiveNttF128::standard(params.k_code());
+    EXPECT(ntt.twiddle(0, 0) == (F128{0, 0}), "twiddle(0,0) == 0");
+  }
+  // SumcheckProver: new -> fold x2 -> introduce_new -> glue -> fold
+  {
+    const size_t n = 1 << 12;
+    std::vector<F128> f = rand_f128(1, n), b = rand_f128(2, n);
+    F128 h1{0, 0};
+    for (size_t i = 0; i < n; ++i) h1 = h1 + f[i] * b[i];
+    auto [prover, msg] = flock::ligerito::SumcheckProver::new_(DeviceVec<F128>(f), DeviceVec<F128>(b), h1);
+    orc_f128 want[2];
+    orc_sc_round_msg(reinterpret_cast<orc_f128*>(f.data()), reinterpret_cast<orc_f128*>(b.data()), n, want);
+    EXPECT(msg.u_0.lo == want[0].lo && msg.u_2.hi == want[1].hi, "first message");
+    std::vector<F128> cf = f, cb = b;
+    uint64_t st = 7;
+    for (int rnd = 0; rnd < 3; ++rnd) {
+      if (rnd == 2) {
+        std::vector<F128> bn = rand_f128(3, cf.size());
+        F128 hn{orc_splitmix64_next(&st), orc_splitmix64_next(&st)};
+        auto m2 = prover.introduce_new(DeviceVec<F128>(bn), hn);
+        orc_sc_round_msg(reinterpret_cast<orc_f128*>(cf.data()), reinterpret_cast<orc_f128*>(bn.data()), cf.size(), want);
+        EXPECT(m2.u_0.lo == want[0].lo && m2.u_2.lo == want[1].lo, "introduce_new message");
+        F128 alpha{orc_splitmix64_next(&st), orc_splitmix64_next(&st)};
+        prover.glue(alpha);
+        orc_sc_glue(reinterpret_cast<orc_f128*>(cb.data()), reinterpret_cast<orc_f128*>(bn.data()), cb.size(), O(alpha));
+        EXPECT(same(prover.combined_basis().to_host(), cb), "glued basis");
+        EXPECT(prover.target() == h1 + alpha * hn, "glued target");
+      }
+      F128 r{orc_splitmix64_next(&st), orc_splitmix64_next(&st)};
+      auto m = prover.fold(r);
+      std::vector<F128> nf(cf.size() / 2), nb(cb.size() / 2);
+      orc_sc_fold_and_msg(reinterpret_cast<orc_f128*>(cf.data()), reinterpret_cast<orc_f128*>(cb.data()), cf.size(),
+                          O(r), reinterpret_cast<orc_f128*>(nf.data()), reinterpret_cast<orc_f128*>(nb.data()), want);
+      cf = nf;
+      cb = nb;
+      EXPECT(m.u_0.lo == want[0].lo && m.u_0.hi == want[0].hi && m.u_2.lo == want[1].lo, "fold message");
+    }
+    EXPECT(same(prover.f().to_host(), cf), "folded f");
+  }
+  // ring switch fold_1b_rows
+  {
+    std::vector<F128> z = rand_f128(11, 5000), e = rand_f128(12, 5000), want(128);
+    auto got = flock::ring_switch::fold_1b_rows(DeviceVec<F128>(z), DeviceVec<F128>(e)).to_host();
+    orc_rs_fold_1b_rows(reinterpret_cast<orc_f128*>(z.data()), reinterpret_cast<orc_f128*>(e.data()), z.size(),
+                        reinterpret_cast<orc_f128*>(want.data()));
+    EXPECT(same(got, want), "fold_1b_rows");
+  }
+  std::printf("host mirror parity ok\n");
+  return 0;
+}
