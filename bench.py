@@ -41,6 +41,22 @@ def parse():
     return ap.parse_args()
 
 
+def measured_encode_traffic():
+    """HBM bytes per RS encode from the committed rocprofv3 counter summary
+    (FETCH_SIZE x 2 + WRITE_SIZE x 1 over the encode's passes, scaled by a plain
+    copy of known size).  None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "commit_config1.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("- Measured HBM traffic per encode"):
+                    gb = float(line.split(":", 1)[1].split("GB")[0].strip())
+                    return int(gb * 1e9), "profiles/commit_config1.md"
+    except (OSError, ValueError):
+        pass
+    return None, None
+
+
 def pctl(xs, q):
     xs = sorted(xs)
     if not xs:
@@ -305,6 +321,7 @@ def main():
 
     kernels = None if args.no_kernels else kernel_table(torch, nat, hp, sptr, st)
 
+    traffic, traffic_src = measured_encode_traffic()
     line = {
         "metric": "verified BLAKE3 compressions/s (2^18 batch, 1 GPU) + p50/p99 proof latency",
         "value": value,
@@ -330,7 +347,7 @@ def main():
         "breakdown_ms": {"rs_encode": enc_ms, "merkle_tree": mer_ms},
         "roofline": {
             "bound": "hbm", "achieved": enc_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": enc_gbs / HBM_PEAK_GBS, "traffic": None,
+            "frac": enc_gbs / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
             "kernel": "RS encode (ntt_pass_kernel passes), algorithmic bytes 16*(N_msg+N_cw) = "
                       f"{enc_bytes} per encode",
         },

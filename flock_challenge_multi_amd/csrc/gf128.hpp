@@ -204,38 +204,6 @@ __device__ __forceinline__ uint4 comb8_mul(uint4 v, const uint4* __restrict__ ta
   return f_reduce(a);
 }
 
-// Unreduced product v * t with eight 16-entry tables tab[j*16 + k] = k * t * x^(4j)
-// (j = nibble position inside a 32-bit word).  Every lookup lands in a 256-byte
-// table, i.e. one 16-byte slot per bank group, so the 32 lookups are free of LDS
-// bank conflicts; no accumulator shifts are needed (word w of v places its
-// entries at word offset w).
-__device__ __forceinline__ void combj_mul_acc(uint4 v, const uint4* __restrict__ tab, U256& acc) {
-  const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
-  U256 a;
-  u256_zero(a);
-  // Two nibble positions per iteration: eight loads in flight per lane.
-#pragma unroll 2
-  for (int j = 0; j < 8; ++j) {
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      const uint32_t nib = (vw[w] >> (4 * j)) & 15u;
-      const uint4 e = tab[j * 16 + nib];
-      a.w[w] ^= e.x;
-      a.w[w + 1] ^= e.y;
-      a.w[w + 2] ^= e.z;
-      a.w[w + 3] ^= e.w;
-    }
-  }
-  u256_xor(acc, a);
-}
-
-__device__ __forceinline__ uint4 combj_mul(uint4 v, const uint4* __restrict__ tab) {
-  U256 a;
-  u256_zero(a);
-  combj_mul_acc(v, tab, a);
-  return f_reduce(a);
-}
-
 // t * x^s for 0 <= s < 32: shift by s, fold the bits above 127 once
 // (U x^128 = U ^ U<<1 ^ U<<2 ^ U<<7 for the <= 31-bit overflow U).
 __device__ __forceinline__ uint4 f_mul_xs(uint4 t, uint32_t s) {
@@ -286,5 +254,108 @@ __device__ __forceinline__ uint4 f_mul_slow(uint4 a, uint4 b) {
 }
 
 __device__ __forceinline__ uint4 ld_f128(const uint4* p) { return *p; }
+
+// ---------------------------------------------------------------------------
+// gfx950-specific forms.  The three-input XOR V_BITOP3_B32 (truth table 0x96)
+// folds two table words per op into the accumulator; the combs below run
+// rolled over the four byte positions, taking the top byte of each word per
+// step (one SDWA op per table address) and shifting the words up by 8, which
+// keeps at most eight table loads in flight and the kernels at <= 128 VGPRs.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+__device__ __forceinline__ uint4 lds_entry(const uint4* tab, uint32_t byte_off) {
+  return *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(tab) + byte_off);
+}
+
+// add + (r mod p) for any 256-bit r.  Same folding as f_reduce, with the
+// second-order overflow ov (at most 7 bits) merged into u0 before the shifts:
+// z = u0 ^ ov, and since ov << 7 stays inside word 0, only u0's own high bits
+// cross into word 1 (through the funnels of word 1).
+__device__ __forceinline__ uint4 f_reduce_add(const uint32_t r[8], uint4 add) {
+  const uint32_t u0 = r[4], u1 = r[5], u2 = r[6], u3 = r[7];
+  const uint32_t ov = xor3(u3 >> 31, u3 >> 30, u3 >> 25);
+  const uint32_t z = u0 ^ ov;
+  uint4 o;
+  o.x = xor3(xor3(r[0], add.x, z), z << 1, z << 2) ^ (z << 7);
+  o.y = xor3(xor3(r[1], add.y, u1), funnel(u1, u0, 31), funnel(u1, u0, 30)) ^ funnel(u1, u0, 25);
+  o.z = xor3(xor3(r[2], add.z, u2), funnel(u2, u1, 31), funnel(u2, u1, 30)) ^ funnel(u2, u1, 25);
+  o.w = xor3(xor3(r[3], add.w, u3), funnel(u3, u2, 31), funnel(u3, u2, 30)) ^ funnel(u3, u2, 25);
+  return o;
+}
+
+// 16 * (high nibble of byte J of w), i.e. (w >> (8J + 4) & 15) << 4.  Plain
+// shifts and masks: the compiler turns J = 2, 3 into single SDWA ops and can
+// still schedule and rematerialise them (inline-asm SDWA cannot be).
+template <int J>
+__device__ __forceinline__ uint32_t hinib_x16(uint32_t w) {
+  return (w >> (8 * J)) & 0xF0u;
+}
+
+// add + v * t from two 16-entry nibble tables, tab[k] = k * t and
+// tab[16 + k] = k * t * x^4: byte J of each word is looked up as its two
+// nibbles (both entries at the word's offset) and the bytes are combined by
+// Horner in x^8, top byte first.  32 lookups in 512 B of table (built per
+// twiddle where a twiddle serves too few products for a 4 KiB byte table).
+// The byte loop stays rolled: unrolled, the compiler hoists all 32 loads and
+// spills; rolled, at most eight are in flight.
+__device__ __forceinline__ uint4 comb4p_mul_add(uint4 v, uint4 add, const uint4* tab) {
+  uint32_t hw[4] = {v.x, v.y, v.z, v.w};
+  uint32_t lw[4] = {v.x << 4, v.y << 4, v.z << 4, v.w << 4};
+  uint32_t a[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a[i] = 0u;
+#pragma unroll 1
+  for (int J = 0; J < 4; ++J) {
+    uint4 l[4], h[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      l[w] = lds_entry(tab, hinib_x16<3>(lw[w]));
+      h[w] = lds_entry(tab + 16, hinib_x16<3>(hw[w]));
+      lw[w] <<= 8;
+      hw[w] <<= 8;
+    }
+#pragma unroll
+    for (int i = 7; i >= 1; --i) a[i] = funnel(a[i], a[i - 1], 24);
+    a[0] = xor3(a[0] << 8, l[0].x, h[0].x);
+    a[1] = xor3(xor3(a[1], l[0].y, h[0].y), l[1].x, h[1].x);
+    a[2] = xor3(xor3(xor3(a[2], l[0].z, h[0].z), l[1].y, h[1].y), l[2].x, h[2].x);
+    a[3] = xor3(xor3(xor3(xor3(a[3], l[0].w, h[0].w), l[1].z, h[1].z), l[2].y, h[2].y), l[3].x, h[3].x);
+    a[4] = xor3(xor3(xor3(a[4], l[1].w, h[1].w), l[2].z, h[2].z), l[3].y, h[3].y);
+    a[5] = xor3(xor3(a[5], l[2].w, h[2].w), l[3].z, h[3].z);
+    a[6] = xor3(a[6], l[3].w, h[3].w);
+  }
+  return f_reduce_add(a, add);
+}
+
+// add + v * t with tab[b] = b * t (b in 0..255, LDS): the byte comb with
+// three-input XORs, rolled over the four byte positions like comb4p_mul_add.
+__device__ __forceinline__ uint4 comb8r_mul_add(uint4 v, uint4 add, const uint4* tab) {
+  uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+  uint32_t a[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a[i] = 0u;
+#pragma unroll 1
+  for (int J = 0; J < 4; ++J) {
+    uint4 e[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      e[w] = lds_entry(tab, (vw[w] >> 20) & 0xFF0u);
+      vw[w] <<= 8;
+    }
+#pragma unroll
+    for (int i = 7; i >= 1; --i) a[i] = funnel(a[i], a[i - 1], 24);
+    a[0] = (a[0] << 8) ^ e[0].x;
+    a[1] = xor3(a[1], e[0].y, e[1].x);
+    a[2] = xor3(a[2], e[0].z, e[1].y) ^ e[2].x;
+    a[3] = xor3(xor3(a[3], e[0].w, e[1].z), e[2].y, e[3].x);
+    a[4] = xor3(a[4], e[1].w, e[2].z) ^ e[3].y;
+    a[5] = xor3(a[5], e[2].w, e[3].z);
+    a[6] ^= e[3].w;
+  }
+  return f_reduce_add(a, add);
+}
 
 }  // namespace flk

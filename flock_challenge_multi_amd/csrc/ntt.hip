@@ -30,9 +30,12 @@
 namespace flk {
 
 constexpr int NTT_WG = 256;
+// Occupancy target of the bitop3 comb kernels: 4 waves per SIMD (<= 128 VGPRs);
+// measured faster than 2 waves with the comb unrolled or 5 waves with spills.
+constexpr int NTT_WAVES = 4;
 
-template <int K>
-__global__ void __launch_bounds__(NTT_WG) ntt_pass_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+template <int K, bool PAIR>  // PAIR: 32-entry nibble-pair tables and comb4p_mul_add
+__global__ void __launch_bounds__(NTT_WG) __attribute__((amdgpu_waves_per_eu(PAIR && K <= 3 ? NTT_WAVES : 1))) ntt_pass_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                          const uint4* __restrict__ tw, uint32_t log_d, uint32_t L,
                                                          uint32_t log_lanes, uint64_t src_pos_mask,
                                                          uint64_t n_threads, uint32_t log_group) {
@@ -42,7 +45,8 @@ __global__ void __launch_bounds__(NTT_WG) ntt_pass_kernel(const uint4* __restric
   const uint32_t tid = threadIdx.x;
   const uint32_t wave = tid >> 6, lane_w = tid & 63u;
   const uint32_t grp = lane_w >> log_group, lane_g = lane_w & (G - 1u);
-  uint4* my_tabs = lds + ((size_t)(wave * groups_per_wave + grp) << (K - 1 + 4));
+  constexpr int TAB = PAIR ? 32 : 16;  // entries per twiddle
+  uint4* my_tabs = lds + (size_t)(wave * groups_per_wave + grp) * ((size_t)TAB << (K - 1));
 
   const uint64_t g = (uint64_t)blockIdx.x * NTT_WG + tid;
   const bool active = g < n_threads;
@@ -62,11 +66,12 @@ __global__ void __launch_bounds__(NTT_WG) ntt_pass_kernel(const uint4* __restric
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     // Tables of the 2^i twiddles of layer L+i met by this set.
-    const uint32_t n_entries = 16u << i;
+    const uint32_t n_entries = (uint32_t)TAB << i;
     const uint64_t tw_base = ((1ull << (L + i)) - 1ull) + (prefix << i);
     for (uint32_t e = lane_g; e < n_entries; e += G) {
-      const uint32_t sb = e >> 4, k = e & 15u;
-      const uint4 t = active ? tw[tw_base + sb] : f_zero();
+      const uint32_t sb = PAIR ? e >> 5 : e >> 4, k = e & 15u;
+      uint4 t = active ? tw[tw_base + sb] : f_zero();
+      if (PAIR && ((e >> 4) & 1u)) t = f_mul_x4(t);
       my_tabs[e] = comb_entry(t, k);
     }
     __syncthreads();
@@ -74,13 +79,17 @@ __global__ void __launch_bounds__(NTT_WG) ntt_pass_kernel(const uint4* __restric
     (void)dummy;
 #pragma unroll
     for (int sb = 0; sb < (1 << i); ++sb) {
-      const uint4* tab = my_tabs + (sb << 4);
+      const uint4* tab = my_tabs + sb * TAB;
 #pragma unroll
       for (int jj = 0; jj < (1 << (K - 1 - i)); ++jj) {
         const int j0 = (sb << (K - i)) + jj;
         const int j1 = j0 + (1 << (K - 1 - i));
-        const uint4 prod = comb_mul<1>(x[j1], tab);
-        x[j0] = f_add(x[j0], prod);
+        if constexpr (PAIR) {
+          x[j0] = comb4p_mul_add(x[j1], x[j0], tab);
+        } else {
+          const uint4 prod = comb_mul<1>(x[j1], tab);
+          x[j0] = f_add(x[j0], prod);
+        }
         x[j1] = f_add(x[j1], x[j0]);
       }
     }
@@ -102,14 +111,14 @@ __global__ void __launch_bounds__(NTT_WG) ntt_pass_kernel(const uint4* __restric
 // built once per work-group at kernel start, and every product is a 16-lookup
 // byte comb.  Table build: the 16-entry nibble tables of t and t*x^4 first, then
 // byte entry b = T_lo[b & 15] ^ T_hi[b >> 4].
-template <int K, int MODE>  // MODE 0: byte comb (256-entry tables); 1: eight nibble tables per twiddle
-__global__ void __launch_bounds__(NTT_WG) ntt_pass_shared_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
-                                                                const uint4* __restrict__ tw, uint32_t log_d,
-                                                                uint32_t L, uint32_t log_lanes, uint64_t src_pos_mask,
-                                                                uint64_t n_threads) {
+// BITOP3: comb8r_mul_add (three-input XORs, rolled byte loop, 4 waves/SIMD);
+// otherwise the two-input comb8_mul (kept for A/B runs, FLOCK_NTT_COMB=0).
+template <int K, bool BITOP3>
+__global__ void __launch_bounds__(NTT_WG) __attribute__((amdgpu_waves_per_eu(BITOP3 && K <= 3 ? NTT_WAVES : 1)))
+ntt_pass_shared_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, const uint4* __restrict__ tw,
+                       uint32_t log_d, uint32_t L, uint32_t log_lanes, uint64_t src_pos_mask, uint64_t n_threads) {
   constexpr int NTW = (1 << K) - 1;
-  constexpr int TAB = MODE == 0 ? 256 : 128;  // entries per twiddle
-  __shared__ uint4 tabs[NTW * TAB];
+  __shared__ uint4 tabs[NTW * 256];
   __shared__ uint4 nib[NTW * 32];
   const uint32_t tid = threadIdx.x;
   const uint64_t g = (uint64_t)blockIdx.x * NTT_WG + tid;
@@ -121,47 +130,31 @@ __global__ void __launch_bounds__(NTT_WG) ntt_pass_shared_kernel(const uint4* __
   // Every set of this work-group has the same prefix; take it from thread 0's set.
   const uint64_t g0 = (uint64_t)blockIdx.x * NTT_WG;
   const uint64_t prefix = (g0 >> log_lanes) >> lo_bits;
+  // element (p_j, c) with p_j = prefix | j | lo: base + j * stride
+  const uint64_t pos0 = (prefix << (log_d - L)) | lo;
+  const uint64_t base = (pos0 << log_lanes) | c;
+  const uint64_t stride = 1ull << (lo_bits + log_lanes);
+  const uint64_t sbase = ((pos0 & src_pos_mask) << log_lanes) | c;
 
   // Loads first so their latency overlaps the table build.
   uint4 x[1 << K];
 #pragma unroll
-  for (int j = 0; j < (1 << K); ++j) {
-    const uint64_t p = (prefix << (log_d - L)) | ((uint64_t)j << lo_bits) | lo;
-    x[j] = active ? src[((p & src_pos_mask) << log_lanes) | c] : f_zero();
-  }
+  for (int j = 0; j < (1 << K); ++j)
+    x[j] = active ? src[sbase + ((((uint64_t)j << lo_bits) & src_pos_mask) << log_lanes)] : f_zero();
 
   // Twiddle q (0 .. 2^K - 2) is layer L+i, sub-block sb with q = 2^i - 1 + sb.
-  if (MODE == 0) {
-    for (uint32_t e = tid; e < (uint32_t)NTW * 32u; e += NTT_WG) {
-      const uint32_t q = e >> 5, half = (e >> 4) & 1u, k = e & 15u;
-      const uint32_t i = 31u - __builtin_clz(q + 1u);
-      const uint32_t sb = q + 1u - (1u << i);
-      uint4 t = tw[((1ull << (L + i)) - 1ull) + (prefix << i) + sb];
-      if (half) t = f_mul_x4(t);
-      nib[e] = comb_entry(t, k);
-    }
-    __syncthreads();
-    for (uint32_t e = tid; e < (uint32_t)NTW * 256u; e += NTT_WG) {
-      const uint32_t q = e >> 8, b = e & 255u;
-      tabs[e] = f_add(nib[q * 32 + (b & 15u)], nib[q * 32 + 16 + (b >> 4)]);
-    }
-  } else {
-    // basis values t * x^s, s = 0..31, then entry (j, k) = XOR of t*x^(4j+b) over bits b of k
-    for (uint32_t e = tid; e < (uint32_t)NTW * 32u; e += NTT_WG) {
-      const uint32_t q = e >> 5, sh = e & 31u;
-      const uint32_t i = 31u - __builtin_clz(q + 1u);
-      const uint32_t sb = q + 1u - (1u << i);
-      nib[e] = f_mul_xs(tw[((1ull << (L + i)) - 1ull) + (prefix << i) + sb], sh);
-    }
-    __syncthreads();
-    for (uint32_t e = tid; e < (uint32_t)NTW * 128u; e += NTT_WG) {
-      const uint32_t q = e >> 7, j = (e >> 4) & 7u, k = e & 15u;
-      uint4 v = f_zero();
-#pragma unroll
-      for (int b = 0; b < 4; ++b)
-        if ((k >> b) & 1u) v = f_add(v, nib[q * 32 + 4 * j + b]);
-      tabs[e] = v;
-    }
+  for (uint32_t e = tid; e < (uint32_t)NTW * 32u; e += NTT_WG) {
+    const uint32_t q = e >> 5, half = (e >> 4) & 1u, k = e & 15u;
+    const uint32_t i = 31u - __builtin_clz(q + 1u);
+    const uint32_t sb = q + 1u - (1u << i);
+    uint4 t = tw[((1ull << (L + i)) - 1ull) + (prefix << i) + sb];
+    if (half) t = f_mul_x4(t);
+    nib[e] = comb_entry(t, k);
+  }
+  __syncthreads();
+  for (uint32_t e = tid; e < (uint32_t)NTW * 256u; e += NTT_WG) {
+    const uint32_t q = e >> 8, b = e & 255u;
+    tabs[e] = f_add(nib[q * 32 + (b & 15u)], nib[q * 32 + 16 + (b >> 4)]);
   }
   __syncthreads();
 
@@ -169,13 +162,15 @@ __global__ void __launch_bounds__(NTT_WG) ntt_pass_shared_kernel(const uint4* __
   for (int i = 0; i < K; ++i) {
 #pragma unroll
     for (int sb = 0; sb < (1 << i); ++sb) {
-      const uint4* tab = tabs + ((1 << i) - 1 + sb) * TAB;
+      const uint4* tab = tabs + ((1 << i) - 1 + sb) * 256;
 #pragma unroll
       for (int jj = 0; jj < (1 << (K - 1 - i)); ++jj) {
         const int j0 = (sb << (K - i)) + jj;
         const int j1 = j0 + (1 << (K - 1 - i));
-        const uint4 prod = MODE == 0 ? comb8_mul(x[j1], tab) : combj_mul(x[j1], tab);
-        x[j0] = f_add(x[j0], prod);
+        if constexpr (BITOP3)
+          x[j0] = comb8r_mul_add(x[j1], x[j0], tab);
+        else
+          x[j0] = f_add(x[j0], comb8_mul(x[j1], tab));
         x[j1] = f_add(x[j1], x[j0]);
       }
     }
@@ -183,10 +178,7 @@ __global__ void __launch_bounds__(NTT_WG) ntt_pass_shared_kernel(const uint4* __
 
   if (active) {
 #pragma unroll
-    for (int j = 0; j < (1 << K); ++j) {
-      const uint64_t p = (prefix << (log_d - L)) | ((uint64_t)j << lo_bits) | lo;
-      dst[(p << log_lanes) | c] = x[j];
-    }
+    for (int j = 0; j < (1 << K); ++j) dst[base + (uint64_t)j * stride] = x[j];
   }
 }
 
@@ -220,15 +212,20 @@ static bool ntt_shared_tables_enabled() {
   return on;
 }
 
-// Product kernel of the shared-table passes: 0 = byte comb (default: fewer VALU
-// ops, measured faster despite LDS bank conflicts on its 4 KiB tables), 1 = eight
-// conflict-free nibble tables (FLOCK_NTT_COMB=1, for schedule sweeps).
-static int ntt_comb_mode() {
-  static const int mode = [] {
-    const char* v = getenv("FLOCK_NTT_COMB");
-    return (v && v[0] == '1') ? 1 : 0;
-  }();
-  return mode;
+// Product kernels: FLOCK_NTT_COMB=0 selects the two-input byte comb for the
+// shared-table passes, FLOCK_NTT_PAIR_TABLES=0 the 4-bit Horner comb for the
+// per-group passes (both for A/B runs; the defaults are the bitop3 forms).
+static bool env_off(const char* name) {
+  const char* v = getenv(name);
+  return v && v[0] == '0';
+}
+static bool ntt_bitop3_comb() {
+  static const bool on = !env_off("FLOCK_NTT_COMB");
+  return on;
+}
+static bool ntt_pair_tables_enabled() {
+  static const bool on = !env_off("FLOCK_NTT_PAIR_TABLES");
+  return on;
 }
 
 template <int K>
@@ -242,18 +239,23 @@ static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t l
   const int log_sets_per_wg = 8 - (int)log_lanes;
   if (ntt_shared_tables_enabled() && n_threads >= (uint64_t)NTT_WG &&
       (log_sets_per_wg <= 0 || (int)lo_bits >= log_sets_per_wg)) {
-    if (ntt_comb_mode() == 0)
-      hipLaunchKernelGGL((ntt_pass_shared_kernel<K, 0>), dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw,
+    if (ntt_bitop3_comb())
+      hipLaunchKernelGGL((ntt_pass_shared_kernel<K, true>), dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw,
                          log_d, L, log_lanes, src_pos_mask, n_threads);
     else
-      hipLaunchKernelGGL((ntt_pass_shared_kernel<K, 1>), dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw,
+      hipLaunchKernelGGL((ntt_pass_shared_kernel<K, false>), dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw,
                          log_d, L, log_lanes, src_pos_mask, n_threads);
     FLK_LAUNCH_CHECK("ntt_pass_shared_kernel");
     return FLK_OK;
   }
-  const size_t lds = (size_t)(NTT_WG / 64) * (64u >> log_group) * ((size_t)16 << (K - 1)) * sizeof(uint4);
-  hipLaunchKernelGGL(ntt_pass_kernel<K>, dim3((unsigned)blocks), dim3(NTT_WG), lds, st, src, dst, tw, log_d, L,
-                     log_lanes, src_pos_mask, n_threads, log_group);
+  const bool pair = ntt_pair_tables_enabled();
+  const size_t lds = (size_t)(NTT_WG / 64) * (64u >> log_group) * ((size_t)(pair ? 32 : 16) << (K - 1)) * sizeof(uint4);
+  if (pair)
+    hipLaunchKernelGGL((ntt_pass_kernel<K, true>), dim3((unsigned)blocks), dim3(NTT_WG), lds, st, src, dst, tw, log_d, L,
+                       log_lanes, src_pos_mask, n_threads, log_group);
+  else
+    hipLaunchKernelGGL((ntt_pass_kernel<K, false>), dim3((unsigned)blocks), dim3(NTT_WG), lds, st, src, dst, tw, log_d,
+                       L, log_lanes, src_pos_mask, n_threads, log_group);
   FLK_LAUNCH_CHECK("ntt_pass_kernel");
   return FLK_OK;
 }
