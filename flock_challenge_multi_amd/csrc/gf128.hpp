@@ -330,6 +330,45 @@ __device__ __forceinline__ uint4 comb4p_mul_add(uint4 v, uint4 add, const uint4*
   return f_reduce_add(a, add);
 }
 
+// add + v * t with eight nibble-position tables tab[j * 16 + k] = k * t * x^(4j)
+// (j = nibble position inside a 32-bit word; 2 KiB per twiddle).  Every
+// 16-entry slice covers all LDS bank groups once, so the 32 lookups of a
+// product are free of bank conflicts (lanes that hit one entry are served by
+// broadcast).  Words are combined by Horner in x^32, top word first (a shift by
+// one word is a register move), two words per rolled iteration.
+__device__ __forceinline__ uint4 combjr_mul_add(uint4 v, uint4 add, const uint4* tab) {
+  uint32_t vw[4] = {v.w, v.z, v.y, v.x};
+  uint32_t a[7];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) a[i] = 0u;
+#pragma unroll 1
+  for (int it = 0; it < 2; ++it) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int i = 6; i >= 1; --i) a[i] = a[i - 1];
+      const uint32_t hi = vw[h], lo = hi << 4;
+      const uint4 e0 = lds_entry(tab + 0 * 16, hinib_x16<0>(lo));
+      const uint4 e1 = lds_entry(tab + 1 * 16, hinib_x16<0>(hi));
+      const uint4 e2 = lds_entry(tab + 2 * 16, hinib_x16<1>(lo));
+      const uint4 e3 = lds_entry(tab + 3 * 16, hinib_x16<1>(hi));
+      const uint4 e4 = lds_entry(tab + 4 * 16, hinib_x16<2>(lo));
+      const uint4 e5 = lds_entry(tab + 5 * 16, hinib_x16<2>(hi));
+      const uint4 e6 = lds_entry(tab + 6 * 16, hinib_x16<3>(lo));
+      const uint4 e7 = lds_entry(tab + 7 * 16, hinib_x16<3>(hi));
+      a[0] = xor3(xor3(xor3(e0.x, e1.x, e2.x), e3.x, e4.x), e5.x, e6.x) ^ e7.x;
+      a[1] = xor3(xor3(xor3(xor3(a[1], e0.y, e1.y), e2.y, e3.y), e4.y, e5.y), e6.y, e7.y);
+      a[2] = xor3(xor3(xor3(xor3(a[2], e0.z, e1.z), e2.z, e3.z), e4.z, e5.z), e6.z, e7.z);
+      a[3] = xor3(xor3(xor3(xor3(a[3], e0.w, e1.w), e2.w, e3.w), e4.w, e5.w), e6.w, e7.w);
+    }
+    vw[0] = vw[2];
+    vw[1] = vw[3];
+  }
+  // a holds words 0..6; reduce with word 7 = 0
+  const uint32_t r[8] = {a[0], a[1], a[2], a[3], a[4], a[5], a[6], 0u};
+  return f_reduce_add(r, add);
+}
+
 // add + v * t with tab[b] = b * t (b in 0..255, LDS): the byte comb with
 // three-input XORs, rolled over the four byte positions like comb4p_mul_add.
 __device__ __forceinline__ uint4 comb8r_mul_add(uint4 v, uint4 add, const uint4* tab) {

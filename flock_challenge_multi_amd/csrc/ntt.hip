@@ -107,18 +107,22 @@ __global__ void __launch_bounds__(NTT_WG) __attribute__((amdgpu_waves_per_eu(PAI
 
 // Passes whose work-group covers sets that all share the block prefix (every
 // pass except the deepest ones): the 2^K - 1 twiddles of the pass are common to
-// the whole work-group, so their 256-entry byte-comb tables (4 KiB each) are
-// built once per work-group at kernel start, and every product is a 16-lookup
-// byte comb.  Table build: the 16-entry nibble tables of t and t*x^4 first, then
-// byte entry b = T_lo[b & 15] ^ T_hi[b >> 4].
-// BITOP3: comb8r_mul_add (three-input XORs, rolled byte loop, 4 waves/SIMD);
-// otherwise the two-input comb8_mul (kept for A/B runs, FLOCK_NTT_COMB=0).
-template <int K, bool BITOP3>
-__global__ void __launch_bounds__(NTT_WG) __attribute__((amdgpu_waves_per_eu(BITOP3 && K <= 3 ? NTT_WAVES : 1)))
+// the whole work-group, so their comb tables are built once per work-group at
+// kernel start.
+//   COMB 0: 256-entry byte tables (4 KiB per twiddle), two-input comb8_mul;
+//   COMB 1: the same tables, comb8r_mul_add (three-input XORs, rolled);
+//   COMB 2: eight 16-entry nibble-position tables (2 KiB per twiddle),
+//           combjr_mul_add: twice the lookups, none of them bank-conflicted.
+// Byte tables: the nibble tables of t and t*x^4 first, then entry
+// b = T_lo[b & 15] ^ T_hi[b >> 4].  Nibble-position tables: the 32 values
+// t*x^s first, then entry (j, k) = XOR of t*x^(4j+i) over the bits i of k.
+template <int K, int COMB>
+__global__ void __launch_bounds__(NTT_WG) __attribute__((amdgpu_waves_per_eu(COMB != 0 && K <= 3 ? NTT_WAVES : 1)))
 ntt_pass_shared_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, const uint4* __restrict__ tw,
                        uint32_t log_d, uint32_t L, uint32_t log_lanes, uint64_t src_pos_mask, uint64_t n_threads) {
   constexpr int NTW = (1 << K) - 1;
-  __shared__ uint4 tabs[NTW * 256];
+  constexpr int TAB = COMB == 2 ? 128 : 256;  // entries per twiddle
+  __shared__ uint4 tabs[NTW * TAB];
   __shared__ uint4 nib[NTW * 32];
   const uint32_t tid = threadIdx.x;
   const uint64_t g = (uint64_t)blockIdx.x * NTT_WG + tid;
@@ -143,18 +147,36 @@ ntt_pass_shared_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, c
     x[j] = active ? src[sbase + ((((uint64_t)j << lo_bits) & src_pos_mask) << log_lanes)] : f_zero();
 
   // Twiddle q (0 .. 2^K - 2) is layer L+i, sub-block sb with q = 2^i - 1 + sb.
-  for (uint32_t e = tid; e < (uint32_t)NTW * 32u; e += NTT_WG) {
-    const uint32_t q = e >> 5, half = (e >> 4) & 1u, k = e & 15u;
-    const uint32_t i = 31u - __builtin_clz(q + 1u);
-    const uint32_t sb = q + 1u - (1u << i);
-    uint4 t = tw[((1ull << (L + i)) - 1ull) + (prefix << i) + sb];
-    if (half) t = f_mul_x4(t);
-    nib[e] = comb_entry(t, k);
-  }
-  __syncthreads();
-  for (uint32_t e = tid; e < (uint32_t)NTW * 256u; e += NTT_WG) {
-    const uint32_t q = e >> 8, b = e & 255u;
-    tabs[e] = f_add(nib[q * 32 + (b & 15u)], nib[q * 32 + 16 + (b >> 4)]);
+  if constexpr (COMB == 2) {
+    for (uint32_t e = tid; e < (uint32_t)NTW * 32u; e += NTT_WG) {
+      const uint32_t q = e >> 5, sh = e & 31u;
+      const uint32_t i = 31u - __builtin_clz(q + 1u);
+      const uint32_t sb = q + 1u - (1u << i);
+      nib[e] = f_mul_xs(tw[((1ull << (L + i)) - 1ull) + (prefix << i) + sb], sh);
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < (uint32_t)NTW * 128u; e += NTT_WG) {
+      const uint32_t q = e >> 7, j = (e >> 4) & 7u, k = e & 15u;
+      uint4 v = f_zero();
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if ((k >> b) & 1u) v = f_add(v, nib[q * 32 + 4 * j + b]);
+      tabs[e] = v;
+    }
+  } else {
+    for (uint32_t e = tid; e < (uint32_t)NTW * 32u; e += NTT_WG) {
+      const uint32_t q = e >> 5, half = (e >> 4) & 1u, k = e & 15u;
+      const uint32_t i = 31u - __builtin_clz(q + 1u);
+      const uint32_t sb = q + 1u - (1u << i);
+      uint4 t = tw[((1ull << (L + i)) - 1ull) + (prefix << i) + sb];
+      if (half) t = f_mul_x4(t);
+      nib[e] = comb_entry(t, k);
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < (uint32_t)NTW * 256u; e += NTT_WG) {
+      const uint32_t q = e >> 8, b = e & 255u;
+      tabs[e] = f_add(nib[q * 32 + (b & 15u)], nib[q * 32 + 16 + (b >> 4)]);
+    }
   }
   __syncthreads();
 
@@ -162,12 +184,14 @@ ntt_pass_shared_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, c
   for (int i = 0; i < K; ++i) {
 #pragma unroll
     for (int sb = 0; sb < (1 << i); ++sb) {
-      const uint4* tab = tabs + ((1 << i) - 1 + sb) * 256;
+      const uint4* tab = tabs + ((1 << i) - 1 + sb) * TAB;
 #pragma unroll
       for (int jj = 0; jj < (1 << (K - 1 - i)); ++jj) {
         const int j0 = (sb << (K - i)) + jj;
         const int j1 = j0 + (1 << (K - 1 - i));
-        if constexpr (BITOP3)
+        if constexpr (COMB == 2)
+          x[j0] = combjr_mul_add(x[j1], x[j0], tab);
+        else if constexpr (COMB == 1)
           x[j0] = comb8r_mul_add(x[j1], x[j0], tab);
         else
           x[j0] = f_add(x[j0], comb8_mul(x[j1], tab));
@@ -212,16 +236,19 @@ static bool ntt_shared_tables_enabled() {
   return on;
 }
 
-// Product kernels: FLOCK_NTT_COMB=0 selects the two-input byte comb for the
-// shared-table passes, FLOCK_NTT_PAIR_TABLES=0 the 4-bit Horner comb for the
-// per-group passes (both for A/B runs; the defaults are the bitop3 forms).
+// Product kernels: FLOCK_NTT_COMB=0|1|2 selects the comb of the shared-table
+// passes (see ntt_pass_shared_kernel), FLOCK_NTT_PAIR_TABLES=0 the 4-bit Horner
+// comb for the per-group passes (both for A/B runs).
 static bool env_off(const char* name) {
   const char* v = getenv(name);
   return v && v[0] == '0';
 }
-static bool ntt_bitop3_comb() {
-  static const bool on = !env_off("FLOCK_NTT_COMB");
-  return on;
+static int ntt_comb() {
+  static const int c = [] {
+    const char* v = getenv("FLOCK_NTT_COMB");
+    return (v && v[0] >= '0' && v[0] <= '2') ? v[0] - '0' : 2;
+  }();
+  return c;
 }
 static bool ntt_pair_tables_enabled() {
   static const bool on = !env_off("FLOCK_NTT_PAIR_TABLES");
@@ -239,11 +266,15 @@ static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t l
   const int log_sets_per_wg = 8 - (int)log_lanes;
   if (ntt_shared_tables_enabled() && n_threads >= (uint64_t)NTT_WG &&
       (log_sets_per_wg <= 0 || (int)lo_bits >= log_sets_per_wg)) {
-    if (ntt_bitop3_comb())
-      hipLaunchKernelGGL((ntt_pass_shared_kernel<K, true>), dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw,
+    const int comb = ntt_comb();
+    if (comb == 2)
+      hipLaunchKernelGGL((ntt_pass_shared_kernel<K, 2>), dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw,
+                         log_d, L, log_lanes, src_pos_mask, n_threads);
+    else if (comb == 1)
+      hipLaunchKernelGGL((ntt_pass_shared_kernel<K, 1>), dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw,
                          log_d, L, log_lanes, src_pos_mask, n_threads);
     else
-      hipLaunchKernelGGL((ntt_pass_shared_kernel<K, false>), dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw,
+      hipLaunchKernelGGL((ntt_pass_shared_kernel<K, 0>), dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw,
                          log_d, L, log_lanes, src_pos_mask, n_threads);
     FLK_LAUNCH_CHECK("ntt_pass_shared_kernel");
     return FLK_OK;
