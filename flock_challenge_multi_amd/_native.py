@@ -63,6 +63,13 @@ SIGNATURES = {
     "flk_merkle_hash_leaves": (_I, [_P, _SZ, _SZ, _P, _P]),
     "flk_merkle_hash_pairs_level": (_I, [_P, _SZ, _P, _P]),
     "flk_merkle_tree": (_I, [_P, _SZ, _SZ, _I, _P, _P]),
+    "flk_merkle_begin": (_I, [_P, _SZ, _SZ, _P, _SZ, _SZ, _P, ctypes.POINTER(ctypes.c_void_p)]),
+    "flk_merkle_session_leaves": (_SZ, [_P]),
+    "flk_merkle_commit_leaves": (_I, [_P, _SZ, _SZ]),
+    "flk_merkle_commit_parent_levels": (_I, [_P]),
+    "flk_merkle_finish": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
+    "flk_merkle_discard": (None, [_P]),
+    "flk_prewire": (_I, [_P, _SZ]),
     "flk_commit": (_I, [_P, _SZ, _SZ, _SZ, _I, _P, _P, _P, _P]),
     "flk_build_eq": (_I, [_P, _SZ, _P, _P]),
     "flk_sc_round_msg": (_I, [_P, _P, _SZ, _P, _P]),

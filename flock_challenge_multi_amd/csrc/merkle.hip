@@ -115,11 +115,13 @@ __global__ void merkle_pairs_kernel(const uint4* __restrict__ children, uint64_t
   parents[2 * i + 1] = make_uint4(o[4], o[5], o[6], o[7]);
 }
 
-static bool leaf_size_ok(size_t leaf_size) { return is_pow2(leaf_size) && leaf_size >= 16 && leaf_size <= 1024; }
+// One BLAKE3 chunk per leaf, read as 16-byte words: 16..1024 bytes in steps of 16
+// (the last 64-byte block is partial when the size is not a multiple of 64).
+static bool leaf_size_ok(size_t leaf_size) { return leaf_size >= 16 && leaf_size <= 1024 && leaf_size % 16 == 0; }
 
 int merkle_leaves_device(const uint8_t* data, size_t leaf_size, size_t n_leaves, uint8_t* out, hipStream_t st) {
   if (!leaf_size_ok(leaf_size))
-    return set_error(FLK_E_SHAPE, "merkle: leaf size %zu unsupported (powers of two 16..1024)", leaf_size);
+    return set_error(FLK_E_SHAPE, "merkle: leaf size %zu unsupported (multiples of 16 up to 1024)", leaf_size);
   if ((uintptr_t)data % 16 || (uintptr_t)out % 16) return set_error(FLK_E_SHAPE, "merkle: buffers must be 16-byte aligned");
   if (n_leaves == 0) return FLK_OK;
   const uint64_t blocks = (n_leaves + LEAF_WG - 1) / LEAF_WG;
@@ -130,12 +132,16 @@ int merkle_leaves_device(const uint8_t* data, size_t leaf_size, size_t n_leaves,
   return FLK_OK;
 }
 
-// Upper levels of a flat tree whose leaves [0, n) are already written.
-int merkle_upper_device(uint8_t* tree_bytes, size_t n, hipStream_t st) {
+// Parent levels of a flat tree whose leaves [0, n) are already written, for as
+// long as a level has at least stop_nodes nodes (stop_nodes = 1: up to the root;
+// the streamed commit leaves the top of the tree to its caller, CF/gpu.rs:1369).
+int merkle_upper_levels_device(uint8_t* tree_bytes, size_t n, size_t stop_nodes, hipStream_t st) {
   uint4* tree = reinterpret_cast<uint4*>(tree_bytes);
+  const uint64_t stop = stop_nodes < 1 ? 1 : stop_nodes;
   uint64_t child_off = 0, cur = n;
-  while (cur > 1) {
-    const int lv_total = log2_exact(cur);
+  while (cur > 1 && cur / 2 >= stop) {
+    int lv_total = log2_exact(cur);
+    while (lv_total > 1 && (cur >> lv_total) < stop) --lv_total;  // last level produced keeps >= stop nodes
     const uint32_t lv = (uint32_t)(lv_total < PAR_MAX_LV ? lv_total : PAR_MAX_LV);
     LevelOffsets lo{};
     uint64_t off = child_off + cur, len = cur;
@@ -166,7 +172,7 @@ int merkle_tree_device(const uint8_t* data, size_t data_len, size_t num_leaves, 
     return set_error(FLK_E_SHAPE, "merkle: num_leaves %zu must be a power of two dividing %zu", num_leaves, data_len);
   rc = merkle_leaves_device(data, data_len / num_leaves, num_leaves, tree, st);
   if (rc) return rc;
-  return merkle_upper_device(tree, num_leaves, st);
+  return merkle_upper_levels_device(tree, num_leaves, 1, st);
 }
 
 }  // namespace flk

@@ -141,6 +141,71 @@ def merkle_tree(data, num_leaves, kind=nat.FLK_HASH_BLAKE3, tree=None):
     return tree
 
 
+SIZE_MAX = (1 << 64) - 1
+
+
+def _host_or_device_ptr(buf):
+    """Address of a torch tensor (CUDA or CPU) or a numpy array."""
+    import ctypes
+    if isinstance(buf, torch.Tensor):
+        if not buf.is_contiguous():
+            raise ValueError("buffers must be contiguous")
+        return ctypes.c_void_p(buf.data_ptr()), buf.numel() * buf.element_size()
+    import numpy as np
+    if isinstance(buf, np.ndarray):
+        if not buf.flags["C_CONTIGUOUS"]:
+            raise ValueError("buffers must be contiguous")
+        return ctypes.c_void_p(buf.ctypes.data), buf.nbytes
+    raise TypeError("expected a torch tensor or a numpy array")
+
+
+def prewire(data):
+    """crate::gpu::prewire (CF/gpu.rs:497-515): accepted; no host registration."""
+    p, n = _host_or_device_ptr(data)
+    nat.call("flk_prewire", p, n)
+
+
+class MerkleSession:
+    """crate::gpu::merkle::Session (CF/gpu.rs:1228-1490): leaves hashed in
+    committed ranges, then the parent levels with >= stop_nodes nodes; data and
+    tree may live on the host (numpy / CPU tensor) or the device."""
+
+    def __init__(self, handle, data, tree):
+        self._h = handle
+        self._keep = (data, tree)  # the buffers must outlive the session
+
+    @staticmethod
+    def begin(data, leaf_size, tree, stop_nodes=SIZE_MAX):
+        import ctypes
+        dp, dn = _host_or_device_ptr(data)
+        tp, tn = _host_or_device_ptr(tree)
+        h = ctypes.c_void_p()
+        nat.call("flk_merkle_begin", dp, dn, leaf_size, tp, tn // 32, stop_nodes, _stream(), ctypes.byref(h))
+        return MerkleSession(h, data, tree)
+
+    def n_leaves(self):
+        return nat.load().flk_merkle_session_leaves(self._h)
+
+    def commit_leaves(self, lo, hi):
+        nat.call("flk_merkle_commit_leaves", self._h, lo, hi)
+
+    def commit_parent_levels(self):
+        nat.call("flk_merkle_commit_parent_levels", self._h)
+
+    def finish(self):
+        """Waits; returns the session's device time in seconds."""
+        import ctypes
+        sec = ctypes.c_double(0.0)
+        h, self._h = self._h, None
+        nat.call("flk_merkle_finish", h, ctypes.byref(sec))
+        return sec.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            nat.load().flk_merkle_discard(self._h)
+            self._h = None
+
+
 @dataclass
 class PcsParams:
     """CF/pcs/commit.rs:34-95 (LOG_PACKING = 7)."""

@@ -110,8 +110,8 @@ int flk_ntt_rs_encode_interleaved(size_t dim, const flk_f128 *msg, size_t n_msg,
 /* ------------------------------------------------------------------------ */
 /* BLAKE3 Merkle tree (CF/merkle.rs).  Leaf = non-root chaining value of the */
 /* leaf bytes; parent = PARENT-flagged merge (:192-203).  Flat 2n-1 layout,  */
-/* leaves first, root at index 2n-2 (:15-22).  Leaf sizes: powers of two     */
-/* from 16 to 1024 bytes.                                                    */
+/* leaves first, root at index 2n-2 (:15-22).  Leaf sizes: multiples of 16   */
+/* from 16 to 1024 bytes (one BLAKE3 chunk).                                 */
 /* ------------------------------------------------------------------------ */
 
 /* merkle::hash_leaves (:498): out[i] = leaf CV of data[i*leaf_size ..]. */
@@ -123,6 +123,34 @@ int flk_merkle_hash_pairs_level(const uint8_t *children, size_t n_children, uint
 /* merkle::merkle_tree (:753): tree_out holds 2*num_leaves - 1 hashes. */
 int flk_merkle_tree(const uint8_t *data, size_t data_len, size_t num_leaves, int hash_kind,
                     uint8_t *tree_out, void *stream);
+
+/* Streaming Merkle session: crate::gpu::merkle::{begin, Session::{n_leaves,
+ * commit_leaves, commit_parent_levels, finish}} (CF/gpu.rs:1228-1490).
+ * data (n_leaves * leaf_size bytes) and tree may each be HOST or DEVICE
+ * memory; host buffers are mirrored on the device and the nodes the session
+ * wrote are copied back by finish.  Parent levels are built while their node
+ * count is >= stop_nodes (SIZE_MAX: leaves only; parent levels need a
+ * power-of-two leaf count); tree_len (in hashes) must cover every node the
+ * session writes.  The caller may write the leaf CVs of ranges it never
+ * commits into a HOST tree before commit_parent_levels (the hybrid CPU join,
+ * :1161-1170).  All work runs in order on `stream`.  A session is used by one
+ * thread at a time. */
+typedef struct flk_merkle_session flk_merkle_session;
+int flk_merkle_begin(const uint8_t *data, size_t data_len, size_t leaf_size, uint8_t *tree,
+                     size_t tree_len, size_t stop_nodes, void *stream, flk_merkle_session **out);
+size_t flk_merkle_session_leaves(const flk_merkle_session *s);
+/* Hash leaves [leaf_lo, leaf_hi) into tree slots [leaf_lo, leaf_hi); non-blocking. */
+int flk_merkle_commit_leaves(flk_merkle_session *s, size_t leaf_lo, size_t leaf_hi);
+/* Every parent level with >= stop_nodes nodes; after all leaves are in place. */
+int flk_merkle_commit_parent_levels(flk_merkle_session *s);
+/* Wait, copy host-tree nodes back, store the session's device time in seconds
+ * (gpu_seconds may be NULL) and free the session, on success and on error. */
+int flk_merkle_finish(flk_merkle_session *s, double *gpu_seconds);
+/* Free a session without finishing it (after waiting for its work). */
+void flk_merkle_discard(flk_merkle_session *s);
+/* crate::gpu::prewire (CF/gpu.rs:497-515): validated and accepted; this library
+ * stages host buffers through HIP copies and registers no host memory. */
+int flk_prewire(const void *data, size_t len);
 
 /* ------------------------------------------------------------------------ */
 /* PCS commit = RS encode + Merkle over codeword rows.                       */
