@@ -60,6 +60,7 @@ SIGNATURES = {
     "flk_ntt_twiddles": (_I, [_SZ, _P]),
     "flk_ntt_forward_interleaved": (_I, [_SZ, _P, _SZ, _SZ, _SZ, _P]),
     "flk_ntt_rs_encode_interleaved": (_I, [_SZ, _P, _SZ, _P, _SZ, _SZ, _P]),
+    "flk_ntt_inverse_interleaved": (_I, [_SZ, _P, _SZ, _SZ, _P]),
     "flk_merkle_hash_leaves": (_I, [_P, _SZ, _SZ, _P, _P]),
     "flk_merkle_hash_pairs_level": (_I, [_P, _SZ, _P, _P]),
     "flk_merkle_tree": (_I, [_P, _SZ, _SZ, _I, _P, _P]),

@@ -34,7 +34,9 @@ constexpr int NTT_WG = 256;
 // measured faster than 2 waves with the comb unrolled or 5 waves with spills.
 constexpr int NTT_WAVES = 4;
 
-template <int K, bool PAIR>  // PAIR: 32-entry nibble-pair tables and comb4p_mul_add
+// PAIR: 32-entry nibble-pair tables and comb4p_mul_add.  INV: the inverse
+// transform's pass (layers deepest first, butterfly v += u; u += v * t).
+template <int K, bool PAIR, bool INV = false>
 __global__ void __launch_bounds__(NTT_WG) __attribute__((amdgpu_waves_per_eu(PAIR && K <= 3 ? NTT_WAVES : 1))) ntt_pass_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                          const uint4* __restrict__ tw, uint32_t log_d, uint32_t L,
                                                          uint32_t log_lanes, uint64_t src_pos_mask,
@@ -64,7 +66,8 @@ __global__ void __launch_bounds__(NTT_WG) __attribute__((amdgpu_waves_per_eu(PAI
   }
 
 #pragma unroll
-  for (int i = 0; i < K; ++i) {
+  for (int ii = 0; ii < K; ++ii) {
+    const int i = INV ? K - 1 - ii : ii;
     // Tables of the 2^i twiddles of layer L+i met by this set.
     const uint32_t n_entries = (uint32_t)TAB << i;
     const uint64_t tw_base = ((1ull << (L + i)) - 1ull) + (prefix << i);
@@ -84,13 +87,14 @@ __global__ void __launch_bounds__(NTT_WG) __attribute__((amdgpu_waves_per_eu(PAI
       for (int jj = 0; jj < (1 << (K - 1 - i)); ++jj) {
         const int j0 = (sb << (K - i)) + jj;
         const int j1 = j0 + (1 << (K - 1 - i));
+        if constexpr (INV) x[j1] = f_add(x[j1], x[j0]);
         if constexpr (PAIR) {
           x[j0] = comb4p_mul_add(x[j1], x[j0], tab);
         } else {
           const uint4 prod = comb_mul<1>(x[j1], tab);
           x[j0] = f_add(x[j0], prod);
         }
-        x[j1] = f_add(x[j1], x[j0]);
+        if constexpr (!INV) x[j1] = f_add(x[j1], x[j0]);
       }
     }
     __syncthreads();
@@ -116,7 +120,8 @@ __global__ void __launch_bounds__(NTT_WG) __attribute__((amdgpu_waves_per_eu(PAI
 // Byte tables: the nibble tables of t and t*x^4 first, then entry
 // b = T_lo[b & 15] ^ T_hi[b >> 4].  Nibble-position tables: the 32 values
 // t*x^s first, then entry (j, k) = XOR of t*x^(4j+i) over the bits i of k.
-template <int K, int COMB>
+// INV: the inverse transform's pass (layers deepest first, v += u; u += v * t).
+template <int K, int COMB, bool INV = false>
 __global__ void __launch_bounds__(NTT_WG) __attribute__((amdgpu_waves_per_eu(COMB != 0 && K <= 3 ? NTT_WAVES : 1)))
 ntt_pass_shared_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, const uint4* __restrict__ tw,
                        uint32_t log_d, uint32_t L, uint32_t log_lanes, uint64_t src_pos_mask, uint64_t n_threads) {
@@ -181,7 +186,8 @@ ntt_pass_shared_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, c
   __syncthreads();
 
 #pragma unroll
-  for (int i = 0; i < K; ++i) {
+  for (int ii = 0; ii < K; ++ii) {
+    const int i = INV ? K - 1 - ii : ii;
 #pragma unroll
     for (int sb = 0; sb < (1 << i); ++sb) {
       const uint4* tab = tabs + ((1 << i) - 1 + sb) * TAB;
@@ -189,13 +195,14 @@ ntt_pass_shared_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, c
       for (int jj = 0; jj < (1 << (K - 1 - i)); ++jj) {
         const int j0 = (sb << (K - i)) + jj;
         const int j1 = j0 + (1 << (K - 1 - i));
+        if constexpr (INV) x[j1] = f_add(x[j1], x[j0]);
         if constexpr (COMB == 2)
           x[j0] = combjr_mul_add(x[j1], x[j0], tab);
         else if constexpr (COMB == 1)
           x[j0] = comb8r_mul_add(x[j1], x[j0], tab);
         else
           x[j0] = f_add(x[j0], comb8_mul(x[j1], tab));
-        x[j1] = f_add(x[j1], x[j0]);
+        if constexpr (!INV) x[j1] = f_add(x[j1], x[j0]);
       }
     }
   }
@@ -257,7 +264,7 @@ static bool ntt_pair_tables_enabled() {
 
 template <int K>
 static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t log_d, uint32_t L, uint32_t log_lanes,
-                       uint64_t src_pos_mask, uint32_t log_group, hipStream_t st) {
+                       uint64_t src_pos_mask, uint32_t log_group, bool inverse, hipStream_t st) {
   const uint64_t n_threads = (1ull << (log_d - K)) << log_lanes;
   const uint64_t blocks = (n_threads + NTT_WG - 1) / NTT_WG;
   // Sets per work-group = NTT_WG / num_ntts; they share the block prefix when the
@@ -267,7 +274,10 @@ static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t l
   if (ntt_shared_tables_enabled() && n_threads >= (uint64_t)NTT_WG &&
       (log_sets_per_wg <= 0 || (int)lo_bits >= log_sets_per_wg)) {
     const int comb = ntt_comb();
-    if (comb == 2)
+    if (inverse)  // the inverse runs the default product kernels only
+      hipLaunchKernelGGL((ntt_pass_shared_kernel<K, 2, true>), dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst,
+                         tw, log_d, L, log_lanes, src_pos_mask, n_threads);
+    else if (comb == 2)
       hipLaunchKernelGGL((ntt_pass_shared_kernel<K, 2>), dim3((unsigned)blocks), dim3(NTT_WG), 0, st, src, dst, tw,
                          log_d, L, log_lanes, src_pos_mask, n_threads);
     else if (comb == 1)
@@ -279,9 +289,12 @@ static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t l
     FLK_LAUNCH_CHECK("ntt_pass_shared_kernel");
     return FLK_OK;
   }
-  const bool pair = ntt_pair_tables_enabled();
+  const bool pair = inverse || ntt_pair_tables_enabled();
   const size_t lds = (size_t)(NTT_WG / 64) * (64u >> log_group) * ((size_t)(pair ? 32 : 16) << (K - 1)) * sizeof(uint4);
-  if (pair)
+  if (inverse)
+    hipLaunchKernelGGL((ntt_pass_kernel<K, true, true>), dim3((unsigned)blocks), dim3(NTT_WG), lds, st, src, dst, tw,
+                       log_d, L, log_lanes, src_pos_mask, n_threads, log_group);
+  else if (pair)
     hipLaunchKernelGGL((ntt_pass_kernel<K, true>), dim3((unsigned)blocks), dim3(NTT_WG), lds, st, src, dst, tw, log_d, L,
                        log_lanes, src_pos_mask, n_threads, log_group);
   else
@@ -330,22 +343,26 @@ static int ntt_schedule(uint32_t n, int kmax, int* ks) {
 }
 
 // Run layers [start, log_d) reading the first pass from `src` (positions masked
-// by src_pos_mask) and writing all passes to `dst`.
+// by src_pos_mask) and writing all passes to `dst`.  inverse: the same passes in
+// reverse order, each running its layers deepest first (the inverse transform,
+// CF/ntt/additive_ntt_f128.rs:6322-6342).
 static int run_layers(const uint4* src, uint4* dst, const uint4* tw, uint32_t log_d, uint32_t start,
-                      uint32_t log_lanes, uint64_t src_pos_mask, hipStream_t st) {
+                      uint32_t log_lanes, uint64_t src_pos_mask, hipStream_t st, bool inverse = false) {
   const uint32_t log_group = log_lanes < 6 ? log_lanes : 6;
-  // The per-group kernel holds (64/G) * 2^(K-1) tables of 256 B per wave: K <= log2(G) + 1.
+  // The per-group kernel holds (64/G) * 2^(K-1) tables of 512 B per wave: K <= log2(G) + 1.
   const int kgroup = (int)log_group + 1;
   int kmax = ntt_radix_cap();
   int ks[64];
   const int n_pass = ntt_schedule(log_d - start, kmax, ks);
   if (n_pass < 0) return set_error(FLK_E_SHAPE, "FLOCK_NTT_SCHEDULE is malformed");
+  // Launch plan in forward order: (first layer, layers), splitting passes the
+  // per-group kernel cannot hold.
+  uint32_t plan_L[64 * 4];
+  int plan_k[64 * 4];
+  int n_plan = 0;
   uint32_t L = start;
-  const uint4* s = src;
-  uint64_t mask = src_pos_mask;
   for (int pi = 0; pi < n_pass; ++pi) {
-    int K = ks[pi];
-    // split a pass the per-group kernel cannot hold into smaller ones
+    const int K = ks[pi];
     const uint32_t lo_bits = log_d - L - (uint32_t)K;
     const int log_sets_per_wg = 8 - (int)log_lanes;
     const bool shared = ntt_shared_tables_enabled() && ((log_d - (uint32_t)K + log_lanes) >= 8) &&
@@ -353,19 +370,26 @@ static int run_layers(const uint4* src, uint4* dst, const uint4* tw, uint32_t lo
     int todo = K;
     while (todo > 0) {
       const int k = shared ? todo : (todo < kgroup ? todo : kgroup);
-      int rc;
-      switch (k) {
-        case 1: rc = launch_pass<1>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
-        case 2: rc = launch_pass<2>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
-        case 3: rc = launch_pass<3>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
-        default: rc = launch_pass<4>(s, dst, tw, log_d, L, log_lanes, mask, log_group, st); break;
-      }
-      if (rc) return rc;
+      plan_L[n_plan] = L;
+      plan_k[n_plan++] = k;
       L += (uint32_t)k;
       todo -= k;
-      s = dst;
-      mask = ~0ull;
     }
+  }
+  const uint4* s = src;
+  uint64_t mask = src_pos_mask;
+  for (int q = 0; q < n_plan; ++q) {
+    const int p = inverse ? n_plan - 1 - q : q;
+    int rc;
+    switch (plan_k[p]) {
+      case 1: rc = launch_pass<1>(s, dst, tw, log_d, plan_L[p], log_lanes, mask, log_group, inverse, st); break;
+      case 2: rc = launch_pass<2>(s, dst, tw, log_d, plan_L[p], log_lanes, mask, log_group, inverse, st); break;
+      case 3: rc = launch_pass<3>(s, dst, tw, log_d, plan_L[p], log_lanes, mask, log_group, inverse, st); break;
+      default: rc = launch_pass<4>(s, dst, tw, log_d, plan_L[p], log_lanes, mask, log_group, inverse, st); break;
+    }
+    if (rc) return rc;
+    s = dst;
+    mask = ~0ull;
   }
   return FLK_OK;
 }
@@ -427,6 +451,24 @@ int flk_ntt_forward_interleaved(size_t dim, flk_f128* data, size_t n_total, size
   uint4* d = reinterpret_cast<uint4*>(data);
   return flk::run_layers(d, d, tw, (uint32_t)log_d, (uint32_t)start_layer, (uint32_t)flk::log2_exact(num_ntts), ~0ull,
                          static_cast<hipStream_t>(stream));
+}
+
+int flk_ntt_inverse_interleaved(size_t dim, flk_f128* data, size_t n_total, size_t num_ntts, void* stream) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!data) return flk::set_error(FLK_E_SHAPE, "inverse_interleaved: null pointer");
+  if (!flk::is_pow2(num_ntts) || n_total % num_ntts)
+    return flk::set_error(FLK_E_SHAPE, "inverse_interleaved: bad shape n_total=%zu num_ntts=%zu", n_total, num_ntts);
+  const int log_d = flk::log2_exact(n_total / num_ntts);
+  if (log_d < 0 || (size_t)log_d > dim)
+    return flk::set_error(FLK_E_SHAPE, "inverse_interleaved: bad domain log_d=%d dim=%zu", log_d, dim);
+  if (dim > 24) return flk::set_error(FLK_E_SHAPE, "inverse_interleaved: dim %zu exceeds 24", dim);
+  if (log_d == 0) return FLK_OK;
+  const uint4* tw = flk::twiddles_device(dim, &rc);
+  if (rc) return rc;
+  uint4* d = reinterpret_cast<uint4*>(data);
+  return flk::run_layers(d, d, tw, (uint32_t)log_d, 0, (uint32_t)flk::log2_exact(num_ntts), ~0ull,
+                         static_cast<hipStream_t>(stream), true);
 }
 
 }  // extern "C"

@@ -101,6 +101,23 @@ class AdditiveNttF128:
             raise ValueError("forward_transform_interleaved: domain exceeds the NTT")
         nat.call("flk_ntt_forward_interleaved", self.dim, _ptr(data), n, num_ntts, start_layer, _stream())
 
+    def forward_transform(self, data):
+        """AdditiveNttF128::forward_transform (:2701): one transform, in place."""
+        self.forward_transform_interleaved(data, 1)
+
+    def inverse_transform_interleaved(self, data, num_ntts):
+        """Inverse of forward_transform_interleaved, in place (inverse_transform, :6322)."""
+        n = _check_f128(data, "data")
+        if not _pow2(num_ntts) or n % num_ntts or not _pow2(n // num_ntts):
+            raise ValueError("inverse_transform_interleaved: bad shape")
+        if _log2(n // num_ntts) > self.dim:
+            raise ValueError("inverse_transform_interleaved: domain exceeds the NTT")
+        nat.call("flk_ntt_inverse_interleaved", self.dim, _ptr(data), n, num_ntts, _stream())
+
+    def inverse_transform(self, data):
+        """AdditiveNttF128::inverse_transform (:6322-6342): exact inverse of forward_transform."""
+        self.inverse_transform_interleaved(data, 1)
+
     def rs_encode_interleaved(self, msg, codeword, num_ntts):
         n_msg = _check_f128(msg, "msg")
         n_cw = _check_f128(codeword, "codeword")

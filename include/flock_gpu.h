@@ -101,6 +101,11 @@ int flk_ntt_twiddles(size_t dim, flk_f128 *out);
 int flk_ntt_forward_interleaved(size_t dim, flk_f128 *data, size_t n_total, size_t num_ntts,
                                 size_t start_layer, void *stream);
 
+/* AdditiveNttF128::inverse_transform (:6322-6342), interleaved like the forward
+ * transform: in place on data[pos * num_ntts + lane] over all log_d layers,
+ * deepest first (v += u; u += v * twiddle).  num_ntts = 1 is inverse_transform. */
+int flk_ntt_inverse_interleaved(size_t dim, flk_f128 *data, size_t n_total, size_t num_ntts, void *stream);
+
 /* AdditiveNttF128::rs_encode_interleaved (:2785-2830): codeword = forward
  * transform of [msg, 0, ...]; n_cw / n_msg is the inverse rate (a power of
  * two >= 2).  msg and codeword must not overlap. */

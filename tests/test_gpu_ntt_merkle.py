@@ -53,6 +53,39 @@ def test_forward_interleaved_in_place(orc, dev, log_d, start, num_ntts):
     assert same(from_dev(t), want)
 
 
+@pytest.mark.parametrize("dim,log_d,num_ntts", [(1, 1, 1), (5, 5, 1), (9, 9, 1), (12, 9, 1), (12, 12, 1),
+                                              (10, 10, 8), (8, 6, 64), (11, 11, 2)])
+def test_inverse_matches_oracle(orc, dev, dim, log_d, num_ntts):
+    """inverse_transform (CF/ntt/additive_ntt_f128.rs:6322) per lane, against the oracle."""
+    _log(f"inverse dim={dim} log_d={log_d} lanes={num_ntts}")
+    data = orc.fill(300 + 7 * dim + log_d, (1 << log_d) * num_ntts)
+    lanes = data.reshape(1 << log_d, num_ntts)
+    want = np.stack([orc.inverse(dim, np.ascontiguousarray(lanes[:, c])) for c in range(num_ntts)], axis=1)
+    t = to_dev(data)
+    hp.AdditiveNttF128.standard(dim).inverse_transform_interleaved(t, num_ntts)
+    torch.cuda.synchronize()
+    assert same(from_dev(t), want.reshape(-1))
+
+
+@pytest.mark.parametrize("log_d,num_ntts", [(17, 64), (20, 8), (14, 1)])
+def test_forward_inverse_round_trip(orc, dev, log_d, num_ntts):
+    """forward then inverse is the identity, and inverse then forward too, at the
+    standalone commit's full shape (2^17 positions x 64 lanes)."""
+    _log(f"round trip log_d={log_d} lanes={num_ntts}")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(log_d)
+    x = torch.randint(-(2**63), 2**63 - 1, ((1 << log_d) * num_ntts, 2), dtype=torch.int64, device="cuda", generator=g)
+    ntt = hp.AdditiveNttF128.standard(log_d)
+    y = x.clone()
+    ntt.forward_transform_interleaved(y, num_ntts)
+    assert not torch.equal(y, x)
+    ntt.inverse_transform_interleaved(y, num_ntts)
+    assert torch.equal(y, x)
+    ntt.inverse_transform_interleaved(y, num_ntts)
+    ntt.forward_transform_interleaved(y, num_ntts)
+    assert torch.equal(y, x)
+
+
 def test_forward_with_larger_dim(orc, dev):
     # a transform smaller than the NTT's domain uses the dim-sized twiddle table
     data = orc.fill(5, 64 * 4)
