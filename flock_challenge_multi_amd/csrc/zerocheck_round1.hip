@@ -13,16 +13,27 @@
 // and the inner sums are GF(2^8) work (Horner in alpha = xtime on packed bytes)
 // plus bit shifts by J; one GF(2^128) product per (x_outer, l).
 //
-// Work-group = one 128-row block (one x_outer) at a time:
+// P_C needs no per-row LDE: the LDE is GF(2^8)-linear and c is a bit, so
+//     P_C[l] = sum_s phi8(M[s][l]) * Q[s],   Q[s] = sum_x eq(x) c_x[s],
+// with M the 64 x 64 LDE matrix (S -> Lambda).  The kernel accumulates Q exactly
+// like P_AB, with the row's bits c_x[s] as 0/1 bytes in place of LDE values, and
+// a 64-thread epilogue applies M.
+//
+// A work-group processes three 128-row blocks (three x_outer) side by side; per block:
 //   phase 1, lane = row (K = lane & 7, J = lane >> 3): the 64-point LDE of the
-//     row's a, b, c bits (S -> Lambda: a fixed GF(2)-linear map 64 bits -> 64
-//     bytes, applied with 16 nibble tables of 64 bytes), v_ab = a_l * b_l in
-//     GF(2^8) by log/exp tables, v_c = c_l; both stored to LDS;
-//   phase 2, lane = (byte word w, which, J-group): Horner over K for 4 J values
-//     and 4 l at once (packed bytes), phi8 lookups shifted by J, a 4-lane XOR
-//     shuffle to finish the J sum, then one byte-comb product by eq_outer.
+//     row's a and b bits (a fixed GF(2)-linear map 64 bits -> 64 bytes, applied
+//     with 16 nibble tables of 64 bytes), v_ab = a_l * b_l in GF(2^8) by
+//     split-nibble product tables (two lookups), and c's bits as bytes; both
+//     stored to LDS;
+//   phase 2a, lane = (byte word w, which, J-group): Horner over K for 4 J values
+//     and 4 l at once (packed bytes), phi8 by two nibble lookups, shifted by J;
+//   phase 2b, lane = (l, which): the sum over the four J-groups, then one
+//     product by eq_outer with nibble-pair tables (comb4p_mul_add).
+// Every data-indexed LDS table has 16 entries of 16 bytes or 256 single bytes,
+// or is laid out so (bank conflicts cost more than the extra lookups).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -34,9 +45,11 @@ namespace flk {
 
 struct Round1Tables {
   uint32_t lde[16 * 16 * 16];  // [nibble pos p][value v][word w]: bytes l = 4w..4w+3
-  uint16_t logt[256];          // log_3 of v; log(0) = 511 (lands in the zero tail of exp)
-  uint8_t expt[1024];          // 3^i for i < 510, zero beyond
-  uint4 phi[256];              // phi8 table
+  uint8_t mul_lo[256 * 16];    // [a][n] = a * n        in GF(2^8)
+  uint8_t mul_hi[256 * 16];    // [a][n] = a * (n << 4) in GF(2^8)
+  uint8_t M[64 * 64];          // [s][l]: the LDE of the unit vector e_s at point l
+  uint4 phi_nib[32];           // phi8(n), then phi8(n << 4), n < 16
+  uint4 phi_pow[8];            // phi8(2^i)
 };
 
 // ---- host GF(2^8) (AES polynomial) and its additive NTT (CF/ntt.rs:22-169) ----
@@ -126,19 +139,20 @@ static void build_round1_tables(Round1Tables& T) {
           for (int l = 0; l < 64; ++l) bytes[l] ^= M[4 * p + j][l];
       memcpy(&T.lde[(p * 16 + val) * 16], bytes, 64);
     }
-  uint8_t e = 1;
-  for (int i = 0; i < 255; ++i) {
-    T.expt[i] = e;
-    T.expt[i + 255] = e;
-    T.logt[e] = (uint16_t)i;
-    e = g8mul(e, 3);
+  for (int a = 0; a < 256; ++a)
+    for (int n = 0; n < 16; ++n) {
+      T.mul_lo[a * 16 + n] = g8mul((uint8_t)a, (uint8_t)n);
+      T.mul_hi[a * 16 + n] = g8mul((uint8_t)a, (uint8_t)(n << 4));
+    }
+  memcpy(T.M, M, sizeof(T.M));
+  auto as_u4 = [](flk_f128 f) {
+    return make_uint4((uint32_t)f.lo, (uint32_t)(f.lo >> 32), (uint32_t)f.hi, (uint32_t)(f.hi >> 32));
+  };
+  for (int n = 0; n < 16; ++n) {
+    T.phi_nib[n] = as_u4(phi8_host((uint8_t)n));
+    T.phi_nib[16 + n] = as_u4(phi8_host((uint8_t)(n << 4)));
   }
-  for (int i = 510; i < 1024; ++i) T.expt[i] = 0;
-  T.logt[0] = 511;
-  for (int v = 0; v < 256; ++v) {
-    const flk_f128 f = phi8_host((uint8_t)v);
-    T.phi[v] = make_uint4((uint32_t)f.lo, (uint32_t)(f.lo >> 32), (uint32_t)f.hi, (uint32_t)(f.hi >> 32));
-  }
+  for (int i = 0; i < 8; ++i) T.phi_pow[i] = as_u4(phi8_host((uint8_t)(1u << i)));
 }
 
 // C_s / D with C_s = phi8(0x1C), D = prod_{i<4} (1 + x^(2^i)).
@@ -185,7 +199,9 @@ static const Round1Tables* round1_tables_device(int* status) {
   return d;
 }
 
-constexpr int R1_WG = 128;
+constexpr int R1_ROWS = 128;             // rows of one block (one x_outer)
+constexpr int R1_SUB = 3;                // blocks a work-group processes per iteration
+constexpr int R1_WG = R1_ROWS * R1_SUB;  // 6 waves; ~75 KiB of LDS -> 2 work-groups per CU
 
 __device__ __forceinline__ void atomic_xor4_r1(uint4* out, uint4 v) {
   uint32_t* o = reinterpret_cast<uint32_t*>(out);
@@ -199,87 +215,110 @@ __device__ __forceinline__ uint32_t xtime4(uint32_t w) {
   return ((w & 0x7F7F7F7Fu) << 1) ^ (((w >> 7) & 0x01010101u) * 0x1Bu);
 }
 
+// Work-group = R1_SUB blocks side by side (lanes sub * 128 + t), sharing the
+// tables; each block follows the phases described at the top of the file.
 __global__ void __launch_bounds__(R1_WG) zc_round1_kernel(const uint64_t* __restrict__ A, const uint64_t* __restrict__ B,
                                                          const uint64_t* __restrict__ C, uint64_t n_blocks,
                                                          const uint4* __restrict__ eq_outer,
                                                          const Round1Tables* __restrict__ T, uint4* __restrict__ out) {
-  __shared__ uint4 lde4[16 * 16 * 4];  // 16 KiB
-  __shared__ uint16_t logt[256];
-  __shared__ uint8_t expt[1024];
-  __shared__ uint4 phi[256];
-  __shared__ uint32_t rowdata[R1_WG * 33];  // [row][ab words 0..15 | c words 16..31], padded
-  __shared__ uint4 nib[32];
-  __shared__ uint4 etab[256];
-  __shared__ uint32_t ypart[2 * 16 * 4 * 4 * 5];
-  const uint32_t t = threadIdx.x;
+  // LDE contributions, chunk-major: lde16[(p * 4 + q) * 16 + v] is bytes 16q..16q+15
+  // of the 64-byte image of nibble value v at nibble position p.  For a fixed
+  // (p, q) the 16 values sit in 16 distinct LDS bank groups, so the data-indexed
+  // lookups of a wave never conflict (16 KiB).
+  __shared__ uint4 lde16[16 * 4 * 16];
+  __shared__ uint8_t mul_lo[256 * 16];  // a * n, a * (n << 4) in GF(2^8): two independent lookups
+  __shared__ uint8_t mul_hi[256 * 16];
+  __shared__ uint4 phin[32];  // phi8(n), phi8(n << 4): phi8(v) = phin[v & 15] ^ phin[16 + (v >> 4)]
+  // Per block: [row][ab words 0..15 | c words 16..31] with pitch RP; after phase
+  // 2a has read it, the same memory holds the partial sums ypart[(q*4 + j)*YP + t].
+  constexpr uint32_t RP = 33, YP = R1_ROWS + 16;  // pitches keep every access free of bank conflicts
+  static_assert(20 * YP <= R1_ROWS * RP, "ypart must fit in rowdata");
+  __shared__ uint32_t rowdata[R1_SUB][R1_ROWS * RP];
+  __shared__ uint4 etab[R1_SUB][32];  // eq_outer * n and eq_outer * x^4 * n (comb4p_mul_add)
+  const uint32_t tid = threadIdx.x, sub = tid / R1_ROWS, t = tid % R1_ROWS;
+  uint32_t* const rows = rowdata[sub];
+  uint32_t* const ypart = rows;
   {
-    const uint4* src = reinterpret_cast<const uint4*>(T->lde);
-    for (uint32_t i = t; i < 16 * 16 * 4; i += R1_WG) lde4[i] = src[i];
-    for (uint32_t i = t; i < 256; i += R1_WG) {
-      logt[i] = T->logt[i];
-      phi[i] = T->phi[i];
+    const uint4* src = reinterpret_cast<const uint4*>(T->lde);  // [p][v][q]
+    for (uint32_t i = tid; i < 16 * 4 * 16; i += R1_WG) {
+      const uint32_t p = i >> 6, q = (i >> 4) & 3u, v = i & 15u;
+      lde16[i] = src[(p * 16 + v) * 4 + q];
     }
-    for (uint32_t i = t; i < 1024; i += R1_WG) expt[i] = T->expt[i];
+    const uint32_t* mlo = reinterpret_cast<const uint32_t*>(T->mul_lo);
+    const uint32_t* mhi = reinterpret_cast<const uint32_t*>(T->mul_hi);
+    for (uint32_t i = tid; i < 1024; i += R1_WG) {
+      reinterpret_cast<uint32_t*>(mul_lo)[i] = mlo[i];
+      reinterpret_cast<uint32_t*>(mul_hi)[i] = mhi[i];
+    }
+    if (tid < 32) phin[tid] = T->phi_nib[tid];
   }
   // phase-2a role: 16 words x 2 (ab, c) x 4 J-groups
   const uint32_t w = t & 15u, which = (t >> 4) & 1u, jg = t >> 5;
-  U256 acc;
-  u256_zero(acc);
+  uint4 acc = f_zero();
   __syncthreads();
 
-  for (uint64_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
-    // ---- phase 1: one row per lane ----
-    const uint64_t row = blk * R1_WG + t;
-    const uint64_t a = A[row], b = B[row], c = C[row];
-    uint32_t ah[16], bh[16], ch[16];
+  for (uint64_t base = (uint64_t)blockIdx.x * R1_SUB; base < n_blocks; base += (uint64_t)gridDim.x * R1_SUB) {
+    const uint64_t blk = base + sub;
+    const bool live = blk < n_blocks;  // a dead block reads zeros and adds nothing
+    // ---- phase 1: one row per lane, row (K, J) = (t & 7, t >> 3) ----
+    const uint64_t row = blk * R1_ROWS + t;
+    const uint64_t a = live ? A[row] : 0ull, b = live ? B[row] : 0ull, c = live ? C[row] : 0ull;
+    uint32_t ah[16], bh[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) ah[q] = bh[q] = ch[q] = 0u;
-#pragma unroll
-    for (int p = 0; p < 16; ++p) {
-      const uint32_t na = (uint32_t)(a >> (4 * p)) & 15u, nb = (uint32_t)(b >> (4 * p)) & 15u,
-                     nc = (uint32_t)(c >> (4 * p)) & 15u;
-      const uint4* ea = lde4 + (p * 16 + na) * 4;
-      const uint4* eb = lde4 + (p * 16 + nb) * 4;
-      const uint4* ec = lde4 + (p * 16 + nc) * 4;
+    for (int q = 0; q < 16; ++q) ah[q] = bh[q] = 0u;
+    // Rolled over nibble-position pairs: at most 16 table loads in flight.
+#pragma unroll 1
+    for (uint32_t p = 0; p < 16; p += 2) {
+      const uint32_t na0 = (uint32_t)(a >> (4 * p)) & 15u, na1 = (uint32_t)(a >> (4 * p + 4)) & 15u;
+      const uint32_t nb0 = (uint32_t)(b >> (4 * p)) & 15u, nb1 = (uint32_t)(b >> (4 * p + 4)) & 15u;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const uint4 x = ea[q], y = eb[q], z = ec[q];
-        ah[4 * q] ^= x.x; ah[4 * q + 1] ^= x.y; ah[4 * q + 2] ^= x.z; ah[4 * q + 3] ^= x.w;
-        bh[4 * q] ^= y.x; bh[4 * q + 1] ^= y.y; bh[4 * q + 2] ^= y.z; bh[4 * q + 3] ^= y.w;
-        ch[4 * q] ^= z.x; ch[4 * q + 1] ^= z.y; ch[4 * q + 2] ^= z.z; ch[4 * q + 3] ^= z.w;
+        const uint4* c0 = lde16 + (p * 4 + q) * 16;
+        const uint4* c1 = c0 + 64;
+        const uint4 x0 = c0[na0], x1 = c1[na1], y0 = c0[nb0], y1 = c1[nb1];
+        ah[4 * q] = xor3(ah[4 * q], x0.x, x1.x);
+        ah[4 * q + 1] = xor3(ah[4 * q + 1], x0.y, x1.y);
+        ah[4 * q + 2] = xor3(ah[4 * q + 2], x0.z, x1.z);
+        ah[4 * q + 3] = xor3(ah[4 * q + 3], x0.w, x1.w);
+        bh[4 * q] = xor3(bh[4 * q], y0.x, y1.x);
+        bh[4 * q + 1] = xor3(bh[4 * q + 1], y0.y, y1.y);
+        bh[4 * q + 2] = xor3(bh[4 * q + 2], y0.z, y1.z);
+        bh[4 * q + 3] = xor3(bh[4 * q + 3], y0.w, y1.w);
       }
     }
-    uint32_t* rd = rowdata + t * 33;
+    uint32_t* rd = rows + t * RP;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       uint32_t v = 0;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const uint32_t x = (ah[q] >> (8 * j)) & 255u, y = (bh[q] >> (8 * j)) & 255u;
-        v |= (uint32_t)expt[logt[x] + logt[y]] << (8 * j);
+        const uint32_t x16 = ((ah[q] >> (8 * j)) & 255u) << 4, y = (bh[q] >> (8 * j)) & 255u;
+        v |= (uint32_t)(mul_lo[x16 | (y & 15u)] ^ mul_hi[x16 | (y >> 4)]) << (8 * j);
       }
       rd[q] = v;
-      rd[16 + q] = ch[q];
+      // bits 4q..4q+3 of c as the bytes 0/1 of one word (spread by a multiply:
+      // the four shifted copies of the nibble do not overlap)
+      rd[16 + q] = (((uint32_t)(c >> (4 * q)) & 15u) * 0x00204081u) & 0x01010101u;
     }
-    // eq_outer[blk] byte-comb table (nibble tables of e and e*x^4, then bytes)
+    // eq_outer[blk] nibble tables for comb4p_mul_add
     if (t < 32) {
-      uint4 e = eq_outer[blk];
+      uint4 e = live ? eq_outer[blk] : f_zero();
       if (t >= 16) e = f_mul_x4(e);
-      nib[t] = comb_entry(e, t & 15u);
+      etab[sub][t] = comb_entry(e, t & 15u);
     }
     __syncthreads();
-    for (uint32_t i = t; i < 256; i += R1_WG) etab[i] = f_add(nib[i & 15u], nib[16 + (i >> 4)]);
-    // ---- phase 2a: lane (w, which, jg): partial Y for 4 l and J in its group ----
-    // Horner over K (alpha = 0x02 -> xtime) on packed bytes: 4 l per word.
+    // ---- phase 2a: lane (w, which, jg): Horner over K in alpha = 0x02 (xtime on
+    // packed bytes: 4 l per word) for the four J of its group ----
     uint32_t wj[4];
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
-      const uint32_t J = 4 * jg + jj;
+      const uint32_t* col = rows + (8 * (4 * jg + jj)) * RP + which * 16 + w;
       uint32_t h = 0;
 #pragma unroll
-      for (int K = 7; K >= 0; --K) h = xtime4(h) ^ rowdata[(K + 8 * J) * 33 + which * 16 + w];
+      for (int K = 7; K >= 0; --K) h = xtime4(h) ^ col[RP * K];
       wj[jj] = h;
     }
+    __syncthreads();  // every lane has read rowdata; ypart reuses it
     uint32_t Y[4][5];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
@@ -290,7 +329,9 @@ __global__ void __launch_bounds__(R1_WG) zc_round1_kernel(const uint64_t* __rest
       const uint32_t J = 4 * jg + jj;  // shift phi8(w) by J < 16 bits
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const uint4 f = phi[(wj[jj] >> (8 * j)) & 255u];
+        const uint32_t byte = (wj[jj] >> (8 * j)) & 255u;
+        const uint4 f0 = phin[byte & 15u], f1 = phin[16 + (byte >> 4)];
+        const uint4 f = make_uint4(f0.x ^ f1.x, f0.y ^ f1.y, f0.z ^ f1.z, f0.w ^ f1.w);
         if (J == 0) {
           Y[j][0] ^= f.x; Y[j][1] ^= f.y; Y[j][2] ^= f.z; Y[j][3] ^= f.w;
         } else {
@@ -305,22 +346,47 @@ __global__ void __launch_bounds__(R1_WG) zc_round1_kernel(const uint64_t* __rest
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int q = 0; q < 5; ++q) ypart[((((which * 16 + w) * 4 + jg) * 4) + j) * 5 + q] = Y[j][q];
+      for (int q = 0; q < 5; ++q) ypart[(q * 4 + j) * YP + t] = Y[j][q];
     __syncthreads();
     // ---- phase 2b: lane (l, which2): finish the J sum, multiply by eq_outer ----
     {
       const uint32_t l = t & 63u, wh = t >> 6, ww = l >> 2, j = l & 3u;
-      U256 y;
-      u256_zero(y);
+      const uint32_t src = ww + 16 * wh;  // writer lane of J-group g is src + 32 g
+      uint32_t y[8];
 #pragma unroll
-      for (int g = 0; g < 4; ++g)
-#pragma unroll
-        for (int q = 0; q < 5; ++q) y.w[q] ^= ypart[((((wh * 16 + ww) * 4 + g) * 4) + j) * 5 + q];
-      comb8_mul_acc(f_reduce(y), etab, acc);
+      for (int q = 0; q < 5; ++q) {
+        const uint32_t* yp = ypart + (q * 4 + j) * YP + src;
+        y[q] = xor3(yp[0], yp[32], yp[64]) ^ yp[96];
+      }
+      y[5] = y[6] = y[7] = 0u;
+      acc = comb4p_mul_add(f_reduce_add(y, f_zero()), acc, etab[sub]);
     }
     __syncthreads();
   }
-  atomic_xor4_r1(out + t, f_reduce(acc));  // out[which * 64 + l], t = which * 64 + l
+  atomic_xor4_r1(out + t, acc);  // out[which * 64 + l], t = which * 64 + l
+}
+
+// P_C[l] = sum_s phi8(M[s][l]) * Q[s] for the 64 accumulated Q values in out_c,
+// in place: R[s][i] = phi8(2^i) * Q[s], and lane l XORs the R[s][i] selected by
+// the bits of M[s][l].
+__global__ void __launch_bounds__(64) zc_round1_lift_c_kernel(uint4* __restrict__ out_c,
+                                                             const Round1Tables* __restrict__ T) {
+  __shared__ uint4 tabs[8 * 16];
+  __shared__ uint4 R[64 * 8];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t e = t; e < 128; e += 64) tabs[e] = comb_entry(T->phi_pow[e >> 4], e & 15u);
+  const uint4 q = out_c[t];
+  __syncthreads();
+  for (int i = 0; i < 8; ++i) R[t * 8 + i] = comb_mul<1>(q, tabs + i * 16);
+  __syncthreads();
+  uint4 acc = f_zero();
+  for (int s = 0; s < 64; ++s) {
+    const uint32_t m = T->M[s * 64 + t];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if ((m >> i) & 1u) acc = f_add(acc, R[s * 8 + i]);
+  }
+  out_c[t] = acc;
 }
 
 __global__ void scale_kernel(uint4* __restrict__ v, uint32_t n, uint4 c) {
@@ -361,13 +427,16 @@ extern "C" int flk_zc_round1(const uint8_t* a_bits, const uint8_t* b_bits, const
   uint4* out = reinterpret_cast<uint4*>(out_ab);
   hipLaunchKernelGGL(zero128_kernel, dim3(1), dim3(128), 0, st, out);
   FLK_LAUNCH_CHECK("zero128_kernel");
-  // LDS (~53 KiB per work-group) allows 3 work-groups per CU: 768 fill the chip;
-  // 2048 leave room for uneven block counts per work-group.
-  uint64_t grid = n_outer < 2048 ? n_outer : 2048;
+  // Two work-groups per CU (LDS): 512 fill the chip; 1024 leave room for
+  // uneven block counts per work-group.
+  const uint64_t n_iter = (n_outer + R1_SUB - 1) / R1_SUB;
+  const uint64_t grid = n_iter < 1024 ? n_iter : 1024;
   hipLaunchKernelGGL(zc_round1_kernel, dim3((unsigned)grid), dim3(R1_WG), 0, st,
                      reinterpret_cast<const uint64_t*>(a_bits), reinterpret_cast<const uint64_t*>(b_bits),
                      reinterpret_cast<const uint64_t*>(c_bits), (uint64_t)n_outer, eq, T, out);
   FLK_LAUNCH_CHECK("zc_round1_kernel");
+  hipLaunchKernelGGL(zc_round1_lift_c_kernel, dim3(1), dim3(64), 0, st, out + 64, T);
+  FLK_LAUNCH_CHECK("zc_round1_lift_c_kernel");
   const flk_f128 sc = round1_scale();
   hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(128), 0, st, out, 128u,
                      make_uint4((uint32_t)sc.lo, (uint32_t)(sc.lo >> 32), (uint32_t)sc.hi, (uint32_t)(sc.hi >> 32)));
