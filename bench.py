@@ -41,20 +41,26 @@ def parse():
     return ap.parse_args()
 
 
-def measured_encode_traffic():
-    """HBM bytes per RS encode from the committed rocprofv3 counter summary
-    (FETCH_SIZE x 2 + WRITE_SIZE x 1 over the encode's passes, scaled by a plain
-    copy of known size).  None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "commit_config1.md")
+PROFILE = "profiles/commit_config1.md"
+VALU_PEAK = 256 * 64 * 2.4e9  # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
+
+
+def encode_counters():
+    """From the committed rocprofv3 counter summary of this workload: HBM bytes per
+    RS encode (FETCH_SIZE / WRITE_SIZE over the encode's passes, scaled by a plain
+    copy of known size) and VALU wave64 instructions per encode.  None for a
+    figure the summary does not hold."""
+    traffic = valu = None
     try:
-        with open(path) as f:
+        with open(os.path.join(ROOT, PROFILE)) as f:
             for line in f:
                 if line.startswith("- Measured HBM traffic per encode"):
-                    gb = float(line.split(":", 1)[1].split("GB")[0].strip())
-                    return int(gb * 1e9), "profiles/commit_config1.md"
+                    traffic = int(float(line.split(":", 1)[1].split("GB")[0].strip()) * 1e9)
+                elif line.startswith("- VALU instructions per encode"):
+                    valu = float(line.split(":", 1)[1].split("wave64")[0].strip())
     except (OSError, ValueError):
         pass
-    return None, None
+    return traffic, valu
 
 
 def pctl(xs, q):
@@ -321,7 +327,7 @@ def main():
 
     kernels = None if args.no_kernels else kernel_table(torch, nat, hp, sptr, st)
 
-    traffic, traffic_src = measured_encode_traffic()
+    traffic, valu = encode_counters()
     line = {
         "metric": "verified BLAKE3 compressions/s (2^18 batch, 1 GPU) + p50/p99 proof latency",
         "value": value,
@@ -347,8 +353,12 @@ def main():
         "breakdown_ms": {"rs_encode": enc_ms, "merkle_tree": mer_ms},
         "roofline": {
             "bound": "hbm", "achieved": enc_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": enc_gbs / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-            "kernel": "RS encode (ntt_pass_kernel passes), algorithmic bytes 16*(N_msg+N_cw) = "
+            "frac": enc_gbs / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": PROFILE if traffic else None,
+            "valu": None if valu is None else {
+                "achieved": valu * 64 / (enc_ms / 1e3), "peak": VALU_PEAK, "unit": "int32 lane-ops/s",
+                "frac": valu * 64 / (enc_ms / 1e3) / VALU_PEAK, "instructions_per_encode": valu,
+                "source": PROFILE},
+            "kernel": "RS encode (one call: six NTT pass launches), algorithmic bytes 16*(N_msg+N_cw) = "
                       f"{enc_bytes} per encode",
         },
         "merkle_roofline": {"achieved": mer_bytes / (mer_ms / 1e3) / 1e9, "unit": "GB/s",

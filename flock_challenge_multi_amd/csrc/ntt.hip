@@ -307,8 +307,9 @@ static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t l
 // Pass schedule for layers [start, log_d): a list of layer counts (each 1..4).
 // The deepest pass (whose blocks are too small to share twiddles across a
 // work-group) takes up to 3 layers; the layers above it go in passes of 3 with
-// the remainder as a leading 1- or 2-layer pass (measured on MI355X: a leading
-// 4-layer pass at 2 waves/SIMD is slower than the extra HBM round trip).
+// the remainder as leading 2-layer passes (a remainder of 1 becomes 2 + 2).
+// Measured on MI355X (2^17 x 64 encode): 2,2,3,3,3,3 0.409 ms, 1,3,3,3,3,3
+// 0.433 ms, 4,3,3,3,3 0.412 ms (its 4-layer pass spills at 4 waves/SIMD).
 // FLOCK_NTT_SCHEDULE="4,3,3,3,3" overrides it (must sum to the layer count).
 static int ntt_schedule(uint32_t n, int kmax, int* ks) {
   int cnt = 0;
@@ -329,7 +330,14 @@ static int ntt_schedule(uint32_t n, int kmax, int* ks) {
   if (n == 0) return 0;
   const uint32_t deep = n < (uint32_t)kmax ? n : (uint32_t)kmax;
   uint32_t rest = n - deep;
-  if (kmax >= 3 && rest % 3 != 0) {
+  if (kmax >= 3 && rest % 3 == 1 && rest >= 4) {
+    // 2 + 2 rather than 1 + 3: the first passes are bound by HBM (the first also
+    // reads the replicated message), so a layer moved into them is nearly free
+    // while the 3-layer passes are bound by table lookups.
+    ks[cnt++] = 2;
+    ks[cnt++] = 2;
+    rest -= 4;
+  } else if (kmax >= 3 && rest % 3 != 0) {
     ks[cnt++] = (int)(rest % 3);
     rest -= rest % 3;
   }
