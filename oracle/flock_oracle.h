@@ -77,6 +77,9 @@ int orc_merkle_verify(const uint8_t root[32], const uint8_t leaf_hash[32], size_
                       const uint8_t *path, size_t path_len);
 
 /* ---- PCS commit = RS encode + Merkle (CF/pcs/commit.rs:187-227,1838-1877) */
+/* orc_commit on n_threads POSIX threads (same results; commit_mt.c). */
+int orc_commit_mt(const orc_f128 *msg, size_t n_msg, size_t num_ntts, size_t log_inv_rate,
+                  orc_f128 *codeword_out, uint8_t *tree_out, uint8_t root_out[32], size_t n_threads);
 int orc_commit(const orc_f128 *msg, size_t n_msg, size_t num_ntts, size_t log_inv_rate,
                orc_f128 *codeword_out, uint8_t *tree_out, uint8_t root_out[32]);
 

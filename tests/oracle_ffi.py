@@ -68,6 +68,7 @@ class Oracle:
             "orc_merkle_proof": (ctypes.c_int, [P, SZ, SZ, P]),
             "orc_merkle_verify": (ctypes.c_int, [P, P, SZ, P, SZ]),
             "orc_commit": (ctypes.c_int, [P, SZ, SZ, SZ, P, P, P]),
+            "orc_commit_mt": (ctypes.c_int, [P, SZ, SZ, SZ, P, P, P, SZ]),
             "orc_build_eq": (None, [P, SZ, P]),
             "orc_partial_eval_lsb": (None, [P, SZ, P, SZ, P]),
             "orc_sc_round_msg": (None, [P, P, SZ, P]),
@@ -203,6 +204,16 @@ class Oracle:
         tree = np.zeros((2 * n_pos - 1, 32), np.uint8)
         root = np.zeros(32, np.uint8)
         assert self.L.orc_commit(_p(msg), len(msg), num_ntts, log_inv_rate, _p(cw), _p(tree), _p(root)) == 0
+        return cw, tree, root.tobytes()
+
+    def commit_mt(self, msg, num_ntts, log_inv_rate, n_threads):
+        n_cw = len(msg) << log_inv_rate
+        n_pos = n_cw // num_ntts
+        cw = np.zeros(n_cw, dtype=F128_DT)
+        tree = np.zeros((2 * n_pos - 1, 32), np.uint8)
+        root = np.zeros(32, np.uint8)
+        assert self.L.orc_commit_mt(_p(msg), len(msg), num_ntts, log_inv_rate, _p(cw), _p(tree), _p(root),
+                                    n_threads) == 0
         return cw, tree, root.tobytes()
 
     def build_eq(self, point):

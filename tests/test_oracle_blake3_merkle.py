@@ -10,6 +10,7 @@ import json
 import os
 
 import numpy as np
+import pytest
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -78,3 +79,14 @@ def test_single_leaf_tree_is_the_leaf(orc):
     d = bytes(range(64))
     tree = orc.merkle_tree(d, 1)
     assert tree.shape == (1, 32) and tree[0].tobytes() == orc.leaf_cv(d)
+
+
+@pytest.mark.parametrize("log_pos,lanes,rate,threads", [(6, 8, 1, 3), (9, 64, 1, 8), (8, 4, 2, 5), (5, 1, 3, 16)])
+def test_threaded_commit_equals_commit(orc, log_pos, lanes, rate, threads):
+    """orc_commit_mt (the CPU baseline's threaded commit) equals orc_commit bit for bit."""
+    msg = orc.fill(4242 + log_pos, (1 << (log_pos - rate)) * lanes)
+    cw1, tree1, root1 = orc.commit(msg, lanes, rate)
+    cw2, tree2, root2 = orc.commit_mt(msg, lanes, rate, threads)
+    assert np.array_equal(cw1.view(np.uint8), cw2.view(np.uint8))
+    assert np.array_equal(tree1, tree2)
+    assert root1 == root2
