@@ -5,13 +5,21 @@
 // 96..127), which is exactly the reference's in-memory layout (lo u64 LE then
 // hi u64 LE), so a 16-byte global load is one element.
 //
-// CDNA4 has no carry-less multiply instruction.  Products use a 4-bit comb:
-// a 16-entry table T[k] = k * t (reduced, 256 B) of one operand lives in LDS;
-// the other operand is scanned nibble by nibble, XOR-ing T[nibble] at word
-// offset w into a 256-bit accumulator that is shifted by 4 bits between the
-// eight nibble positions.  The accumulator is congruent to the product mod p,
-// so sums of products are XOR-accumulated unreduced and reduced once
-// (the deferred-reduction identity of CF/field/gf2_128.rs:46-53).
+// CDNA4 has no carry-less multiply instruction.  Products are combs: tables of
+// multiples of one operand live in LDS, the other operand is cut into nibbles
+// or bytes that index them, and the entries are XOR-ed at their offsets into a
+// 256-bit accumulator.  The accumulator is congruent to the product mod p, so
+// sums of products are XOR-accumulated unreduced and reduced once (the
+// deferred-reduction identity of CF/field/gf2_128.rs:46-53).  Forms:
+//   comb_mul*      16-entry table k*t, 4-bit Horner (per-lane tables too);
+//   comb8_mul*     256-entry byte table (16 lookups; data-indexed 4 KiB tables
+//                  collide in the LDS banks);
+//   comb4p_mul_add k*t and k*t*x^4, byte Horner (32 conflict-free lookups);
+//   combjr_mul_add eight nibble-position tables k*t*x^(4j), no shifts
+//                  (32 conflict-free lookups);
+//   comb8r_mul_add the byte comb with three-input XORs.
+// A 16-entry table of 16-byte entries covers the LDS bank groups once, so its
+// data-indexed lookups never conflict.
 //
 // All routines are plain integer VALU work; nothing here touches MFMA.
 #pragma once
@@ -28,7 +36,6 @@ __device__ __forceinline__ uint4 f_zero() { return make_uint4(0u, 0u, 0u, 0u); }
 __device__ __forceinline__ uint4 f_add(uint4 a, uint4 b) {
   return make_uint4(a.x ^ b.x, a.y ^ b.y, a.z ^ b.z, a.w ^ b.w);
 }
-__device__ __forceinline__ bool f_is_zero(uint4 a) { return (a.x | a.y | a.z | a.w) == 0u; }
 
 // (hi:lo) >> s for 0 < s < 32, i.e. hi << (32-s) | lo >> s.
 __device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t s) {
@@ -231,29 +238,6 @@ __device__ __forceinline__ void table_store_private(uint4* tab_base, uint32_t nt
 #pragma unroll
   for (int k = 0; k < 16; ++k) tab_base[k * nthreads + tid] = e[k];
 }
-
-// Register-only product for scalar set-up work (slow path, a few calls per
-// kernel): shift-and-add over the 128 bits of b.
-__device__ __forceinline__ uint4 f_mul_slow(uint4 a, uint4 b) {
-  U256 acc;
-  u256_zero(acc);
-  const uint32_t bw[4] = {b.x, b.y, b.z, b.w};
-  uint32_t s[5] = {a.x, a.y, a.z, a.w, 0u};
-  for (int bit = 0; bit < 128; ++bit) {
-    const uint32_t word = bit >> 5, off = bit & 31;
-    if ((bw[word] >> off) & 1u) {
-      // a << bit lands at words word..word+4 shifted by off
-      uint32_t sh[5];
-      sh[0] = s[0] << off;
-      for (int i = 1; i < 5; ++i) sh[i] = off ? ((s[i] << off) | (s[i - 1] >> (32 - off))) : s[i];
-      for (int i = 0; i < 5; ++i)
-        if (word + i < 8) acc.w[word + i] ^= sh[i];
-    }
-  }
-  return f_reduce(acc);
-}
-
-__device__ __forceinline__ uint4 ld_f128(const uint4* p) { return *p; }
 
 // ---------------------------------------------------------------------------
 // gfx950-specific forms.  The three-input XOR V_BITOP3_B32 (truth table 0x96)

@@ -12,9 +12,12 @@
 // Consecutive threads are consecutive lanes of the same positions, so every
 // access is a contiguous 16-byte-per-lane stream.
 //
-// Twiddle products use the comb multiply of gf128.hpp.  All lanes of one set
-// share the twiddles, so the G = min(num_ntts, 64) lanes of a set build the
-// 16-entry tables of the twiddles of the current layer together in LDS.
+// Twiddle products are combs (gf128.hpp).  In every pass but the deepest, all
+// sets of a work-group share their twiddles and the work-group builds their
+// nibble-position tables once (combjr_mul_add); in the deepest passes the
+// G = min(num_ntts, 64) lanes of a set build nibble-pair tables for each layer
+// (comb4p_mul_add).  The inverse transform runs the same passes in reverse
+// order with the inverse butterfly (v += u; u += v*twiddle).
 //
 // The RS encode reads the message directly in its first pass: after the
 // log_inv_rate copy-only layers of the zero-padded transform, position p holds
