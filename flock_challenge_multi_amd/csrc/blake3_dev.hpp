@@ -90,4 +90,53 @@ __device__ __forceinline__ void b3_parent(const uint32_t l[8], const uint32_t r[
   b3_compress_cv(out, m, 0u, 0u, 64u, B3_PARENT);
 }
 
+// ---------------------------------------------------------------------------
+// Quad-cooperative parent compression for latency-bound tree tops: the four
+// lanes of an aligned lane quad (q = lane & 3) each run one column of the
+// 4 x 4 state (a = v[q], b = v[4+q], c = v[8+q], d = v[12+q]); for the
+// diagonal step rows b, c, d are rotated by 1, 2, 3 lanes with DPP quad
+// permutes and rotated back after it.  Every lane must hold the same 16 message
+// words (left CV, then right CV) and all four lanes of the quad must be active.
+// Lane q returns output words q and 4 + q.
+// ---------------------------------------------------------------------------
+template <int K>  // lane i reads lane (i + K) mod 4 of its quad
+__device__ __forceinline__ uint32_t quad_rot(uint32_t x) {
+  constexpr int ctrl = ((0 + K) & 3) | (((1 + K) & 3) << 2) | (((2 + K) & 3) << 4) | (((3 + K) & 3) << 6);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, ctrl, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ uint32_t sel_quad(bool q1, bool q2, uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3) {
+  return q2 ? (q1 ? w3 : w2) : (q1 ? w1 : w0);
+}
+
+#define FLK_B3_QROUND(s0, s1, s2, s3, s4, s5, s6, s7, s8, s9, s10, s11, s12, s13, s14, s15)            \
+  do {                                                                                                 \
+    FLK_B3_G(a, b, c, d, sel_quad(q1, q2, m[s0], m[s2], m[s4], m[s6]),                                 \
+             sel_quad(q1, q2, m[s1], m[s3], m[s5], m[s7]));                                            \
+    b = quad_rot<1>(b);                                                                                \
+    c = quad_rot<2>(c);                                                                                \
+    d = quad_rot<3>(d);                                                                                \
+    FLK_B3_G(a, b, c, d, sel_quad(q1, q2, m[s8], m[s10], m[s12], m[s14]),                              \
+             sel_quad(q1, q2, m[s9], m[s11], m[s13], m[s15]));                                         \
+    b = quad_rot<3>(b);                                                                                \
+    c = quad_rot<2>(c);                                                                                \
+    d = quad_rot<1>(d);                                                                                \
+  } while (0)
+
+__device__ __forceinline__ uint2 b3_parent_quad(const uint32_t m[16], uint32_t q) {
+  const bool q1 = (q & 1u) != 0u, q2 = (q & 2u) != 0u;
+  uint32_t a = sel_quad(q1, q2, 0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au);
+  uint32_t b = sel_quad(q1, q2, 0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u);
+  uint32_t c = a;                                       // v[8..11] = IV[0..3]
+  uint32_t d = sel_quad(q1, q2, 0u, 0u, 64u, B3_PARENT);  // counter 0, block_len 64, PARENT
+  FLK_B3_QROUND(0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15);
+  FLK_B3_QROUND(2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8);
+  FLK_B3_QROUND(3, 4, 10, 12, 13, 2, 7, 14, 6, 5, 9, 0, 11, 15, 8, 1);
+  FLK_B3_QROUND(10, 7, 12, 9, 14, 3, 13, 15, 4, 0, 11, 2, 5, 8, 1, 6);
+  FLK_B3_QROUND(12, 13, 9, 11, 15, 10, 14, 8, 7, 2, 5, 3, 0, 1, 6, 4);
+  FLK_B3_QROUND(9, 14, 11, 5, 8, 12, 15, 1, 13, 3, 0, 10, 2, 6, 4, 7);
+  FLK_B3_QROUND(11, 15, 5, 0, 1, 9, 8, 6, 14, 10, 2, 12, 3, 4, 7, 13);
+  return make_uint2(a ^ c, b ^ d);  // out[q] = v[q] ^ v[8+q], out[4+q] = v[4+q] ^ v[12+q]
+}
+
 }  // namespace flk

@@ -52,52 +52,42 @@ __global__ void __launch_bounds__(LEAF_WG) merkle_leaves_kernel(const uint4* __r
 
 // One work-group turns 2^LV children (starting at child index blockIdx*2^LV of the
 // level at `child`) into the LV levels above them.  level_off[k] is the tree
-// offset (in hashes) of level k above the child level, k = 1..LV.
-constexpr int PAR_MAX_LV = 9;  // 512 children, 256 threads
+// offset (in hashes) of level k above the child level, k = 1..LV.  Each node is
+// hashed by a lane quad (b3_parent_quad): the levels of a tree top are a chain
+// of dependent compressions, so four lanes per compression cut the latency.
+constexpr int PAR_MAX_LV = 9;  // 512 children, 256 nodes x 4 lanes
 
 struct LevelOffsets {
   uint64_t off[PAR_MAX_LV + 1];
 };
 
-__global__ void __launch_bounds__(256) merkle_levels_kernel(uint4* __restrict__ tree, uint64_t child_off,
-                                                           uint32_t lv, LevelOffsets lo) {
-  __shared__ uint32_t cvs[256 * 8];
-  const uint32_t t = threadIdx.x;
-  const uint32_t half = 1u << (lv - 1);  // parents produced at the first level
+__global__ void __launch_bounds__(1024) merkle_levels_kernel(uint4* __restrict__ tree, uint64_t child_off, uint32_t lv,
+                                                            LevelOffsets lo) {
+  __shared__ uint32_t cvs[2][256 * 8];
+  const uint32_t t = threadIdx.x, node = t >> 2, q = t & 3u;
   const uint64_t first_child = (uint64_t)blockIdx.x << lv;
-  if (t < half) {
-    const uint4* c = tree + 2 * (child_off + first_child + 2 * t);
-    const uint4 a0 = c[0], a1 = c[1], b0 = c[2], b1 = c[3];
-    uint32_t l[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-    uint32_t r[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-    uint32_t o[8];
-    b3_parent(l, r, o);
-    const uint64_t node = lo.off[1] + ((uint64_t)blockIdx.x << (lv - 1)) + t;
-    tree[2 * node] = make_uint4(o[0], o[1], o[2], o[3]);
-    tree[2 * node + 1] = make_uint4(o[4], o[5], o[6], o[7]);
+  for (uint32_t k = 1; k <= lv; ++k) {
+    const uint32_t n = 1u << (lv - k);  // nodes produced at this level
+    if (node < n) {
+      uint32_t m[16];
+      if (k == 1) {
+        const uint4* c = tree + 2 * (child_off + first_child + 2 * node);
+        const uint4 a0 = c[0], a1 = c[1], b0 = c[2], b1 = c[3];
+        m[0] = a0.x; m[1] = a0.y; m[2] = a0.z; m[3] = a0.w; m[4] = a1.x; m[5] = a1.y; m[6] = a1.z; m[7] = a1.w;
+        m[8] = b0.x; m[9] = b0.y; m[10] = b0.z; m[11] = b0.w; m[12] = b1.x; m[13] = b1.y; m[14] = b1.z; m[15] = b1.w;
+      } else {
+        const uint32_t* src = cvs[k & 1u] + node * 16;  // children 2 node, 2 node + 1, 8 words each
 #pragma unroll
-    for (int w = 0; w < 8; ++w) cvs[t * 8 + w] = o[w];
-  }
-  __syncthreads();
-  for (uint32_t k = 2; k <= lv; ++k) {
-    const uint32_t n = 1u << (lv - k);
-    uint32_t o[8];
-    if (t < n) {
-      uint32_t l[8], r[8];
-#pragma unroll
-      for (int w = 0; w < 8; ++w) {
-        l[w] = cvs[(2 * t) * 8 + w];
-        r[w] = cvs[(2 * t + 1) * 8 + w];
+        for (int i = 0; i < 16; ++i) m[i] = src[i];
       }
-      b3_parent(l, r, o);
-      const uint64_t node = lo.off[k] + ((uint64_t)blockIdx.x << (lv - k)) + t;
-      tree[2 * node] = make_uint4(o[0], o[1], o[2], o[3]);
-      tree[2 * node + 1] = make_uint4(o[4], o[5], o[6], o[7]);
-    }
-    __syncthreads();
-    if (t < n) {
-#pragma unroll
-      for (int w = 0; w < 8; ++w) cvs[t * 8 + w] = o[w];
+      const uint2 o = b3_parent_quad(m, q);
+      const uint64_t hash = lo.off[k] + ((uint64_t)blockIdx.x << (lv - k)) + node;
+      uint32_t* dst = reinterpret_cast<uint32_t*>(tree + 2 * hash);
+      dst[q] = o.x;
+      dst[4 + q] = o.y;
+      uint32_t* next = cvs[(k + 1) & 1u] + node * 8;
+      next[q] = o.x;
+      next[4 + q] = o.y;
     }
     __syncthreads();
   }
@@ -150,7 +140,7 @@ int merkle_upper_levels_device(uint8_t* tree_bytes, size_t n, size_t stop_nodes,
       len >>= 1;
       off += len;
     }
-    const uint32_t threads = 1u << (lv - 1);
+    const uint32_t threads = 4u << (lv - 1);  // a lane quad per node of the first level
     const uint64_t blocks = cur >> lv;
     hipLaunchKernelGGL(merkle_levels_kernel, dim3((unsigned)blocks), dim3(threads < 64 ? 64 : threads), 0, st, tree,
                        child_off, lv, lo);
