@@ -29,18 +29,22 @@ __global__ void __launch_bounds__(LEAF_WG) merkle_leaves_kernel(const uint4* __r
   uint32_t cv[8];
   b3_iv(cv);
   const uint32_t nblocks = (leaf_bytes + 63u) / 64u;
+  // Block b + 1 is loaded while block b is compressed (the chain is sequential).
+  uint4 nxt[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) nxt[q] = (uint32_t)q < leaf_words4 ? p[q] : make_uint4(0u, 0u, 0u, 0u);
   for (uint32_t b = 0; b < nblocks; ++b) {
     uint32_t m[16];
-    const uint32_t base4 = b * 4u;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      uint4 v = make_uint4(0u, 0u, 0u, 0u);
-      if (base4 + q < leaf_words4) v = p[base4 + q];
-      m[4 * q + 0] = v.x;
-      m[4 * q + 1] = v.y;
-      m[4 * q + 2] = v.z;
-      m[4 * q + 3] = v.w;
+      m[4 * q + 0] = nxt[q].x;
+      m[4 * q + 1] = nxt[q].y;
+      m[4 * q + 2] = nxt[q].z;
+      m[4 * q + 3] = nxt[q].w;
     }
+    const uint32_t base4 = (b + 1u) * 4u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) nxt[q] = base4 + q < leaf_words4 ? p[base4 + q] : make_uint4(0u, 0u, 0u, 0u);
     const uint32_t rem = leaf_bytes - b * 64u;
     const uint32_t blen = rem < 64u ? rem : 64u;
     const uint32_t flags = (b == 0 ? B3_CHUNK_START : 0u) | (b + 1 == nblocks ? B3_CHUNK_END : 0u);
