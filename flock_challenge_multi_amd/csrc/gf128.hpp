@@ -42,6 +42,12 @@ __device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t s)
   return __builtin_amdgcn_alignbit(hi, lo, s);
 }
 
+// a ^ b ^ c in one v_bitop3_b32 (truth table 0x96); the compiler does not
+// fuse two-input XOR chains into it by itself.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 // x * a (CF/field/gf2_128.rs:174-184).
 __device__ __forceinline__ uint4 f_mul_x(uint4 a) {
   uint32_t carry = a.w >> 31;
@@ -135,31 +141,34 @@ __device__ __forceinline__ void comb_mul_acc(uint4 v, const uint4* __restrict__ 
 }
 
 // Same with a run-time stride (per-thread tables laid out entry-major).
+// Each Horner step folds the four looked-up entries into the 7-word window
+// with three-input XORs.
 __device__ __forceinline__ void comb_mul_acc_rt(uint4 v, const uint4* __restrict__ tab, uint32_t stride,
                                                 U256& acc) {
   const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
-  U256 a;
-  u256_zero(a);
+  uint32_t a[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a[i] = 0u;
   // Not unrolled over j: keeps only four table loads in flight per lane, which
   // bounds register pressure when several products are inlined together.
 #pragma unroll 1
   for (int j = 7; j >= 0; --j) {
-    if (j != 7) {
+    const uint4 e0 = tab[((vw[0] >> (4 * j)) & 15u) * stride];
+    const uint4 e1 = tab[((vw[1] >> (4 * j)) & 15u) * stride];
+    const uint4 e2 = tab[((vw[2] >> (4 * j)) & 15u) * stride];
+    const uint4 e3 = tab[((vw[3] >> (4 * j)) & 15u) * stride];
 #pragma unroll
-      for (int i = 7; i >= 1; --i) a.w[i] = funnel(a.w[i], a.w[i - 1], 28);
-      a.w[0] <<= 4;
-    }
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      const uint32_t nib = (vw[w] >> (4 * j)) & 15u;
-      const uint4 e = tab[nib * stride];
-      a.w[w] ^= e.x;
-      a.w[w + 1] ^= e.y;
-      a.w[w + 2] ^= e.z;
-      a.w[w + 3] ^= e.w;
-    }
+    for (int i = 7; i >= 1; --i) a[i] = funnel(a[i], a[i - 1], 28);
+    a[0] = (a[0] << 4) ^ e0.x;
+    a[1] = xor3(a[1], e0.y, e1.x);
+    a[2] = xor3(a[2], e0.z, e1.y) ^ e2.x;
+    a[3] = xor3(xor3(a[3], e0.w, e1.z), e2.y, e3.x);
+    a[4] = xor3(a[4], e1.w, e2.z) ^ e3.y;
+    a[5] = xor3(a[5], e2.w, e3.z);
+    a[6] ^= e3.w;
   }
-  u256_xor(acc, a);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) acc.w[i] ^= a[i];
 }
 
 template <int STRIDE>
@@ -246,10 +255,6 @@ __device__ __forceinline__ void table_store_private(uint4* tab_base, uint32_t nt
 // step (one SDWA op per table address) and shifting the words up by 8, which
 // keeps at most eight table loads in flight and the kernels at <= 128 VGPRs.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
-  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
-}
-
 __device__ __forceinline__ uint4 lds_entry(const uint4* tab, uint32_t byte_off) {
   return *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(tab) + byte_off);
 }
