@@ -280,7 +280,22 @@ __device__ __forceinline__ uint4 f_reduce_add(const uint32_t r[8], uint4 add) {
 // still schedule and rematerialise them (inline-asm SDWA cannot be).
 template <int J>
 __device__ __forceinline__ uint32_t hinib_x16(uint32_t w) {
-  return (w >> (8 * J)) & 0xF0u;
+  // One instruction per table address: an SDWA byte select of the AND.  The
+  // compiler would fold a preceding shift into the select's shift amount and
+  // emit shift + AND instead, so bytes 1-3 are written out.
+  uint32_t r;
+  if constexpr (J == 1)
+    asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_1 src1_sel:DWORD"
+        : "=v"(r) : "v"(w), "v"(0xF0u));
+  else if constexpr (J == 2)
+    asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_2 src1_sel:DWORD"
+        : "=v"(r) : "v"(w), "v"(0xF0u));
+  else if constexpr (J == 3)
+    asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_3 src1_sel:DWORD"
+        : "=v"(r) : "v"(w), "v"(0xF0u));
+  else
+    r = w & 0xF0u;
+  return r;
 }
 
 // add + v * t from two 16-entry nibble tables, tab[k] = k * t and
@@ -298,23 +313,26 @@ __device__ __forceinline__ uint4 comb4p_mul_add(uint4 v, uint4 add, const uint4*
   for (int i = 0; i < 8; ++i) a[i] = 0u;
 #pragma unroll 1
   for (int J = 0; J < 4; ++J) {
-    uint4 l[4], h[4];
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      l[w] = lds_entry(tab, hinib_x16<3>(lw[w]));
-      h[w] = lds_entry(tab + 16, hinib_x16<3>(hw[w]));
-      lw[w] <<= 8;
-      hw[w] <<= 8;
-    }
 #pragma unroll
     for (int i = 7; i >= 1; --i) a[i] = funnel(a[i], a[i - 1], 24);
-    a[0] = xor3(a[0] << 8, l[0].x, h[0].x);
-    a[1] = xor3(xor3(a[1], l[0].y, h[0].y), l[1].x, h[1].x);
-    a[2] = xor3(xor3(xor3(a[2], l[0].z, h[0].z), l[1].y, h[1].y), l[2].x, h[2].x);
-    a[3] = xor3(xor3(xor3(xor3(a[3], l[0].w, h[0].w), l[1].z, h[1].z), l[2].y, h[2].y), l[3].x, h[3].x);
-    a[4] = xor3(xor3(xor3(a[4], l[1].w, h[1].w), l[2].z, h[2].z), l[3].y, h[3].y);
-    a[5] = xor3(xor3(a[5], l[2].w, h[2].w), l[3].z, h[3].z);
-    a[6] = xor3(a[6], l[3].w, h[3].w);
+    a[0] <<= 8;
+    // Low nibbles, then high nibbles: half the entries live at a time.
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      uint4 e[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        e[w] = half ? lds_entry(tab + 16, hinib_x16<3>(hw[w])) : lds_entry(tab, hinib_x16<3>(lw[w]));
+        if (half) hw[w] <<= 8; else lw[w] <<= 8;
+      }
+      a[0] ^= e[0].x;
+      a[1] = xor3(a[1], e[0].y, e[1].x);
+      a[2] = xor3(a[2], e[0].z, e[1].y) ^ e[2].x;
+      a[3] = xor3(xor3(a[3], e[0].w, e[1].z), e[2].y, e[3].x);
+      a[4] = xor3(a[4], e[1].w, e[2].z) ^ e[3].y;
+      a[5] = xor3(a[5], e[2].w, e[3].z);
+      a[6] ^= e[3].w;
+    }
   }
   return f_reduce_add(a, add);
 }

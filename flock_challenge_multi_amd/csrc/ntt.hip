@@ -112,6 +112,90 @@ __global__ void __launch_bounds__(NTT_WG) __attribute__((amdgpu_waves_per_eu(PAI
   }
 }
 
+// Deepest passes with one set per wave (num_ntts >= 64), one wave per
+// work-group: before each layer the wave builds the nibble-position tables
+// (k * t * x^(4j), j = 0..7; 2 KiB per twiddle) of the 2^i twiddles that layer
+// meets, so every product is the shift-free combjr_mul_add.  The tables sit at
+// LDS address 0 of the work-group, so every lookup address is the nibble alone
+// plus an immediate offset.  Table build: lane item (sb, j, half) forms
+// b_i = t_sb * x^(4j+i), i = 0..3, and writes entries half*8 .. half*8+7 as
+// XORs of them (Gray order, one XOR per entry).  The twiddles of all K layers
+// are loaded with the data, before the first table build.
+constexpr int NTT_POS_WG = 64;
+template <int K, bool INV = false>
+__global__ void __launch_bounds__(NTT_POS_WG) __attribute__((amdgpu_waves_per_eu(NTT_WAVES)))
+ntt_pass_pos_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, const uint4* __restrict__ tw,
+                    uint32_t log_d, uint32_t L, uint32_t log_lanes, uint64_t src_pos_mask, uint64_t n_threads) {
+  constexpr int TAB = 128;  // entries per twiddle
+  __shared__ uint4 tabs[TAB << (K - 1)];
+  const uint32_t lane = threadIdx.x;
+
+  const uint64_t g = (uint64_t)blockIdx.x * NTT_POS_WG + lane;
+  const bool active = g < n_threads;
+  const uint64_t c = g & ((1ull << log_lanes) - 1ull);
+  const uint64_t s = g >> log_lanes;
+  const uint32_t lo_bits = log_d - L - K;
+  const uint64_t lo = s & ((1ull << lo_bits) - 1ull);
+  const uint64_t prefix = s >> lo_bits;
+
+  uint4 x[1 << K];
+#pragma unroll
+  for (int j = 0; j < (1 << K); ++j) {
+    const uint64_t p = (prefix << (log_d - L)) | ((uint64_t)j << lo_bits) | lo;
+    x[j] = active ? src[((p & src_pos_mask) << log_lanes) | c] : f_zero();
+  }
+  // Item of lane `lane` in layer i: twiddle sb = lane >> 4 (lanes < 16 << i).
+  uint4 t_item[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const uint32_t sb = lane >> 4;
+    t_item[i] = (active && lane < (16u << i)) ? tw[((1ull << (L + i)) - 1ull) + (prefix << i) + sb] : f_zero();
+  }
+
+#pragma unroll
+  for (int ii = 0; ii < K; ++ii) {
+    const int i = INV ? K - 1 - ii : ii;
+    if (lane < (16u << i)) {
+      const uint32_t sb = lane >> 4, j = (lane >> 1) & 7u, half = lane & 1u;
+      const uint4 b0 = f_mul_xs(t_item[i], 4u * j);
+      const uint4 b1 = f_mul_x(b0), b2 = f_mul_x(b1);
+      uint4 e = half ? f_mul_x(b2) : f_zero();  // entry half*8 + 0
+      uint4* out = tabs + sb * TAB + j * 16u + half * 8u;
+      // Gray order over the low three index bits: 0,1,3,2,6,7,5,4.
+      out[0] = e;
+      e = f_add(e, b0); out[1] = e;
+      e = f_add(e, b1); out[3] = e;
+      e = f_add(e, b0); out[2] = e;
+      e = f_add(e, b2); out[6] = e;
+      e = f_add(e, b0); out[7] = e;
+      e = f_add(e, b1); out[5] = e;
+      e = f_add(e, b0); out[4] = e;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int sb = 0; sb < (1 << i); ++sb) {
+      const uint4* tab = tabs + sb * TAB;
+#pragma unroll
+      for (int jj = 0; jj < (1 << (K - 1 - i)); ++jj) {
+        const int j0 = (sb << (K - i)) + jj;
+        const int j1 = j0 + (1 << (K - 1 - i));
+        if constexpr (INV) x[j1] = f_add(x[j1], x[j0]);
+        x[j0] = combjr_mul_add(x[j1], x[j0], tab);
+        if constexpr (!INV) x[j1] = f_add(x[j1], x[j0]);
+      }
+    }
+    __syncthreads();
+  }
+
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < (1 << K); ++j) {
+      const uint64_t p = (prefix << (log_d - L)) | ((uint64_t)j << lo_bits) | lo;
+      dst[(p << log_lanes) | c] = x[j];
+    }
+  }
+}
+
 // Passes whose work-group covers sets that all share the block prefix (every
 // pass except the deepest ones): the 2^K - 1 twiddles of the pass are common to
 // the whole work-group, so their comb tables are built once per work-group at
@@ -264,6 +348,11 @@ static bool ntt_pair_tables_enabled() {
   static const bool on = !env_off("FLOCK_NTT_PAIR_TABLES");
   return on;
 }
+// FLOCK_NTT_POS_TABLES=0 keeps the deepest passes on the nibble-pair kernel.
+static bool ntt_pos_tables_enabled() {
+  static const bool on = !env_off("FLOCK_NTT_POS_TABLES");
+  return on;
+}
 
 template <int K>
 static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t log_d, uint32_t L, uint32_t log_lanes,
@@ -291,6 +380,20 @@ static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t l
                          log_d, L, log_lanes, src_pos_mask, n_threads);
     FLK_LAUNCH_CHECK("ntt_pass_shared_kernel");
     return FLK_OK;
+  }
+  if constexpr (K <= 3) {
+    // One set per wave (num_ntts >= 64): per-wave nibble-position tables.
+    if (log_group == 6 && ntt_pos_tables_enabled()) {
+      const unsigned pos_blocks = (unsigned)((n_threads + NTT_POS_WG - 1) / NTT_POS_WG);
+      if (inverse)
+        hipLaunchKernelGGL((ntt_pass_pos_kernel<K, true>), dim3(pos_blocks), dim3(NTT_POS_WG), 0, st, src, dst, tw,
+                           log_d, L, log_lanes, src_pos_mask, n_threads);
+      else
+        hipLaunchKernelGGL((ntt_pass_pos_kernel<K>), dim3(pos_blocks), dim3(NTT_POS_WG), 0, st, src, dst, tw, log_d, L,
+                           log_lanes, src_pos_mask, n_threads);
+      FLK_LAUNCH_CHECK("ntt_pass_pos_kernel");
+      return FLK_OK;
+    }
   }
   const bool pair = inverse || ntt_pair_tables_enabled();
   const size_t lds = (size_t)(NTT_WG / 64) * (64u >> log_group) * ((size_t)(pair ? 32 : 16) << (K - 1)) * sizeof(uint4);
