@@ -275,9 +275,7 @@ __device__ __forceinline__ uint4 f_reduce_add(const uint32_t r[8], uint4 add) {
   return o;
 }
 
-// 16 * (high nibble of byte J of w), i.e. (w >> (8J + 4) & 15) << 4.  Plain
-// shifts and masks: the compiler turns J = 2, 3 into single SDWA ops and can
-// still schedule and rematerialise them (inline-asm SDWA cannot be).
+// 16 * (high nibble of byte J of w), i.e. (w >> (8J + 4) & 15) << 4.
 template <int J>
 __device__ __forceinline__ uint32_t hinib_x16(uint32_t w) {
   // One instruction per table address: an SDWA byte select of the AND.  The

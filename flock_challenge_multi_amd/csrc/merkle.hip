@@ -18,7 +18,10 @@ namespace flk {
 int ntt_rs_encode_device(size_t dim, const uint4* msg, size_t n_msg, uint4* cw, size_t n_cw, size_t num_ntts,
                          hipStream_t st);
 
-constexpr int LEAF_WG = 128;
+// 256 lanes (four waves) per work-group: with 2^17 one-KiB leaves the kernel is
+// a single round of two waves per SIMD, and 128-lane work-groups ran 20-30%
+// longer right after some kernels (the waves did not all start together).
+constexpr int LEAF_WG = 256;
 
 __global__ void __launch_bounds__(LEAF_WG) merkle_leaves_kernel(const uint4* __restrict__ data, uint32_t leaf_words4,
                                                                uint32_t leaf_bytes, uint64_t n_leaves,
