@@ -341,6 +341,10 @@ __device__ __forceinline__ uint4 comb4p_mul_add(uint4 v, uint4 add, const uint4*
 // product are free of bank conflicts (lanes that hit one entry are served by
 // broadcast).  Words are combined by Horner in x^32, top word first (a shift by
 // one word is a register move), two words per rolled iteration.
+// JS: entries between the starts of consecutive nibble-position tables (16, or
+// 17 where the tables are written by lanes that each own one position: the pad
+// spreads those writes over all LDS bank groups; lookups only change offsets).
+template <int JS = 16>
 __device__ __forceinline__ uint4 combjr_mul_add(uint4 v, uint4 add, const uint4* tab) {
   uint32_t vw[4] = {v.w, v.z, v.y, v.x};
   uint32_t a[7];
@@ -353,14 +357,14 @@ __device__ __forceinline__ uint4 combjr_mul_add(uint4 v, uint4 add, const uint4*
 #pragma unroll
       for (int i = 6; i >= 1; --i) a[i] = a[i - 1];
       const uint32_t hi = vw[h], lo = hi << 4;
-      const uint4 e0 = lds_entry(tab + 0 * 16, hinib_x16<0>(lo));
-      const uint4 e1 = lds_entry(tab + 1 * 16, hinib_x16<0>(hi));
-      const uint4 e2 = lds_entry(tab + 2 * 16, hinib_x16<1>(lo));
-      const uint4 e3 = lds_entry(tab + 3 * 16, hinib_x16<1>(hi));
-      const uint4 e4 = lds_entry(tab + 4 * 16, hinib_x16<2>(lo));
-      const uint4 e5 = lds_entry(tab + 5 * 16, hinib_x16<2>(hi));
-      const uint4 e6 = lds_entry(tab + 6 * 16, hinib_x16<3>(lo));
-      const uint4 e7 = lds_entry(tab + 7 * 16, hinib_x16<3>(hi));
+      const uint4 e0 = lds_entry(tab + 0 * JS, hinib_x16<0>(lo));
+      const uint4 e1 = lds_entry(tab + 1 * JS, hinib_x16<0>(hi));
+      const uint4 e2 = lds_entry(tab + 2 * JS, hinib_x16<1>(lo));
+      const uint4 e3 = lds_entry(tab + 3 * JS, hinib_x16<1>(hi));
+      const uint4 e4 = lds_entry(tab + 4 * JS, hinib_x16<2>(lo));
+      const uint4 e5 = lds_entry(tab + 5 * JS, hinib_x16<2>(hi));
+      const uint4 e6 = lds_entry(tab + 6 * JS, hinib_x16<3>(lo));
+      const uint4 e7 = lds_entry(tab + 7 * JS, hinib_x16<3>(hi));
       a[0] = xor3(xor3(xor3(e0.x, e1.x, e2.x), e3.x, e4.x), e5.x, e6.x) ^ e7.x;
       a[1] = xor3(xor3(xor3(xor3(a[1], e0.y, e1.y), e2.y, e3.y), e4.y, e5.y), e6.y, e7.y);
       a[2] = xor3(xor3(xor3(xor3(a[2], e0.z, e1.z), e2.z, e3.z), e4.z, e5.z), e6.z, e7.z);

@@ -119,14 +119,18 @@ __global__ void __launch_bounds__(NTT_WG) __attribute__((amdgpu_waves_per_eu(PAI
 // LDS address 0 of the work-group, so every lookup address is the nibble alone
 // plus an immediate offset.  Table build: lane item (sb, j, half) forms
 // b_i = t_sb * x^(4j+i), i = 0..3, and writes entries half*8 .. half*8+7 as
-// XORs of them (Gray order, one XOR per entry).  The twiddles of all K layers
-// are loaded with the data, before the first table build.
+// XORs of them (Gray order, one XOR per entry).  Position j's table starts at
+// entry 17j (one pad entry), so the 16 lanes of each write cover all 16 LDS
+// bank groups; with 16-entry strides every write hit two of them (0.21 of the
+// CU cycles in bank conflicts).  The twiddles of all K layers are loaded with
+// the data, before the first table build.
 constexpr int NTT_POS_WG = 64;
 template <int K, bool INV = false>
 __global__ void __launch_bounds__(NTT_POS_WG) __attribute__((amdgpu_waves_per_eu(NTT_WAVES)))
 ntt_pass_pos_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, const uint4* __restrict__ tw,
                     uint32_t log_d, uint32_t L, uint32_t log_lanes, uint64_t src_pos_mask, uint64_t n_threads) {
-  constexpr int TAB = 128;  // entries per twiddle
+  constexpr int JS = 17;         // entries per nibble-position table (16 + 1 pad)
+  constexpr int TAB = 8 * JS;    // entries per twiddle
   __shared__ uint4 tabs[TAB << (K - 1)];
   const uint32_t lane = threadIdx.x;
 
@@ -160,7 +164,7 @@ ntt_pass_pos_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, cons
       const uint4 b0 = f_mul_xs(t_item[i], 4u * j);
       const uint4 b1 = f_mul_x(b0), b2 = f_mul_x(b1);
       uint4 e = half ? f_mul_x(b2) : f_zero();  // entry half*8 + 0
-      uint4* out = tabs + sb * TAB + j * 16u + half * 8u;
+      uint4* out = tabs + sb * TAB + j * JS + half * 8u;
       // Gray order over the low three index bits: 0,1,3,2,6,7,5,4.
       out[0] = e;
       e = f_add(e, b0); out[1] = e;
@@ -180,7 +184,7 @@ ntt_pass_pos_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, cons
         const int j0 = (sb << (K - i)) + jj;
         const int j1 = j0 + (1 << (K - 1 - i));
         if constexpr (INV) x[j1] = f_add(x[j1], x[j0]);
-        x[j0] = combjr_mul_add(x[j1], x[j0], tab);
+        x[j0] = combjr_mul_add<JS>(x[j1], x[j0], tab);
         if constexpr (!INV) x[j1] = f_add(x[j1], x[j0]);
       }
     }
