@@ -335,6 +335,27 @@ __device__ __forceinline__ uint4 comb4p_mul_add(uint4 v, uint4 add, const uint4*
   return f_reduce_add(a, add);
 }
 
+// XOR of tab[j * 16 + nibble j of w] over the eight nibbles j of w: one word of
+// a GF(2)-linear map applied with 16-entry subset-sum tables (conflict-free
+// lookups, one address instruction each).
+__device__ __forceinline__ uint4 nibtab8_sum(uint32_t w, const uint4* tab) {
+  const uint32_t lo = w << 4;
+  const uint4 e0 = lds_entry(tab + 0 * 16, hinib_x16<0>(lo));
+  const uint4 e1 = lds_entry(tab + 1 * 16, hinib_x16<0>(w));
+  const uint4 e2 = lds_entry(tab + 2 * 16, hinib_x16<1>(lo));
+  const uint4 e3 = lds_entry(tab + 3 * 16, hinib_x16<1>(w));
+  const uint4 e4 = lds_entry(tab + 4 * 16, hinib_x16<2>(lo));
+  const uint4 e5 = lds_entry(tab + 5 * 16, hinib_x16<2>(w));
+  const uint4 e6 = lds_entry(tab + 6 * 16, hinib_x16<3>(lo));
+  const uint4 e7 = lds_entry(tab + 7 * 16, hinib_x16<3>(w));
+  uint4 r;
+  r.x = xor3(xor3(xor3(e0.x, e1.x, e2.x), e3.x, e4.x), e5.x, e6.x) ^ e7.x;
+  r.y = xor3(xor3(xor3(e0.y, e1.y, e2.y), e3.y, e4.y), e5.y, e6.y) ^ e7.y;
+  r.z = xor3(xor3(xor3(e0.z, e1.z, e2.z), e3.z, e4.z), e5.z, e6.z) ^ e7.z;
+  r.w = xor3(xor3(xor3(e0.w, e1.w, e2.w), e3.w, e4.w), e5.w, e6.w) ^ e7.w;
+  return r;
+}
+
 // add + v * t with eight nibble-position tables tab[j * 16 + k] = k * t * x^(4j)
 // (j = nibble position inside a 32-bit word; 2 KiB per twiddle).  Every
 // 16-entry slice covers all LDS bank groups once, so the 32 lookups of a

@@ -226,16 +226,17 @@ __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_kernel(const uint4* __r
 }
 
 // ---------------------------------------------------------------------------
-// Ring switch fold_b128_elems: out_i = sum_b bit_b(s_i) eq_r[b].  The 16 byte
-// tables (table[p][v] = sum of eq_r[8p + bit] over the bits of v, 64 KiB) are
-// built once per work-group; each element is 16 lookups.
+// Ring switch fold_b128_elems: out_i = sum_b bit_b(s_i) eq_r[b].  32 nibble
+// tables (table[p][v] = sum of eq_r[4p + bit] over the bits of v, 8 KiB) are
+// built once per work-group; each element is 32 lookups.  Byte tables halve
+// the lookups, but their data-indexed 4 KiB tables collide in the LDS banks
+// (0.38 of the CU cycles at 2^25 elements).
 // ---------------------------------------------------------------------------
 constexpr int FB_WG = 256;
 
 __global__ void __launch_bounds__(FB_WG) rs_fold_b128_kernel(const uint4* __restrict__ sfx, const uint4* __restrict__ eqr,
                                                             uint64_t n, uint4* __restrict__ out) {
-  __shared__ uint4 tables[16 * 256];  // byte tables, 64 KiB (gfx950 has 160 KiB of LDS per CU)
-  __shared__ uint4 nibt[32 * 16];     // nibble tables
+  __shared__ uint4 nibt[32 * 16];
   for (uint32_t e = threadIdx.x; e < 32u * 16u; e += FB_WG) {
     const uint32_t np = e >> 4, k = e & 15u;  // nibble position np covers bits 4np..4np+3
     uint4 v = f_zero();
@@ -245,34 +246,26 @@ __global__ void __launch_bounds__(FB_WG) rs_fold_b128_kernel(const uint4* __rest
     nibt[e] = v;
   }
   __syncthreads();
-  for (uint32_t e = threadIdx.x; e < 16u * 256u; e += FB_WG) {
-    const uint32_t p = e >> 8, v = e & 255u;
-    tables[e] = f_add(nibt[(2 * p) * 16 + (v & 15u)], nibt[(2 * p + 1) * 16 + (v >> 4)]);
-  }
-  __syncthreads();
   for (uint64_t i = (uint64_t)blockIdx.x * FB_WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * FB_WG) {
     const uint4 s = sfx[i];
-    const uint32_t w[4] = {s.x, s.y, s.z, s.w};
-    uint4 acc = f_zero();
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc = f_add(acc, tables[(4 * q + b) * 256 + ((w[q] >> (8 * b)) & 255u)]);
-    out[i] = acc;
+    const uint4 r0 = nibtab8_sum(s.x, nibt), r1 = nibtab8_sum(s.y, nibt + 128);
+    const uint4 r2 = nibtab8_sum(s.z, nibt + 256), r3 = nibtab8_sum(s.w, nibt + 384);
+    out[i] = make_uint4(xor3(r0.x, r1.x, r2.x) ^ r3.x, xor3(r0.y, r1.y, r2.y) ^ r3.y, xor3(r0.z, r1.z, r2.z) ^ r3.z,
+                        xor3(r0.w, r1.w, r2.w) ^ r3.w);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Zerocheck fold at z: out[x] = sum_{s < 2^k_skip} bit(x*2^k_skip + s) * w[s].
-// Byte tables of the weights (2^k_skip / 8 tables of 256 entries) per work-group.
+// Nibble tables of the weights (2 * row_bytes tables of 16 entries) per
+// work-group; 8-byte rows take 16 conflict-free lookups.
 // ---------------------------------------------------------------------------
 constexpr int FZ_WG = 256;
 
 __global__ void __launch_bounds__(FZ_WG) zc_fold_at_z_kernel(const uint8_t* __restrict__ bits, uint32_t row_bytes,
                                                             uint64_t n_rows, const uint4* __restrict__ w,
                                                             uint4* __restrict__ out) {
-  __shared__ uint4 tables[8 * 256];  // row_bytes (<= 8) byte tables
-  __shared__ uint4 nibt[8 * 32];
+  __shared__ uint4 nibt[16 * 16];  // row_bytes (<= 8) * 2 nibble tables
   for (uint32_t e = threadIdx.x; e < row_bytes * 32u; e += FZ_WG) {
     const uint32_t np = e >> 4, k = e & 15u;
     uint4 v = f_zero();
@@ -282,22 +275,16 @@ __global__ void __launch_bounds__(FZ_WG) zc_fold_at_z_kernel(const uint8_t* __re
     nibt[e] = v;
   }
   __syncthreads();
-  for (uint32_t e = threadIdx.x; e < row_bytes * 256u; e += FZ_WG) {
-    const uint32_t p = e >> 8, v = e & 255u;
-    tables[e] = f_add(nibt[(2 * p) * 16 + (v & 15u)], nibt[(2 * p + 1) * 16 + (v >> 4)]);
-  }
-  __syncthreads();
   for (uint64_t x = (uint64_t)blockIdx.x * FZ_WG + threadIdx.x; x < n_rows; x += (uint64_t)gridDim.x * FZ_WG) {
     const uint8_t* row = bits + x * row_bytes;
-    uint4 acc = f_zero();
+    uint4 acc;
     if (row_bytes == 8) {
       const uint2 v = *reinterpret_cast<const uint2*>(row);
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc = f_add(acc, tables[b * 256 + ((v.x >> (8 * b)) & 255u)]);
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc = f_add(acc, tables[(4 + b) * 256 + ((v.y >> (8 * b)) & 255u)]);
+      acc = f_add(nibtab8_sum(v.x, nibt), nibtab8_sum(v.y, nibt + 128));
     } else {
-      for (uint32_t b = 0; b < row_bytes; ++b) acc = f_add(acc, tables[b * 256 + row[b]]);
+      acc = f_zero();
+      for (uint32_t b = 0; b < row_bytes; ++b)
+        acc = f_add(acc, f_add(nibt[(2 * b) * 16 + (row[b] & 15u)], nibt[(2 * b + 1) * 16 + (row[b] >> 4)]));
     }
     out[x] = acc;
   }
