@@ -161,15 +161,27 @@ __global__ void __launch_bounds__(256) lc_bind_top_kernel(uint4* __restrict__ v,
 // ---------------------------------------------------------------------------
 // Ring switch fold_1b_rows: s_hat_v[r] = sum_i bit_r(z_i) eq_i, r < 128.
 // A tile of 256 elements is staged in LDS; every group of 4 elements gets a
-// 16-entry subset-sum table of its eq values; lane (r, half) turns bit r of the
-// 4 elements of each group into a nibble and looks it up.  out must be zeroed.
+// 16-entry subset-sum table of its eq values.  The tile's bits are transposed
+// once into nibbles nib[g][r] = (bit r of the group's 4 elements), packed
+// eight to a word, so lane (r, half) reads one word per group and looks its
+// nibble up.  out must be zeroed.
 // ---------------------------------------------------------------------------
 constexpr int F1_WG = 256;
+
+// Bits b of the byte x moved to bit 4b (b < 8).
+__device__ __forceinline__ uint32_t spread_byte4(uint32_t x) {
+  x &= 0xFFu;
+  x = (x | (x << 12)) & 0x000F000Fu;
+  x = (x | (x << 6)) & 0x03030303u;
+  x = (x | (x << 3)) & 0x11111111u;
+  return x;
+}
 
 __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_kernel(const uint4* __restrict__ zp,
                                                                const uint4* __restrict__ eq, uint64_t n,
                                                                uint4* __restrict__ out) {
   __shared__ uint32_t zt[F1_WG * 4];
+  __shared__ uint32_t nibw[(F1_WG / 4) * 16];  // [group][r / 8]: nibbles of r = 8q .. 8q+7
   __shared__ uint4 tabs[(F1_WG / 4) * 16];
   __shared__ uint4 red[128];
   const uint32_t t = threadIdx.x;
@@ -210,13 +222,26 @@ __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_kernel(const uint4* __r
     tg[2] = f_add(base, e1);
     tg[3] = f_add(f_add(base, e0), e1);
     __syncthreads();
-    const uint32_t word = r >> 5, bit = r & 31u;
+    // Transpose: item (g, qw) packs the nibbles of r = 8qw .. 8qw+7, i.e. byte
+    // qw & 3 of word qw >> 2 of the group's four elements, bit b of element k
+    // at bit 4b + k.
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const uint32_t item = (uint32_t)it * F1_WG + t, g = item >> 4, qw = item & 15u;
+      const uint32_t w = qw >> 2, sh = 8u * (qw & 3u);
+      uint32_t v = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v |= spread_byte4(zt[(4 * g + k) * 4 + w] >> sh) << k;
+      nibw[item] = v;
+    }
+    __syncthreads();
+    const uint32_t qr = r >> 3, shr = 4u * (r & 7u);
 #pragma unroll 4
-    for (uint32_t g = halfsel * 32; g < halfsel * 32 + 32; ++g) {
-      const uint32_t nibv = ((zt[(4 * g + 0) * 4 + word] >> bit) & 1u) | (((zt[(4 * g + 1) * 4 + word] >> bit) & 1u) << 1) |
-                            (((zt[(4 * g + 2) * 4 + word] >> bit) & 1u) << 2) |
-                            (((zt[(4 * g + 3) * 4 + word] >> bit) & 1u) << 3);
-      acc = f_add(acc, tabs[g * 16 + nibv]);
+    for (uint32_t g = halfsel * 32; g < halfsel * 32 + 32; g += 2) {
+      const uint4 x0 = tabs[g * 16 + ((nibw[g * 16 + qr] >> shr) & 15u)];
+      const uint4 x1 = tabs[(g + 1) * 16 + ((nibw[(g + 1) * 16 + qr] >> shr) & 15u)];
+      acc = make_uint4(xor3(acc.x, x0.x, x1.x), xor3(acc.y, x0.y, x1.y), xor3(acc.z, x0.z, x1.z),
+                       xor3(acc.w, x0.w, x1.w));
     }
     __syncthreads();
   }
