@@ -66,39 +66,60 @@ static int zero_n(uint4* p, uint64_t n, hipStream_t st) {
 // LC_CHUNK stripes at a time (one coalesced eq load, one barrier) so that the
 // lookups of a chunk run back to back with their z loads in flight.  (16 i per
 // lane, one 16-byte load per stripe, measured 2.6x slower at m = 32.)
-// grid = (k / 1024, stripe splits); out must be zeroed.
+// A work-group is LC_GROUPS stripe groups of LC_I lanes over the same i: each
+// group folds its own quarter of the work-group's stripes, and the groups are
+// summed in LDS before the device-scope atomics, which cost about as much as
+// the lookups when every 256 lanes issued their own (m = 32: 2048 work-groups
+// of 256 lanes 0.52 ms, 4096 0.80 ms, 8192 1.39 ms).
+// grid = (k / (4 * LC_I), stripe splits); out must be zeroed.
 // ---------------------------------------------------------------------------
-constexpr int LC_WG = 256;
+constexpr int LC_I = 256;
+constexpr int LC_GROUPS = 4;
+constexpr int LC_WG = LC_I * LC_GROUPS;
 constexpr int LC_CHUNK = 32;
 
 __global__ void __launch_bounds__(LC_WG) lc_partial_fold_kernel(const uint32_t* __restrict__ z, uint64_t k,
                                                                uint64_t n_stripes, uint64_t stripes_per_split,
                                                                const uint4* __restrict__ eq, uint4* __restrict__ out) {
-  __shared__ uint4 eqs[LC_CHUNK * 8];
-  __shared__ uint4 tab[LC_CHUNK][32];  // [stripe][low-nibble table | high-nibble table]
-  const uint32_t tid = threadIdx.x;
-  const uint64_t i4 = ((uint64_t)blockIdx.x * LC_WG + tid);  // index of the 4-byte group
+  __shared__ uint4 eqs[LC_GROUPS][LC_CHUNK * 8];
+  // [group][stripe][low-nibble table | high-nibble table]; reused for the
+  // cross-group sums at the end
+  __shared__ uint4 tab[LC_GROUPS][LC_CHUNK][32];
+  static_assert(LC_CHUNK * 32 >= LC_I * 4, "the group sums must fit in one group's tables");
+  const uint32_t tid = threadIdx.x, li = tid % LC_I, grp = tid / LC_I;
+  const uint64_t i4 = ((uint64_t)blockIdx.x * LC_I + li);  // index of the 4-byte group
   const bool active = i4 * 4 < k;
-  const uint64_t s0 = (uint64_t)blockIdx.y * stripes_per_split;
-  uint64_t s1 = s0 + stripes_per_split;
-  if (s1 > n_stripes) s1 = n_stripes;
+  // This group's stripes: quarter grp of the split.
+  const uint64_t sp0 = (uint64_t)blockIdx.y * stripes_per_split;
+  uint64_t sp1 = sp0 + stripes_per_split;
+  if (sp1 > n_stripes) sp1 = n_stripes;
+  const uint64_t per_g = (stripes_per_split + LC_GROUPS - 1) / LC_GROUPS;
+  uint64_t s0 = sp0 + grp * per_g, s1 = s0 + per_g;
+  if (s0 > sp1) s0 = sp1;
+  if (s1 > sp1) s1 = sp1;
   const uint64_t k4 = k / 4;
   uint4 acc[4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) acc[b] = f_zero();
-  for (uint64_t c0 = s0; c0 < s1; c0 += LC_CHUNK) {
-    const uint32_t nc = (uint32_t)(s1 - c0 < (uint64_t)LC_CHUNK ? s1 - c0 : (uint64_t)LC_CHUNK);
+  // Every group runs the same number of chunk iterations (barriers are
+  // work-group wide); a group whose stripes are used up does empty chunks.
+  const uint64_t n_iter = (per_g + LC_CHUNK - 1) / LC_CHUNK;
+  uint4* const eg = eqs[grp];
+  uint4(*const tg)[32] = tab[grp];
+  for (uint64_t it = 0; it < n_iter; ++it) {
+    const uint64_t c0 = s0 + it * LC_CHUNK;
+    const uint32_t nc = c0 >= s1 ? 0u : (uint32_t)(s1 - c0 < (uint64_t)LC_CHUNK ? s1 - c0 : (uint64_t)LC_CHUNK);
     __syncthreads();  // the previous chunk's lookups are done
-    for (uint32_t e = tid; e < nc * 8; e += LC_WG) eqs[e] = eq[8 * c0 + e];
+    for (uint32_t e = li; e < nc * 8; e += LC_I) eg[e] = eq[8 * c0 + e];
     __syncthreads();
-    for (uint32_t e = tid; e < nc * 32; e += LC_WG) {
+    for (uint32_t e = li; e < nc * 32; e += LC_I) {
       const uint32_t sc = e >> 5, half = (e >> 4) & 1u, kk = e & 15u;
-      const uint4* src = eqs + sc * 8 + 4 * half;
+      const uint4* src = eg + sc * 8 + 4 * half;
       uint4 v = f_zero();
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         if ((kk >> r) & 1u) v = f_add(v, src[r]);
-      tab[sc][e & 31u] = v;
+      tg[sc][e & 31u] = v;
     }
     __syncthreads();
     if (active) {
@@ -106,7 +127,7 @@ __global__ void __launch_bounds__(LC_WG) lc_partial_fold_kernel(const uint32_t* 
 #pragma unroll 8
       for (uint32_t sc = 0; sc < nc; ++sc) {
         const uint32_t w = zc[(uint64_t)sc * k4];
-        const uint4* t = tab[sc];
+        const uint4* t = tg[sc];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
           const uint32_t byte = (w >> (8 * b)) & 255u;
@@ -117,9 +138,23 @@ __global__ void __launch_bounds__(LC_WG) lc_partial_fold_kernel(const uint32_t* 
       }
     }
   }
-  if (active) {
+  // Sum the groups: groups 1.. write their sums into their own tables, group 0
+  // adds them and issues the atomics.
+  __syncthreads();
+  if (grp > 0) {
+    uint4* red = &tab[grp][0][0];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) atomic_xor4(out + i4 * 4 + b, acc[b]);
+    for (int b = 0; b < 4; ++b) red[b * LC_I + li] = acc[b];
+  }
+  __syncthreads();
+  if (grp == 0 && active) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      uint4 v = acc[b];
+#pragma unroll
+      for (int g = 1; g < LC_GROUPS; ++g) v = f_add(v, (&tab[g][0][0])[b * LC_I + li]);
+      atomic_xor4(out + i4 * 4 + b, v);
+    }
   }
 }
 
@@ -371,8 +406,9 @@ int flk_lc_partial_fold(const uint8_t* z_packed, size_t m, size_t k_log, const f
   const uint64_t n_stripes = (uint64_t(1) << (m - k_log)) / 8;
   uint4* o = reinterpret_cast<uint4*>(out);
   if ((rc = zero_n(o, k, st))) return rc;
-  const uint64_t gx = (k / 4 + LC_WG - 1) / LC_WG;
-  uint64_t splits = (2048 + gx - 1) / gx;
+  const uint64_t gx = (k / 4 + LC_I - 1) / LC_I;
+  // ~512 work-groups of 1024 lanes (2 per CU), each split folded by 4 groups
+  uint64_t splits = (512 + gx - 1) / gx;
   if (splits > n_stripes) splits = n_stripes;
   if (splits > 65535) splits = 65535;
   const uint64_t per = (n_stripes + splits - 1) / splits;
