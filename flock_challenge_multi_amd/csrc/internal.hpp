@@ -36,6 +36,16 @@ inline int log2_exact(size_t n) {
     if (e_ != hipSuccess) return ::flk::device_failure(e_, where); \
   } while (0)
 
+// Work-group partial sums of the reduction kernels.  A kernel launched on a grid
+// of G work-groups stores the XOR over its work-group of output k at
+// parts[k * G + blockIdx.x] (plain stores); partials_sum_free then writes
+// out[k] = XOR of the G partials and releases the scratch.  The scratch comes
+// from the stream-ordered pool (hipMallocAsync / hipFreeAsync on the call's
+// stream), so calls on different streams never share it.  Device-scope atomics
+// from every work-group into the same few words cost ~7 ns each, serialised.
+int partials_alloc(uint4** parts, unsigned grid, int n_out, hipStream_t st);
+int partials_sum_free(uint4* parts, unsigned grid, int n_out, uint4* out, hipStream_t st);
+
 // Host-side GF(2^128) product (used for set-up constants only).
 flk_f128 host_mul(flk_f128 a, flk_f128 b);
 flk_f128 host_inv(flk_f128 a);

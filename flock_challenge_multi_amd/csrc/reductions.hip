@@ -17,6 +17,8 @@
 // work-group.  Products of two data values use the per-lane comb of gf128.hpp.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "gf128.hpp"
 #include "internal.hpp"
 
@@ -35,7 +37,9 @@ __device__ __forceinline__ void atomic_xor4(uint4* out, uint4 v) {
 }
 
 // XOR-reduce over the work-group (blockDim multiple of 64) into *out.
-__device__ __forceinline__ void wg_reduce_xor(uint4 v, uint4* red, uint4* out) {
+// XOR-reduce v over the work-group; thread 0 stores it at parts[k * gridDim.x +
+// blockIdx.x] (the work-group partials that partials_sum_free adds up).
+__device__ __forceinline__ void wg_reduce_store(uint4 v, uint4* red, uint4* parts, uint32_t k) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v = f_add(v, shfl_xor4_r(v, m));
   if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = v;
@@ -43,7 +47,7 @@ __device__ __forceinline__ void wg_reduce_xor(uint4 v, uint4* red, uint4* out) {
   if (threadIdx.x == 0) {
     uint4 acc = red[0];
     for (uint32_t w = 1; w < blockDim.x / 64u; ++w) acc = f_add(acc, red[w]);
-    atomic_xor4(out, acc);
+    parts[(size_t)k * gridDim.x + blockIdx.x] = acc;
   }
 }
 
@@ -162,6 +166,17 @@ __global__ void __launch_bounds__(LC_WG) lc_partial_fold_kernel(const uint32_t* 
 // Lincheck product sumcheck, top variable: message and binding.
 // ---------------------------------------------------------------------------
 constexpr int RD_WG = 128;
+// Work-groups of the product-sum kernels (their sums go through partials, so
+// the count costs no atomics): FLOCK_RD_GRID overrides it for sweeps.
+static unsigned rd_grid_cap() {
+  static const unsigned c = [] {
+    const char* v = getenv("FLOCK_RD_GRID");
+    const long k = v ? atol(v) : 8192;
+    return (unsigned)(k < 1 ? 1 : (k > 65535 ? 65535 : k));
+  }();
+  return c;
+}
+#define RD_GRID_CAP rd_grid_cap()
 
 __global__ void __launch_bounds__(RD_WG) lc_round_msg_msb_kernel(const uint4* __restrict__ comb,
                                                                 const uint4* __restrict__ z, uint64_t half,
@@ -179,8 +194,8 @@ __global__ void __launch_bounds__(RD_WG) lc_round_msg_msb_kernel(const uint4* __
     table_store_private(tabs, RD_WG, threadIdx.x, f_add(hi, lo));
     comb_mul_acc_rt(f_add(zh, zl), my, RD_WG, einf);
   }
-  wg_reduce_xor(f_reduce(e1), red[0], out + 0);
-  wg_reduce_xor(f_reduce(einf), red[1], out + 1);
+  wg_reduce_store(f_reduce(e1), red[0], out, 0);
+  wg_reduce_store(f_reduce(einf), red[1], out, 1);
 }
 
 __global__ void __launch_bounds__(256) lc_bind_top_kernel(uint4* __restrict__ v, uint64_t half, uint4 r) {
@@ -373,8 +388,8 @@ __global__ void __launch_bounds__(RD_WG) zc_round_pair_kernel(const uint4* __res
     table_store_private(tabs, RD_WG, threadIdx.x, eas);
     comb_mul_acc_rt(f_add(b0, b1), my, RD_WG, ginf);
   }
-  wg_reduce_xor(f_reduce(g1), red[0], out + 0);
-  wg_reduce_xor(f_reduce(ginf), red[1], out + 1);
+  wg_reduce_store(f_reduce(g1), red[0], out, 0);
+  wg_reduce_store(f_reduce(ginf), red[1], out, 1);
 }
 
 static unsigned grid_for(uint64_t items, int wg, unsigned cap) {
@@ -427,11 +442,13 @@ int flk_lc_round_msg_msb(const flk_f128* comb, const flk_f128* z, size_t n, flk_
   if (!is_pow2(n) || n < 2) return set_error(FLK_E_SHAPE, "lc_round_msg_msb: n=%zu must be a power of two >= 2", n);
   hipStream_t st = static_cast<hipStream_t>(stream);
   uint4* o = reinterpret_cast<uint4*>(out2);
-  if ((rc = zero_n(o, 2, st))) return rc;
-  hipLaunchKernelGGL(lc_round_msg_msb_kernel, dim3(grid_for(n / 2, RD_WG, 2048)), dim3(RD_WG), 0, st,
-                     reinterpret_cast<const uint4*>(comb), reinterpret_cast<const uint4*>(z), (uint64_t)(n / 2), o);
+  const unsigned grid = grid_for(n / 2, RD_WG, RD_GRID_CAP);
+  uint4* parts = nullptr;
+  if ((rc = partials_alloc(&parts, grid, 2, st))) return rc;
+  hipLaunchKernelGGL(lc_round_msg_msb_kernel, dim3(grid), dim3(RD_WG), 0, st, reinterpret_cast<const uint4*>(comb),
+                     reinterpret_cast<const uint4*>(z), (uint64_t)(n / 2), parts);
   FLK_LAUNCH_CHECK("lc_round_msg_msb_kernel");
-  return FLK_OK;
+  return partials_sum_free(parts, grid, 2, o, st);
 }
 
 int flk_lc_bind_top(flk_f128* v, size_t n, flk_f128 r, void* stream) {
@@ -499,12 +516,14 @@ int flk_zc_round_pair(const flk_f128* a, const flk_f128* b, size_t n, const flk_
   if (!is_pow2(n) || n < 2) return set_error(FLK_E_SHAPE, "zc_round_pair: n=%zu must be a power of two >= 2", n);
   hipStream_t st = static_cast<hipStream_t>(stream);
   uint4* o = reinterpret_cast<uint4*>(out2);
-  if ((rc = zero_n(o, 2, st))) return rc;
-  hipLaunchKernelGGL(zc_round_pair_kernel, dim3(grid_for(n / 2, RD_WG, 2048)), dim3(RD_WG), 0, st,
-                     reinterpret_cast<const uint4*>(a), reinterpret_cast<const uint4*>(b),
-                     reinterpret_cast<const uint4*>(eq_rest), (uint64_t)(n / 2), o);
+  const unsigned grid = grid_for(n / 2, RD_WG, RD_GRID_CAP);
+  uint4* parts = nullptr;
+  if ((rc = partials_alloc(&parts, grid, 2, st))) return rc;
+  hipLaunchKernelGGL(zc_round_pair_kernel, dim3(grid), dim3(RD_WG), 0, st, reinterpret_cast<const uint4*>(a),
+                     reinterpret_cast<const uint4*>(b), reinterpret_cast<const uint4*>(eq_rest), (uint64_t)(n / 2),
+                     parts);
   FLK_LAUNCH_CHECK("zc_round_pair_kernel");
-  return FLK_OK;
+  return partials_sum_free(parts, grid, 2, o, st);
 }
 
 }  // extern "C"

@@ -185,6 +185,50 @@ const uint4* twiddles_device(size_t dim, int* status) {
   return d;
 }
 
+// out[k] = XOR of parts[k * grid .. k * grid + grid), one work-group per k.
+__global__ void __launch_bounds__(256) partials_sum_kernel(const uint4* __restrict__ parts, uint32_t grid,
+                                                           uint4* __restrict__ out) {
+  __shared__ uint4 red[4];
+  const uint4* p = parts + (size_t)blockIdx.x * grid;
+  uint4 acc = make_uint4(0u, 0u, 0u, 0u);
+  for (uint32_t i = threadIdx.x; i < grid; i += 256u) {
+    const uint4 v = p[i];
+    acc = make_uint4(acc.x ^ v.x, acc.y ^ v.y, acc.z ^ v.z, acc.w ^ v.w);
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+    acc = make_uint4(acc.x ^ __shfl_xor(acc.x, m), acc.y ^ __shfl_xor(acc.y, m), acc.z ^ __shfl_xor(acc.z, m),
+                     acc.w ^ __shfl_xor(acc.w, m));
+  if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint4 r = red[0];
+    for (int w = 1; w < 4; ++w) r = make_uint4(r.x ^ red[w].x, r.y ^ red[w].y, r.z ^ red[w].z, r.w ^ red[w].w);
+    out[blockIdx.x] = r;
+  }
+}
+
+int partials_alloc(uint4** parts, unsigned grid, int n_out, hipStream_t st) {
+  *parts = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(parts), (size_t)grid * (size_t)n_out * sizeof(uint4), st);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *parts = nullptr;
+    return set_error(FLK_E_NOMEM, "reduction scratch: hipMallocAsync of %zu bytes failed",
+                     (size_t)grid * (size_t)n_out * sizeof(uint4));
+  }
+  return FLK_OK;
+}
+
+int partials_sum_free(uint4* parts, unsigned grid, int n_out, uint4* out, hipStream_t st) {
+  hipLaunchKernelGGL(partials_sum_kernel, dim3((unsigned)n_out), dim3(256), 0, st, parts, (uint32_t)grid, out);
+  hipError_t e = hipGetLastError();
+  const hipError_t ef = hipFreeAsync(parts, st);
+  if (e != hipSuccess) return device_failure(e, "partials_sum_kernel");
+  if (ef != hipSuccess) return device_failure(ef, "reduction scratch: hipFreeAsync");
+  return FLK_OK;
+}
+
 }  // namespace flk
 
 extern "C" {

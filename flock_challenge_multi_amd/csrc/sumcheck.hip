@@ -13,6 +13,8 @@
 // Products by a challenge (r, alpha, eq factors) use one work-group-shared table.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "gf128.hpp"
 #include "internal.hpp"
 
@@ -24,8 +26,9 @@ __device__ __forceinline__ uint4 shfl_xor4(uint4 v, int m) {
   return make_uint4(__shfl_xor(v.x, m), __shfl_xor(v.y, m), __shfl_xor(v.z, m), __shfl_xor(v.w, m));
 }
 
-// XOR-reduce `v` over the work-group and XOR it into *out (4 atomics).
-__device__ __forceinline__ void wg_xor_reduce_to(uint4 v, uint4* red_lds, uint4* out) {
+// XOR-reduce `v` over the work-group and store it at parts[k * gridDim.x +
+// blockIdx.x] (the work-group partials that partials_sum_free adds up).
+__device__ __forceinline__ void wg_xor_reduce_store(uint4 v, uint4* red_lds, uint4* parts, uint32_t k) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v = f_add(v, shfl_xor4(v, m));
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -34,17 +37,8 @@ __device__ __forceinline__ void wg_xor_reduce_to(uint4 v, uint4* red_lds, uint4*
   if (threadIdx.x == 0) {
     uint4 acc = red_lds[0];
     for (uint32_t w = 1; w < (blockDim.x + 63u) / 64u; ++w) acc = f_add(acc, red_lds[w]);
-    uint32_t* o = reinterpret_cast<uint32_t*>(out);
-    atomicXor(o + 0, acc.x);
-    atomicXor(o + 1, acc.y);
-    atomicXor(o + 2, acc.z);
-    atomicXor(o + 3, acc.w);
+    parts[(size_t)k * gridDim.x + blockIdx.x] = acc;
   }
-}
-
-__global__ void zero_kernel(uint4* p, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // Build the shared table of a uniform multiplier t (threads 0..15 write).
@@ -55,6 +49,7 @@ __device__ __forceinline__ void shared_table(uint4* tab, uint4 t) {
 // ---------------------------------------------------------------------------
 // round_msg_lsb / round_msg_and_eval_lsb
 // ---------------------------------------------------------------------------
+// out: work-group partials (2 or 3 outputs x gridDim.x), summed afterwards.
 template <bool EVAL>
 __global__ void __launch_bounds__(SC_WG) round_msg_kernel(const uint4* __restrict__ f, const uint4* __restrict__ b,
                                                          uint64_t n_pairs, uint4* __restrict__ out) {
@@ -78,27 +73,29 @@ __global__ void __launch_bounds__(SC_WG) round_msg_kernel(const uint4* __restric
     }
   }
   const uint4 u0 = f_reduce(a0), u2 = f_reduce(a2);
-  wg_xor_reduce_to(u0, red[0], out + 0);
-  wg_xor_reduce_to(u2, red[1], out + 1);
-  if (EVAL) wg_xor_reduce_to(f_add(u0, f_reduce(a1)), red[2], out + 2);
+  wg_xor_reduce_store(u0, red[0], out, 0);
+  wg_xor_reduce_store(u2, red[1], out, 1);
+  if (EVAL) wg_xor_reduce_store(f_add(u0, f_reduce(a1)), red[2], out, 2);
 }
 
+// Work-groups of the product-sum kernels (sums go through partials, so the
+// count costs no atomics); FLOCK_SC_GRID overrides it for sweeps.
 static unsigned reduce_grid(uint64_t items) {
+  static const uint64_t cap = [] {
+    const char* v = getenv("FLOCK_SC_GRID");
+    const long k = v ? atol(v) : 8192;
+    return (uint64_t)(k < 1 ? 1 : (k > 65535 ? 65535 : k));
+  }();
   uint64_t g = (items + SC_WG - 1) / SC_WG;
-  if (g > 4096) g = 4096;
+  if (g > cap) g = cap;
   if (g == 0) g = 1;
   return (unsigned)g;
-}
-
-static int launch_zero(uint4* p, uint32_t n, hipStream_t st) {
-  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, p, n);
-  FLK_LAUNCH_CHECK("zero_kernel");
-  return FLK_OK;
 }
 
 // ---------------------------------------------------------------------------
 // fold_and_msg_lsb: one lane owns one output pair (4 inputs of f and b).
 // ---------------------------------------------------------------------------
+// out: work-group partials (2 outputs x gridDim.x), summed afterwards.
 __global__ void __launch_bounds__(SC_WG) fold_and_msg_kernel(const uint4* __restrict__ f, const uint4* __restrict__ b,
                                                             uint64_t n_quads, uint4 r, uint4* __restrict__ f_out,
                                                             uint4* __restrict__ b_out, uint4* __restrict__ out) {
@@ -130,8 +127,8 @@ __global__ void __launch_bounds__(SC_WG) fold_and_msg_kernel(const uint4* __rest
     table_store_private(tabs, SC_WG, tid, f_add(nf0, nf1));
     comb_mul_acc_rt(f_add(nb0, nb1), my, SC_WG, a2);
   }
-  wg_xor_reduce_to(f_reduce(a0), red[0], out + 0);
-  wg_xor_reduce_to(f_reduce(a2), red[1], out + 1);
+  wg_xor_reduce_store(f_reduce(a0), red[0], out, 0);
+  wg_xor_reduce_store(f_reduce(a2), red[1], out, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -225,12 +222,14 @@ int flk_sc_round_msg(const flk_f128* f, const flk_f128* b, size_t n, flk_f128* o
   if (!flk::is_pow2(n) || n < 2) return flk::set_error(FLK_E_SHAPE, "round_msg: n=%zu must be a power of two >= 2", n);
   hipStream_t st = static_cast<hipStream_t>(stream);
   uint4* o = reinterpret_cast<uint4*>(out2);
-  if ((rc = flk::launch_zero(o, 2, st))) return rc;
   const uint64_t pairs = n / 2;
-  hipLaunchKernelGGL(flk::round_msg_kernel<false>, dim3(flk::reduce_grid(pairs)), dim3(SC_WG), 0, st,
-                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, o);
+  const unsigned grid = flk::reduce_grid(pairs);
+  uint4* parts = nullptr;
+  if ((rc = flk::partials_alloc(&parts, grid, 2, st))) return rc;
+  hipLaunchKernelGGL(flk::round_msg_kernel<false>, dim3(grid), dim3(SC_WG), 0, st,
+                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, parts);
   FLK_LAUNCH_CHECK("round_msg_kernel");
-  return FLK_OK;
+  return flk::partials_sum_free(parts, grid, 2, o, st);
 }
 
 int flk_sc_round_msg_and_eval(const flk_f128* f, const flk_f128* b, size_t n, flk_f128* out3, void* stream) {
@@ -241,12 +240,14 @@ int flk_sc_round_msg_and_eval(const flk_f128* f, const flk_f128* b, size_t n, fl
     return flk::set_error(FLK_E_SHAPE, "round_msg_and_eval: n=%zu must be a power of two >= 2", n);
   hipStream_t st = static_cast<hipStream_t>(stream);
   uint4* o = reinterpret_cast<uint4*>(out3);
-  if ((rc = flk::launch_zero(o, 3, st))) return rc;
   const uint64_t pairs = n / 2;
-  hipLaunchKernelGGL(flk::round_msg_kernel<true>, dim3(flk::reduce_grid(pairs)), dim3(SC_WG), 0, st,
-                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, o);
+  const unsigned grid = flk::reduce_grid(pairs);
+  uint4* parts = nullptr;
+  if ((rc = flk::partials_alloc(&parts, grid, 3, st))) return rc;
+  hipLaunchKernelGGL(flk::round_msg_kernel<true>, dim3(grid), dim3(SC_WG), 0, st,
+                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, parts);
   FLK_LAUNCH_CHECK("round_msg_kernel<eval>");
-  return FLK_OK;
+  return flk::partials_sum_free(parts, grid, 3, o, st);
 }
 
 int flk_sc_fold_and_msg(const flk_f128* f, const flk_f128* b, size_t n, flk_f128 r, flk_f128* f_out, flk_f128* b_out,
@@ -257,13 +258,15 @@ int flk_sc_fold_and_msg(const flk_f128* f, const flk_f128* b, size_t n, flk_f128
   if (!flk::is_pow2(n) || n < 4) return flk::set_error(FLK_E_SHAPE, "fold_and_msg: n=%zu must be a power of two >= 4", n);
   hipStream_t st = static_cast<hipStream_t>(stream);
   uint4* o = reinterpret_cast<uint4*>(out2);
-  if ((rc = flk::launch_zero(o, 2, st))) return rc;
   const uint64_t quads = n / 4;
-  hipLaunchKernelGGL(flk::fold_and_msg_kernel, dim3(flk::reduce_grid(quads)), dim3(SC_WG), 0, st,
+  const unsigned grid = flk::reduce_grid(quads);
+  uint4* parts = nullptr;
+  if ((rc = flk::partials_alloc(&parts, grid, 2, st))) return rc;
+  hipLaunchKernelGGL(flk::fold_and_msg_kernel, dim3(grid), dim3(SC_WG), 0, st,
                      reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), quads, flk::to_u4(r),
-                     reinterpret_cast<uint4*>(f_out), reinterpret_cast<uint4*>(b_out), o);
+                     reinterpret_cast<uint4*>(f_out), reinterpret_cast<uint4*>(b_out), parts);
   FLK_LAUNCH_CHECK("fold_and_msg_kernel");
-  return FLK_OK;
+  return flk::partials_sum_free(parts, grid, 2, o, st);
 }
 
 int flk_sc_glue(flk_f128* b, const flk_f128* b_new, size_t n, flk_f128 alpha, void* stream) {
