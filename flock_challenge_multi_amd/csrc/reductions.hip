@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(256) lc_bind_top_kernel(uint4* __restrict__ v,
 // 16-entry subset-sum table of its eq values.  The tile's bits are transposed
 // once into nibbles nib[g][r] = (bit r of the group's 4 elements), packed
 // eight to a word, so lane (r, half) reads one word per group and looks its
-// nibble up.  out must be zeroed.
+// nibble up.  out: work-group partials, out[r * gridDim.x + blockIdx.x].
 // ---------------------------------------------------------------------------
 constexpr int F1_WG = 256;
 
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_kernel(const uint4* __r
   }
   if (halfsel == 1) red[r] = acc;
   __syncthreads();
-  if (halfsel == 0) atomic_xor4(out + r, f_add(acc, red[r]));
+  if (halfsel == 0) out[(size_t)r * gridDim.x + blockIdx.x] = f_add(acc, red[r]);
 }
 
 // ---------------------------------------------------------------------------
@@ -469,13 +469,15 @@ int flk_rs_fold_1b_rows(const flk_f128* packed_witness, const flk_f128* suffix_t
   if (!packed_witness || !suffix_tensor || !out128) return set_error(FLK_E_SHAPE, "rs_fold_1b_rows: null pointer");
   hipStream_t st = static_cast<hipStream_t>(stream);
   uint4* o = reinterpret_cast<uint4*>(out128);
-  if ((rc = zero_n(o, 128, st))) return rc;
-  if (n == 0) return FLK_OK;
-  hipLaunchKernelGGL(rs_fold_1b_rows_kernel, dim3(grid_for(n, F1_WG, 2048)), dim3(F1_WG), 0, st,
+  if (n == 0) return zero_n(o, 128, st);
+  const unsigned grid = grid_for(n, F1_WG, 2048);
+  uint4* parts = nullptr;
+  if ((rc = partials_alloc(&parts, grid, 128, st))) return rc;
+  hipLaunchKernelGGL(rs_fold_1b_rows_kernel, dim3(grid), dim3(F1_WG), 0, st,
                      reinterpret_cast<const uint4*>(packed_witness), reinterpret_cast<const uint4*>(suffix_tensor),
-                     (uint64_t)n, o);
+                     (uint64_t)n, parts);
   FLK_LAUNCH_CHECK("rs_fold_1b_rows_kernel");
-  return FLK_OK;
+  return partials_sum_free(parts, grid, 128, o, st);
 }
 
 int flk_rs_fold_b128_elems(const flk_f128* suffix_tensor, const flk_f128* eq_r_dprime, size_t n, flk_f128* out,

@@ -203,20 +203,13 @@ constexpr int R1_ROWS = 128;             // rows of one block (one x_outer)
 constexpr int R1_SUB = 3;                // blocks a work-group processes per iteration
 constexpr int R1_WG = R1_ROWS * R1_SUB;  // 6 waves; ~75 KiB of LDS -> 2 work-groups per CU
 
-__device__ __forceinline__ void atomic_xor4_r1(uint4* out, uint4 v) {
-  uint32_t* o = reinterpret_cast<uint32_t*>(out);
-  atomicXor(o + 0, v.x);
-  atomicXor(o + 1, v.y);
-  atomicXor(o + 2, v.z);
-  atomicXor(o + 3, v.w);
-}
-
 __device__ __forceinline__ uint32_t xtime4(uint32_t w) {
   return ((w & 0x7F7F7F7Fu) << 1) ^ (((w >> 7) & 0x01010101u) * 0x1Bu);
 }
 
 // Work-group = R1_SUB blocks side by side (lanes sub * 128 + t), sharing the
 // tables; each block follows the phases described at the top of the file.
+// out: work-group partials, out[k * gridDim.x + blockIdx.x] for the 128 outputs k.
 __global__ void __launch_bounds__(R1_WG) zc_round1_kernel(const uint64_t* __restrict__ A, const uint64_t* __restrict__ B,
                                                          const uint64_t* __restrict__ C, uint64_t n_blocks,
                                                          const uint4* __restrict__ eq_outer,
@@ -233,6 +226,7 @@ __global__ void __launch_bounds__(R1_WG) zc_round1_kernel(const uint64_t* __rest
   // 2a has read it, the same memory holds the partial sums ypart[(q*4 + j)*YP + t].
   constexpr uint32_t RP = 33, YP = R1_ROWS + 16;  // pitches keep every access free of bank conflicts
   static_assert(20 * YP <= R1_ROWS * RP, "ypart must fit in rowdata");
+  static_assert((R1_SUB - 1) * R1_ROWS * 4 <= R1_ROWS * RP, "the sub-block sums must fit in rowdata[0]");
   __shared__ uint32_t rowdata[R1_SUB][R1_ROWS * RP];
   __shared__ uint4 etab[R1_SUB][32];  // eq_outer * n and eq_outer * x^4 * n (comb4p_mul_add)
   const uint32_t tid = threadIdx.x, sub = tid / R1_ROWS, t = tid % R1_ROWS;
@@ -363,7 +357,16 @@ __global__ void __launch_bounds__(R1_WG) zc_round1_kernel(const uint64_t* __rest
     }
     __syncthreads();
   }
-  atomic_xor4_r1(out + t, acc);  // out[which * 64 + l], t = which * 64 + l
+  // Sum the sub-blocks' accumulators for output t = which * 64 + l, then store
+  // the work-group's partial at out[t * gridDim.x + blockIdx.x].
+  uint4* const red = reinterpret_cast<uint4*>(rowdata[0]);
+  if (sub > 0) red[(sub - 1) * R1_ROWS + t] = acc;
+  __syncthreads();
+  if (sub == 0) {
+#pragma unroll
+    for (int k = 1; k < R1_SUB; ++k) acc = f_add(acc, red[(k - 1) * R1_ROWS + t]);
+    out[(size_t)t * gridDim.x + blockIdx.x] = acc;
+  }
 }
 
 // P_C[l] = sum_s phi8(M[s][l]) * Q[s] for the 64 accumulated Q values in out_c,
@@ -396,8 +399,6 @@ __global__ void scale_kernel(uint4* __restrict__ v, uint32_t n, uint4 c) {
   if (threadIdx.x < n) v[threadIdx.x] = comb_mul<1>(v[threadIdx.x], tab);
 }
 
-__global__ void zero128_kernel(uint4* p) { p[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u); }
-
 }  // namespace flk
 
 
@@ -425,16 +426,20 @@ extern "C" int flk_zc_round1(const uint8_t* a_bits, const uint8_t* b_bits, const
   rc = flk_build_eq(r_outer, m - 13, reinterpret_cast<flk_f128*>(eq), stream);
   if (rc) return rc;
   uint4* out = reinterpret_cast<uint4*>(out_ab);
-  hipLaunchKernelGGL(zero128_kernel, dim3(1), dim3(128), 0, st, out);
-  FLK_LAUNCH_CHECK("zero128_kernel");
   // Two work-groups per CU (LDS): 512 fill the chip; 1024 leave room for
   // uneven block counts per work-group.
   const uint64_t n_iter = (n_outer + R1_SUB - 1) / R1_SUB;
-  const uint64_t grid = n_iter < 1024 ? n_iter : 1024;
-  hipLaunchKernelGGL(zc_round1_kernel, dim3((unsigned)grid), dim3(R1_WG), 0, st,
-                     reinterpret_cast<const uint64_t*>(a_bits), reinterpret_cast<const uint64_t*>(b_bits),
-                     reinterpret_cast<const uint64_t*>(c_bits), (uint64_t)n_outer, eq, T, out);
+  const unsigned grid = (unsigned)(n_iter < 1024 ? n_iter : 1024);
+  uint4* parts = nullptr;
+  if ((rc = partials_alloc(&parts, grid, 128, st))) {
+    (void)hipFreeAsync(eq, st);
+    return rc;
+  }
+  hipLaunchKernelGGL(zc_round1_kernel, dim3(grid), dim3(R1_WG), 0, st, reinterpret_cast<const uint64_t*>(a_bits),
+                     reinterpret_cast<const uint64_t*>(b_bits), reinterpret_cast<const uint64_t*>(c_bits),
+                     (uint64_t)n_outer, eq, T, parts);
   FLK_LAUNCH_CHECK("zc_round1_kernel");
+  if ((rc = partials_sum_free(parts, grid, 128, out, st))) return rc;
   hipLaunchKernelGGL(zc_round1_lift_c_kernel, dim3(1), dim3(64), 0, st, out + 64, T);
   FLK_LAUNCH_CHECK("zc_round1_lift_c_kernel");
   const flk_f128 sc = round1_scale();
