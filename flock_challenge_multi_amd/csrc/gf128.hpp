@@ -335,6 +335,52 @@ __device__ __forceinline__ uint4 comb4p_mul_add(uint4 v, uint4 add, const uint4*
   return f_reduce_add(a, add);
 }
 
+// The 32 table offsets of v for the nibble-position combs, top word first:
+// ad[8 * h + j] = 16 * (nibble j of word 3 - h of v).
+__device__ __forceinline__ void pos_offsets(uint4 v, uint32_t ad[32]) {
+  const uint32_t vw[4] = {v.w, v.z, v.y, v.x};
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    const uint32_t hi = vw[h], lo = hi << 4;
+    ad[8 * h + 0] = hinib_x16<0>(lo);
+    ad[8 * h + 1] = hinib_x16<0>(hi);
+    ad[8 * h + 2] = hinib_x16<1>(lo);
+    ad[8 * h + 3] = hinib_x16<1>(hi);
+    ad[8 * h + 4] = hinib_x16<2>(lo);
+    ad[8 * h + 5] = hinib_x16<2>(hi);
+    ad[8 * h + 6] = hinib_x16<3>(lo);
+    ad[8 * h + 7] = hinib_x16<3>(hi);
+  }
+}
+
+// v * t from v's precomputed offsets (pos_offsets) and t's nibble-position
+// tables (tab[j * 16 + k] = k * t * x^(4j)): the combjr_mul_add product for
+// several t with one v.
+__device__ __forceinline__ uint4 combjr_mul_pre(const uint32_t ad[32], const uint4* tab) {
+  uint32_t a[7];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) a[i] = 0u;
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+#pragma unroll
+    for (int i = 6; i >= 1; --i) a[i] = a[i - 1];
+    const uint4 e0 = lds_entry(tab + 0 * 16, ad[8 * h + 0]);
+    const uint4 e1 = lds_entry(tab + 1 * 16, ad[8 * h + 1]);
+    const uint4 e2 = lds_entry(tab + 2 * 16, ad[8 * h + 2]);
+    const uint4 e3 = lds_entry(tab + 3 * 16, ad[8 * h + 3]);
+    const uint4 e4 = lds_entry(tab + 4 * 16, ad[8 * h + 4]);
+    const uint4 e5 = lds_entry(tab + 5 * 16, ad[8 * h + 5]);
+    const uint4 e6 = lds_entry(tab + 6 * 16, ad[8 * h + 6]);
+    const uint4 e7 = lds_entry(tab + 7 * 16, ad[8 * h + 7]);
+    a[0] = xor3(xor3(xor3(e0.x, e1.x, e2.x), e3.x, e4.x), e5.x, e6.x) ^ e7.x;
+    a[1] = xor3(xor3(xor3(xor3(a[1], e0.y, e1.y), e2.y, e3.y), e4.y, e5.y), e6.y, e7.y);
+    a[2] = xor3(xor3(xor3(xor3(a[2], e0.z, e1.z), e2.z, e3.z), e4.z, e5.z), e6.z, e7.z);
+    a[3] = xor3(xor3(xor3(xor3(a[3], e0.w, e1.w), e2.w, e3.w), e4.w, e5.w), e6.w, e7.w);
+  }
+  const uint32_t r[8] = {a[0], a[1], a[2], a[3], a[4], a[5], a[6], 0u};
+  return f_reduce_add(r, f_zero());
+}
+
 // XOR of tab[j * 16 + nibble j of w] over the eight nibbles j of w: one word of
 // a GF(2)-linear map applied with 16-entry subset-sum tables (conflict-free
 // lookups, one address instruction each).

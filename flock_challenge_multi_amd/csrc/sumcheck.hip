@@ -170,14 +170,28 @@ __global__ void __launch_bounds__(256) eq_seed_kernel(uint4* __restrict__ out, E
 }
 
 // out[i + k*len] = out[i] * E[k] for k < 2^t, i < len (in place; reads [0,len)).
+// The 2^t factors get nibble-position tables (2 KiB each); each lane computes
+// its value's 32 table offsets once and runs the 2^t shift-free products.
 __global__ void __launch_bounds__(256) eq_expand_kernel(uint4* __restrict__ out, uint64_t len, uint32_t t, EqFactors E) {
-  __shared__ uint4 tabs[16 * 16];
-  for (uint32_t e = threadIdx.x; e < (16u << t); e += 256) tabs[e] = comb_entry(E.e[e >> 4], e & 15u);
+  __shared__ uint4 nib[16 * 32];    // E[c] * x^s
+  __shared__ uint4 tabs[16 * 128];  // [c][j][k] = k * E[c] * x^(4j)
+  const uint32_t nk = 1u << t;
+  for (uint32_t e = threadIdx.x; e < nk * 32u; e += 256) nib[e] = f_mul_xs(E.e[e >> 5], e & 31u);
   __syncthreads();
-  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= len) return;
-  const uint4 v = out[i];
-  for (uint32_t k = 0; k < (1u << t); ++k) out[i + k * len] = comb_mul<1>(v, tabs + 16 * k);
+  for (uint32_t e = threadIdx.x; e < nk * 128u; e += 256) {
+    const uint32_t c = e >> 7, j = (e >> 4) & 7u, k = e & 15u;
+    uint4 v = f_zero();
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if ((k >> b) & 1u) v = f_add(v, nib[c * 32 + 4 * j + b]);
+    tabs[e] = v;
+  }
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < len; i += (uint64_t)gridDim.x * 256) {
+    uint32_t ad[32];
+    pos_offsets(out[i], ad);
+    for (uint32_t k = 0; k < nk; ++k) out[i + k * len] = combjr_mul_pre(ad, tabs + 128 * k);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -317,8 +331,10 @@ int flk_build_eq(const flk_f128* point, size_t d, flk_f128* out, void* stream) {
       E.e[k] = flk::to_u4(acc);
     }
     const uint64_t len = uint64_t(1) << done;
-    hipLaunchKernelGGL(flk::eq_expand_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, o, len, (uint32_t)t,
-                       E);
+    // Each work-group builds 2^t tables of 2 KiB, so it strides over several
+    // inputs: at most 1024 work-groups.
+    const uint64_t blocks = (len + 255) / 256 < 1024 ? (len + 255) / 256 : 1024;
+    hipLaunchKernelGGL(flk::eq_expand_kernel, dim3((unsigned)blocks), dim3(256), 0, st, o, len, (uint32_t)t, E);
     FLK_LAUNCH_CHECK("eq_expand_kernel");
     done += t;
   }
