@@ -335,6 +335,25 @@ __device__ __forceinline__ uint4 comb4p_mul_add(uint4 v, uint4 add, const uint4*
   return f_reduce_add(a, add);
 }
 
+// Work-group build of the nibble-position tables of n constants t[c] in LDS
+// (the combjr_mul_add layout: tab[c * 128 + j * 16 + k] = k * t[c] * x^(4j)),
+// through nib[c * 32 + s] = t[c] * x^s.  get(c) returns t[c].  Contains two
+// barriers; every thread of the work-group must call it.
+template <typename Get>
+__device__ __forceinline__ void build_pos_tables(uint4* tab, uint4* nib, uint32_t n, Get get) {
+  for (uint32_t e = threadIdx.x; e < n * 32u; e += blockDim.x) nib[e] = f_mul_xs(get(e >> 5), e & 31u);
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < n * 128u; e += blockDim.x) {
+    const uint32_t c = e >> 7, j = (e >> 4) & 7u, k = e & 15u;
+    uint4 v = f_zero();
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if ((k >> b) & 1u) v = f_add(v, nib[c * 32 + 4 * j + b]);
+    tab[e] = v;
+  }
+  __syncthreads();
+}
+
 // The 32 table offsets of v for the nibble-position combs, top word first:
 // ad[8 * h + j] = 16 * (nibble j of word 3 - h of v).
 __device__ __forceinline__ void pos_offsets(uint4 v, uint32_t ad[32]) {

@@ -199,12 +199,12 @@ __global__ void __launch_bounds__(RD_WG) lc_round_msg_msb_kernel(const uint4* __
 }
 
 __global__ void __launch_bounds__(256) lc_bind_top_kernel(uint4* __restrict__ v, uint64_t half, uint4 r) {
-  __shared__ uint4 rt[16];
-  if (threadIdx.x < 16) rt[threadIdx.x] = comb_entry(r, threadIdx.x);
-  __syncthreads();
+  __shared__ uint4 rt[128];  // nibble-position tables of r
+  __shared__ uint4 rnib[32];
+  build_pos_tables(rt, rnib, 1, [&](uint32_t) { return r; });
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < half; i += (uint64_t)gridDim.x * 256) {
     const uint4 lo = v[i], hi = v[i + half];
-    v[i] = f_add(lo, comb_mul<1>(f_add(hi, lo), rt));
+    v[i] = combjr_mul_add(f_add(hi, lo), lo, rt);
   }
 }
 

@@ -41,11 +41,6 @@ __device__ __forceinline__ void wg_xor_reduce_store(uint4 v, uint4* red_lds, uin
   }
 }
 
-// Build the shared table of a uniform multiplier t (threads 0..15 write).
-__device__ __forceinline__ void shared_table(uint4* tab, uint4 t) {
-  if (threadIdx.x < 16) tab[threadIdx.x] = comb_entry(t, threadIdx.x);
-}
-
 // ---------------------------------------------------------------------------
 // round_msg_lsb / round_msg_and_eval_lsb
 // ---------------------------------------------------------------------------
@@ -100,11 +95,11 @@ __global__ void __launch_bounds__(SC_WG) fold_and_msg_kernel(const uint4* __rest
                                                             uint64_t n_quads, uint4 r, uint4* __restrict__ f_out,
                                                             uint4* __restrict__ b_out, uint4* __restrict__ out) {
   __shared__ uint4 tabs[16 * SC_WG];
-  __shared__ uint4 rtab[16];
+  __shared__ uint4 rtab[128];  // nibble-position tables of r
+  __shared__ uint4 rnib[32];
   __shared__ uint4 red[2][SC_WG / 64];
   const uint32_t tid = threadIdx.x;
-  shared_table(rtab, r);
-  __syncthreads();
+  build_pos_tables(rtab, rnib, 1, [&](uint32_t) { return r; });
   U256 a0, a2;
   u256_zero(a0);
   u256_zero(a2);
@@ -114,10 +109,10 @@ __global__ void __launch_bounds__(SC_WG) fold_and_msg_kernel(const uint4* __rest
     const uint4* bp = b + 4 * q;
     const uint4 fa = fp[0], fb = fp[1], fc = fp[2], fd = fp[3];
     const uint4 ba = bp[0], bb = bp[1], bc = bp[2], bd = bp[3];
-    const uint4 nf0 = f_add(fa, comb_mul<1>(f_add(fa, fb), rtab));
-    const uint4 nf1 = f_add(fc, comb_mul<1>(f_add(fc, fd), rtab));
-    const uint4 nb0 = f_add(ba, comb_mul<1>(f_add(ba, bb), rtab));
-    const uint4 nb1 = f_add(bc, comb_mul<1>(f_add(bc, bd), rtab));
+    const uint4 nf0 = combjr_mul_add(f_add(fa, fb), fa, rtab);
+    const uint4 nf1 = combjr_mul_add(f_add(fc, fd), fc, rtab);
+    const uint4 nb0 = combjr_mul_add(f_add(ba, bb), ba, rtab);
+    const uint4 nb1 = combjr_mul_add(f_add(bc, bd), bc, rtab);
     f_out[2 * q] = nf0;
     f_out[2 * q + 1] = nf1;
     b_out[2 * q] = nb0;
@@ -136,11 +131,11 @@ __global__ void __launch_bounds__(SC_WG) fold_and_msg_kernel(const uint4* __rest
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) glue_kernel(uint4* __restrict__ b, const uint4* __restrict__ bn, uint64_t n,
                                                   uint4 alpha) {
-  __shared__ uint4 atab[16];
-  shared_table(atab, alpha);
-  __syncthreads();
+  __shared__ uint4 atab[128];  // nibble-position tables of alpha
+  __shared__ uint4 anib[32];
+  build_pos_tables(atab, anib, 1, [&](uint32_t) { return alpha; });
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
-    b[i] = f_add(b[i], comb_mul<1>(bn[i], atab));
+    b[i] = combjr_mul_add(bn[i], b[i], atab);
 }
 
 // ---------------------------------------------------------------------------
@@ -200,9 +195,9 @@ __global__ void __launch_bounds__(256) eq_expand_kernel(uint4* __restrict__ out,
 template <int T>
 __global__ void __launch_bounds__(256) fold_pass_kernel(const uint4* __restrict__ in, uint4* __restrict__ out,
                                                        uint64_t n_out, EqFactors rs) {
-  __shared__ uint4 tabs[16 * T];
-  for (uint32_t e = threadIdx.x; e < 16u * T; e += 256) tabs[e] = comb_entry(rs.e[e >> 4], e & 15u);
-  __syncthreads();
+  __shared__ uint4 tabs[128 * T];  // nibble-position tables of the T challenges
+  __shared__ uint4 nib[32 * T];
+  build_pos_tables(tabs, nib, T, [&](uint32_t c) { return rs.e[c]; });
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n_out) return;
   uint4 x[1 << T];
@@ -213,7 +208,7 @@ __global__ void __launch_bounds__(256) fold_pass_kernel(const uint4* __restrict_
 #pragma unroll
     for (int j = 0; j < (1 << (T - 1 - t)); ++j) {
       const uint4 e0 = x[2 * j], e1 = x[2 * j + 1];
-      x[j] = f_add(e0, comb_mul<1>(f_add(e0, e1), tabs + 16 * t));
+      x[j] = combjr_mul_add(f_add(e0, e1), e0, tabs + 128 * t);
     }
   }
   out[i] = x[0];
