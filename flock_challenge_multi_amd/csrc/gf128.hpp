@@ -354,6 +354,30 @@ __device__ __forceinline__ void build_pos_tables(uint4* tab, uint4* nib, uint32_
   __syncthreads();
 }
 
+// acc += v * t unreduced (seven words, bits 0..223), with t's nibble-position
+// tables (tab[j * 16 + k] = k * t * x^(4j)): word w of v contributes its eight
+// entries at word offset w, so sums of products need no shifts and one
+// reduction at the end (f_reduce of the seven words and a zero eighth).
+__device__ __forceinline__ void combjr_acc(uint4 v, const uint4* tab, uint32_t acc[7]) {
+  const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const uint32_t hi = vw[w], lo = hi << 4;
+    const uint4 e0 = lds_entry(tab + 0 * 16, hinib_x16<0>(lo));
+    const uint4 e1 = lds_entry(tab + 1 * 16, hinib_x16<0>(hi));
+    const uint4 e2 = lds_entry(tab + 2 * 16, hinib_x16<1>(lo));
+    const uint4 e3 = lds_entry(tab + 3 * 16, hinib_x16<1>(hi));
+    const uint4 e4 = lds_entry(tab + 4 * 16, hinib_x16<2>(lo));
+    const uint4 e5 = lds_entry(tab + 5 * 16, hinib_x16<2>(hi));
+    const uint4 e6 = lds_entry(tab + 6 * 16, hinib_x16<3>(lo));
+    const uint4 e7 = lds_entry(tab + 7 * 16, hinib_x16<3>(hi));
+    acc[w + 0] = xor3(xor3(xor3(xor3(acc[w + 0], e0.x, e1.x), e2.x, e3.x), e4.x, e5.x), e6.x, e7.x);
+    acc[w + 1] = xor3(xor3(xor3(xor3(acc[w + 1], e0.y, e1.y), e2.y, e3.y), e4.y, e5.y), e6.y, e7.y);
+    acc[w + 2] = xor3(xor3(xor3(xor3(acc[w + 2], e0.z, e1.z), e2.z, e3.z), e4.z, e5.z), e6.z, e7.z);
+    acc[w + 3] = xor3(xor3(xor3(xor3(acc[w + 3], e0.w, e1.w), e2.w, e3.w), e4.w, e5.w), e6.w, e7.w);
+  }
+}
+
 // The 32 table offsets of v for the nibble-position combs, top word first:
 // ad[8 * h + j] = 16 * (nibble j of word 3 - h of v).
 __device__ __forceinline__ void pos_offsets(uint4 v, uint32_t ad[32]) {

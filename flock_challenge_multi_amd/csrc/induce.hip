@@ -74,26 +74,35 @@ template <int ITEMS>
 __global__ void __launch_bounds__(IND_WG) induce_basis_kernel(const uint4* __restrict__ lo, const uint4* __restrict__ hi,
                                                              uint32_t nq, uint32_t s_lo, uint32_t s_hi,
                                                              uint4* __restrict__ basis) {
-  __shared__ uint4 tab[2][16];
+  constexpr uint32_t QC = 8;  // queries per table build
+  __shared__ uint4 tab[QC * 128];  // nibble-position tables of hi[q][jh], q in the chunk
+  __shared__ uint4 nib[QC * 32];
   const uint32_t jh = blockIdx.x;
   const uint32_t n_lo = 1u << s_lo;
-  U256 acc[ITEMS];
+  uint32_t acc[ITEMS][7];
 #pragma unroll
-  for (int t = 0; t < ITEMS; ++t) u256_zero(acc[t]);
-  for (uint32_t q = 0; q < nq; ++q) {
-    uint4* tb = tab[q & 1];
-    if (threadIdx.x < 16) tb[threadIdx.x] = comb_entry(hi[((size_t)q << s_hi) + jh], threadIdx.x);
-    __syncthreads();
+  for (int t = 0; t < ITEMS; ++t)
 #pragma unroll
-    for (int t = 0; t < ITEMS; ++t) {
-      const uint32_t jl = threadIdx.x + t * IND_WG;
-      if (jl < n_lo) comb_mul_acc<1>(lo[((size_t)q << s_lo) + jl], tb, acc[t]);
+    for (int w = 0; w < 7; ++w) acc[t][w] = 0u;
+  for (uint32_t q0 = 0; q0 < nq; q0 += QC) {
+    const uint32_t nc = nq - q0 < QC ? nq - q0 : QC;
+    __syncthreads();  // the previous chunk's lookups are done
+    build_pos_tables(tab, nib, nc, [&](uint32_t c) { return hi[((size_t)(q0 + c) << s_hi) + jh]; });
+    for (uint32_t c = 0; c < nc; ++c) {
+#pragma unroll
+      for (int t = 0; t < ITEMS; ++t) {
+        const uint32_t jl = threadIdx.x + t * IND_WG;
+        if (jl < n_lo) combjr_acc(lo[((size_t)(q0 + c) << s_lo) + jl], tab + 128 * c, acc[t]);
+      }
     }
   }
 #pragma unroll
   for (int t = 0; t < ITEMS; ++t) {
     const uint32_t jl = threadIdx.x + t * IND_WG;
-    if (jl < n_lo) basis[((size_t)jh << s_lo) + jl] = f_reduce(acc[t]);
+    if (jl < n_lo) {
+      const uint32_t r[8] = {acc[t][0], acc[t][1], acc[t][2], acc[t][3], acc[t][4], acc[t][5], acc[t][6], 0u};
+      basis[((size_t)jh << s_lo) + jl] = f_reduce_add(r, f_zero());
+    }
   }
 }
 
