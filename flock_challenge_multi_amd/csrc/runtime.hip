@@ -185,13 +185,16 @@ const uint4* twiddles_device(size_t dim, int* status) {
   return d;
 }
 
-// out[k] = XOR of parts[k * grid .. k * grid + grid), one work-group per k.
-__global__ void __launch_bounds__(256) partials_sum_kernel(const uint4* __restrict__ parts, uint32_t grid,
-                                                           uint4* __restrict__ out) {
-  __shared__ uint4 red[4];
+// out[k] = XOR of parts[k * grid .. k * grid + grid), one work-group of 1024
+// lanes per k; each lane's loads are independent (issued back to back).
+constexpr int PSUM_WG = 1024;
+__global__ void __launch_bounds__(PSUM_WG) partials_sum_kernel(const uint4* __restrict__ parts, uint32_t grid,
+                                                               uint4* __restrict__ out) {
+  __shared__ uint4 red[PSUM_WG / 64];
   const uint4* p = parts + (size_t)blockIdx.x * grid;
   uint4 acc = make_uint4(0u, 0u, 0u, 0u);
-  for (uint32_t i = threadIdx.x; i < grid; i += 256u) {
+#pragma unroll 8
+  for (uint32_t i = threadIdx.x; i < grid; i += PSUM_WG) {
     const uint4 v = p[i];
     acc = make_uint4(acc.x ^ v.x, acc.y ^ v.y, acc.z ^ v.z, acc.w ^ v.w);
   }
@@ -203,7 +206,7 @@ __global__ void __launch_bounds__(256) partials_sum_kernel(const uint4* __restri
   __syncthreads();
   if (threadIdx.x == 0) {
     uint4 r = red[0];
-    for (int w = 1; w < 4; ++w) r = make_uint4(r.x ^ red[w].x, r.y ^ red[w].y, r.z ^ red[w].z, r.w ^ red[w].w);
+    for (int w = 1; w < PSUM_WG / 64; ++w) r = make_uint4(r.x ^ red[w].x, r.y ^ red[w].y, r.z ^ red[w].z, r.w ^ red[w].w);
     out[blockIdx.x] = r;
   }
 }
@@ -221,7 +224,7 @@ int partials_alloc(uint4** parts, unsigned grid, int n_out, hipStream_t st) {
 }
 
 int partials_sum_free(uint4* parts, unsigned grid, int n_out, uint4* out, hipStream_t st) {
-  hipLaunchKernelGGL(partials_sum_kernel, dim3((unsigned)n_out), dim3(256), 0, st, parts, (uint32_t)grid, out);
+  hipLaunchKernelGGL(partials_sum_kernel, dim3((unsigned)n_out), dim3(PSUM_WG), 0, st, parts, (uint32_t)grid, out);
   hipError_t e = hipGetLastError();
   const hipError_t ef = hipFreeAsync(parts, st);
   if (e != hipSuccess) return device_failure(e, "partials_sum_kernel");
