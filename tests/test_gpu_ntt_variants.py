@@ -1,0 +1,34 @@
+"""The NTT product-kernel variants behind the library's environment switches
+(read once per process) against the oracle: each setting runs
+tests/ntt_variant_check.py in a child process of its own, one after another."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+CHECK = os.path.join(HERE, "ntt_variant_check.py")
+
+VARIANTS = [
+    {"FLOCK_NTT_POS_TABLES": "0"},                                  # deepest pass on nibble-pair tables
+    {"FLOCK_NTT_POS_TABLES": "0", "FLOCK_NTT_PAIR_TABLES": "0"},    # ... on 16-entry tables, 4-bit Horner
+    {"FLOCK_NTT_SETS": "1"},                                        # one set per thread in shared passes
+    {"FLOCK_NTT_SETS": "4"},
+    {"FLOCK_NTT_COMB": "0"},                                        # byte comb, two-input XORs
+    {"FLOCK_NTT_COMB": "1"},                                        # byte comb, three-input XORs
+    {"FLOCK_NTT_SHARED_TABLES": "0"},                               # every pass on per-group tables
+    {"FLOCK_NTT_SCHEDULE": "3,3,3,2,2"},                            # 2-layer deepest pass (the 2^13 x 64 commit)
+    {"FLOCK_NTT_RADIX_LOG": "2"},
+]
+
+
+@pytest.mark.parametrize("env", VARIANTS, ids=lambda e: ",".join(f"{k}={v}" for k, v in e.items()))
+def test_ntt_variant_matches_oracle(dev, env):
+    print(f"variant {env}", flush=True)
+    r = subprocess.run([sys.executable, CHECK], env={**os.environ, **env}, capture_output=True, text=True,
+                       timeout=100)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "MISMATCH" not in r.stdout
