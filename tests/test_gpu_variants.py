@@ -1,6 +1,6 @@
-"""The NTT product-kernel variants behind the library's environment switches
-(read once per process) against the oracle: each setting runs
-tests/ntt_variant_check.py in a child process of its own, one after another."""
+"""Kernel variants behind the library's environment switches (read once per
+process) against the oracle: each setting runs tests/variant_check.py in a
+child process of its own, one after another."""
 import os
 import subprocess
 import sys
@@ -9,7 +9,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
-CHECK = os.path.join(HERE, "ntt_variant_check.py")
+CHECK = os.path.join(HERE, "variant_check.py")
 
 VARIANTS = [
     {"FLOCK_NTT_POS_TABLES": "0"},                                  # deepest pass on nibble-pair tables
@@ -27,8 +27,27 @@ VARIANTS = [
 @pytest.mark.parametrize("env", VARIANTS, ids=lambda e: ",".join(f"{k}={v}" for k, v in e.items()))
 def test_ntt_variant_matches_oracle(dev, env):
     print(f"variant {env}", flush=True)
-    r = subprocess.run([sys.executable, CHECK], env={**os.environ, **env}, capture_output=True, text=True,
+    r = subprocess.run([sys.executable, CHECK, "ntt"], env={**os.environ, **env}, capture_output=True, text=True,
                        timeout=100)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "MISMATCH" not in r.stdout
+
+
+# Work-group counts of the product-sum kernels (their sums go through per-work-group
+# partials): tiny and uneven grids make every work-group stride over many items.
+REDUCTION_VARIANTS = [
+    {"FLOCK_SC_GRID": "3", "FLOCK_RD_GRID": "5"},
+    {"FLOCK_SC_GRID": "1", "FLOCK_RD_GRID": "1"},
+    {"FLOCK_SC_GRID": "65535", "FLOCK_RD_GRID": "65535"},
+]
+
+
+@pytest.mark.parametrize("env", REDUCTION_VARIANTS, ids=lambda e: ",".join(f"{k}={v}" for k, v in e.items()))
+def test_reduction_variant_matches_oracle(dev, env):
+    print(f"variant {env}", flush=True)
+    r = subprocess.run([sys.executable, CHECK, "reductions"], env={**os.environ, **env}, capture_output=True,
+                       text=True, timeout=100)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "MISMATCH" not in r.stdout

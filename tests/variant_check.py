@@ -1,12 +1,17 @@
-"""Parity check of the NTT product-kernel variants that the library selects by
-environment switches (read once per process), run as a child process by
-tests/test_gpu_ntt_variants.py with one setting each.
+"""Parity checks of the kernel variants that the library selects by environment
+switches (read once per process), run as a child process by
+tests/test_gpu_variants.py with one setting each:
 
-Checks, against the oracle: the commit (codeword + root) at 2^13 x 64 and
-2^11 x 8 (rate 1/2), the interleaved forward transform and inverse at
-2^12 x 64, and a forward/inverse round trip.  Uses the C ABI through ctypes
-only (flk_alloc / flk_copy_*), no torch.  Exit status 0 means every check
-matched; the lines printed name each check.
+  variant_check.py ntt         the commit (codeword + root) at 2^13 x 64 and
+                               2^11 x 8 (rate 1/2), the interleaved forward
+                               transform and inverse at 2^12 x 64, and a
+                               forward/inverse round trip;
+  variant_check.py reductions  round_msg (+eval), fold_and_msg, zerocheck round
+                               pair, lincheck round message and ring-switch
+                               fold_1b_rows at 2^13 and 2^16.
+
+Uses the C ABI through ctypes only (flk_alloc / flk_copy_*), no torch.  Exit
+status 0 means every check matched the oracle; the lines printed name each check.
 """
 import ctypes
 import os
@@ -57,9 +62,7 @@ def same(a, b):
     return a.shape == b.shape and bool((a["lo"] == b["lo"]).all() and (a["hi"] == b["hi"]).all())
 
 
-def main():
-    lib = nat.load()
-    orc = Oracle()
+def check_ntt(lib, orc):
     ok = True
     for log_cols, log_lanes in [(13, 6), (11, 3)]:
         lanes = 1 << log_lanes
@@ -100,6 +103,57 @@ def main():
     print(f"inverse 2^{log_d} x {lanes}: {'ok' if good else 'MISMATCH'}", flush=True)
     ok = ok and good
     dd.free()
+    return ok
+
+
+def check_reductions(lib, orc):
+    import random
+    ok = True
+
+    def run(name, got, want):
+        nonlocal ok
+        good = same(got, want)
+        print(f"{name}: {'ok' if good else 'MISMATCH'}", flush=True)
+        ok = ok and good
+
+    for logn in (13, 16):
+        n = 1 << logn
+        rng = random.Random(logn)
+        f, b = fill(10 + logn, n), fill(20 + logn, n)
+        df, db, dout = Dev(lib, f.nbytes), Dev(lib, b.nbytes), Dev(lib, 128 * 16)
+        df.put(f)
+        db.put(b)
+        nat.check(lib.flk_sc_round_msg(df.p, db.p, n, dout.p, None), "round_msg")
+        run(f"round_msg 2^{logn}", dout.get(F128_DT, 2), orc.round_msg(f, b))
+        nat.check(lib.flk_sc_round_msg_and_eval(df.p, db.p, n, dout.p, None), "round_msg_and_eval")
+        run(f"round_msg_and_eval 2^{logn}", dout.get(F128_DT, 3), orc.round_msg_and_eval(f, b))
+        r = rng.getrandbits(128)
+        dfo, dbo = Dev(lib, f.nbytes // 2), Dev(lib, b.nbytes // 2)
+        nat.check(lib.flk_sc_fold_and_msg(df.p, db.p, n, nat.F128.from_int(r), dfo.p, dbo.p, dout.p, None),
+                  "fold_and_msg")
+        wfo, wbo, wout = orc.fold_and_msg(f, b, r)
+        run(f"fold_and_msg 2^{logn} (folds)",
+            np.concatenate([dfo.get(F128_DT, n // 2), dbo.get(F128_DT, n // 2)]), np.concatenate([wfo, wbo]))
+        run(f"fold_and_msg 2^{logn} (message)", dout.get(F128_DT, 2), wout)
+        eq_rest = orc.build_eq([rng.getrandbits(128) for _ in range(logn - 1)])
+        de = Dev(lib, eq_rest.nbytes)
+        de.put(eq_rest)
+        nat.check(lib.flk_zc_round_pair(df.p, db.p, n, de.p, dout.p, None), "zc_round_pair")
+        run(f"zc_round_pair 2^{logn}", dout.get(F128_DT, 2), orc.zc_round_pair(f, b, eq_rest))
+        nat.check(lib.flk_lc_round_msg_msb(df.p, db.p, n, dout.p, None), "lc_round_msg_msb")
+        run(f"lc_round_msg_msb 2^{logn}", dout.get(F128_DT, 2), orc.lc_round_msg_msb(f, b))
+        nat.check(lib.flk_rs_fold_1b_rows(df.p, db.p, n, dout.p, None), "rs_fold_1b_rows")
+        run(f"rs_fold_1b_rows 2^{logn}", dout.get(F128_DT, 128), orc.rs_fold_1b_rows(f, b))
+        for d in (df, db, dout, dfo, dbo, de):
+            d.free()
+    return ok
+
+
+def main():
+    lib = nat.load()
+    orc = Oracle()
+    mode = sys.argv[1] if len(sys.argv) > 1 else "ntt"
+    ok = check_ntt(lib, orc) if mode == "ntt" else check_reductions(lib, orc)
     return 0 if ok else 1
 
 
