@@ -153,10 +153,11 @@ __device__ __forceinline__ void comb_mul_acc_rt(uint4 v, const uint4* __restrict
   // bounds register pressure when several products are inlined together.
 #pragma unroll 1
   for (int j = 7; j >= 0; --j) {
-    const uint4 e0 = tab[((vw[0] >> (4 * j)) & 15u) * stride];
-    const uint4 e1 = tab[((vw[1] >> (4 * j)) & 15u) * stride];
-    const uint4 e2 = tab[((vw[2] >> (4 * j)) & 15u) * stride];
-    const uint4 e3 = tab[((vw[3] >> (4 * j)) & 15u) * stride];
+    // entry nib of the lane's table: one bit-field extract and one shift-add
+    const uint4 e0 = tab[__builtin_amdgcn_ubfe(vw[0], 4 * j, 4) * stride];
+    const uint4 e1 = tab[__builtin_amdgcn_ubfe(vw[1], 4 * j, 4) * stride];
+    const uint4 e2 = tab[__builtin_amdgcn_ubfe(vw[2], 4 * j, 4) * stride];
+    const uint4 e3 = tab[__builtin_amdgcn_ubfe(vw[3], 4 * j, 4) * stride];
 #pragma unroll
     for (int i = 7; i >= 1; --i) a[i] = funnel(a[i], a[i - 1], 28);
     a[0] = (a[0] << 4) ^ e0.x;
