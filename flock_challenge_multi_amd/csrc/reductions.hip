@@ -218,6 +218,17 @@ __global__ void __launch_bounds__(256) lc_bind_top_kernel(uint4* __restrict__ v,
 // ---------------------------------------------------------------------------
 constexpr int F1_WG = 256;
 
+// Lane K of the caller's aligned lane quad, broadcast to the quad (DPP).
+template <int K>
+__device__ __forceinline__ uint32_t quad_bcast(uint32_t x) {
+  constexpr int ctrl = K | (K << 2) | (K << 4) | (K << 6);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, ctrl, 0xF, 0xF, false);
+}
+template <int K>
+__device__ __forceinline__ uint4 quad_bcast4(uint4 v) {
+  return make_uint4(quad_bcast<K>(v.x), quad_bcast<K>(v.y), quad_bcast<K>(v.z), quad_bcast<K>(v.w));
+}
+
 // Bits b of the byte x moved to bit 4b (b < 8).
 __device__ __forceinline__ uint32_t spread_byte4(uint32_t x) {
   x &= 0xFFu;
@@ -230,7 +241,7 @@ __device__ __forceinline__ uint32_t spread_byte4(uint32_t x) {
 __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_kernel(const uint4* __restrict__ zp,
                                                                const uint4* __restrict__ eq, uint64_t n,
                                                                uint4* __restrict__ out) {
-  __shared__ uint32_t zt[F1_WG * 4];
+  __shared__ __align__(16) uint32_t zt[F1_WG * 4];
   __shared__ uint32_t nibw[(F1_WG / 4) * 16];  // [group][r / 8]: nibbles of r = 8q .. 8q+7
   __shared__ uint4 tabs[(F1_WG / 4) * 16];
   __shared__ uint4 red[128];
@@ -245,23 +256,12 @@ __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_kernel(const uint4* __r
       zv = zp[i];
       ev = eq[i];
     }
-    zt[t * 4 + 0] = zv.x;
-    zt[t * 4 + 1] = zv.y;
-    zt[t * 4 + 2] = zv.z;
-    zt[t * 4 + 3] = zv.w;
+    reinterpret_cast<uint4*>(zt)[t] = zv;  // one 16-byte store per lane (conflict-free)
     // Table of group g = t/4 holds sums over subsets of its 4 eq values; this lane
-    // writes the 4 entries whose bits 2,3 equal q = t%4.  Gather the group's eq
-    // values from the 4 lanes of the group with cross-lane shuffles.
-    const int base_lane = (int)(t & ~3u) & 63;
-    uint4 e0, e1, e2, e3;
-    e0 = make_uint4(__shfl(ev.x, base_lane + 0), __shfl(ev.y, base_lane + 0), __shfl(ev.z, base_lane + 0),
-                    __shfl(ev.w, base_lane + 0));
-    e1 = make_uint4(__shfl(ev.x, base_lane + 1), __shfl(ev.y, base_lane + 1), __shfl(ev.z, base_lane + 1),
-                    __shfl(ev.w, base_lane + 1));
-    e2 = make_uint4(__shfl(ev.x, base_lane + 2), __shfl(ev.y, base_lane + 2), __shfl(ev.z, base_lane + 2),
-                    __shfl(ev.w, base_lane + 2));
-    e3 = make_uint4(__shfl(ev.x, base_lane + 3), __shfl(ev.y, base_lane + 3), __shfl(ev.z, base_lane + 3),
-                    __shfl(ev.w, base_lane + 3));
+    // writes the 4 entries whose bits 2,3 equal q = t%4.  The group is the
+    // lane's aligned quad, so its eq values arrive by DPP quad broadcasts
+    // (VALU moves; cross-lane shuffles would go through the LDS).
+    const uint4 e0 = quad_bcast4<0>(ev), e1 = quad_bcast4<1>(ev), e2 = quad_bcast4<2>(ev), e3 = quad_bcast4<3>(ev);
     const uint32_t q = t & 3u;
     uint4 base = f_zero();
     if (q & 1u) base = f_add(base, e2);
