@@ -10,6 +10,13 @@
  * enqueued on `stream` (a hipStream_t; NULL = the default stream) without a
  * host synchronisation.  Nothing retains a pointer after return.
  *
+ * Temporary device memory: the entry points that reduce to a few outputs
+ * (sumcheck round messages, zerocheck round 1 and round pair, lincheck round
+ * message, ring-switch fold_1b_rows) store per-work-group partial sums in
+ * scratch taken from the stream-ordered pool of `stream` (hipMallocAsync,
+ * released with hipFreeAsync on the same stream); zerocheck round 1 also
+ * builds its eq table there.  A failed allocation returns FLK_E_NOMEM.
+ *
  * Element layout: flk_f128 is 16 bytes, `lo` (x^0..x^63) then `hi`
  * (x^64..x^127), little-endian — identical to the reference F128
  * (CF/field/gf2_128.rs:22-27).  Hashes are 32 raw bytes.
