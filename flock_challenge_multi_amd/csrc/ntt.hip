@@ -14,10 +14,13 @@
 //
 // Twiddle products are combs (gf128.hpp).  In every pass but the deepest, all
 // sets of a work-group share their twiddles and the work-group builds their
-// nibble-position tables once (combjr_mul_add); in the deepest passes the
-// G = min(num_ntts, 64) lanes of a set build nibble-pair tables for each layer
-// (comb4p_mul_add).  The inverse transform runs the same passes in reverse
-// order with the inverse butterfly (v += u; u += v*twiddle).
+// nibble-position tables once (combjr_mul_add); each thread runs two sets on
+// them.  In the deepest passes a set's twiddles are its own: with 64 lanes or
+// more one wave (= one work-group) owns a set and builds the nibble-position
+// tables of each layer's twiddles (ntt_pass_pos_kernel); with fewer lanes the
+// G = num_ntts lanes of a set build nibble-pair tables (comb4p_mul_add).  The
+// inverse transform runs the same passes in reverse order with the inverse
+// butterfly (v += u; u += v*twiddle).
 //
 // The RS encode reads the message directly in its first pass: after the
 // log_inv_rate copy-only layers of the zero-padded transform, position p holds

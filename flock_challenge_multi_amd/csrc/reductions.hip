@@ -13,8 +13,9 @@
 // sums of known values selected by bits ("sum eq[i] over the set bits") use
 // subset-sum tables in LDS, the GPU form of the reference's method-of-four-
 // Russians tables: a 16-entry table per nibble of selector bits (conflict-free,
-// 256 B) or a 256-entry table per byte where the table is shared by a whole
-// work-group.  Products of two data values use the per-lane comb of gf128.hpp.
+// 256 B; byte tables collide in the LDS banks when data index them).  Products
+// of two data values use the per-lane comb of gf128.hpp.  Sums over all items
+// end as per-work-group partials added by a second launch (partials_sum_free).
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>

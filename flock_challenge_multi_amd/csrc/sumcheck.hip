@@ -4,13 +4,15 @@
 //
 // Every reduction is an XOR sum, so partial sums are combined in any order:
 // each lane XOR-accumulates unreduced 256-bit comb products, reduces once,
-// the wave folds with cross-lane shuffles and one lane per work-group XORs the
-// result into the output with atomicXor.  The result is bit-identical to the
-// sequential sum (CF/pcs.rs:96-107: schedule width cannot change wire bytes).
+// the wave folds with cross-lane shuffles, one lane per work-group stores the
+// work-group's sum as a partial, and a second launch XORs the partials
+// (partials_sum_free).  The result is bit-identical to the sequential sum
+// (CF/pcs.rs:96-107: schedule width cannot change wire bytes).
 //
 // Data x data products (f.b) use a per-lane comb table of one operand in LDS,
 // laid out entry-major so a wave's 64 lanes touch 64 consecutive 16-byte slots.
-// Products by a challenge (r, alpha, eq factors) use one work-group-shared table.
+// Products by a challenge (r, alpha, eq factors) use work-group-shared
+// nibble-position tables (build_pos_tables, combjr_mul_add).
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
