@@ -211,8 +211,31 @@ __global__ void __launch_bounds__(PSUM_WG) partials_sum_kernel(const uint4* __re
   }
 }
 
+// The default stream-ordered pool of each device keeps what the reduction
+// scratch frees (release threshold raised once per device), so repeated calls
+// reuse it instead of returning it to the driver at every synchronisation.
+static void keep_pool_memory() {
+  static std::mutex mu;
+  static bool done[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    (void)hipGetLastError();
+    return;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  if (done[dev]) return;
+  done[dev] = true;
+  hipMemPool_t pool = nullptr;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
+  (void)hipGetLastError();
+}
+
 int partials_alloc(uint4** parts, unsigned grid, int n_out, hipStream_t st) {
   *parts = nullptr;
+  keep_pool_memory();
   hipError_t e = hipMallocAsync(reinterpret_cast<void**>(parts), (size_t)grid * (size_t)n_out * sizeof(uint4), st);
   if (e != hipSuccess) {
     (void)hipGetLastError();
