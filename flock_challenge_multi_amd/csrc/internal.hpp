@@ -43,7 +43,8 @@ inline int log2_exact(size_t n) {
 // from the stream-ordered pool (hipMallocAsync / hipFreeAsync on the call's
 // stream), so calls on different streams never share it.  Device-scope atomics
 // from every work-group into the same few words cost ~7 ns each, serialised.
-int partials_alloc(uint4** parts, unsigned grid, int n_out, hipStream_t st);
+// With grid == 1 the partials are out itself (no scratch, no second launch).
+int partials_alloc(uint4** parts, unsigned grid, int n_out, uint4* out, hipStream_t st);
 int partials_sum_free(uint4* parts, unsigned grid, int n_out, uint4* out, hipStream_t st);
 
 // Host-side GF(2^128) product (used for set-up constants only).

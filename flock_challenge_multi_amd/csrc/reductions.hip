@@ -445,7 +445,7 @@ int flk_lc_round_msg_msb(const flk_f128* comb, const flk_f128* z, size_t n, flk_
   uint4* o = reinterpret_cast<uint4*>(out2);
   const unsigned grid = grid_for(n / 2, RD_WG, RD_GRID_CAP);
   uint4* parts = nullptr;
-  if ((rc = partials_alloc(&parts, grid, 2, st))) return rc;
+  if ((rc = partials_alloc(&parts, grid, 2, o, st))) return rc;
   hipLaunchKernelGGL(lc_round_msg_msb_kernel, dim3(grid), dim3(RD_WG), 0, st, reinterpret_cast<const uint4*>(comb),
                      reinterpret_cast<const uint4*>(z), (uint64_t)(n / 2), parts);
   FLK_LAUNCH_CHECK("lc_round_msg_msb_kernel");
@@ -473,7 +473,7 @@ int flk_rs_fold_1b_rows(const flk_f128* packed_witness, const flk_f128* suffix_t
   if (n == 0) return zero_n(o, 128, st);
   const unsigned grid = grid_for(n, F1_WG, 2048);
   uint4* parts = nullptr;
-  if ((rc = partials_alloc(&parts, grid, 128, st))) return rc;
+  if ((rc = partials_alloc(&parts, grid, 128, o, st))) return rc;
   hipLaunchKernelGGL(rs_fold_1b_rows_kernel, dim3(grid), dim3(F1_WG), 0, st,
                      reinterpret_cast<const uint4*>(packed_witness), reinterpret_cast<const uint4*>(suffix_tensor),
                      (uint64_t)n, parts);
@@ -521,7 +521,7 @@ int flk_zc_round_pair(const flk_f128* a, const flk_f128* b, size_t n, const flk_
   uint4* o = reinterpret_cast<uint4*>(out2);
   const unsigned grid = grid_for(n / 2, RD_WG, RD_GRID_CAP);
   uint4* parts = nullptr;
-  if ((rc = partials_alloc(&parts, grid, 2, st))) return rc;
+  if ((rc = partials_alloc(&parts, grid, 2, o, st))) return rc;
   hipLaunchKernelGGL(zc_round_pair_kernel, dim3(grid), dim3(RD_WG), 0, st, reinterpret_cast<const uint4*>(a),
                      reinterpret_cast<const uint4*>(b), reinterpret_cast<const uint4*>(eq_rest), (uint64_t)(n / 2),
                      parts);

@@ -233,8 +233,12 @@ static void keep_pool_memory() {
   (void)hipGetLastError();
 }
 
-int partials_alloc(uint4** parts, unsigned grid, int n_out, hipStream_t st) {
+int partials_alloc(uint4** parts, unsigned grid, int n_out, uint4* out, hipStream_t st) {
   *parts = nullptr;
+  if (grid == 1) {  // parts[k * 1 + 0] is out[k]: no scratch, no second launch
+    *parts = out;
+    return FLK_OK;
+  }
   keep_pool_memory();
   hipError_t e = hipMallocAsync(reinterpret_cast<void**>(parts), (size_t)grid * (size_t)n_out * sizeof(uint4), st);
   if (e != hipSuccess) {
@@ -247,6 +251,7 @@ int partials_alloc(uint4** parts, unsigned grid, int n_out, hipStream_t st) {
 }
 
 int partials_sum_free(uint4* parts, unsigned grid, int n_out, uint4* out, hipStream_t st) {
+  if (parts == out) return FLK_OK;  // one work-group wrote out directly
   hipLaunchKernelGGL(partials_sum_kernel, dim3((unsigned)n_out), dim3(PSUM_WG), 0, st, parts, (uint32_t)grid, out);
   hipError_t e = hipGetLastError();
   const hipError_t ef = hipFreeAsync(parts, st);

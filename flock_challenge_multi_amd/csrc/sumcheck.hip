@@ -236,7 +236,7 @@ int flk_sc_round_msg(const flk_f128* f, const flk_f128* b, size_t n, flk_f128* o
   const uint64_t pairs = n / 2;
   const unsigned grid = flk::reduce_grid(pairs);
   uint4* parts = nullptr;
-  if ((rc = flk::partials_alloc(&parts, grid, 2, st))) return rc;
+  if ((rc = flk::partials_alloc(&parts, grid, 2, o, st))) return rc;
   hipLaunchKernelGGL(flk::round_msg_kernel<false>, dim3(grid), dim3(SC_WG), 0, st,
                      reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, parts);
   FLK_LAUNCH_CHECK("round_msg_kernel");
@@ -254,7 +254,7 @@ int flk_sc_round_msg_and_eval(const flk_f128* f, const flk_f128* b, size_t n, fl
   const uint64_t pairs = n / 2;
   const unsigned grid = flk::reduce_grid(pairs);
   uint4* parts = nullptr;
-  if ((rc = flk::partials_alloc(&parts, grid, 3, st))) return rc;
+  if ((rc = flk::partials_alloc(&parts, grid, 3, o, st))) return rc;
   hipLaunchKernelGGL(flk::round_msg_kernel<true>, dim3(grid), dim3(SC_WG), 0, st,
                      reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, parts);
   FLK_LAUNCH_CHECK("round_msg_kernel<eval>");
@@ -272,7 +272,7 @@ int flk_sc_fold_and_msg(const flk_f128* f, const flk_f128* b, size_t n, flk_f128
   const uint64_t quads = n / 4;
   const unsigned grid = flk::reduce_grid(quads);
   uint4* parts = nullptr;
-  if ((rc = flk::partials_alloc(&parts, grid, 2, st))) return rc;
+  if ((rc = flk::partials_alloc(&parts, grid, 2, o, st))) return rc;
   hipLaunchKernelGGL(flk::fold_and_msg_kernel, dim3(grid), dim3(SC_WG), 0, st,
                      reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), quads, flk::to_u4(r),
                      reinterpret_cast<uint4*>(f_out), reinterpret_cast<uint4*>(b_out), parts);

@@ -431,7 +431,7 @@ extern "C" int flk_zc_round1(const uint8_t* a_bits, const uint8_t* b_bits, const
   const uint64_t n_iter = (n_outer + R1_SUB - 1) / R1_SUB;
   const unsigned grid = (unsigned)(n_iter < 1024 ? n_iter : 1024);
   uint4* parts = nullptr;
-  if ((rc = partials_alloc(&parts, grid, 128, st))) {
+  if ((rc = partials_alloc(&parts, grid, 128, out, st))) {
     (void)hipFreeAsync(eq, st);
     return rc;
   }
