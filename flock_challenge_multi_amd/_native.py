@@ -88,6 +88,7 @@ SIGNATURES = {
     "flk_zc_fold_at_z": (_I, [_P, _SZ, _SZ, _P, _P, _P]),
     "flk_zc_round_pair": (_I, [_P, _P, _SZ, _P, _P, _P]),
     "flk_zc_round1": (_I, [_P, _P, _P, _SZ, _P, _P, _P, _P]),
+    "flk_blake3_witness": (_I, [_P, _SZ, _SZ, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

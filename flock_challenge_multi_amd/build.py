@@ -69,5 +69,20 @@ def build_host_mirror_test():
     return exe
 
 
+def build_witness_lane_test():
+    """Compile tests/cpp/witness_lane_test (host check of the per-compression
+    witness code in csrc/witness_lane.hpp against the oracle).  Needs
+    oracle/liboracle.so."""
+    root = os.path.dirname(HERE)
+    exe = os.path.join(root, "tests", "cpp", "witness_lane_test")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wno-unknown-pragmas", "-I", os.path.join(root, "oracle"),
+           "-I", CSRC, os.path.join(root, "tests", "cpp", "witness_lane_test.cpp"), "-o", exe,
+           "-L", os.path.join(root, "oracle"), "-loracle", "-Wl,-rpath,$ORIGIN/../../oracle"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"witness lane test build failed:\n{r.stderr}")
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

@@ -11,6 +11,9 @@ Names, argument meaning and error behaviour follow the reference entry points
   build_eq_table                           CF/lincheck.rs:2080
   partial_eval_lsb                         CF/pcs/ligerito.rs:1162
   SumcheckProver (new/fold/introduce_new/glue)  CF/pcs/ligerito.rs:8893-9260
+  generate_witness_with_ab_packed[_and_lincheck]
+                                           CP/r1cs_hashes/blake3.rs:2184,4617
+                                           (CP = crates/flock-prover/src/)
 
 Buffers are torch tensors on the GPU: F128 arrays are int64 tensors of shape
 (n, 2) holding (lo, hi); hashes are uint8 tensors of shape (n, 32).  Work is
@@ -466,6 +469,41 @@ def zerocheck_round1(a_bits, b_bits, c_bits, m, r_outer):
     r = (F128 * max(len(r_outer), 1))(*[F128.from_int(v) for v in r_outer])
     nat.call("flk_zc_round1", _ptr(a_bits), _ptr(b_bits), _ptr(c_bits), m, r, _ptr(out), _ptr(out[64:]), _stream())
     return out[:64], out[64:]
+
+
+# ---- BLAKE3 R1CS witness (CP/r1cs_hashes/blake3.rs, CP = crates/flock-prover/src/) --
+COMPRESSION_WORDS = 28  # cv[8], m[16], counter (lo, hi), block_len, flags
+
+
+def _witness(blocks, n_blocks_log, lincheck):
+    if not (isinstance(blocks, torch.Tensor) and blocks.is_cuda and blocks.dtype == torch.int32
+            and blocks.dim() == 2 and blocks.shape[1] == COMPRESSION_WORDS and blocks.is_contiguous()):
+        raise ValueError("blocks must be a contiguous CUDA int32 tensor of shape (n, 28)")
+    n_blocks = blocks.shape[0]
+    if not 3 <= n_blocks_log <= 30:
+        raise ValueError("n_blocks_log must be in 3..30 (the lincheck stripe needs >= 8 slots)")
+    n_total = 1 << n_blocks_log
+    if n_blocks > n_total:
+        raise ValueError(f"{n_blocks} blocks > 2^{n_blocks_log} = {n_total} slots")
+    z, a, b = (f128_empty(n_total * 128, blocks.device) for _ in range(3))
+    zl = torch.empty(n_total * 2048, dtype=torch.uint8, device=blocks.device) if lincheck else None
+    nat.call("flk_blake3_witness", _ptr(blocks), n_blocks, n_blocks_log, _ptr(z), _ptr(a), _ptr(b),
+             _ptr(zl) if lincheck else None, _stream())
+    return z, a, b, zl
+
+
+def generate_witness_with_ab_packed_and_lincheck(blocks, n_blocks_log):
+    """(z, a, b, z_lincheck) for the compressions in `blocks` (one 112-byte record
+    per row: cv, m, counter lo/hi, block_len, flags) padded to 2^n_blocks_log
+    slots with the all-zero compression (blake3.rs:4617-4647).  z, a, b are F128
+    tensors of 2^n_blocks_log * 128 elements (2^14 packed row bits per slot);
+    z_lincheck is the uint8 stripe that partial_fold_packed_z reads."""
+    return _witness(blocks, n_blocks_log, True)
+
+
+def generate_witness_with_ab_packed(blocks, n_blocks_log):
+    """(z, a, b) as above without the stripe (blake3.rs:2184-2206)."""
+    return _witness(blocks, n_blocks_log, False)[:3]
 
 
 class SumcheckProver:

@@ -280,6 +280,33 @@ int flk_zc_round_pair(const flk_f128 *a, const flk_f128 *b, size_t n, const flk_
 int flk_zc_round1(const uint8_t *a_bits, const uint8_t *b_bits, const uint8_t *c_bits, size_t m,
                   const flk_f128 *r_outer, flk_f128 *out_ab, flk_f128 *out_c, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* BLAKE3 R1CS witness (CP/r1cs_hashes/blake3.rs, CP = crates/flock-prover/src/). */
+/* ------------------------------------------------------------------------ */
+
+/* One compression input, the reference's Compression tuple (blake3.rs:1860-1861):
+ * (cv, m, counter, block_len, flags); 112 bytes. */
+typedef struct {
+    uint32_t cv[8];
+    uint32_t m[16];
+    uint64_t counter;
+    uint32_t block_len;
+    uint32_t flags;
+} flk_compression;
+
+/* generate_witness_with_ab_packed_and_lincheck (blake3.rs:4617-4647; with
+ * z_lincheck == NULL, generate_witness_with_ab_packed, :2184-2206): for each of
+ * the 2^n_blocks_log slots (3 <= n_blocks_log <= 30) the K = 2^14 row values z,
+ * a = A z, b = B z of one compression, packed LSB-first, slot j at flk_f128
+ * index 128 j of z, a and b (2^n_blocks_log * 128 elements each).  Slots
+ * n_blocks.. hold the all-zero compression (blake3.rs:4632-4636).  z_lincheck
+ * (2^n_blocks_log * 2048 bytes) receives the lincheck stripe: byte g * 2^14 + p
+ * has bit r = z bit p of slot 8 g + r (CP/r1cs_hashes/common.rs:304-320), the
+ * layout flk_lc_partial_fold reads.  blocks (device, n_blocks records) and every
+ * output must be 16-byte aligned. */
+int flk_blake3_witness(const flk_compression *blocks, size_t n_blocks, size_t n_blocks_log, flk_f128 *z,
+                       flk_f128 *a, flk_f128 *b, uint8_t *z_lincheck, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -119,6 +119,27 @@ int orc_gf8_inverse(size_t k, uint8_t beta, uint8_t *v);
 int orc_zc_round1(const uint8_t *a, const uint8_t *b, const uint8_t *c, size_t m, size_t k_skip,
                   const orc_f128 *r, orc_f128 *p_ab, orc_f128 *p_c);
 
+
+/* ---- BLAKE3 R1CS witness generation (witness.c; CP/r1cs_hashes/blake3.rs) */
+/* One compression input (blake3.rs:1860-1861): cv, message words, counter,
+ * block_len, flags; 112 bytes. */
+typedef struct {
+    uint32_t cv[8];
+    uint32_t m[16];
+    uint64_t counter;
+    uint32_t block_len;
+    uint32_t flags;
+} orc_compression;
+/* out = {K, const position, G base, G stride, out_hi base, useful bits}. */
+void orc_witness_layout(size_t out[6]);
+/* (z, a, b) of one block, 256 u64 words each (zeroed first). */
+void orc_blake3_block_witness(const orc_compression *blk, uint64_t *z, uint64_t *a, uint64_t *b);
+/* 2^n_blocks_log slots (>= 8), padding slots = the all-zero compression; z, a, b
+ * hold 2^n_blocks_log * 256 u64 words each; stripe (nullable) 2^n_blocks_log *
+ * 2048 bytes. Returns 0, or 1 on a bad shape. */
+int orc_blake3_witness(const orc_compression *blocks, size_t n_blocks, size_t n_blocks_log,
+                       uint64_t *z, uint64_t *a, uint64_t *b, uint8_t *stripe);
+
 #ifdef __cplusplus
 }
 #endif

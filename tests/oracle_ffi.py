@@ -14,6 +14,11 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB = os.path.join(ORACLE_DIR, "liboracle.so")
 
 F128_DT = np.dtype([("lo", "<u8"), ("hi", "<u8")])
+# One BLAKE3 compression input (112 bytes): cv, message words, counter,
+# block_len, flags.
+COMPRESSION_DT = np.dtype([("cv", "<u4", 8), ("m", "<u4", 16), ("counter", "<u8"),
+                           ("block_len", "<u4"), ("flags", "<u4")])
+assert COMPRESSION_DT.itemsize == 112
 
 
 class COrcF128(ctypes.Structure):
@@ -88,6 +93,10 @@ class Oracle:
             "orc_gf8_forward": (ctypes.c_int, [SZ, ctypes.c_uint8, P]),
             "orc_gf8_inverse": (ctypes.c_int, [SZ, ctypes.c_uint8, P]),
             "orc_zc_round1": (ctypes.c_int, [P, P, P, SZ, SZ, P, P, P]),
+            "orc_witness_layout": (None, [P]),
+            "orc_blake3_block_witness": (None, [P, P, P, P]),
+            "orc_blake3_witness": (ctypes.c_int, [P, SZ, SZ, P, P, P, P]),
+            "orc_blake3_compress": (None, [P, P, U64, ctypes.c_uint32, ctypes.c_uint32, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -160,6 +169,42 @@ class Oracle:
         cw = np.zeros(n_cw, dtype=F128_DT)
         assert self.L.orc_ntt_rs_encode_interleaved(dim, _p(msg), len(msg), _p(cw), n_cw, num_ntts) == 0
         return cw
+
+    # ---- BLAKE3 R1CS witness (oracle/witness.c) ----
+    def witness_layout(self):
+        out = np.zeros(6, dtype=np.uint64)
+        self.L.orc_witness_layout(_p(out))
+        return [int(x) for x in out]
+
+    def block_witness(self, comp):
+        """(z, a, b) of one compression record (COMPRESSION_DT scalar array of 1)."""
+        c = np.ascontiguousarray(comp, dtype=COMPRESSION_DT).reshape(1)
+        z, a, b = (np.zeros(256, np.uint64) for _ in range(3))
+        self.L.orc_blake3_block_witness(_p(c), _p(z), _p(a), _p(b))
+        return z, a, b
+
+    def witness(self, comps, n_blocks_log, stripe=True):
+        n_total = 1 << n_blocks_log
+        c = np.ascontiguousarray(comps, dtype=COMPRESSION_DT)
+        if c.size == 0:
+            c = np.zeros(1, COMPRESSION_DT)
+            n = 0
+        else:
+            n = c.size
+        z, a, b = (np.zeros(n_total * 256, np.uint64) for _ in range(3))
+        st = np.zeros(n_total * 2048, np.uint8) if stripe else None
+        rc = self.L.orc_blake3_witness(_p(c), n, n_blocks_log, _p(z), _p(a), _p(b),
+                                       _p(st) if stripe else None)
+        if rc:
+            raise ValueError("orc_blake3_witness rejected the shape")
+        return z, a, b, st
+
+    def compress(self, cv, block_bytes, counter, block_len, flags):
+        cvv = np.asarray(cv, np.uint32).copy()
+        blk = np.frombuffer(bytes(block_bytes), np.uint8).copy()
+        out = np.zeros(16, np.uint32)
+        self.L.orc_blake3_compress(_p(cvv), _p(blk), counter, block_len, flags, _p(out))
+        return out
 
     def blake3(self, data):
         b = np.frombuffer(bytes(data), dtype=np.uint8).copy() if len(data) else np.zeros(1, np.uint8)

@@ -1,0 +1,164 @@
+"""GPU parity: BLAKE3 R1CS witness generation (flk_blake3_witness) against the
+oracle, bit-exact, through the C ABI.
+
+Small batches are compared whole (z, a, b and the lincheck stripe), dense and
+ragged (padding slots), with and without the stripe.  At the full sizes
+(2^15 compressions = m 29, 2^18 = m 32) the checks are size-independent:
+R1CS satisfaction a AND b = z over every row, the constant wire set and the
+padding rows clear in every slot, the stripe transposed back equal to z, and
+sampled slots equal to the oracle's witness of the same compression.  The
+witness then feeds the zerocheck round 1 and the lincheck fold on the device,
+whose outputs equal the oracle's on the oracle's witness.
+"""
+import random
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from flock_challenge_multi_amd import _native as nat
+from flock_challenge_multi_amd import hotpath as hp
+from gpu_util import from_dev, same, to_dev
+from oracle_ffi import COMPRESSION_DT
+from test_gpu_reductions import _fixed_inner_challenges
+
+pytestmark = pytest.mark.gpu
+
+K = 1 << 14
+CONST = 512
+USEFUL = 15409
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _comps(seed, n):
+    rng = np.random.default_rng(seed)
+    c = np.zeros(n, COMPRESSION_DT)
+    raw = rng.integers(0, 2**32, (n, 28), dtype=np.uint32)
+    c.view(np.uint32).reshape(n, 28)[:] = raw
+    return c
+
+
+def _dev_blocks(c):
+    if c.size == 0:
+        return torch.zeros((0, 28), dtype=torch.int32, device="cuda")
+    return torch.from_numpy(c.view(np.int32).reshape(-1, 28).copy()).cuda()
+
+
+def _u64(t):
+    return t.detach().cpu().numpy().view(np.uint64).reshape(-1)
+
+
+@pytest.mark.parametrize("n_log,n", [(3, 8), (3, 5), (3, 0), (4, 13), (6, 64), (6, 40), (7, 128), (9, 300)])
+def test_witness_matches_oracle(orc, dev, n_log, n):
+    _log(f"blake3_witness n_log={n_log} n={n}")
+    c = _comps(1000 + 17 * n_log + n, n)
+    want = orc.witness(c, n_log)
+    z, a, b, zl = hp.generate_witness_with_ab_packed_and_lincheck(_dev_blocks(c), n_log)
+    torch.cuda.synchronize()
+    for got, w, name in zip((z, a, b), want[:3], "zab"):
+        assert np.array_equal(_u64(got), w), name
+    assert np.array_equal(zl.cpu().numpy(), want[3]), "stripe"
+
+
+def test_witness_without_stripe(orc, dev):
+    c = _comps(7, 20)
+    want = orc.witness(c, 5, stripe=False)
+    z, a, b = hp.generate_witness_with_ab_packed(_dev_blocks(c), 5)
+    torch.cuda.synchronize()
+    for got, w in zip((z, a, b), want[:3]):
+        assert np.array_equal(_u64(got), w)
+
+
+def test_witness_special_inputs(orc, dev):
+    c = np.zeros(16, COMPRESSION_DT)
+    c.view(np.uint32).reshape(16, 28)[1::2] = 0xFFFFFFFF  # all-ones records between zero ones
+    c[2]["cv"] = [0x6A09E667, 0xBB67AE85, 0x3C6EF372, 0xA54FF53A, 0x510E527F, 0x9B05688C, 0x1F83D9AB, 0x5BE0CD19]
+    c[2]["flags"] = 11
+    want = orc.witness(c, 4)
+    z, a, b, zl = hp.generate_witness_with_ab_packed_and_lincheck(_dev_blocks(c), 4)
+    torch.cuda.synchronize()
+    for got, w in zip((z, a, b), want[:3]):
+        assert np.array_equal(_u64(got), w)
+    assert np.array_equal(zl.cpu().numpy(), want[3])
+
+
+def test_witness_shape_errors(dev):
+    blocks = _dev_blocks(_comps(1, 9))
+    for args in [(blocks, 2), (blocks, 3)]:
+        with pytest.raises(ValueError):
+            hp.generate_witness_with_ab_packed_and_lincheck(*args)
+    z = hp.f128_empty(8 * 128)
+    # the C ABI rejects the same shapes without latching the library
+    st = dev.flk_blake3_witness(blocks.data_ptr(), 9, 3, z.data_ptr(), z.data_ptr(), z.data_ptr(), None, None)
+    assert st == nat.FLK_E_SHAPE
+    st = dev.flk_blake3_witness(blocks.data_ptr(), 8, 3, z.data_ptr() + 8, z.data_ptr(), z.data_ptr(), None, None)
+    assert st == nat.FLK_E_SHAPE
+    assert dev.flk_is_disabled() == 0
+
+
+def _stripe_back(zl_chunk):
+    """(G, K) stripe bytes -> (8 G, K / 8) z bytes (the inverse transpose)."""
+    g = zl_chunk.shape[0]
+    sh = torch.arange(8, device=zl_chunk.device, dtype=torch.uint8)
+    bits = (zl_chunk.unsqueeze(-1) >> sh) & 1                     # (G, K, 8): bit r
+    bits = bits.permute(0, 2, 1).reshape(g * 8, K // 8, 8)         # (8G, K/8, 8): bit p%8
+    return (bits << sh).sum(-1, dtype=torch.uint8)
+
+
+@pytest.mark.parametrize("n_log", [15, 18])
+def test_witness_full_size_properties(orc, dev, n_log):
+    _log(f"blake3_witness full size n_log={n_log}")
+    n_total = 1 << n_log
+    n = n_total - 3  # ragged: the last three slots are padding
+    g = torch.Generator(device="cuda").manual_seed(n_log)
+    blocks = torch.randint(-2**31, 2**31, (n, 28), dtype=torch.int32, device="cuda", generator=g)
+    z, a, b, zl = hp.generate_witness_with_ab_packed_and_lincheck(blocks, n_log)
+    torch.cuda.synchronize()
+    # R1CS (C = I): a AND b = z in every row of every slot
+    assert torch.equal(a & b, z)
+    zb = z.view(torch.uint8).reshape(n_total, K // 8)
+    assert bool(((zb[:, CONST // 8] >> (CONST % 8)) & 1).all()), "constant wire"
+    assert not bool(zb[:, (USEFUL + 7) // 8:].any()), "padding rows"
+    assert not bool((zb[:, USEFUL // 8] >> (USEFUL % 8)).any()), "padding rows"
+    # the stripe is z transposed
+    zl2 = zl.reshape(n_total // 8, K)
+    step = 512
+    for g0 in range(0, n_total // 8, step):
+        assert torch.equal(_stripe_back(zl2[g0:g0 + step]), zb[8 * g0: 8 * (g0 + step)]), g0
+    # sampled slots against the oracle (the last is a padding slot)
+    host = blocks.cpu().numpy().view(np.uint32)
+    rng = random.Random(n_log)
+    slots = sorted(set(rng.randrange(n) for _ in range(24))) + [n - 1, n_total - 1]
+    zz, aa, bb = _u64(z), _u64(a), _u64(b)
+    for j in slots:
+        rec = np.zeros(1, COMPRESSION_DT)
+        if j < n:
+            rec.view(np.uint32)[:] = host[j]
+        wz, wa, wb = orc.block_witness(rec)
+        sl = slice(256 * j, 256 * (j + 1))
+        assert np.array_equal(zz[sl], wz) and np.array_equal(aa[sl], wa) and np.array_equal(bb[sl], wb), j
+
+
+def test_witness_feeds_zerocheck_and_lincheck(orc, dev):
+    """Device witness -> device zerocheck round 1 / lincheck fold equals the
+    oracle's pipeline on the oracle's witness (m = 17: 8 compressions)."""
+    n_log, m = 3, 17
+    c = _comps(77, 6)
+    wz, wa, wb, wst = orc.witness(c, n_log)
+    z, a, b, zl = hp.generate_witness_with_ab_packed_and_lincheck(_dev_blocks(c), n_log)
+    rng = random.Random(5)
+    r_outer = [rng.getrandbits(128) for _ in range(m - 13)]
+    p_ab, p_c = hp.zerocheck_round1(a.view(torch.uint8).reshape(-1), b.view(torch.uint8).reshape(-1),
+                                    z.view(torch.uint8).reshape(-1), m, r_outer)
+    eq = orc.build_eq([rng.getrandbits(128) for _ in range(m - 14)])
+    fold = hp.partial_fold_packed_z(zl, m, 14, to_dev(eq))
+    torch.cuda.synchronize()
+    r = [0] * 6 + _fixed_inner_challenges() + r_outer
+    want_ab, want_c = orc.zc_round1(wa.view(np.uint8), wb.view(np.uint8), wz.view(np.uint8), m, 6, r)
+    assert same(from_dev(p_ab), want_ab) and same(from_dev(p_c), want_c)
+    assert same(from_dev(fold), orc.lc_partial_fold(wst, m, 14, eq))
