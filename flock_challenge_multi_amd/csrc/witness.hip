@@ -25,6 +25,8 @@
 // bound by HBM writes: 8 KiB per compression with the stripe, 6 KiB without.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "internal.hpp"
 #include "witness_lane.hpp"
 
@@ -42,8 +44,6 @@ constexpr int STRIPE_U4_PER_GROUP = wit::K_BITS / 16;
 static_assert(WIN + 4 <= RING, "a G call emits at most 4 words after a checkpoint");
 static_assert(sizeof(flk_compression) == 112, "compression record is 7 x 16 bytes");
 
-__constant__ wit::Sched k_sched = wit::make_sched();
-
 // Message word i of the lane's compression, from LDS ([16][64]: conflict-free).
 struct LdsMsg {
   const uint32_t* m;
@@ -54,21 +54,32 @@ struct LdsMsg {
 struct WaveSink {
   uint64_t* ring;  // LDS [3][64][RSTRIDE]
   int lane;
+  int hi;          // end of the part's words
   int flushed;     // first word not yet written out (a multiple of WIN)
   uint64_t blk0, n_total;
   uint4 *z, *a, *b, *stripe;
 
+  // Words outside the part are stored too (no branch): a part's first step
+  // writes them before its first word, in slots of no pending word, and its
+  // last step after its last word, in slots of words already written out.
   __device__ __forceinline__ void put(int widx, uint64_t zw, uint64_t aw, uint64_t bw) {
     uint64_t* p = ring + lane * RSTRIDE + widx % RING;
     p[0] = zw;
     p[ARR] = aw;
     p[2 * ARR] = bw;
   }
-  __device__ __forceinline__ void checkpoint(int widx) {
-    if (widx >= flushed + WIN) {
+  // The ring is idle before a part's first word: its first 4 KB hold the
+  // state words while the part fast-forwards ([16][64] u32, own lane only).
+  __device__ __forceinline__ wit::StridedState state_scratch() {
+    return wit::StridedState{reinterpret_cast<uint32_t*>(ring) + lane, WIT_WG};
+  }
+  __device__ __forceinline__ void state_scratch_done() { __syncthreads(); }
+  __device__ __forceinline__ bool checkpoint(int words) {
+    if (words >= flushed + WIN && flushed < hi) {
       flush(flushed);
       flushed += WIN;
     }
+    return flushed >= hi;
   }
   // Write words [w0, w0 + WIN) of all 64 blocks, and their stripe bytes.
   __device__ void flush(int w0) {
@@ -112,14 +123,17 @@ struct WaveSink {
   }
 };
 
+// Work-group (c, p) = block index c * parts + p builds words [256 p / parts,
+// 256 (p + 1) / parts) of compressions 64 c .. 64 c + 63.
 __global__ __launch_bounds__(WIT_WG) void blake3_witness_kernel(const uint4* __restrict__ comps, uint64_t n_blocks,
-                                                                uint64_t n_total, uint4* __restrict__ z,
-                                                                uint4* __restrict__ a, uint4* __restrict__ b,
-                                                                uint4* __restrict__ stripe) {
+                                                                uint64_t n_total, int log_parts,
+                                                                uint4* __restrict__ z, uint4* __restrict__ a,
+                                                                uint4* __restrict__ b, uint4* __restrict__ stripe) {
   __shared__ uint64_t ring[3 * ARR];
   __shared__ uint32_t msg[16 * WIT_WG];
   const int lane = threadIdx.x;
-  const uint64_t blk0 = (uint64_t)blockIdx.x * WIT_WG;
+  const int part = blockIdx.x & ((1 << log_parts) - 1);
+  const uint64_t blk0 = (uint64_t)(blockIdx.x >> log_parts) * WIT_WG;
   const uint64_t blk = blk0 + lane;
 
   uint32_t w[28];
@@ -144,14 +158,28 @@ __global__ __launch_bounds__(WIT_WG) void blake3_witness_kernel(const uint4* __r
   const uint32_t t_lo = w[24], t_hi = w[25], blen = w[26], flags = w[27];
 
   const LdsMsg mm{msg, lane};
-  uint32_t s[16];
-  wit::compress_state(cv, mm, t_lo, t_hi, blen, flags, k_sched, s);
-  uint32_t out_lo[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) out_lo[i] = s[i] ^ s[i + 8];
+  const int span = wit::WORDS >> log_parts;
+  WaveSink sink{ring, lane, span * (part + 1), span * part, blk0, n_total, z, a, b, stripe};
+  wit::block_rows(cv, mm, t_lo, t_hi, blen, flags, span * part, sink);
+}
 
-  WaveSink sink{ring, lane, 0, blk0, n_total, z, a, b, stripe};
-  wit::block_rows(cv, mm, t_lo, t_hi, blen, flags, out_lo, k_sched, sink);
+// Parts per compression group: enough work-groups to give every SIMD one
+// (4 per CU on 256 CUs) when the batch is small, one part when it is large.
+// FLOCK_WIT_PARTS overrides (1, 2, 4, 8 or 16).
+int witness_log_parts(size_t groups) {
+  static const int env = [] {
+    const char* e = getenv("FLOCK_WIT_PARTS");
+    const int v = e ? atoi(e) : 0;
+    return (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) ? v : 0;
+  }();
+  int parts = env;
+  if (!parts) {
+    parts = 1;
+    while (parts < 4 && groups * parts < 1024) parts *= 2;
+  }
+  int l = 0;
+  while ((1 << l) < parts) ++l;
+  return l;
 }
 
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
@@ -176,9 +204,11 @@ int flk_blake3_witness(const flk_compression* blocks, size_t n_blocks, size_t n_
   if ((n_blocks && !aligned16(blocks)) || !aligned16(z) || !aligned16(a) || !aligned16(b) ||
       (z_lincheck && !aligned16(z_lincheck)))
     return set_error(FLK_E_SHAPE, "blake3_witness: buffers must be 16-byte aligned");
-  const unsigned grid = (unsigned)((n_total + WIT_WG - 1) / WIT_WG);
+  const size_t groups = (n_total + WIT_WG - 1) / WIT_WG;
+  const int log_parts = witness_log_parts(groups);
+  const unsigned grid = (unsigned)(groups << log_parts);
   hipLaunchKernelGGL(blake3_witness_kernel, dim3(grid), dim3(WIT_WG), 0, static_cast<hipStream_t>(stream),
-                     reinterpret_cast<const uint4*>(blocks), (uint64_t)n_blocks, (uint64_t)n_total,
+                     reinterpret_cast<const uint4*>(blocks), (uint64_t)n_blocks, (uint64_t)n_total, log_parts,
                      reinterpret_cast<uint4*>(z), reinterpret_cast<uint4*>(a), reinterpret_cast<uint4*>(b),
                      reinterpret_cast<uint4*>(z_lincheck));
   FLK_LAUNCH_CHECK("blake3_witness_kernel");

@@ -36,23 +36,12 @@ constexpr int G_STRIDE = 250;
 constexpr int OUT_HI_BASE = GS_BASE + 56 * G_STRIDE;  // 15153
 constexpr int USEFUL_BITS = OUT_HI_BASE + 256;        // 15409
 
-struct Sched {
-  uint8_t v[7][16];
-};
-// Message word consumed at position j of round r: PERM^r applied to the
-// identity (blake3.rs:149,325-340).
-constexpr Sched make_sched() {
-  const uint8_t perm[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
-  Sched s{};
-  uint8_t p[16] = {};
-  for (int i = 0; i < 16; ++i) p[i] = (uint8_t)i;
-  for (int r = 0; r < 7; ++r) {
-    for (int i = 0; i < 16; ++i) s.v[r][i] = p[i];
-    uint8_t n[16] = {};
-    for (int i = 0; i < 16; ++i) n[i] = p[perm[i]];
-    for (int i = 0; i < 16; ++i) p[i] = n[i];
-  }
-  return s;
+// Message schedule: position j of round r consumes message word PERM^r(j)
+// (blake3.rs:149,325-340).
+constexpr uint8_t MSG_PERM[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+constexpr int sched_idx(int r, int j) {
+  for (int i = 0; i < r; ++i) j = MSG_PERM[j];
+  return j;
 }
 
 FLK_WHD uint32_t iv(int i) {
@@ -62,176 +51,272 @@ FLK_WHD uint32_t iv(int i) {
 
 FLK_WHD uint32_t rotr32(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
 
-// The compression's state after the 7 rounds (before the finalisation XORs),
-// in the textbook lane order; `msg(i)` returns message word i.
-template <class Msg>
-FLK_WHD void compress_state(const uint32_t cv[8], const Msg& msg, uint32_t t_lo, uint32_t t_hi,
-                            uint32_t blen, uint32_t flags, const Sched& sc, uint32_t s[16]) {
+// The compression's initial state (blake3.rs:282-300).
+FLK_WHD void init_state(uint32_t s[16], const uint32_t cv[8], uint32_t t_lo, uint32_t t_hi, uint32_t blen,
+                        uint32_t flags) {
   for (int i = 0; i < 8; ++i) s[i] = cv[i];
   for (int i = 0; i < 4; ++i) s[8 + i] = iv(i);
   s[12] = t_lo;
   s[13] = t_hi;
   s[14] = blen;
   s[15] = flags;
-  const int lanes[8][4] = {{0, 4, 8, 12}, {1, 5, 9, 13}, {2, 6, 10, 14}, {3, 7, 11, 15},
-                           {0, 5, 10, 15}, {1, 6, 11, 12}, {2, 7, 8, 13}, {3, 4, 9, 14}};
-#pragma unroll
-  for (int r = 0; r < 7; ++r) {
-#pragma unroll
-    for (int g = 0; g < 8; ++g) {
-      const int a = lanes[g][0], b = lanes[g][1], c = lanes[g][2], d = lanes[g][3];
-      s[a] = s[a] + s[b] + msg(sc.v[r][2 * g]);
-      s[d] = rotr32(s[d] ^ s[a], 16);
-      s[c] = s[c] + s[d];
-      s[b] = rotr32(s[b] ^ s[c], 12);
-      s[a] = s[a] + s[b] + msg(sc.v[r][2 * g + 1]);
-      s[d] = rotr32(s[d] ^ s[a], 8);
-      s[c] = s[c] + s[d];
-      s[b] = rotr32(s[b] ^ s[c], 7);
-    }
+}
+
+// G call G of the compression (textbook lane order, blake3.rs:250-270).
+template <int G, class Msg>
+FLK_WHD void g_call(uint32_t s[16], const Msg& msg) {
+  constexpr int r = G / 8, g = G % 8;
+  constexpr int a = g < 4 ? g : g - 4;
+  constexpr int b = g < 4 ? 4 + g : 4 + (g - 3) % 4;
+  constexpr int c = g < 4 ? 8 + g : 8 + (g - 2) % 4;
+  constexpr int d = g < 4 ? 12 + g : 12 + (g - 1) % 4;
+  s[a] = s[a] + s[b] + msg(sched_idx(r, 2 * g));
+  s[d] = rotr32(s[d] ^ s[a], 16);
+  s[c] = s[c] + s[d];
+  s[b] = rotr32(s[b] ^ s[c], 12);
+  s[a] = s[a] + s[b] + msg(sched_idx(r, 2 * g + 1));
+  s[d] = rotr32(s[d] ^ s[a], 8);
+  s[c] = s[c] + s[d];
+  s[b] = rotr32(s[b] ^ s[c], 7);
+}
+
+// All 56 G calls, straight-line (the part that writes words 4..7, out_lo,
+// needs the final state before its first word).
+template <int G = 0, class Msg>
+FLK_WHD void compress_rounds(uint32_t s[16], const Msg& msg) {
+  if constexpr (G < 56) {
+    g_call<G>(s, msg);
+    compress_rounds<G + 1>(s, msg);
   }
 }
 
-// The three row vectors (z, a, b) of one block as a stream of u64 words.
-template <class Sink>
-struct RowStream {
-  Sink& sink;
-  uint64_t z, a, b;  // pending low bits of the current word
-  int used;          // number of pending bits, < 64
-  int widx;          // index of the next word
+// State words in caller storage: word i at p[i * stride].
+struct StridedState {
+  uint32_t* p;
+  int stride;
+  FLK_WHD uint32_t get(int i) const { return p[i * stride]; }
+  FLK_WHD void set(int i, uint32_t v) const { p[i * stride] = v; }
+};
 
-  FLK_WHD void word(uint64_t zw, uint64_t aw, uint64_t bw) {
-    sink.put(widx, zw, aw, bw);
-    ++widx;
+// Runs G calls 0 .. n_g - 1 on the state in `st` as one rolled loop (lanes
+// and message indices from scalar arithmetic): a part of a block that starts
+// after G call k fast-forwards with it.
+template <class Msg>
+FLK_WHD void advance_state(const StridedState& st, const Msg& msg, int n_g) {
+  uint64_t perm = 0;  // MSG_PERM as 16 nibbles
+  for (int i = 0; i < 16; ++i) perm |= (uint64_t)MSG_PERM[i] << (4 * i);
+#pragma unroll 1
+  for (int G = 0; G < n_g; ++G) {
+    const int r = G >> 3, g = G & 7, j = g & 3;
+    const int a = j;
+    const int b = 4 + (g < 4 ? j : (j + 1) & 3);
+    const int c = 8 + (g < 4 ? j : (j + 2) & 3);
+    const int d = 12 + (g < 4 ? j : (j + 3) & 3);
+    int mi = 2 * g, mj = 2 * g + 1;
+    for (int i = 0; i < r; ++i) {
+      mi = (int)((perm >> (4 * mi)) & 15);
+      mj = (int)((perm >> (4 * mj)) & 15);
+    }
+    uint32_t va = st.get(a), vb = st.get(b), vc = st.get(c), vd = st.get(d);
+    va = va + vb + msg(mi);
+    vd = rotr32(vd ^ va, 16);
+    vc = vc + vd;
+    vb = rotr32(vb ^ vc, 12);
+    va = va + vb + msg(mj);
+    vd = rotr32(vd ^ va, 8);
+    vc = vc + vd;
+    vb = rotr32(vb ^ vc, 7);
+    st.set(a, va);
+    st.set(b, vb);
+    st.set(c, vc);
+    st.set(d, vd);
   }
-  // Append W bits of each vector (values already masked to W bits).
-  template <int W>
+}
+
+// The three row vectors (z, a, b) of one block, written as u64 words at bit
+// positions known at compile time: every push names its first bit.
+template <class Sink>
+struct Rows {
+  Sink& sink;
+  uint64_t z, a, b;  // bits of the current word below the next position
+
+  // W bits of each vector (values already masked to W bits) at bit POS.
+  template <int POS, int W>
   FLK_WHD void push(uint32_t zv, uint32_t av, uint32_t bv) {
-    z |= (uint64_t)zv << used;
-    a |= (uint64_t)av << used;
-    b |= (uint64_t)bv << used;
-    used += W;
-    if (used >= 64) {
-      word(z, a, b);
-      used -= 64;
-      const int sh = W - used;  // bits of this value already in the emitted word, 1..W
+    constexpr int used = POS & 63;
+    if constexpr (used == 0) {
+      z = zv;
+      a = av;
+      b = bv;
+    } else {
+      z |= (uint64_t)zv << used;
+      a |= (uint64_t)av << used;
+      b |= (uint64_t)bv << used;
+    }
+    if constexpr (used + W >= 64) {
+      sink.put(POS >> 6, z, a, b);
+      constexpr int sh = 64 - used;  // bits of this value in the emitted word, 1..W
       z = (uint64_t)zv >> sh;
       a = (uint64_t)av >> sh;
       b = (uint64_t)bv >> sh;
     }
   }
   // A lin row word (blake3.rs:1889-1890): z = a = v, b = 1.
-  FLK_WHD void lin(uint32_t v) { push<32>(v, v, 0xffffffffu); }
+  template <int POS>
+  FLK_WHD void lin(uint32_t v) {
+    push<POS, 32>(v, v, 0xffffffffu);
+  }
   // One 32-bit addition: its 31 carry rows (common.rs:80-88) and the sum.
+  template <int POS>
   FLK_WHD uint32_t add(uint32_t x, uint32_t y) {
     const uint32_t sum = x + y;
     const uint32_t cin = sum ^ x ^ y;
     const uint32_t left = (x ^ cin) & 0x7fffffffu;
     const uint32_t right = (y ^ cin) & 0x7fffffffu;
-    push<31>(left & right, left, right);
+    push<POS, 31>(left & right, left, right);
     return sum;
   }
 };
 
-// Rotate row `r` of the 4x4 state left by `n` columns (new column j = old
-// column j + n).
-FLK_WHD void rot_row(uint32_t s[16], int r, int n) {
-  uint32_t t[4];
-  for (int j = 0; j < 4; ++j) t[j] = s[4 * r + ((j + n) & 3)];
-  for (int j = 0; j < 4; ++j) s[4 * r + j] = t[j];
-}
-
-// All 256 words of one block's (z, a, b).  out_lo[w] = final state[w] ^
-// state[w + 8], from compress_state (its row words precede the G records).
+// The row words of one block in 59 steps: step 0 the words before the G
+// records, steps 1..56 the G calls, step 57 out_hi, step 58 the zero padding
+// words.  run<STEP>() returns the number of words complete after the step; s
+// holds the state before the step's G call.
 template <class Sink, class Msg>
-FLK_WHD void block_rows(const uint32_t cv[8], const Msg& msg, uint32_t t_lo, uint32_t t_hi, uint32_t blen,
-                        uint32_t flags, const uint32_t out_lo[8], const Sched& sc, Sink& sink) {
-  RowStream<Sink> st{sink, 0, 0, 0, 0, 0};
-  const uint64_t ones = ~0ull;
-  // words 0..3: cv; 4..7: out_lo (aligned lin rows)
-  for (int i = 0; i < 4; ++i) {
-    const uint64_t v = (uint64_t)cv[2 * i] | ((uint64_t)cv[2 * i + 1] << 32);
-    st.word(v, v, ones);
-  }
-  for (int i = 0; i < 4; ++i) {
-    const uint64_t v = (uint64_t)out_lo[2 * i] | ((uint64_t)out_lo[2 * i + 1] << 32);
-    st.word(v, v, ones);
-  }
-  // words 8..17: the constant wire at bit 512, then m[0..16), t_lo, t_hi,
-  // block_len, flags from bit 513 on (input rows: z = a = value, b = 1)
-  uint32_t prev_top = 1;  // bit 0 of word 8 is the constant wire
-  for (int i = 0; i < 10; ++i) {
-    const uint32_t lo = 2 * i < 16 ? msg(2 * i) : (2 * i == 16 ? t_lo : blen);
-    const uint32_t hi = 2 * i + 1 < 16 ? msg(2 * i + 1) : (2 * i + 1 == 17 ? t_hi : flags);
-    const uint64_t v = (uint64_t)prev_top | ((uint64_t)lo << 1) | ((uint64_t)hi << 33);
-    st.word(v, v, ones);
-    prev_top = hi >> 31;
-  }
-  st.z = st.a = prev_top;  // flags bit 31 at bit 1152
-  st.b = 1;
-  st.used = 1;
-  sink.checkpoint(st.widx);
-
-  // The 56 G calls.  The state is kept rotated so that every call works on
-  // column 0: rows are rotated left by (g, g, g, g) for column call g and by
-  // (j, 1 + j, 2 + j, 3 + j) for diagonal call 4 + j.
+struct BlockSteps {
+  Rows<Sink>& st;
+  Sink& sink;
+  const Msg& msg;
+  const uint32_t* cv;
+  uint32_t t_lo, t_hi, blen, flags;
+  uint32_t out_lo[8];  // final state[w] ^ state[w + 8], for step 0
   uint32_t s[16];
-  for (int i = 0; i < 8; ++i) s[i] = cv[i];
-  for (int i = 0; i < 4; ++i) s[8 + i] = iv(i);
-  s[12] = t_lo;
-  s[13] = t_hi;
-  s[14] = blen;
-  s[15] = flags;
-#pragma unroll 1
-  for (int r = 0; r < 7; ++r) {
-#pragma unroll 1
-    for (int g = 0; g < 8; ++g) {
-      const uint32_t mx = msg(sc.v[r][2 * g]), my = msg(sc.v[r][2 * g + 1]);
-      const uint32_t a0 = s[0], b0 = s[4], c0 = s[8], d0 = s[12];
-      const uint32_t tmp0 = st.add(a0, b0);
-      const uint32_t a1 = st.add(tmp0, mx);
-      const uint32_t d1 = rotr32(d0 ^ a1, 16);
-      const uint32_t c1 = st.add(c0, d1);
-      const uint32_t b1 = rotr32(b0 ^ c1, 12);
-      const uint32_t tmp1 = st.add(a1, b1);
-      const uint32_t a2 = st.add(tmp1, my);
-      const uint32_t d2 = rotr32(d1 ^ a2, 8);
-      const uint32_t c2 = st.add(c1, d2);
-      const uint32_t bn = rotr32(b1 ^ c2, 7);
-      st.lin(bn);
-      st.lin(d2);
-      s[0] = a2;
-      s[4] = bn;
-      s[8] = c2;
-      s[12] = d2;
-      if (g == 3) {  // (3,3,3,3) -> (0,1,2,3)
-        rot_row(s, 0, 1);
-        rot_row(s, 1, 2);
-        rot_row(s, 2, 3);
-      } else if (g == 7) {  // (3,4,5,6) = (3,0,1,2) -> (0,0,0,0)
-        rot_row(s, 0, 1);
-        rot_row(s, 2, 3);
-        rot_row(s, 3, 2);
-      } else {
-        rot_row(s, 0, 1);
-        rot_row(s, 1, 1);
-        rot_row(s, 2, 1);
-        rot_row(s, 3, 1);
+
+  template <int STEP>
+  FLK_WHD int run() {
+    if constexpr (STEP == 0) {
+      const uint64_t ones = ~0ull;
+      // words 0..3: cv; 4..7: out_lo (aligned lin rows)
+      for (int i = 0; i < 4; ++i) {
+        const uint64_t v = (uint64_t)cv[2 * i] | ((uint64_t)cv[2 * i + 1] << 32);
+        sink.put(i, v, v, ones);
       }
-      sink.checkpoint(st.widx);
+      for (int i = 0; i < 4; ++i) {
+        const uint64_t v = (uint64_t)out_lo[2 * i] | ((uint64_t)out_lo[2 * i + 1] << 32);
+        sink.put(4 + i, v, v, ones);
+      }
+      // words 8..17: the constant wire at bit 512, then m[0..16), t_lo, t_hi,
+      // block_len, flags from bit 513 on (input rows: z = a = value, b = 1)
+      uint32_t prev_top = 1;
+      for (int i = 0; i < 10; ++i) {
+        const uint32_t lo = 2 * i < 16 ? msg(2 * i) : (2 * i == 16 ? t_lo : blen);
+        const uint32_t hi = 2 * i + 1 < 16 ? msg(2 * i + 1) : (2 * i + 1 == 17 ? t_hi : flags);
+        const uint64_t v = (uint64_t)prev_top | ((uint64_t)lo << 1) | ((uint64_t)hi << 33);
+        sink.put(8 + i, v, v, ones);
+        prev_top = hi >> 31;
+      }
+      st.z = st.a = prev_top;  // flags bit 31 at bit 1152
+      st.b = 1;
+      return 18;
+    } else if constexpr (STEP <= 56) {
+      constexpr int G = STEP - 1, r = G / 8, gi = G % 8;
+      constexpr int la = gi < 4 ? gi : gi - 4;
+      constexpr int lb = gi < 4 ? 4 + gi : 4 + (gi - 3) % 4;
+      constexpr int lc = gi < 4 ? 8 + gi : 8 + (gi - 2) % 4;
+      constexpr int ld = gi < 4 ? 12 + gi : 12 + (gi - 1) % 4;
+      constexpr int base = GS_BASE + G_STRIDE * G;
+      const uint32_t mx = msg(sched_idx(r, 2 * gi)), my = msg(sched_idx(r, 2 * gi + 1));
+      const uint32_t tmp0 = st.template add<base + 0>(s[la], s[lb]);
+      const uint32_t a1 = st.template add<base + 31>(tmp0, mx);
+      const uint32_t d1 = rotr32(s[ld] ^ a1, 16);
+      const uint32_t c1 = st.template add<base + 62>(s[lc], d1);
+      const uint32_t b1 = rotr32(s[lb] ^ c1, 12);
+      const uint32_t tmp1 = st.template add<base + 93>(a1, b1);
+      const uint32_t a2 = st.template add<base + 124>(tmp1, my);
+      const uint32_t d2 = rotr32(d1 ^ a2, 8);
+      const uint32_t c2 = st.template add<base + 155>(c1, d2);
+      const uint32_t bn = rotr32(b1 ^ c2, 7);
+      st.template lin<base + 186>(bn);
+      st.template lin<base + 218>(d2);
+      s[la] = a2;
+      s[lb] = bn;
+      s[lc] = c2;
+      s[ld] = d2;
+      return (base + G_STRIDE) >> 6;
+    } else if constexpr (STEP == 57) {
+      // out_hi = state[w + 8] ^ cv[w] (lin rows), then the last partial word
+#define FLK_OUT_HI(w) st.template lin<OUT_HI_BASE + 32 * (w)>(s[8 + (w)] ^ cv[w])
+      FLK_OUT_HI(0);
+      FLK_OUT_HI(1);
+      FLK_OUT_HI(2);
+      FLK_OUT_HI(3);
+      FLK_OUT_HI(4);
+      FLK_OUT_HI(5);
+      FLK_OUT_HI(6);
+      FLK_OUT_HI(7);
+#undef FLK_OUT_HI
+      static_assert(USEFUL_BITS % 64 != 0, "the out_hi rows end inside a word");
+      sink.put(USEFUL_BITS >> 6, st.z, st.a, st.b);
+      return (USEFUL_BITS >> 6) + 1;
+    } else {
+      for (int w = (USEFUL_BITS >> 6) + 1; w < WORDS; ++w) sink.put(w, 0, 0, 0);
+      return WORDS;
     }
   }
-  // out_hi = state[w + 8] ^ cv[w] (lin rows), then the zero padding rows.
-  for (int w = 0; w < 8; ++w) {
-    st.lin(s[8 + w] ^ cv[w]);
-    sink.checkpoint(st.widx);
+};
+
+constexpr int N_STEPS = 59;
+
+template <int LO, int HI, class Steps>
+FLK_WHD int run_step(int step, Steps& f) {
+  if constexpr (HI - LO == 1) {
+    return f.template run<LO>();
+  } else {
+    constexpr int MID = (LO + HI) / 2;
+    if (step < MID) return run_step<LO, MID>(step, f);
+    return run_step<MID, HI>(step, f);
   }
-  st.word(st.z, st.a, st.b);  // USEFUL_BITS is not a multiple of 64
-  sink.checkpoint(st.widx);
+}
+
+// The first step whose words can complete word w_lo - 1 (so that word w_lo
+// and later ones are built entirely by the steps that follow).
+FLK_WHD int first_step(int w_lo) {
+  const int bit = 64 * w_lo - 1;
+  if (bit < GS_BASE) return 0;
+  if (bit >= OUT_HI_BASE) return 57;
+  return 1 + (bit - GS_BASE) / G_STRIDE;
+}
+
+// Words [w_lo, w_hi) of one block's (z, a, b).  The sink keeps the words of
+// its range, gets a checkpoint after every step (the only flush site; at most
+// 19 words pending since the last 16-word boundary it has passed) and answers
+// true once its range is written out.  The part with w_lo == 0 first runs the
+// whole compression for out_lo (words 4..7); a part that starts after G call
+// k fast-forwards the state in the sink's scratch (Sink::state_scratch,
+// released by Sink::state_scratch_done).
+template <class Sink, class Msg>
+FLK_WHD void block_rows(const uint32_t cv[8], const Msg& msg, uint32_t t_lo, uint32_t t_hi, uint32_t blen,
+                        uint32_t flags, int w_lo, Sink& sink) {
+  Rows<Sink> st{sink, 0, 0, 0};
+  BlockSteps<Sink, Msg> f{st, sink, msg, cv, t_lo, t_hi, blen, flags, {}, {}};
+  init_state(f.s, cv, t_lo, t_hi, blen, flags);
+  const int s0 = first_step(w_lo);
+  if (s0 == 0) {
+    uint32_t e[16];
+    for (int i = 0; i < 16; ++i) e[i] = f.s[i];
+    compress_rounds(e, msg);
+    for (int i = 0; i < 8; ++i) f.out_lo[i] = e[i] ^ e[i + 8];
+  } else if (s0 > 1) {
+    const StridedState ss = sink.state_scratch();
+    for (int i = 0; i < 16; ++i) ss.set(i, f.s[i]);
+    advance_state(ss, msg, s0 - 1);
+    for (int i = 0; i < 16; ++i) f.s[i] = ss.get(i);
+    sink.state_scratch_done();
+  }
 #pragma unroll 1
-  while (st.widx < WORDS) {
-    st.word(0, 0, 0);
-    sink.checkpoint(st.widx);
+  for (int step = s0; step < N_STEPS; ++step) {
+    const int words = run_step<0, N_STEPS>(step, f);
+    if (sink.checkpoint(words)) break;
   }
 }
 

@@ -1,11 +1,13 @@
 // Host-side check of the per-compression witness code shared with the device
 // kernel (flock_challenge_multi_amd/csrc/witness_lane.hpp) against the CPU
-// oracle (oracle/witness.c): the row streams of whole blocks, the order and
-// spacing of the sink calls that the kernel's LDS ring relies on, the
-// compression pre-pass and the stripe transpose.  Prints "witness lane ok".
+// oracle (oracle/witness.c): the row words of whole blocks built in 1, 2, 4,
+// 8 and 16 parts, the order and spacing of the sink calls that the kernel's
+// LDS ring relies on, the compression state and the stripe transpose.  Prints
+// "witness lane ok".
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 
 #include "flock_oracle.h"
 #include "witness_lane.hpp"
@@ -22,26 +24,43 @@ static int fails = 0;
     }                                 \
   } while (0)
 
-// Mirrors the kernel's sink: words land in a 20-slot ring per lane and every
-// checkpoint writes out at most one 16-word window.
+// Mirrors the kernel's sink for the part [lo, hi) of a block: words land in
+// a 20-slot ring and every checkpoint writes out at most one 16-word window.
 struct HostSink {
   static constexpr int RING = 20, WIN = 16;
-  uint64_t z[WORDS], a[WORDS], b[WORDS];
-  int next = 0, flushed = 0;
+  uint64_t* z;
+  uint64_t* a;
+  uint64_t* b;
+  uint8_t* seen;
+  int lo, hi;
+  int flushed, last = -1;
+  bool done = false;
   void put(int widx, uint64_t zw, uint64_t aw, uint64_t bw) {
-    CHECK(widx == next, "word %d out of order (expected %d)", widx, next);
+    CHECK(widx > last, "word %d after word %d", widx, last);
+    if (widx < lo || widx >= hi) {
+      // the kernel stores it anyway: its ring slot must hold no pending word
+      for (int j = flushed; j <= last && j < hi; ++j)
+        CHECK(j < lo || j % RING != widx % RING, "word %d (outside %d..%d) overwrites pending word %d", widx, lo, hi, j);
+      last = widx;
+      return;
+    }
+    last = widx;
+    CHECK(!done, "word %d after the part ended", widx);
     CHECK(widx - flushed < RING, "word %d would overwrite unflushed slot (flushed %d)", widx, flushed);
-    CHECK(widx < WORDS, "word %d past the block", widx);
-    if (widx < 0 || widx >= WORDS) return;
     z[widx] = zw;
     a[widx] = aw;
     b[widx] = bw;
-    next = widx + 1;
+    ++seen[widx];
   }
-  void checkpoint(int widx) {
-    if (widx >= flushed + WIN) flushed += WIN;
-    CHECK(widx - flushed < WIN, "checkpoint at %d leaves %d pending", widx, widx - flushed);
+  bool checkpoint(int words) {
+    if (words >= flushed + WIN && flushed < hi) flushed += WIN;
+    CHECK(flushed >= hi || words - flushed < WIN, "checkpoint at %d leaves %d pending", words, words - flushed);
+    done = flushed >= hi;
+    return done;
   }
+  uint32_t scratch[16];
+  StridedState state_scratch() { return StridedState{scratch, 1}; }
+  void state_scratch_done() {}
 };
 
 struct ArrMsg {
@@ -58,29 +77,34 @@ static uint64_t next_u64() {
 }
 
 static void check_block(const orc_compression& c) {
-  static constexpr Sched sc = make_sched();
   const uint32_t t_lo = (uint32_t)c.counter, t_hi = (uint32_t)(c.counter >> 32);
   ArrMsg msg{c.m};
   uint32_t s[16];
-  compress_state(c.cv, msg, t_lo, t_hi, c.block_len, c.flags, sc, s);
+  init_state(s, c.cv, t_lo, t_hi, c.block_len, c.flags);
+  advance_state(StridedState{s, 1}, msg, 56);
   uint32_t out16[16];
   uint8_t blk[64];
   std::memcpy(blk, c.m, 64);
   orc_blake3_compress(c.cv, blk, c.counter, c.block_len, c.flags, out16);
-  uint32_t out_lo[8];
-  for (int w = 0; w < 8; ++w) {
-    out_lo[w] = s[w] ^ s[w + 8];
-    CHECK(out_lo[w] == out16[w], "compress_state out_lo[%d]", w);
-  }
-  HostSink sink;
-  block_rows(c.cv, msg, t_lo, t_hi, c.block_len, c.flags, out_lo, sc, sink);
-  CHECK(sink.next == WORDS && sink.flushed == WORDS, "stream ended at %d, flushed %d", sink.next, sink.flushed);
+  for (int w = 0; w < 8; ++w) CHECK((s[w] ^ s[w + 8]) == out16[w], "advance_state out_lo[%d]", w);
   uint64_t z[WORDS], a[WORDS], b[WORDS];
   orc_blake3_block_witness(&c, z, a, b);
-  for (int i = 0; i < WORDS; ++i) {
-    CHECK(sink.z[i] == z[i], "z word %d: %016llx vs %016llx", i, (unsigned long long)sink.z[i], (unsigned long long)z[i]);
-    CHECK(sink.a[i] == a[i], "a word %d", i);
-    CHECK(sink.b[i] == b[i], "b word %d", i);
+  for (int parts : {1, 2, 4, 8, 16}) {
+    uint64_t gz[WORDS] = {}, ga[WORDS] = {}, gb[WORDS] = {};
+    uint8_t seen[WORDS] = {};
+    for (int p = 0; p < parts; ++p) {
+      const int lo = WORDS / parts * p, hi = WORDS / parts * (p + 1);
+      HostSink sink{gz, ga, gb, seen, lo, hi, lo};
+      block_rows(c.cv, msg, t_lo, t_hi, c.block_len, c.flags, lo, sink);
+      CHECK(sink.done, "part %d/%d did not finish", p, parts);
+    }
+    for (int i = 0; i < WORDS; ++i) {
+      CHECK(seen[i] == 1, "parts %d: word %d written %d times", parts, i, seen[i]);
+      CHECK(gz[i] == z[i], "parts %d: z word %d: %016llx vs %016llx", parts, i, (unsigned long long)gz[i],
+            (unsigned long long)z[i]);
+      CHECK(ga[i] == a[i], "parts %d: a word %d", parts, i);
+      CHECK(gb[i] == b[i], "parts %d: b word %d", parts, i);
+    }
   }
 }
 
