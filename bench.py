@@ -13,7 +13,9 @@ The line also carries p50/p99 per-commit latency, the roofline of the
 dominant kernel (NTT passes, timed with HIP events on the launch stream), the
 m = 32 L0 commit (the 2^18-compression batch's commit, 1 GiB codeword) and a
 CPU baseline: the oracle's commit (C with PCLMULQDQ, threaded over up to 16 host
-cores, with its 1-thread figure beside it) on the same workload.
+cores, with its 1-thread figure beside it) on the same workload.  `witness`
+times the device BLAKE3 R1CS witness generation (z, a, b and the lincheck
+stripe) of 2^15 synthetic compressions, with its HBM roofline.
 
 `ms_per_step` is the wall time of `steps` commits queued back to back between
 two device synchronisations; p50/p99 come from a second loop of the same
@@ -201,8 +203,69 @@ def kernel_table(torch, nat, hp, sptr, st):
                                             P(tree.data_ptr()), sptr), reps=3)
     add("a4_merkle_m32", ms, 16 * n_cw + 32 * (2 * n_pos - 1), {"blake3_compressions_per_s": (17 * n_pos - 1) / (ms / 1e3)})
     del msg, cw, tree
+    # witness generation of the 2^18 compressions (z, a, b + stripe: 2 GiB written)
+    w = witness_leg(torch, nat, hp, sptr, st, 18, 5, 0)
+    add("w1_blake3_witness_2^18", w["ms"], w["roofline"]["bytes_per_launch"],
+        {"compressions_per_s": w["compressions_per_s"]})
     torch.cuda.synchronize()
     return res
+
+
+def witness_leg(torch, nat, hp, sptr, st, n_log, reps, cpu_sample_log):
+    """BLAKE3 R1CS witness generation (z, a = Az, b = Bz and the lincheck
+    stripe) of 2^n_log synthetic compressions, timed with HIP events on the
+    launch stream; algorithmic bytes per compression = 112 read + 4 x 2048
+    written.  A few slots are checked against the oracle, and the oracle times
+    a 2^cpu_sample_log sample (0: no CPU leg)."""
+    P = ctypes.c_void_p
+    n = 1 << n_log
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0xB1A4E3)
+    blocks = torch.randint(-(2**31), 2**31, (n, 28), dtype=torch.int32, device="cuda", generator=g)
+    z, a, b = (hp.f128_empty(n * 128) for _ in range(3))
+    zl = torch.empty(n * 2048, dtype=torch.uint8, device="cuda")
+
+    def once():
+        nat.call("flk_blake3_witness", P(blocks.data_ptr()), n, n_log, P(z.data_ptr()), P(a.data_ptr()),
+                 P(b.data_ptr()), P(zl.data_ptr()), sptr)
+
+    ms = _time(torch, st, once, reps)
+    nbytes = n * (112 + 4 * 2048)
+    gbs = nbytes / (ms / 1e3) / 1e9
+    out = {"workload": f"witness of 2^{n_log} BLAKE3 compressions (m = {n_log + 14}): z, a, b packed "
+                       f"({n * 2048 >> 20} MiB each) + lincheck stripe",
+           "compressions": n, "ms": ms, "compressions_per_s": n / (ms / 1e3),
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": nbytes}}
+    if cpu_sample_log:
+        try:
+            import numpy as np
+
+            import oracle_ffi
+            orc = oracle_ffi.Oracle()
+            host = blocks.cpu().numpy().view(np.uint32)
+            comps = np.zeros(n, oracle_ffi.COMPRESSION_DT)
+            comps.view(np.uint32).reshape(n, 28)[:] = host
+            zz = z.cpu().numpy().view(np.uint64).reshape(n, 256)
+            ok = True
+            for j in (0, 1, n // 3, n // 2 + 7, n - 1):
+                wz, wa, wb = orc.block_witness(comps[j:j + 1])
+                ok = ok and np.array_equal(zz[j], wz)
+                ok = ok and np.array_equal(a[128 * j:128 * (j + 1)].cpu().numpy().view(np.uint64).reshape(-1), wa)
+                ok = ok and np.array_equal(b[128 * j:128 * (j + 1)].cpu().numpy().view(np.uint64).reshape(-1), wb)
+            ns = 1 << cpu_sample_log
+            t1 = time.perf_counter()
+            _, _, _, st_ref = orc.witness(comps[:ns], cpu_sample_log)
+            cpu_s = time.perf_counter() - t1
+            ok = ok and np.array_equal(zl[:ns * 2048].cpu().numpy(), st_ref)
+            out["verified"] = bool(ok)
+            out["cpu_baseline"] = {"value": ns / cpu_s, "unit": "compressions/s", "cores": 1, "kind": "port",
+                                   "sample": f"the oracle's bit-at-a-time witness (C) of the first 2^{cpu_sample_log} "
+                                             f"compressions with the stripe: {cpu_s:.2f} s on 1 thread"}
+        except Exception as ex:  # the oracle is only a checker here
+            out["verified"] = f"oracle unavailable: {ex}"
+    del blocks, z, a, b, zl
+    return out
 
 
 def main():
@@ -358,6 +421,7 @@ def main():
         del msg32, cw32, tree32
 
     kernels = None if args.no_kernels else kernel_table(torch, nat, hp, sptr, st)
+    witness = witness_leg(torch, nat, hp, sptr, st, 15, 50, 0 if args.no_cpu_baseline else 13)
 
     traffic, valu = encode_counters()
     line = {
@@ -396,6 +460,7 @@ def main():
         "merkle_roofline": {"achieved": mer_bytes / (mer_ms / 1e3) / 1e9, "unit": "GB/s",
                             "compressions_per_s": (n_pos * 17 - 1) / (mer_ms / 1e3)},
         "m32_l0_commit": m32,
+        "witness": witness,
         "kernels_m32": kernels,
         "cpu_baseline": cpu,
     }
