@@ -15,7 +15,8 @@ m = 32 L0 commit (the 2^18-compression batch's commit, 1 GiB codeword) and a
 CPU baseline: the oracle's commit (C with PCLMULQDQ, threaded over up to 16 host
 cores, with its 1-thread figure beside it) on the same workload.  `witness`
 times the device BLAKE3 R1CS witness generation (z, a, b and the lincheck
-stripe) of 2^15 synthetic compressions, with its HBM roofline.
+stripe) of 2^15 synthetic compressions, with its HBM roofline; `pow` the device
+BLAKE3 proof-of-work hash rate and grind latency at 19 bits.
 
 `ms_per_step` is the wall time of `steps` commits queued back to back between
 two device synchronisations; p50/p99 come from a second loop of the same
@@ -268,6 +269,47 @@ def witness_leg(torch, nat, hp, sptr, st, n_log, reps, cpu_sample_log):
     return out
 
 
+def pow_leg(hp, cpu_sample_bits):
+    """Proof-of-work grinding (BLAKE3 transcripts): the device hash rate over a
+    2^30-nonce scan with no possible match (bits = 64), and grind_pow's
+    latency at the L0 fold-grind difficulty of the m = 32 profile (19 bits)
+    over 8 transcript digests, host call to returned nonce.  The oracle
+    grinds one digest at cpu_sample_bits on 1 thread (0: no CPU leg)."""
+    import random
+    d = bytes(range(32))
+    hp.pow_scan(d, 0, 1 << 20, 64)
+    n = 1 << 30
+    t0 = time.perf_counter()
+    assert hp.pow_scan(d, 0, n, 64) is None
+    dt = time.perf_counter() - t0
+    lat, nonces = [], []
+    for s in range(8):
+        dd = bytes(random.Random(s).randrange(256) for _ in range(32))
+        t0 = time.perf_counter()
+        nonces.append(hp.grind_pow_nonce(dd, 19))
+        lat.append(time.perf_counter() - t0)
+    out = {"workload": "BLAKE3 PoW: blake3(digest || u64le(nonce) || 24 zero bytes), >= bits leading zeros",
+           "scan_nonces": n, "scan_ms": dt * 1e3, "hashes_per_s": n / dt,
+           "grind19_ms_mean": sum(lat) / len(lat) * 1e3, "grind19_ms_min": min(lat) * 1e3,
+           "grind19_nonce_mean": sum(nonces) / len(nonces)}
+    if cpu_sample_bits:
+        try:
+            import oracle_ffi
+            orc = oracle_ffi.Oracle()
+            dd = bytes(random.Random(0).randrange(256) for _ in range(32))
+            t0 = time.perf_counter()
+            nn = orc.pow_grind(dd, cpu_sample_bits)
+            cs = time.perf_counter() - t0
+            out["verified"] = nn == hp.grind_pow_nonce(dd, cpu_sample_bits) and nonces[0] == orc.pow_grind(
+                bytes(random.Random(0).randrange(256) for _ in range(32)), 19)
+            out["cpu_baseline"] = {"value": (nn + 1) / cs, "unit": "hashes/s", "cores": 1, "kind": "port",
+                                   "sample": f"the oracle's scalar grind of one digest at {cpu_sample_bits} bits: "
+                                             f"{nn + 1} hashes in {cs:.2f} s on 1 thread"}
+        except Exception as ex:  # the oracle is only a checker here
+            out["verified"] = f"oracle unavailable: {ex}"
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -422,6 +464,7 @@ def main():
 
     kernels = None if args.no_kernels else kernel_table(torch, nat, hp, sptr, st)
     witness = witness_leg(torch, nat, hp, sptr, st, 15, 50, 0 if args.no_cpu_baseline else 13)
+    pow_ = pow_leg(hp, 0 if args.no_cpu_baseline else 22)
 
     traffic, valu = encode_counters()
     line = {
@@ -461,6 +504,7 @@ def main():
                             "compressions_per_s": (n_pos * 17 - 1) / (mer_ms / 1e3)},
         "m32_l0_commit": m32,
         "witness": witness,
+        "pow": pow_,
         "kernels_m32": kernels,
         "cpu_baseline": cpu,
     }

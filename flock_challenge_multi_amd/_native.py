@@ -89,6 +89,8 @@ SIGNATURES = {
     "flk_zc_round_pair": (_I, [_P, _P, _SZ, _P, _P, _P]),
     "flk_zc_round1": (_I, [_P, _P, _P, _SZ, _P, _P, _P, _P]),
     "flk_blake3_witness": (_I, [_P, _SZ, _SZ, _P, _P, _P, _P, _P]),
+    "flk_pow_scan": (_I, [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), _P]),
+    "flk_pow_grind": (_I, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), _P]),
 }
 
 _lib = None

@@ -14,6 +14,7 @@ Names, argument meaning and error behaviour follow the reference entry points
   generate_witness_with_ab_packed[_and_lincheck]
                                            CP/r1cs_hashes/blake3.rs:2184,4617
                                            (CP = crates/flock-prover/src/)
+  pow_scan / grind_pow_nonce               CF/challenger.rs:384-474,1648
 
 Buffers are torch tensors on the GPU: F128 arrays are int64 tensors of shape
 (n, 2) holding (lo, hi); hashes are uint8 tensors of shape (n, 32).  Work is
@@ -504,6 +505,37 @@ def generate_witness_with_ab_packed_and_lincheck(blocks, n_blocks_log):
 def generate_witness_with_ab_packed(blocks, n_blocks_log):
     """(z, a, b) as above without the stripe (blake3.rs:2184-2206)."""
     return _witness(blocks, n_blocks_log, False)[:3]
+
+
+# ---- proof-of-work grinding (CF/challenger.rs, BLAKE3 transcripts) ------------
+def _digest_buf(state_digest):
+    import ctypes
+    d = bytes(state_digest)
+    if len(d) != 32:
+        raise ValueError("state_digest must be 32 bytes")
+    return (ctypes.c_uint8 * 32).from_buffer_copy(d)
+
+
+def pow_scan(state_digest, start, length, bits):
+    """pow_scan: the smallest nonce in [start, start + length) whose BLAKE3 PoW
+    hash has `bits` leading zero bits, or None (CF/challenger.rs:1648-1660)."""
+    import ctypes
+    if not 0 <= bits <= 64:
+        raise ValueError("bits must be in 0..64")
+    out = ctypes.c_uint64()
+    nat.call("flk_pow_scan", _digest_buf(state_digest), start, length, bits, ctypes.byref(out), _stream())
+    return None if out.value == 2**64 - 1 else out.value
+
+
+def grind_pow_nonce(state_digest, bits):
+    """The nonce grind_pow finds for this transcript digest (CF/challenger.rs:384-474):
+    the smallest satisfying one, 0 when bits == 0.  The caller absorbs it."""
+    import ctypes
+    if not 0 <= bits <= 48:
+        raise ValueError("bits must be in 0..48")
+    out = ctypes.c_uint64()
+    nat.call("flk_pow_grind", _digest_buf(state_digest), bits, ctypes.byref(out), _stream())
+    return out.value
 
 
 class SumcheckProver:

@@ -307,6 +307,22 @@ typedef struct {
 int flk_blake3_witness(const flk_compression *blocks, size_t n_blocks, size_t n_blocks_log, flk_f128 *z,
                        flk_f128 *a, flk_f128 *b, uint8_t *z_lincheck, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Proof-of-work grinding, BLAKE3 transcripts (CF/challenger.rs).            */
+/* ------------------------------------------------------------------------ */
+
+/* pow_scan / blake3_pow_scan (:605-640,1648-1660): *nonce_out = the smallest n in
+ * [start, start + len) (the range ends at 2^64 - 1 at most) whose hash
+ * blake3(state_digest || u64le(n) || 24 zero bytes) has >= bits leading zero
+ * bits (byte 0's most significant bit first), or UINT64_MAX when there is none.
+ * state_digest and nonce_out are HOST pointers; synchronises `stream`. */
+int flk_pow_scan(const uint8_t *state_digest, uint64_t start, uint64_t len, uint32_t bits, uint64_t *nonce_out,
+                 void *stream);
+/* grind_pow's nonce (:384-474): the smallest satisfying nonce from 0, or 0 when
+ * bits == 0; 1 <= bits <= 48 otherwise.  The caller absorbs u64le(nonce) into its
+ * transcript (observe_bytes), as grind_pow does.  HOST pointers; synchronises. */
+int flk_pow_grind(const uint8_t *state_digest, uint32_t bits, uint64_t *nonce_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

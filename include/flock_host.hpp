@@ -314,4 +314,38 @@ inline std::pair<std::vector<F128>, std::vector<F128>> round1(const DeviceVec<ui
 }
 }  // namespace zerocheck
 
+// ---------------------------------------------------------------------------
+// flock_prover::r1cs_hashes::blake3 witness generation
+// (crates/flock-prover/src/r1cs_hashes/blake3.rs:2184-2206,4617-4647)
+// ---------------------------------------------------------------------------
+namespace r1cs_hashes {
+namespace blake3 {
+using Compression = flk_compression;  // (cv, m, counter, block_len, flags)
+
+struct Witness {
+  DeviceVec<F128> z, a, b;       // 2^n_blocks_log x 128 elements each
+  DeviceVec<uint8_t> z_lincheck;  // 2^n_blocks_log x 2048 bytes (empty without the stripe)
+};
+
+inline Witness generate_witness_with_ab_packed_and_lincheck(const DeviceVec<Compression>& blocks,
+                                                            size_t n_blocks_log, bool lincheck = true,
+                                                            void* stream = nullptr) {
+  if (n_blocks_log < 3 || n_blocks_log > 30)
+    throw std::invalid_argument("generate_witness: n_blocks_log must be in 3..30");
+  const size_t n_total = size_t(1) << n_blocks_log;
+  if (blocks.len() > n_total) throw std::invalid_argument("generate_witness: more compressions than slots");
+  Witness w{DeviceVec<F128>(n_total * 128), DeviceVec<F128>(n_total * 128), DeviceVec<F128>(n_total * 128),
+            lincheck ? DeviceVec<uint8_t>(n_total * 2048) : DeviceVec<uint8_t>()};
+  check(flk_blake3_witness(blocks.data(), blocks.len(), n_blocks_log, w.z.data(), w.a.data(), w.b.data(),
+                           lincheck ? w.z_lincheck.data() : nullptr, stream),
+        "generate_witness");
+  return w;
+}
+inline Witness generate_witness_with_ab_packed(const DeviceVec<Compression>& blocks, size_t n_blocks_log,
+                                               void* stream = nullptr) {
+  return generate_witness_with_ab_packed_and_lincheck(blocks, n_blocks_log, false, stream);
+}
+}  // namespace blake3
+}  // namespace r1cs_hashes
+
 }  // namespace flock

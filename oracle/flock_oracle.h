@@ -140,6 +140,17 @@ void orc_blake3_block_witness(const orc_compression *blk, uint64_t *z, uint64_t 
 int orc_blake3_witness(const orc_compression *blocks, size_t n_blocks, size_t n_blocks_log,
                        uint64_t *z, uint64_t *a, uint64_t *b, uint8_t *stripe);
 
+
+/* ---- Proof-of-work grinding (pow.c; CF/challenger.rs:384-474,549-601) ---- */
+/* blake3(state_digest || nonce_le || 24 zero bytes). */
+void orc_pow_hash(const uint8_t digest[32], uint64_t nonce, uint8_t out[32]);
+/* 1 when h has >= bits leading zero bits (byte 0's most significant bit first). */
+int orc_leading_zero_bits_ok(const uint8_t h[32], uint32_t bits);
+/* verify_pow's check (bits == 0 accepts only nonce 0). */
+int orc_pow_ok(const uint8_t digest[32], uint64_t nonce, uint32_t bits);
+/* grind_pow's nonce: the smallest satisfying nonce (0 when bits == 0). */
+uint64_t orc_pow_grind(const uint8_t digest[32], uint32_t bits);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,6 +4,7 @@
 // fold/introduce/glue sequence of CF/pcs/ligerito.rs:8906-9275).  Exit code 0 = pass.
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "flock_host.hpp"
@@ -102,6 +103,27 @@ int main() {
     orc_rs_fold_1b_rows(reinterpret_cast<orc_f128*>(z.data()), reinterpret_cast<orc_f128*>(e.data()), z.size(),
                         reinterpret_cast<orc_f128*>(want.data()));
     EXPECT(same(got, want), "fold_1b_rows");
+  }
+  // BLAKE3 witness: 13 compressions in 16 slots (3 padding), with the stripe
+  {
+    std::vector<flk_compression> comps(13);
+    uint64_t st = 99;
+    for (auto& c : comps) {
+      uint32_t* w = reinterpret_cast<uint32_t*>(&c);
+      for (int i = 0; i < 28; ++i) w[i] = (uint32_t)orc_splitmix64_next(&st);
+    }
+    auto wit = flock::r1cs_hashes::blake3::generate_witness_with_ab_packed_and_lincheck(
+        DeviceVec<flk_compression>(comps), 4);
+    std::vector<uint64_t> z(16 * 256), a(16 * 256), b(16 * 256);
+    std::vector<uint8_t> stripe(16 * 2048);
+    EXPECT(orc_blake3_witness(reinterpret_cast<const orc_compression*>(comps.data()), comps.size(), 4, z.data(),
+                              a.data(), b.data(), stripe.data()) == 0,
+           "oracle witness");
+    auto gz = wit.z.to_host(), ga = wit.a.to_host(), gb = wit.b.to_host();
+    EXPECT(std::memcmp(gz.data(), z.data(), z.size() * 8) == 0, "witness z");
+    EXPECT(std::memcmp(ga.data(), a.data(), a.size() * 8) == 0, "witness a");
+    EXPECT(std::memcmp(gb.data(), b.data(), b.size() * 8) == 0, "witness b");
+    EXPECT(wit.z_lincheck.to_host() == stripe, "witness lincheck stripe");
   }
   std::printf("host mirror parity ok\n");
   return 0;

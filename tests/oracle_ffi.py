@@ -97,6 +97,10 @@ class Oracle:
             "orc_blake3_block_witness": (None, [P, P, P, P]),
             "orc_blake3_witness": (ctypes.c_int, [P, SZ, SZ, P, P, P, P]),
             "orc_blake3_compress": (None, [P, P, U64, ctypes.c_uint32, ctypes.c_uint32, P]),
+            "orc_pow_hash": (None, [P, U64, P]),
+            "orc_leading_zero_bits_ok": (ctypes.c_int, [P, ctypes.c_uint32]),
+            "orc_pow_ok": (ctypes.c_int, [P, U64, ctypes.c_uint32]),
+            "orc_pow_grind": (U64, [P, ctypes.c_uint32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -198,6 +202,25 @@ class Oracle:
         if rc:
             raise ValueError("orc_blake3_witness rejected the shape")
         return z, a, b, st
+
+    # ---- proof-of-work grinding (oracle/pow.c) ----
+    def pow_hash(self, digest, nonce):
+        d = np.frombuffer(bytes(digest), np.uint8).copy()
+        out = np.zeros(32, np.uint8)
+        self.L.orc_pow_hash(_p(d), nonce, _p(out))
+        return out.tobytes()
+
+    def lz_ok(self, h, bits):
+        d = np.frombuffer(bytes(h), np.uint8).copy()
+        return bool(self.L.orc_leading_zero_bits_ok(_p(d), bits))
+
+    def pow_ok(self, digest, nonce, bits):
+        d = np.frombuffer(bytes(digest), np.uint8).copy()
+        return bool(self.L.orc_pow_ok(_p(d), nonce, bits))
+
+    def pow_grind(self, digest, bits):
+        d = np.frombuffer(bytes(digest), np.uint8).copy()
+        return int(self.L.orc_pow_grind(_p(d), bits))
 
     def compress(self, cv, block_bytes, counter, block_len, flags):
         cvv = np.asarray(cv, np.uint32).copy()
