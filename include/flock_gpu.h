@@ -315,7 +315,9 @@ int flk_blake3_witness(const flk_compression *blocks, size_t n_blocks, size_t n_
  * [start, start + len) (the range ends at 2^64 - 1 at most) whose hash
  * blake3(state_digest || u64le(n) || 24 zero bytes) has >= bits leading zero
  * bits (byte 0's most significant bit first), or UINT64_MAX when there is none.
- * state_digest and nonce_out are HOST pointers; synchronises `stream`. */
+ * state_digest and nonce_out are HOST pointers; synchronises `stream`.  The PoW
+ * calls keep an 8-byte device word and a pinned host word per device for the
+ * process's life and run one at a time per process. */
 int flk_pow_scan(const uint8_t *state_digest, uint64_t start, uint64_t len, uint32_t bits, uint64_t *nonce_out,
                  void *stream);
 /* grind_pow's nonce (:384-474): the smallest satisfying nonce from 0, or 0 when
