@@ -91,6 +91,18 @@ SIGNATURES = {
     "flk_blake3_witness": (_I, [_P, _SZ, _SZ, _P, _P, _P, _P, _P]),
     "flk_pow_scan": (_I, [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), _P]),
     "flk_pow_grind": (_I, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), _P]),
+    "flk_fs_new": (_I, [_P, _SZ, ctypes.POINTER(ctypes.c_void_p)]),
+    "flk_fs_clone": (_I, [_P, ctypes.POINTER(ctypes.c_void_p)]),
+    "flk_fs_free": (None, [_P]),
+    "flk_fs_observe_label": (_I, [_P, _P, _SZ]),
+    "flk_fs_observe_f128": (_I, [_P, _F]),
+    "flk_fs_observe_f128_slice": (_I, [_P, _P, _SZ]),
+    "flk_fs_observe_bytes": (_I, [_P, _P, _SZ]),
+    "flk_fs_sample_f128": (_I, [_P, ctypes.POINTER(_F)]),
+    "flk_fs_sample_f128_vec": (_I, [_P, _SZ, _P]),
+    "flk_fs_state_digest": (_I, [_P, _P]),
+    "flk_fs_grind_pow": (_I, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), _P]),
+    "flk_fs_verify_pow": (_I, [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int)]),
 }
 
 _lib = None

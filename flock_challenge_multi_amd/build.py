@@ -34,13 +34,26 @@ def _compile(src, obj):
     return err
 
 
+def _compile_host(src, obj):
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{r.stderr}")
+    return r.stderr
+
+
 def build(verbose=False):
+    """.hip sources go through hipcc for gfx950; .cpp sources (host-only code,
+    the Fiat-Shamir transcript) through g++; one shared library."""
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    hsrcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".cpp"))
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = [os.path.join(objdir, os.path.basename(s)[:-4] + ".o") for s in srcs]
-    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        errs = list(ex.map(_compile, srcs, objs))
+    hobjs = [os.path.join(objdir, os.path.basename(s)[:-4] + ".host.o") for s in hsrcs]
+    with ThreadPoolExecutor(max_workers=min(8, len(srcs) + len(hsrcs))) as ex:
+        errs = list(ex.map(_compile, srcs, objs)) + list(ex.map(_compile_host, hsrcs, hobjs))
+    objs = objs + hobjs
     if verbose:
         for e in errs:
             if e.strip():

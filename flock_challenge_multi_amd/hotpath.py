@@ -15,6 +15,7 @@ Names, argument meaning and error behaviour follow the reference entry points
                                            CP/r1cs_hashes/blake3.rs:2184,4617
                                            (CP = crates/flock-prover/src/)
   pow_scan / grind_pow_nonce               CF/challenger.rs:384-474,1648
+  FsChallenger (BLAKE3 transcript)         CF/challenger.rs:226-523
 
 Buffers are torch tensors on the GPU: F128 arrays are int64 tensors of shape
 (n, 2) holding (lo, hi); hashes are uint8 tensors of shape (n, 32).  Work is
@@ -536,6 +537,86 @@ def grind_pow_nonce(state_digest, bits):
     out = ctypes.c_uint64()
     nat.call("flk_pow_grind", _digest_buf(state_digest), bits, ctypes.byref(out), _stream())
     return out.value
+
+
+class FsChallenger:
+    """FsChallenger::with_hash(domain, HashKind::Blake3) (CF/challenger.rs:226-523):
+    the Fiat-Shamir transcript on the host; grind_pow runs the nonce search on
+    the device (flk_pow_grind), everything else is host BLAKE3."""
+
+    def __init__(self, domain):
+        import ctypes
+        d = bytes(domain)
+        h = ctypes.c_void_p()
+        buf = (ctypes.c_uint8 * max(len(d), 1)).from_buffer_copy(d or b"\0")
+        nat.call("flk_fs_new", buf, len(d), ctypes.byref(h))
+        self._h = h
+
+    @classmethod
+    def with_hash(cls, domain):
+        return cls(domain)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            nat.load().flk_fs_free(h)
+            self._h = None
+
+    def clone(self):
+        import ctypes
+        c = FsChallenger.__new__(FsChallenger)
+        h = ctypes.c_void_p()
+        nat.call("flk_fs_clone", self._h, ctypes.byref(h))
+        c._h = h
+        return c
+
+    @staticmethod
+    def _bytes(b):
+        import ctypes
+        b = bytes(b)
+        return (ctypes.c_uint8 * max(len(b), 1)).from_buffer_copy(b or b"\0"), len(b)
+
+    def observe_label(self, label):
+        nat.call("flk_fs_observe_label", self._h, *self._bytes(label))
+
+    def observe_bytes(self, data):
+        nat.call("flk_fs_observe_bytes", self._h, *self._bytes(data))
+
+    def observe_f128(self, v):
+        nat.call("flk_fs_observe_f128", self._h, F128.from_int(v))
+
+    def observe_f128_slice(self, vals):
+        arr = (F128 * max(len(vals), 1))(*[F128.from_int(v) for v in vals])
+        nat.call("flk_fs_observe_f128_slice", self._h, arr, len(vals))
+
+    def sample_f128(self):
+        out = F128()
+        import ctypes
+        nat.call("flk_fs_sample_f128", self._h, ctypes.byref(out))
+        return out.to_int()
+
+    def sample_f128_vec(self, n):
+        arr = (F128 * max(n, 1))()
+        nat.call("flk_fs_sample_f128_vec", self._h, n, arr)
+        return [arr[i].to_int() for i in range(n)]
+
+    def state_digest(self):
+        import ctypes
+        out = (ctypes.c_uint8 * 32)()
+        nat.call("flk_fs_state_digest", self._h, out)
+        return bytes(out)
+
+    def grind_pow(self, bits):
+        import ctypes
+        n = ctypes.c_uint64()
+        nat.call("flk_fs_grind_pow", self._h, bits, ctypes.byref(n), _stream())
+        return n.value
+
+    def verify_pow(self, nonce, bits):
+        import ctypes
+        ok = ctypes.c_int()
+        nat.call("flk_fs_verify_pow", self._h, nonce, bits, ctypes.byref(ok))
+        return bool(ok.value)
 
 
 class SumcheckProver:

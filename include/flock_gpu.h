@@ -325,6 +325,41 @@ int flk_pow_scan(const uint8_t *state_digest, uint64_t start, uint64_t len, uint
  * transcript (observe_bytes), as grind_pow does.  HOST pointers; synchronises. */
 int flk_pow_grind(const uint8_t *state_digest, uint32_t bits, uint64_t *nonce_out, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Fiat-Shamir transcript over BLAKE3, host side (CF/challenger.rs).         */
+/* FsChallenger::with_hash(domain, HashKind::Blake3) and its Challenger      */
+/* methods (:170-177,226-383,384-474,502-517); every pointer is HOST memory. */
+/* ------------------------------------------------------------------------ */
+typedef struct flk_fs flk_fs;
+
+/* with_hash(domain, Blake3) (:237-250): absorbs 0x01 || u64le(len) || domain. */
+int flk_fs_new(const uint8_t *domain, size_t len, flk_fs **out);
+/* Clone (the transcript state is copied). */
+int flk_fs_clone(const flk_fs *t, flk_fs **out);
+void flk_fs_free(flk_fs *t);
+/* observe_label: 0x02 || u64le(len) || label. */
+int flk_fs_observe_label(flk_fs *t, const uint8_t *label, size_t len);
+/* observe_f128: 0x03 0x01 || lo || hi (little-endian). */
+int flk_fs_observe_f128(flk_fs *t, flk_f128 v);
+/* observe_f128_slice: 0x03 0x02 || u64le(n) || values. */
+int flk_fs_observe_f128_slice(flk_fs *t, const flk_f128 *v, size_t n);
+/* observe_bytes: 0x05 || u64le(len) || bytes. */
+int flk_fs_observe_bytes(flk_fs *t, const uint8_t *bytes, size_t len);
+/* sample_f128 (:355-366): absorb 0x04 0x01, read 16 bytes of the state's XOF,
+ * absorb them; the element is (lo, hi) little-endian. */
+int flk_fs_sample_f128(flk_fs *t, flk_f128 *out);
+/* sample_f128_vec (:368-382): absorb 0x04 0x02 || u64le(n), read 16 n XOF bytes,
+ * absorb them. */
+int flk_fs_sample_f128_vec(flk_fs *t, size_t n, flk_f128 *out);
+/* state_digest (:309-317): the 32-byte hash of everything absorbed. */
+int flk_fs_state_digest(const flk_fs *t, uint8_t out[32]);
+/* grind_pow (:384-474): flk_pow_grind on the state digest (device work on
+ * `stream`), then observe_bytes(u64le(nonce)); bits == 0 gives nonce 0. */
+int flk_fs_grind_pow(flk_fs *t, uint32_t bits, uint64_t *nonce_out, void *stream);
+/* verify_pow (:502-523): *ok_out = the PoW check (nonce == 0 when bits == 0);
+ * observe_bytes(u64le(nonce)) whatever the outcome.  Host only. */
+int flk_fs_verify_pow(flk_fs *t, uint64_t nonce, uint32_t bits, int *ok_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -134,6 +134,53 @@ void orc_blake3_hash(const uint8_t *data, size_t len, uint8_t out[32]) {
     words_to_bytes(cv, out);
 }
 
+/* Extended output (XOF): the root node's last compression, its input kept,
+ * evaluated with ROOT and counter = output block index, 64 bytes a block. */
+void orc_blake3_xof(const uint8_t *data, size_t len, uint8_t *out, size_t out_len) {
+    uint32_t cv[8];
+    uint8_t blk[64];
+    uint32_t bl, fl;
+    size_t nchunks = len == 0 ? 1 : (len + CHUNK_LEN - 1) / CHUNK_LEN;
+    memset(blk, 0, sizeof blk);
+    if (nchunks == 1) {
+        /* single chunk: every block but the last chains from the IV */
+        size_t nblocks = len == 0 ? 1 : (len + 63) / 64;
+        uint32_t o16[16];
+        memcpy(cv, IV, sizeof cv);
+        for (size_t i = 0; i + 1 < nblocks; i++) {
+            orc_blake3_compress(cv, data + 64 * i, 0, 64, i == 0 ? CHUNK_START : 0, o16);
+            memcpy(cv, o16, 32);
+        }
+        size_t off = 64 * (nblocks - 1);
+        bl = (uint32_t)(len - off);
+        if (bl) memcpy(blk, data + off, bl);
+        fl = (nblocks == 1 ? CHUNK_START : 0) | CHUNK_END;
+    } else {
+        size_t left_chunks = 1;
+        while (left_chunks * 2 < nchunks) left_chunks *= 2;
+        size_t left_len = left_chunks * CHUNK_LEN;
+        uint32_t l[8], r[8];
+        subtree_cv(data, left_len, 0, 0, l);
+        subtree_cv(data + left_len, len - left_len, left_chunks, 0, r);
+        memcpy(cv, IV, sizeof cv);
+        words_to_bytes(l, blk);
+        words_to_bytes(r, blk + 32);
+        bl = 64;
+        fl = PARENT;
+    }
+    for (uint64_t b = 0; out_len; b++) {
+        uint32_t o16[16];
+        uint8_t bytes[64];
+        orc_blake3_compress(cv, blk, b, bl, fl | ROOT, o16);
+        for (int i = 0; i < 16; i++)
+            for (int j = 0; j < 4; j++) bytes[4 * i + j] = (uint8_t)(o16[i] >> (8 * j));
+        size_t take = out_len < 64 ? out_len : 64;
+        memcpy(out, bytes, take);
+        out += take;
+        out_len -= take;
+    }
+}
+
 void orc_blake3_leaf_cv(const uint8_t *data, size_t len, uint8_t out[32]) {
     uint32_t cv[8];
     subtree_cv(data, len, 0, 0, cv);

@@ -63,6 +63,8 @@ orc_f128 orc_subspace_poly_hat(size_t i, orc_f128 z);
 void orc_blake3_compress(const uint32_t cv[8], const uint8_t block[64], uint64_t counter,
                          uint32_t block_len, uint32_t flags, uint32_t out16[16]);
 void orc_blake3_hash(const uint8_t *data, size_t len, uint8_t out[32]);
+/* Extended output: out_len bytes of the XOF (the first 32 are the hash). */
+void orc_blake3_xof(const uint8_t *data, size_t len, uint8_t *out, size_t out_len);
 /* Non-root chaining value of `data` (Hasher::update + finalize_non_root). */
 void orc_blake3_leaf_cv(const uint8_t *data, size_t len, uint8_t out[32]);
 /* PARENT-flagged non-root merge of two chaining values. */

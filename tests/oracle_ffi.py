@@ -98,6 +98,7 @@ class Oracle:
             "orc_blake3_witness": (ctypes.c_int, [P, SZ, SZ, P, P, P, P]),
             "orc_blake3_compress": (None, [P, P, U64, ctypes.c_uint32, ctypes.c_uint32, P]),
             "orc_pow_hash": (None, [P, U64, P]),
+            "orc_blake3_xof": (None, [P, SZ, P, SZ]),
             "orc_leading_zero_bits_ok": (ctypes.c_int, [P, ctypes.c_uint32]),
             "orc_pow_ok": (ctypes.c_int, [P, U64, ctypes.c_uint32]),
             "orc_pow_grind": (U64, [P, ctypes.c_uint32]),
@@ -228,6 +229,12 @@ class Oracle:
         out = np.zeros(16, np.uint32)
         self.L.orc_blake3_compress(_p(cvv), _p(blk), counter, block_len, flags, _p(out))
         return out
+
+    def blake3_xof(self, data, n):
+        b = np.frombuffer(bytes(data), dtype=np.uint8).copy() if len(data) else np.zeros(1, np.uint8)
+        out = np.zeros(max(n, 1), np.uint8)
+        self.L.orc_blake3_xof(_p(b), len(data), _p(out), n)
+        return out[:n].tobytes()
 
     def blake3(self, data):
         b = np.frombuffer(bytes(data), dtype=np.uint8).copy() if len(data) else np.zeros(1, np.uint8)

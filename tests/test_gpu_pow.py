@@ -76,3 +76,25 @@ def test_scan_many_offsets(orc, dev):
         bits = r.randrange(1, 9)
         want = next((n for n in range(start, start + 3000) if orc.pow_ok(d, n, bits)), None)
         assert hp.pow_scan(d, start, 3000, bits) == want
+
+
+@pytest.mark.parametrize("bits", [0, 5, 10, 14, 18])
+def test_transcript_grind_roundtrip(orc, dev, bits):
+    """fs_challenger_pow_roundtrip (CF/challenger.rs:2185-2209) with the device
+    grind: the verifier accepts the nonce, the nonce is the oracle's, and the
+    challenges after it agree."""
+    def script():
+        t = hp.FsChallenger(b"pow-test")
+        t.observe_label(b"flock-pow-test")
+        t.observe_bytes(b"some root data")
+        return t
+    prover, verifier = script(), script()
+    d = prover.state_digest()
+    nonce = prover.grind_pow(bits)
+    assert nonce == orc.pow_grind(d, bits)
+    assert verifier.verify_pow(nonce, bits)
+    for _ in range(4):
+        assert prover.sample_f128() == verifier.sample_f128()
+    if bits:
+        wrong = script()
+        assert not wrong.verify_pow(nonce + 1, bits) or orc.pow_ok(d, nonce + 1, bits)
