@@ -315,6 +315,63 @@ inline std::pair<std::vector<F128>, std::vector<F128>> round1(const DeviceVec<ui
 }  // namespace zerocheck
 
 // ---------------------------------------------------------------------------
+// crate::challenger::FsChallenger over BLAKE3 (CF/challenger.rs:226-523):
+// host transcript, device nonce search in grind_pow
+// ---------------------------------------------------------------------------
+namespace challenger {
+class FsChallenger {
+ public:
+  explicit FsChallenger(const std::string& domain) {
+    check(flk_fs_new(reinterpret_cast<const uint8_t*>(domain.data()), domain.size(), &t_), "fs_new");
+  }
+  static FsChallenger with_hash(const std::string& domain) { return FsChallenger(domain); }
+  ~FsChallenger() {
+    if (t_) flk_fs_free(t_);
+  }
+  FsChallenger(FsChallenger&& o) noexcept : t_(o.t_) { o.t_ = nullptr; }
+  FsChallenger(const FsChallenger& o) { check(flk_fs_clone(o.t_, &t_), "fs_clone"); }
+  FsChallenger& operator=(const FsChallenger&) = delete;
+
+  void observe_label(const std::string& l) {
+    check(flk_fs_observe_label(t_, reinterpret_cast<const uint8_t*>(l.data()), l.size()), "observe_label");
+  }
+  void observe_bytes(const uint8_t* b, size_t n) { check(flk_fs_observe_bytes(t_, b, n), "observe_bytes"); }
+  void observe_f128(F128 v) { check(flk_fs_observe_f128(t_, v), "observe_f128"); }
+  void observe_f128_slice(const std::vector<F128>& v) {
+    check(flk_fs_observe_f128_slice(t_, v.data(), v.size()), "observe_f128_slice");
+  }
+  F128 sample_f128() {
+    F128 v;
+    check(flk_fs_sample_f128(t_, &v), "sample_f128");
+    return v;
+  }
+  std::vector<F128> sample_f128_vec(size_t n) {
+    std::vector<F128> v(n);
+    check(flk_fs_sample_f128_vec(t_, n, v.data()), "sample_f128_vec");
+    return v;
+  }
+  std::array<uint8_t, 32> state_digest() const {
+    std::array<uint8_t, 32> d;
+    check(flk_fs_state_digest(t_, d.data()), "state_digest");
+    return d;
+  }
+  uint64_t grind_pow(uint32_t bits, void* stream = nullptr) {
+    uint64_t n = 0;
+    check(flk_fs_grind_pow(t_, bits, &n, stream), "grind_pow");
+    return n;
+  }
+  bool verify_pow(uint64_t nonce, uint32_t bits) {
+    int ok = 0;
+    check(flk_fs_verify_pow(t_, nonce, bits, &ok), "verify_pow");
+    return ok != 0;
+  }
+
+ private:
+  flk_fs* t_ = nullptr;
+};
+}  // namespace challenger
+
+// ---------------------------------------------------------------------------
 // flock_prover::r1cs_hashes::blake3 witness generation
 // (crates/flock-prover/src/r1cs_hashes/blake3.rs:2184-2206,4617-4647)
 // ---------------------------------------------------------------------------

@@ -125,6 +125,23 @@ int main() {
     EXPECT(std::memcmp(gb.data(), b.data(), b.size() * 8) == 0, "witness b");
     EXPECT(wit.z_lincheck.to_host() == stripe, "witness lincheck stripe");
   }
+  // FsChallenger: device grind, host verify, challenges in lockstep
+  {
+    auto script = [] {
+      flock::challenger::FsChallenger t("pow-test");
+      t.observe_label("flock-pow-test");
+      const uint8_t root[4] = {'r', 'o', 'o', 't'};
+      t.observe_bytes(root, 4);
+      return t;
+    };
+    auto prover = script();
+    auto verifier = script();
+    auto d = prover.state_digest();
+    const uint64_t n = prover.grind_pow(12);
+    EXPECT(n == orc_pow_grind(d.data(), 12), "grind_pow nonce == oracle");
+    EXPECT(verifier.verify_pow(n, 12), "verify_pow accepts");
+    EXPECT(prover.sample_f128() == verifier.sample_f128(), "challenges agree after the PoW");
+  }
   std::printf("host mirror parity ok\n");
   return 0;
 }
