@@ -103,6 +103,9 @@ SIGNATURES = {
     "flk_fs_state_digest": (_I, [_P, _P]),
     "flk_fs_grind_pow": (_I, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), _P]),
     "flk_fs_verify_pow": (_I, [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int)]),
+    "flk_fs_sample_distinct_queries": (_I, [_P, ctypes.c_uint64, _SZ, _P]),
+    "flk_gather_rows": (_I, [_P, _SZ, _SZ, _P, _SZ, _P, _P]),
+    "flk_merkle_multi_proof": (_I, [_P, _SZ, _P, _SZ, _P, _SZ, ctypes.POINTER(_SZ), _P]),
 }
 
 _lib = None

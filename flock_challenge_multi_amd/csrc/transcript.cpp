@@ -14,6 +14,9 @@
 //   sample_f128_vec(n)     absorb 0x04 0x02 || u64le(n), read 16 n XOF bytes,
 //                          absorb them
 //   state_digest()         the 32-byte BLAKE3 hash of the stream so far
+//   sample_distinct_queries(block_len, count)   sample_f128 until `count`
+//                          distinct q = lo % block_len (first occurrence kept),
+//                          sorted (CF/pcs/ligerito.rs:9665-9720)
 //   grind_pow(bits)        nonce = flk_pow_grind(state_digest, bits) (device),
 //                          then observe_bytes(u64le(nonce))
 //   verify_pow(n, bits)    the PoW check against state_digest (n == 0 when
@@ -24,6 +27,7 @@
 // the device kernels have their own copy (blake3_dev.hpp).  The SHA-256
 // transcript of the reference is not provided: the prover's BLAKE3 profile
 // uses this one.
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <new>
@@ -344,6 +348,26 @@ int flk_fs_sample_f128(flk_fs* t, flk_f128* out) {
     hi |= (uint64_t)buf[8 + j] << (8 * j);
   }
   *out = flk_f128{lo, hi};
+  return FLK_OK;
+}
+
+int flk_fs_sample_distinct_queries(flk_fs* t, uint64_t block_len, size_t count, uint64_t* out) {
+  if (!t || (count && !out)) return set_error(FLK_E_SHAPE, "fs_sample_distinct_queries: null pointer");
+  if (count > block_len)
+    return set_error(FLK_E_SHAPE, "fs_sample_distinct_queries: count %zu > block_len %llu", count,
+                     (unsigned long long)block_len);
+  std::vector<uint64_t> got;
+  got.reserve(count);
+  while (got.size() < count) {
+    flk_f128 v;
+    flk_fs_sample_f128(t, &v);
+    const uint64_t q = v.lo % block_len;
+    bool seen = false;
+    for (uint64_t x : got) seen = seen || x == q;
+    if (!seen) got.push_back(q);
+  }
+  std::sort(got.begin(), got.end());
+  std::copy(got.begin(), got.end(), out);
   return FLK_OK;
 }
 

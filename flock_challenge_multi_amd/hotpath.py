@@ -16,6 +16,7 @@ Names, argument meaning and error behaviour follow the reference entry points
                                            (CP = crates/flock-prover/src/)
   pow_scan / grind_pow_nonce               CF/challenger.rs:384-474,1648
   FsChallenger (BLAKE3 transcript)         CF/challenger.rs:226-523
+  gather_rows / merkle_multi_proof         CF/merkle.rs:1020-1107 (query openings)
 
 Buffers are torch tensors on the GPU: F128 arrays are int64 tensors of shape
 (n, 2) holding (lo, hi); hashes are uint8 tensors of shape (n, 32).  Work is
@@ -29,6 +30,11 @@ import torch
 
 from . import _native as nat
 from ._native import F128, FlockGpuError  # noqa: F401  (re-exported)
+
+
+import ctypes as _ctypes  # noqa: E402
+
+ctypes_u64 = _ctypes.c_uint64
 
 
 def _stream():
@@ -539,6 +545,35 @@ def grind_pow_nonce(state_digest, bits):
     return out.value
 
 
+def _u64_host(vals):
+    arr = (ctypes_u64 * max(len(vals), 1))(*[int(v) for v in vals])
+    return arr
+
+
+def gather_rows(codeword, row_len, positions, out=None):
+    """The opened rows of `positions` (host ints) from a SoA codeword tensor."""
+    n_el = _check_f128(codeword, "codeword")
+    if row_len <= 0 or n_el % row_len:
+        raise ValueError("gather_rows: the codeword length must be a multiple of row_len")
+    n = len(positions)
+    out = f128_empty(n * row_len, codeword.device) if out is None else out
+    nat.call("flk_gather_rows", _ptr(codeword), n_el // row_len, row_len, _u64_host(positions), n, _ptr(out),
+             _stream())
+    return out
+
+
+def merkle_multi_proof(tree, num_leaves, positions):
+    """merkle_multi_proof (CF/merkle.rs:1020-1060) of a device tree: (n, 32) uint8 tensor of the siblings."""
+    import ctypes
+    depth = max(num_leaves.bit_length() - 1, 1)
+    cap = max(len(positions) * depth, 1)
+    out = hashes_empty(cap, tree.device)
+    k = ctypes.c_size_t()
+    nat.call("flk_merkle_multi_proof", _ptr(tree), num_leaves, _u64_host(positions), len(positions), _ptr(out),
+             cap, ctypes.byref(k), _stream())
+    return out[: k.value]
+
+
 class FsChallenger:
     """FsChallenger::with_hash(domain, HashKind::Blake3) (CF/challenger.rs:226-523):
     the Fiat-Shamir transcript on the host; grind_pow runs the nonce search on
@@ -605,6 +640,12 @@ class FsChallenger:
         out = (ctypes.c_uint8 * 32)()
         nat.call("flk_fs_state_digest", self._h, out)
         return bytes(out)
+
+    def sample_distinct_queries(self, block_len, count):
+        """sample_distinct_queries (CF/pcs/ligerito.rs:9665-9720), sorted."""
+        arr = (ctypes_u64 * max(count, 1))()
+        nat.call("flk_fs_sample_distinct_queries", self._h, block_len, count, arr)
+        return [int(arr[i]) for i in range(count)]
 
     def grind_pow(self, bits):
         import ctypes

@@ -351,6 +351,10 @@ int flk_fs_sample_f128(flk_fs *t, flk_f128 *out);
 /* sample_f128_vec (:368-382): absorb 0x04 0x02 || u64le(n), read 16 n XOF bytes,
  * absorb them. */
 int flk_fs_sample_f128_vec(flk_fs *t, size_t n, flk_f128 *out);
+/* sample_distinct_queries (CF/pcs/ligerito.rs:9665-9720): sample_f128 until
+ * `count` distinct q = lo mod block_len are drawn (first occurrences), written
+ * to out sorted ascending; count <= block_len. */
+int flk_fs_sample_distinct_queries(flk_fs *t, uint64_t block_len, size_t count, uint64_t *out);
 /* state_digest (:309-317): the 32-byte hash of everything absorbed. */
 int flk_fs_state_digest(const flk_fs *t, uint8_t out[32]);
 /* grind_pow (:384-474): flk_pow_grind on the state digest (device work on
@@ -359,6 +363,22 @@ int flk_fs_grind_pow(flk_fs *t, uint32_t bits, uint64_t *nonce_out, void *stream
 /* verify_pow (:502-523): *ok_out = the PoW check (nonce == 0 when bits == 0);
  * observe_bytes(u64le(nonce)) whatever the outcome.  Host only. */
 int flk_fs_verify_pow(flk_fs *t, uint64_t nonce, uint32_t bits, int *ok_out);
+
+/* ------------------------------------------------------------------------ */
+/* Ligerito opening glue (CF/merkle.rs, CF/pcs/ligerito.rs).                 */
+/* ------------------------------------------------------------------------ */
+
+/* The opened rows: out[i * row_len + j] = codeword[positions[i] * row_len + j]
+ * (SoA codeword of n_rows rows of row_len elements, e.g. row_len = num_ntts).
+ * positions is a HOST array of n entries < n_rows; synchronises `stream`. */
+int flk_gather_rows(const flk_f128 *codeword, size_t n_rows, size_t row_len, const uint64_t *positions, size_t n,
+                    flk_f128 *out, void *stream);
+/* merkle_multi_proof (CF/merkle.rs:1020-1060) of a device flat tree (2 num_leaves
+ * - 1 hashes): the sibling hashes in emission order into proof_out (device,
+ * room for max_hashes), their count in *n_hashes_out (host).  positions is a
+ * HOST array (any order, repeats allowed); synchronises `stream`. */
+int flk_merkle_multi_proof(const uint8_t *tree, size_t num_leaves, const uint64_t *positions, size_t n,
+                           uint8_t *proof_out, size_t max_hashes, size_t *n_hashes_out, void *stream);
 
 #ifdef __cplusplus
 }

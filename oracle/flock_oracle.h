@@ -153,6 +153,17 @@ int orc_pow_ok(const uint8_t digest[32], uint64_t nonce, uint32_t bits);
 /* grind_pow's nonce: the smallest satisfying nonce (0 when bits == 0). */
 uint64_t orc_pow_grind(const uint8_t digest[32], uint32_t bits);
 
+
+/* ---- Ligerito opening glue (open.c; CF/merkle.rs:1020-1191) ------------- */
+/* Flat sibling indices of merkle_multi_proof, in emission order; out holds up
+ * to n * log2(num_leaves); returns the count. */
+size_t orc_merkle_multi_proof_indices(size_t num_leaves, const uint64_t *pos, size_t n, uint64_t *out);
+/* verify_merkle_multi_proof over BLAKE3 parents: pos sorted unique. */
+int orc_merkle_multi_verify(const uint8_t root[32], size_t num_leaves, const uint64_t *pos, const uint8_t *leaf_hashes,
+                            size_t n, const uint8_t *proof, size_t n_proof);
+/* out[i] = row pos[i] (row_len elements) of a SoA codeword. */
+void orc_gather_rows(const orc_f128 *codeword, size_t row_len, const uint64_t *pos, size_t n, orc_f128 *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,9 @@ class Oracle:
             "orc_blake3_compress": (None, [P, P, U64, ctypes.c_uint32, ctypes.c_uint32, P]),
             "orc_pow_hash": (None, [P, U64, P]),
             "orc_blake3_xof": (None, [P, SZ, P, SZ]),
+            "orc_merkle_multi_proof_indices": (SZ, [SZ, P, SZ, P]),
+            "orc_merkle_multi_verify": (ctypes.c_int, [P, SZ, P, P, SZ, P, SZ]),
+            "orc_gather_rows": (None, [P, SZ, P, SZ, P]),
             "orc_leading_zero_bits_ok": (ctypes.c_int, [P, ctypes.c_uint32]),
             "orc_pow_ok": (ctypes.c_int, [P, U64, ctypes.c_uint32]),
             "orc_pow_grind": (U64, [P, ctypes.c_uint32]),
@@ -229,6 +232,22 @@ class Oracle:
         out = np.zeros(16, np.uint32)
         self.L.orc_blake3_compress(_p(cvv), _p(blk), counter, block_len, flags, _p(out))
         return out
+
+    # ---- opening glue (oracle/open.c) ----
+    def multi_proof_indices(self, num_leaves, positions):
+        pos = np.asarray(positions, np.uint64).copy() if len(positions) else np.zeros(1, np.uint64)
+        depth = max(num_leaves.bit_length() - 1, 1)
+        out = np.zeros(max(len(positions) * depth, 1), np.uint64)
+        k = self.L.orc_merkle_multi_proof_indices(num_leaves, _p(pos), len(positions), _p(out))
+        return [int(x) for x in out[:k]]
+
+    def multi_verify(self, root, num_leaves, sorted_positions, leaf_hashes, proof):
+        pos = np.asarray(sorted_positions, np.uint64).copy() if len(sorted_positions) else np.zeros(1, np.uint64)
+        lh = np.ascontiguousarray(leaf_hashes, np.uint8).reshape(-1) if len(sorted_positions) else np.zeros(32, np.uint8)
+        pr = np.ascontiguousarray(proof, np.uint8).reshape(-1) if len(proof) else np.zeros(32, np.uint8)
+        r = np.frombuffer(bytes(root), np.uint8).copy()
+        return bool(self.L.orc_merkle_multi_verify(_p(r), num_leaves, _p(pos), _p(lh), len(sorted_positions),
+                                                   _p(pr), len(proof)))
 
     def blake3_xof(self, data, n):
         b = np.frombuffer(bytes(data), dtype=np.uint8).copy() if len(data) else np.zeros(1, np.uint8)

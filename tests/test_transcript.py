@@ -109,3 +109,17 @@ def test_verify_pow_on_host(orc):
         m = Model(orc, b"pow-test")
         m.label(b"flock-pow-test"), m.bytes_(b"some root data"), m.bytes_(struct.pack("<Q", n))
         assert good.state_digest() == m.digest()
+
+
+def test_sample_distinct_queries(orc):
+    for block_len, count in ((1 << 17, 148), (64, 64), (1000, 3), (7, 0)):
+        t, m = hp.FsChallenger(b"q"), Model(orc, b"q")
+        t.observe_bytes(b"root"), m.bytes_(b"root")
+        got = t.sample_distinct_queries(block_len, count)
+        want = []
+        while len(want) < count:
+            q = (m.sample() & (2**64 - 1)) % block_len
+            if q not in want:
+                want.append(q)
+        assert got == sorted(want)
+        assert t.state_digest() == m.digest()
