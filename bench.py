@@ -393,6 +393,24 @@ def main():
     mer_bytes = 16 * n_cw + 32 * (2 * n_pos - 1)
     enc_gbs = enc_bytes / (enc_ms / 1e3) / 1e9
 
+    # ---- opening glue on the committed codeword and tree: 218 queries (the
+    # m = 32 profile's L0 query count), rows gathered + the multi-proof --------
+    import random as _random
+    qpos = sorted(_random.Random(5).sample(range(n_pos), 218))
+    t_rows, t_proof = [], []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        rows = hp.gather_rows(cw, 1 << LOG_LANES, qpos)
+        t_rows.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        proof = hp.merkle_multi_proof(tree, n_pos, qpos)
+        t_proof.append(time.perf_counter() - t0)
+    open_leg = {"queries": len(qpos), "row_bytes": 16 << LOG_LANES, "proof_hashes": int(proof.shape[0]),
+                "gather_rows_ms": sorted(t_rows)[len(t_rows) // 2] * 1e3,
+                "multi_proof_ms": sorted(t_proof)[len(t_proof) // 2] * 1e3,
+                "note": "host call to synchronised result, median of 20"}
+    del rows, proof
+
     # ---- optional calibration copy: a plain kernel of known traffic ------------
     if args.calib_copy_mib:
         nb = args.calib_copy_mib << 20
@@ -505,6 +523,7 @@ def main():
         "m32_l0_commit": m32,
         "witness": witness,
         "pow": pow_,
+        "open": open_leg,
         "kernels_m32": kernels,
         "cpu_baseline": cpu,
     }
