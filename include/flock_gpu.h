@@ -15,7 +15,10 @@
  * message, ring-switch fold_1b_rows) store per-work-group partial sums in
  * scratch taken from the stream-ordered pool of `stream` (hipMallocAsync,
  * released with hipFreeAsync on the same stream); zerocheck round 1 also
- * builds its eq table there.  A failed allocation returns FLK_E_NOMEM.
+ * builds its eq table there, and the opening gathers their index lists.  The
+ * proof-of-work calls keep one 8-byte device word and one pinned host word per
+ * device.  A failed allocation returns FLK_E_NOMEM.  The PoW, opening and
+ * transcript-grind calls return host results and synchronise `stream`.
  *
  * Element layout: flk_f128 is 16 bytes, `lo` (x^0..x^63) then `hi`
  * (x^64..x^127), little-endian — identical to the reference F128

@@ -51,3 +51,17 @@ def test_reduction_variant_matches_oracle(dev, env):
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "MISMATCH" not in r.stdout
+
+
+# Word-range parts of the witness kernel (default: 2 at 2^15 compressions, 1 at 2^18).
+WITNESS_VARIANTS = [{"FLOCK_WIT_PARTS": v} for v in ("1", "2", "4", "16")]
+
+
+@pytest.mark.parametrize("env", WITNESS_VARIANTS, ids=lambda e: ",".join(f"{k}={v}" for k, v in e.items()))
+def test_witness_variant_matches_oracle(dev, env):
+    print(f"variant {env}", flush=True)
+    r = subprocess.run([sys.executable, CHECK, "witness"], env={**os.environ, **env}, capture_output=True,
+                       text=True, timeout=100)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "MISMATCH" not in r.stdout

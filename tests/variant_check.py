@@ -8,7 +8,9 @@ tests/test_gpu_variants.py with one setting each:
                                forward/inverse round trip;
   variant_check.py reductions  round_msg (+eval), fold_and_msg, zerocheck round
                                pair, lincheck round message and ring-switch
-                               fold_1b_rows at 2^13 and 2^16.
+                               fold_1b_rows at 2^13 and 2^16;
+  variant_check.py witness     the BLAKE3 witness (z, a, b, stripe) of 100 and
+                               1000 compressions in 2^7 and 2^10 slots.
 
 Uses the C ABI through ctypes only (flk_alloc / flk_copy_*), no torch.  Exit
 status 0 means every check matched the oracle; the lines printed name each check.
@@ -149,11 +151,38 @@ def check_reductions(lib, orc):
     return ok
 
 
+def check_witness(lib, orc):
+    """flk_blake3_witness (z, a, b, stripe) at 2^7 and 2^10 slots, ragged."""
+    from oracle_ffi import COMPRESSION_DT
+    ok = True
+    for n_log, n in ((7, 100), (10, 1000)):
+        n_total = 1 << n_log
+        c = np.zeros(n, COMPRESSION_DT)
+        c.view(np.uint32).reshape(n, 28)[:] = np.random.default_rng(n).integers(0, 2**32, (n, 28), dtype=np.uint32)
+        want = orc.witness(c, n_log)
+        dc = Dev(lib, c.nbytes)
+        dc.put(c)
+        outs = [Dev(lib, n_total * 2048) for _ in range(4)]
+        nat.check(lib.flk_blake3_witness(dc.p, n, n_log, outs[0].p, outs[1].p, outs[2].p, outs[3].p, None),
+                  "blake3_witness")
+        for name, d, w in zip(("z", "a", "b"), outs[:3], want[:3]):
+            good = bool(np.array_equal(d.get(np.uint64, n_total * 256), w))
+            print(f"witness {name} 2^{n_log}: {'ok' if good else 'MISMATCH'}", flush=True)
+            ok = ok and good
+        good = bool(np.array_equal(outs[3].get(np.uint8, n_total * 2048), want[3]))
+        print(f"witness stripe 2^{n_log}: {'ok' if good else 'MISMATCH'}", flush=True)
+        ok = ok and good
+        for d in [dc] + outs:
+            d.free()
+    return ok
+
+
 def main():
     lib = nat.load()
     orc = Oracle()
     mode = sys.argv[1] if len(sys.argv) > 1 else "ntt"
-    ok = check_ntt(lib, orc) if mode == "ntt" else check_reductions(lib, orc)
+    check = {"ntt": check_ntt, "reductions": check_reductions, "witness": check_witness}[mode]
+    ok = check(lib, orc)
     return 0 if ok else 1
 
 
