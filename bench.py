@@ -218,11 +218,11 @@ def witness_leg(torch, nat, hp, sptr, st, n_log, reps, cpu_sample_log):
     launch stream; algorithmic bytes per compression = 112 read + 4 x 2048
     written.  A few slots are checked against the oracle, and the oracle times
     a 2^cpu_sample_log sample (0: no CPU leg)."""
+    from flock_challenge_multi_amd.inputs import generate_compressions
     P = ctypes.c_void_p
     n = 1 << n_log
-    g = torch.Generator(device="cuda")
-    g.manual_seed(0xB1A4E3)
-    blocks = torch.randint(-(2**31), 2**31, (n, 28), dtype=torch.int32, device="cuda", generator=g)
+    # the reference harness's inputs: seeded cv / message / counter, block_len 64, flags 11
+    blocks = torch.from_numpy(generate_compressions(n_log, 0xB1A4E3).view("int32")).cuda()
     z, a, b = (hp.f128_empty(n * 128) for _ in range(3))
     zl = torch.empty(n * 2048, dtype=torch.uint8, device="cuda")
 
@@ -233,8 +233,8 @@ def witness_leg(torch, nat, hp, sptr, st, n_log, reps, cpu_sample_log):
     ms = _time(torch, st, once, reps)
     nbytes = n * (112 + 4 * 2048)
     gbs = nbytes / (ms / 1e3) / 1e9
-    out = {"workload": f"witness of 2^{n_log} BLAKE3 compressions (m = {n_log + 14}): z, a, b packed "
-                       f"({n * 2048 >> 20} MiB each) + lincheck stripe",
+    out = {"workload": f"witness of 2^{n_log} BLAKE3 compressions (m = {n_log + 14}; generate_compressions("
+                       f"{n_log}, seed) inputs): z, a, b packed ({n * 2048 >> 20} MiB each) + lincheck stripe",
            "compressions": n, "ms": ms, "compressions_per_s": n / (ms / 1e3),
            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": nbytes}}

@@ -162,3 +162,19 @@ def test_witness_feeds_zerocheck_and_lincheck(orc, dev):
     want_ab, want_c = orc.zc_round1(wa.view(np.uint8), wb.view(np.uint8), wz.view(np.uint8), m, 6, r)
     assert same(from_dev(p_ab), want_ab) and same(from_dev(p_c), want_c)
     assert same(from_dev(fold), orc.lc_partial_fold(wst, m, 14, eq))
+
+
+def test_witness_of_the_harness_inputs(orc, dev):
+    """The plain-prover input set of the reference harness: 2^12 compressions of
+    generate_compressions(12, seed) (block_len 64, flags 11), whole witness and
+    stripe against the oracle."""
+    from flock_challenge_multi_amd.inputs import generate_compressions
+    raw = generate_compressions(12, 0x5EED)
+    c = np.zeros(len(raw), COMPRESSION_DT)
+    c.view(np.uint32).reshape(-1, 28)[:] = raw
+    want = orc.witness(c, 12)
+    z, a, b, zl = hp.generate_witness_with_ab_packed_and_lincheck(_dev_blocks(c), 12)
+    torch.cuda.synchronize()
+    for got, w, name in zip((z, a, b), want[:3], "zab"):
+        assert np.array_equal(_u64(got), w), name
+    assert np.array_equal(zl.cpu().numpy(), want[3]), "stripe"
