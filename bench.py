@@ -313,7 +313,7 @@ def pow_leg(hp, cpu_sample_bits):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        print(json.dumps({"error": "this benchmark measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
     import torch
 

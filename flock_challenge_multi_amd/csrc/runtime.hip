@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(PSUM_WG) partials_sum_kernel(const uint4* __re
 
 // The default stream-ordered pool of each device keeps what the reduction
 // scratch frees (release threshold raised once per device), so repeated calls
-// reuse it instead of returning it to the driver at every synchronisation.
+// reuse it instead of returning it to the HIP runtime at every synchronisation.
 static void keep_pool_memory() {
   static std::mutex mu;
   static bool done[64] = {};
