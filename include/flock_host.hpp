@@ -315,6 +315,56 @@ inline std::pair<std::vector<F128>, std::vector<F128>> round1(const DeviceVec<ui
 }  // namespace zerocheck
 
 // ---------------------------------------------------------------------------
+// Proof of work and query openings (CF/challenger.rs:384-474,1648;
+// CF/merkle.rs:1020-1107)
+// ---------------------------------------------------------------------------
+namespace challenger {
+// pow_scan: the smallest nonce in [start, start + len) or no value.
+inline bool pow_scan(const std::array<uint8_t, 32>& digest, uint64_t start, uint64_t len, uint32_t bits,
+                     uint64_t* nonce, void* stream = nullptr) {
+  uint64_t n = 0;
+  check(flk_pow_scan(digest.data(), start, len, bits, &n, stream), "pow_scan");
+  *nonce = n;
+  return n != ~0ull;
+}
+}  // namespace challenger
+
+namespace merkle {
+// merkle_multi_proof of a device tree (2 num_leaves - 1 hashes), returned to
+// the host (the hashes go into the proof).
+inline std::vector<Hash> merkle_multi_proof(const DeviceVec<Hash>& tree, size_t num_leaves,
+                                            const std::vector<uint64_t>& positions, void* stream = nullptr) {
+  size_t depth = 0;
+  while ((size_t(1) << depth) < num_leaves) ++depth;
+  const size_t cap = positions.size() * (depth ? depth : 1);
+  DeviceVec<Hash> tmp(cap ? cap : 1);
+  size_t k = 0;
+  check(flk_merkle_multi_proof(reinterpret_cast<const uint8_t*>(tree.data()), num_leaves, positions.data(),
+                               positions.size(), reinterpret_cast<uint8_t*>(tmp.data()), cap, &k, stream),
+        "merkle_multi_proof");
+  std::vector<Hash> out(k);
+  if (k) {
+    check(flk_copy_to_host(out.data(), tmp.data(), k * sizeof(Hash), stream), "flk_copy_to_host");
+    check(flk_synchronize(stream), "flk_synchronize");
+  }
+  return out;
+}
+}  // namespace merkle
+
+namespace pcs {
+// The opened rows of `positions` from a SoA codeword of rows of row_len elements.
+inline DeviceVec<F128> gather_rows(const DeviceVec<F128>& codeword, size_t row_len,
+                                   const std::vector<uint64_t>& positions, void* stream = nullptr) {
+  if (row_len == 0 || codeword.len() % row_len) throw std::invalid_argument("gather_rows: row_len");
+  DeviceVec<F128> out(positions.size() * row_len ? positions.size() * row_len : 1);
+  check(flk_gather_rows(codeword.data(), codeword.len() / row_len, row_len, positions.data(), positions.size(),
+                        out.data(), stream),
+        "gather_rows");
+  return out;
+}
+}  // namespace pcs
+
+// ---------------------------------------------------------------------------
 // crate::challenger::FsChallenger over BLAKE3 (CF/challenger.rs:226-523):
 // host transcript, device nonce search in grind_pow
 // ---------------------------------------------------------------------------

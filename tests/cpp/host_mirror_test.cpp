@@ -142,6 +142,32 @@ int main() {
     EXPECT(verifier.verify_pow(n, 12), "verify_pow accepts");
     EXPECT(prover.sample_f128() == verifier.sample_f128(), "challenges agree after the PoW");
   }
+  // query openings: pow_scan, row gather and multi-proof on a committed codeword
+  {
+    std::array<uint8_t, 32> d{};
+    for (int i = 0; i < 32; ++i) d[i] = (uint8_t)(3 * i + 1);
+    uint64_t n = 0;
+    EXPECT(flock::challenger::pow_scan(d, 0, 1 << 20, 9, &n), "pow_scan finds a nonce");
+    EXPECT(n == orc_pow_grind(d.data(), 9), "pow_scan == the oracle's smallest nonce");
+    flock::pcs::PcsParams params{18, 1, 3};  // 512 positions x 8 lanes
+    std::vector<F128> z = rand_f128(0xABCD, size_t(1) << params.log_msg_len());
+    auto [commitment, pd] = flock::pcs::commit(DeviceVec<F128>(z), params);
+    const std::vector<uint64_t> q = {5, 6, 7, 100, 300, 511, 6};
+    auto rows = flock::pcs::gather_rows(pd.codeword, params.num_ntts(), q).to_host();
+    auto cw = pd.codeword.to_host();
+    bool rows_ok = true;
+    for (size_t i = 0; i < q.size(); ++i)
+      for (size_t j = 0; j < params.num_ntts(); ++j)
+        rows_ok = rows_ok && rows[i * params.num_ntts() + j] == cw[q[i] * params.num_ntts() + j];
+    EXPECT(rows_ok, "gather_rows == codeword rows");
+    auto proof = flock::merkle::merkle_multi_proof(pd.merkle_tree, params.n_leaves(), q);
+    std::vector<uint64_t> idx(q.size() * 20);
+    const size_t k = orc_merkle_multi_proof_indices(params.n_leaves(), q.data(), q.size(), idx.data());
+    auto tree = pd.merkle_tree.to_host();
+    bool proof_ok = proof.size() == k;
+    for (size_t i = 0; proof_ok && i < k; ++i) proof_ok = proof[i] == tree[idx[i]];
+    EXPECT(proof_ok, "merkle_multi_proof == the oracle's sibling hashes");
+  }
   std::printf("host mirror parity ok\n");
   return 0;
 }
