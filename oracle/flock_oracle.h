@@ -142,6 +142,11 @@ void orc_blake3_block_witness(const orc_compression *blk, uint64_t *z, uint64_t 
 int orc_blake3_witness(const orc_compression *blocks, size_t n_blocks, size_t n_blocks_log,
                        uint64_t *z, uint64_t *a, uint64_t *b, uint8_t *stripe);
 
+/* Word-form restatement of the same rows (witness_words.c). */
+void orc_blake3_block_witness_words(const orc_compression *blk, uint64_t *z, uint64_t *a, uint64_t *b);
+/* orc_blake3_witness on n_threads POSIX threads, word form (same outputs). */
+int orc_blake3_witness_mt(const orc_compression *blocks, size_t n_blocks, size_t n_blocks_log, uint64_t *z,
+                          uint64_t *a, uint64_t *b, uint8_t *stripe, size_t n_threads);
 
 /* ---- Proof-of-work grinding (pow.c; CF/challenger.rs:384-474,549-601) ---- */
 /* blake3(state_digest || nonce_le || 24 zero bytes). */

@@ -98,6 +98,8 @@ class Oracle:
             "orc_blake3_witness": (ctypes.c_int, [P, SZ, SZ, P, P, P, P]),
             "orc_blake3_compress": (None, [P, P, U64, ctypes.c_uint32, ctypes.c_uint32, P]),
             "orc_pow_hash": (None, [P, U64, P]),
+            "orc_blake3_block_witness_words": (None, [P, P, P, P]),
+            "orc_blake3_witness_mt": (ctypes.c_int, [P, SZ, SZ, P, P, P, P, SZ]),
             "orc_blake3_xof": (None, [P, SZ, P, SZ]),
             "orc_merkle_multi_proof_indices": (SZ, [SZ, P, SZ, P]),
             "orc_merkle_multi_verify": (ctypes.c_int, [P, SZ, P, P, SZ, P, SZ]),
@@ -190,6 +192,20 @@ class Oracle:
         z, a, b = (np.zeros(256, np.uint64) for _ in range(3))
         self.L.orc_blake3_block_witness(_p(c), _p(z), _p(a), _p(b))
         return z, a, b
+
+    def witness_mt(self, comps, n_blocks_log, threads, stripe=True):
+        """Word-form restatement on `threads` POSIX threads (same outputs as witness)."""
+        n_total = 1 << n_blocks_log
+        c = np.ascontiguousarray(comps, dtype=COMPRESSION_DT)
+        n = c.size
+        if n == 0:
+            c = np.zeros(1, COMPRESSION_DT)
+        z, a, b = (np.zeros(n_total * 256, np.uint64) for _ in range(3))
+        st = np.zeros(n_total * 2048, np.uint8) if stripe else None
+        if self.L.orc_blake3_witness_mt(_p(c), n, n_blocks_log, _p(z), _p(a), _p(b), _p(st) if stripe else None,
+                                        threads):
+            raise ValueError("orc_blake3_witness_mt rejected the shape")
+        return z, a, b, st
 
     def witness(self, comps, n_blocks_log, stripe=True):
         n_total = 1 << n_blocks_log

@@ -264,3 +264,19 @@ def test_shapes_rejected(orc):
     z, a, b, st = orc.witness(comps[:0], 3)  # no compressions: all padding
     zero = orc.block_witness(np.zeros(1, COMPRESSION_DT))
     assert np.array_equal(z[:256], zero[0])
+
+
+def test_word_form_equals_bit_form(orc):
+    """The two restatements (bit by bit; word records as the reference's packed
+    builder) agree on whole batches, stripe included, on 1 and 5 threads."""
+    comps = random_compressions(41, 37)
+    want = orc.witness(comps, 6)
+    for threads in (1, 5):
+        got = orc.witness_mt(comps, 6, threads)
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w)
+    zero = np.zeros(1, COMPRESSION_DT)
+    z2, a2, b2 = (np.zeros(256, np.uint64) for _ in range(3))
+    orc.L.orc_blake3_block_witness_words(zero.ctypes.data, z2.ctypes.data, a2.ctypes.data, b2.ctypes.data)
+    for g, w in zip((z2, a2, b2), orc.block_witness(zero)):
+        assert np.array_equal(g, w)
