@@ -213,6 +213,18 @@ def kernel_table(torch, nat, hp, sptr, st):
     return res
 
 
+def _cpu_median(fn, min_s=2.0, min_runs=3, max_runs=50):
+    """Runs fn until min_s seconds and min_runs runs have passed; returns its
+    last result, the median seconds per run and the run count."""
+    times, t0 = [], time.perf_counter()
+    while len(times) < min_runs or (time.perf_counter() - t0 < min_s and len(times) < max_runs):
+        t1 = time.perf_counter()
+        res = fn()
+        times.append(time.perf_counter() - t1)
+    times.sort()
+    return res, times[len(times) // 2], len(times)
+
+
 def witness_leg(torch, nat, hp, sptr, st, n_log, reps, cpu_sample_log):
     """BLAKE3 R1CS witness generation (z, a = Az, b = Bz and the lincheck
     stripe) of 2^n_log synthetic compressions, timed with HIP events on the
@@ -264,17 +276,15 @@ def witness_leg(torch, nat, hp, sptr, st, n_log, reps, cpu_sample_log):
             ok = ok and np.array_equal(zl[:ns * 2048].cpu().numpy(), st_ref)
             # the whole batch on the box's 16-core share: the word-form oracle, threaded
             threads = max(1, min(16, os.cpu_count() or 1))
-            t1 = time.perf_counter()
-            zm, am, bm, stm = orc.witness_mt(comps, n_log, threads)
-            cpu_mt = time.perf_counter() - t1
+            (zm, am, bm, stm), cpu_mt, n_rep = _cpu_median(lambda: orc.witness_mt(comps, n_log, threads))
             ok = ok and np.array_equal(zz.reshape(-1), zm) and np.array_equal(zl.cpu().numpy(), stm)
             ok = ok and np.array_equal(a.cpu().numpy().view(np.uint64).reshape(-1), am)
             ok = ok and np.array_equal(b.cpu().numpy().view(np.uint64).reshape(-1), bm)
             out["verified"] = bool(ok)
             out["cpu_baseline"] = {"value": n / cpu_mt, "unit": "compressions/s", "cores": threads, "kind": "port",
                                    "sample": f"the whole batch of 2^{n_log} compressions by the oracle's word-form "
-                                             f"witness (C, POSIX threads) with the stripe: {cpu_mt:.3f} s on "
-                                             f"{threads} threads",
+                                             f"witness (C, POSIX threads) with the stripe: {cpu_mt:.4f} s on "
+                                             f"{threads} threads (median of {n_rep} runs)",
                                    "single_thread_value": ns / cpu_s1,
                                    "single_thread_sample": f"the oracle's bit-at-a-time witness of the first "
                                                            f"2^{cpu_sample_log} compressions: {cpu_s1:.2f} s"}
@@ -454,14 +464,14 @@ def main():
         if not args.no_cpu_baseline:
             # the box's CPU share is 16 cores whatever os.cpu_count() reports
             threads = max(1, min(16, os.cpu_count() or 1))
-            t1 = time.perf_counter()
-            _, _, root_mt = orc.commit_mt(host_msg, 1 << LOG_LANES, LOG_RATE, threads)
-            cpu_mt = time.perf_counter() - t1
+            (_, _, root_mt), cpu_mt, n_rep = _cpu_median(
+                lambda: orc.commit_mt(host_msg, 1 << LOG_LANES, LOG_RATE, threads))
             verified = verified and root_mt == root
             cpu = {"value": compressions / cpu_mt, "unit": "committed BLAKE3 compressions/s", "cores": threads,
                    "kind": "port",
                    "sample": f"one full commit of the same 2^22-coefficient input by the oracle (C, PCLMULQDQ): "
-                             f"{cpu_mt:.3f} s on {threads} threads, {cpu_s1:.2f} s on 1 thread",
+                             f"{cpu_mt:.3f} s on {threads} threads (median of {n_rep} runs), {cpu_s1:.2f} s on "
+                             f"1 thread",
                    "single_thread_value": compressions / cpu_s1, "cpu_model": cpu_model()}
     except Exception as ex:  # the oracle is only a checker here
         verified = f"oracle unavailable: {ex}"
