@@ -159,6 +159,14 @@ def kernel_table(torch, nat, hp, sptr, st):
         "flk_rs_fold_1b_rows", P(f.data_ptr()), P(b.data_ptr()), N, P(o128.data_ptr()), sptr)), 32 * N + 2048)
     add("a11_fold_b128_elems_2^25", _time(torch, st, lambda: nat.call(
         "flk_rs_fold_b128_elems", P(b.data_ptr()), P(o128.data_ptr()), N, P(f.data_ptr()), sptr)), 32 * N + 2048)
+    # ring switch of two claims + the combined basis and target (host transcript included;
+    # bytes: the packed witness read once and the basis written once)
+    fs = hp.FsChallenger(b"flock-bench")
+    xo = (nat.F128 * 52)(*[F(rng.getrandbits(128)) for _ in range(52)])
+    shv, cl, ga, tg = (nat.F128 * 256)(), (nat.F128 * 2)(), (nat.F128 * 2)(), nat.F128()
+    add("f3_rs_combine_2x2^25", _time(torch, st, lambda: nat.call(
+        "flk_pcs_combined_basis_and_target", fs._h, P(f.data_ptr()), 25, xo, 2, shv, cl, ga, P(b.data_ptr()),
+        ctypes.byref(tg), sptr), reps=3), 32 * N)
     del f, b, fo, bo
     # a8 / a9 zerocheck, a10 lincheck on 2^32-bit witness columns
     nb = (1 << m) // 8

@@ -436,6 +436,56 @@ def fold_b128_elems(suffix_tensor, eq_r_dprime, out=None):
     return out
 
 
+def _rs_claims(packed_witness, x_outers):
+    n = _check_f128(packed_witness, "packed_witness")
+    if not _pow2(n):
+        raise ValueError("ring switch: packed witness length must be a power of two")
+    log_len = _log2(n)
+    x_outers = [list(x) for x in x_outers]
+    if not x_outers:
+        raise ValueError("ring switch: at least one claim is required")
+    for x in x_outers:
+        if len(x) != log_len + 1:
+            raise ValueError(f"ring switch: x_outer must have {log_len + 1} coordinates (2^L = len(packed_witness))")
+    flat = [v for x in x_outers for v in x]
+    arr = (F128 * len(flat))(*[F128.from_int(v) for v in flat])
+    return log_len, arr, len(x_outers)
+
+
+def _rs_outputs(k):
+    return (F128 * (128 * k))(), (F128 * k)(), (F128 * k)()
+
+
+def prove_batched(packed_witness, x_outers, challenger, b_combined=None, want_basis=True):
+    """ring_switch::prove_batched (CF/pcs/ring_switch.rs:3727-3735, 4310-4383):
+    returns (s_hat_vs, sumcheck_claims, gammas, b_combined), where b_combined =
+    sum_k of the gamma-baked rs_eq_ind (a device tensor, or None when
+    want_basis is False)."""
+    log_len, arr, k = _rs_claims(packed_witness, x_outers)
+    if want_basis and b_combined is None:
+        b_combined = f128_empty(1 << log_len, packed_witness.device)
+    shv, cl, ga = _rs_outputs(k)
+    nat.call("flk_rs_prove_batched", challenger._h, _ptr(packed_witness), log_len, arr, k, shv, cl, ga,
+             _ptr(b_combined) if want_basis else None, _stream())
+    s_hat_vs = [[shv[128 * j + r].to_int() for r in range(128)] for j in range(k)]
+    return s_hat_vs, [c.to_int() for c in cl], [g.to_int() for g in ga], (b_combined if want_basis else None)
+
+
+def compute_combined_basis_and_target(packed_witness, x_outers, challenger, b_combined=None):
+    """compute_combined_basis_and_target (CF/pcs.rs:778-870) for ring-switched
+    claims: returns (s_hat_vs, sumcheck_claims, gammas, b_combined, target)."""
+    log_len, arr, k = _rs_claims(packed_witness, x_outers)
+    if b_combined is None:
+        b_combined = f128_empty(1 << log_len, packed_witness.device)
+    shv, cl, ga = _rs_outputs(k)
+    tgt = F128()
+    import ctypes
+    nat.call("flk_pcs_combined_basis_and_target", challenger._h, _ptr(packed_witness), log_len, arr, k, shv, cl, ga,
+             _ptr(b_combined), ctypes.byref(tgt), _stream())
+    s_hat_vs = [[shv[128 * j + r].to_int() for r in range(128)] for j in range(k)]
+    return s_hat_vs, [c.to_int() for c in cl], [g.to_int() for g in ga], b_combined, tgt.to_int()
+
+
 # ---- zerocheck (CF/zerocheck) ----------------------------------------------------
 def lagrange_weights(k_skip, z):
     """lagrange_weights_naive on the phi8 nodes (host), as a list of ints."""

@@ -254,6 +254,36 @@ int flk_rs_fold_1b_rows(const flk_f128 *packed_witness, const flk_f128 *suffix_t
 int flk_rs_fold_b128_elems(const flk_f128 *suffix_tensor, const flk_f128 *eq_r_dprime, size_t n,
                            flk_f128 *out, void *stream);
 
+struct flk_fs;
+
+/* ring_switch::prove_batched (:3727-3735; transcript order of the per-claim tail
+ * :4310-4383) over a dense packed witness of 2^log_len elements (device) and
+ * n_claims opening points x_outers (HOST, n_claims rows of log_len + 1
+ * coordinates, the reference's x_outer of length m - 6).  For claim k:
+ * observe_label("flock-ring-switch-v0"), s_hat_v_k = fold_1b_rows(z, eq(x_outer_k[1..])),
+ * observe_f128_slice(s_hat_v_k), r''_k = sample_f128_vec(7),
+ * claims_out[k] = <tensor_algebra_transpose(s_hat_v_k), eq(r''_k)>; then
+ * gammas_out[k] = sample_f128() for every claim.  s_hat_v_out (HOST) receives
+ * n_claims x 128 elements.  b_combined (device, 2^log_len, or NULL) receives
+ * sum_k fold_b128_elems(eq(x_outer_k[1..]), gamma_k eq(r''_k)), the sum of the
+ * gamma-baked rs_eq_ind that compute_combined_basis_and_target forms
+ * (CF/pcs.rs:865-870 and its b_combined sum).  1 <= n_claims <= 64.  The suffix
+ * tensors (n_claims x 2^log_len elements) live in stream-ordered scratch for
+ * the call; the call synchronises `stream`.  PaddingSpec only lets the reference
+ * skip all-zero chunks; the values equal the dense ones computed here. */
+int flk_rs_prove_batched(struct flk_fs *fs, const flk_f128 *packed_witness, size_t log_len,
+                         const flk_f128 *x_outers, size_t n_claims, flk_f128 *s_hat_v_out,
+                         flk_f128 *claims_out, flk_f128 *gammas_out, flk_f128 *b_combined, void *stream);
+
+/* compute_combined_basis_and_target (CF/pcs.rs:778-870) for ring-switched
+ * claims (no packed-direct claims): observe_label("flock-pcs-open-batch-v0"),
+ * flk_rs_prove_batched, and target_out (HOST) = sum_k gamma_k claims_out[k].
+ * b_combined is required.  On a shape error the transcript is untouched. */
+int flk_pcs_combined_basis_and_target(struct flk_fs *fs, const flk_f128 *packed_witness, size_t log_len,
+                                      const flk_f128 *x_outers, size_t n_claims, flk_f128 *s_hat_v_out,
+                                      flk_f128 *claims_out, flk_f128 *gammas_out, flk_f128 *b_combined,
+                                      flk_f128 *target_out, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Zerocheck (CF/zerocheck.rs, CF/zerocheck/multilinear.rs).                */
 /* ------------------------------------------------------------------------ */

@@ -169,6 +169,29 @@ int orc_merkle_multi_verify(const uint8_t root[32], size_t num_leaves, const uin
 /* out[i] = row pos[i] (row_len elements) of a SoA codeword. */
 void orc_gather_rows(const orc_f128 *codeword, size_t row_len, const uint64_t *pos, size_t n, orc_f128 *out);
 
+/* ---- ring switch proving and claim combination (ring_switch.c) --------- */
+/* The BLAKE3 Fiat-Shamir transcript as a byte log (CF/challenger.rs:237-383). */
+typedef struct orc_fs orc_fs;
+orc_fs *orc_fs_new(const uint8_t *domain, size_t len);
+void orc_fs_free(orc_fs *t);
+int orc_fs_label(orc_fs *t, const uint8_t *label, size_t len);
+int orc_fs_f128(orc_fs *t, orc_f128 v);
+int orc_fs_f128_slice(orc_fs *t, const orc_f128 *v, size_t n);
+int orc_fs_sample(orc_fs *t, orc_f128 *out);
+int orc_fs_sample_vec(orc_fs *t, size_t n, orc_f128 *out);
+void orc_fs_digest(const orc_fs *t, uint8_t out[32]);
+/* tensor_algebra_transpose: out[b] bit i = bit b of s_hat_v[i]. */
+void orc_rs_transpose(const orc_f128 s_hat_v[128], orc_f128 out[128]);
+/* ring_switch::prove_batched; b_combined (2^log_len, or NULL) = sum of the
+ * gamma-baked rs_eq_ind.  x_outers: n_claims rows of log_len + 1. */
+int orc_rs_prove_batched(orc_fs *t, const orc_f128 *z, size_t log_len, const orc_f128 *x_outers, size_t n_claims,
+                         orc_f128 *s_hat_v, orc_f128 *claims, orc_f128 *gammas, orc_f128 *b_combined);
+int orc_pcs_combined_basis_and_target(orc_fs *t, const orc_f128 *z, size_t log_len, const orc_f128 *x_outers,
+                                      size_t n_claims, orc_f128 *s_hat_v, orc_f128 *claims, orc_f128 *gammas,
+                                      orc_f128 *b_combined, orc_f128 *target);
+/* The verifier's claim check: sum_i build_claim_weights(z_skip, x_outer_0)[i] s_hat_v[i]. */
+orc_f128 orc_rs_claim_check(orc_f128 z_skip, orc_f128 x_outer_0, const orc_f128 s_hat_v[128]);
+
 #ifdef __cplusplus
 }
 #endif

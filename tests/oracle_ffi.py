@@ -104,6 +104,18 @@ class Oracle:
             "orc_merkle_multi_proof_indices": (SZ, [SZ, P, SZ, P]),
             "orc_merkle_multi_verify": (ctypes.c_int, [P, SZ, P, P, SZ, P, SZ]),
             "orc_gather_rows": (None, [P, SZ, P, SZ, P]),
+            "orc_fs_new": (P, [P, SZ]),
+            "orc_fs_free": (None, [P]),
+            "orc_fs_label": (ctypes.c_int, [P, P, SZ]),
+            "orc_fs_f128": (ctypes.c_int, [P, COrcF128]),
+            "orc_fs_f128_slice": (ctypes.c_int, [P, P, SZ]),
+            "orc_fs_sample": (ctypes.c_int, [P, P]),
+            "orc_fs_sample_vec": (ctypes.c_int, [P, SZ, P]),
+            "orc_fs_digest": (None, [P, P]),
+            "orc_rs_transpose": (None, [P, P]),
+            "orc_rs_prove_batched": (ctypes.c_int, [P, P, SZ, P, SZ, P, P, P, P]),
+            "orc_pcs_combined_basis_and_target": (ctypes.c_int, [P, P, SZ, P, SZ, P, P, P, P, P]),
+            "orc_rs_claim_check": (COrcF128, [COrcF128, COrcF128, P]),
             "orc_leading_zero_bits_ok": (ctypes.c_int, [P, ctypes.c_uint32]),
             "orc_pow_ok": (ctypes.c_int, [P, U64, ctypes.c_uint32]),
             "orc_pow_grind": (U64, [P, ctypes.c_uint32]),
@@ -405,6 +417,48 @@ class Oracle:
         self.L.orc_rs_fold_b128_elems(_p(suffix), _p(eq_r), len(suffix), _p(out))
         return out
 
+    # ---- ring switch proving and claim combination (ring_switch.c) ----
+    def fs(self, domain):
+        return OracleFs(self, domain)
+
+    def rs_transpose(self, s_hat_v):
+        a = ints_to_f128(s_hat_v) if not isinstance(s_hat_v, np.ndarray) else s_hat_v
+        out = np.zeros(128, dtype=F128_DT)
+        self.L.orc_rs_transpose(_p(a), _p(out))
+        return f128_to_ints(out)
+
+    def _rs_args(self, z, x_outers):
+        log_len = int(len(z)).bit_length() - 1
+        assert len(z) == 1 << log_len and all(len(x) == log_len + 1 for x in x_outers)
+        xs = ints_to_f128([v for x in x_outers for v in x])
+        k = len(x_outers)
+        return log_len, xs, k, np.zeros(128 * k, F128_DT), np.zeros(k, F128_DT), np.zeros(k, F128_DT)
+
+    def rs_prove_batched(self, fs, z, x_outers, basis=True):
+        """(s_hat_vs, claims, gammas, b_combined or None) of ring_switch::prove_batched."""
+        log_len, xs, k, shv, cl, ga = self._rs_args(z, x_outers)
+        b = np.zeros(1 << log_len, F128_DT) if basis else None
+        assert self.L.orc_rs_prove_batched(fs.h, _p(z), log_len, _p(xs), k, _p(shv), _p(cl), _p(ga),
+                                           _p(b) if basis else None) == 0
+        v = f128_to_ints(shv)
+        return [v[128 * j:128 * (j + 1)] for j in range(k)], f128_to_ints(cl), f128_to_ints(ga), b
+
+    def pcs_combined_basis_and_target(self, fs, z, x_outers):
+        """(s_hat_vs, claims, gammas, b_combined, target) of compute_combined_basis_and_target."""
+        log_len, xs, k, shv, cl, ga = self._rs_args(z, x_outers)
+        b = np.zeros(1 << log_len, F128_DT)
+        t = np.zeros(1, F128_DT)
+        assert self.L.orc_pcs_combined_basis_and_target(fs.h, _p(z), log_len, _p(xs), k, _p(shv), _p(cl), _p(ga),
+                                                        _p(b), _p(t)) == 0
+        v = f128_to_ints(shv)
+        return [v[128 * j:128 * (j + 1)] for j in range(k)], f128_to_ints(cl), f128_to_ints(ga), b, \
+            f128_to_ints(t)[0]
+
+    def rs_claim_check(self, z_skip, x_outer_0, s_hat_v):
+        a = ints_to_f128(s_hat_v)
+        r = self.L.orc_rs_claim_check(self._F(z_skip), self._F(x_outer_0), _p(a))
+        return r.lo | (r.hi << 64)
+
     def zc_lagrange_weights(self, k_skip, z):
         out = np.zeros(1 << k_skip, dtype=F128_DT)
         assert self.L.orc_zc_lagrange_weights(k_skip, self._F(z), _p(out)) == 0
@@ -438,3 +492,44 @@ class Oracle:
         args = [np.ascontiguousarray(x, dtype=np.uint8) for x in (a, b, c)]
         assert self.L.orc_zc_round1(_p(args[0]), _p(args[1]), _p(args[2]), m, k_skip, _p(rr), _p(pab), _p(pc)) == 0
         return pab, pc
+
+
+class OracleFs:
+    """The oracle's Fiat-Shamir transcript (a BLAKE3 byte log, ring_switch.c)."""
+
+    def __init__(self, orc, domain):
+        d = bytes(domain)
+        self.orc = orc
+        self.h = orc.L.orc_fs_new(d or None, len(d))
+        assert self.h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.orc.L.orc_fs_free(self.h)
+            self.h = None
+
+    def observe_label(self, label):
+        b = bytes(label)
+        assert self.orc.L.orc_fs_label(self.h, b or None, len(b)) == 0
+
+    def observe_f128(self, v):
+        assert self.orc.L.orc_fs_f128(self.h, Oracle._F(v)) == 0
+
+    def observe_f128_slice(self, vals):
+        a = ints_to_f128(vals)
+        assert self.orc.L.orc_fs_f128_slice(self.h, _p(a) if len(vals) else None, len(vals)) == 0
+
+    def sample_f128(self):
+        out = np.zeros(1, F128_DT)
+        assert self.orc.L.orc_fs_sample(self.h, _p(out)) == 0
+        return f128_to_ints(out)[0]
+
+    def sample_f128_vec(self, n):
+        out = np.zeros(max(n, 1), F128_DT)
+        assert self.orc.L.orc_fs_sample_vec(self.h, n, _p(out)) == 0
+        return f128_to_ints(out)[:n]
+
+    def state_digest(self):
+        out = np.zeros(32, np.uint8)
+        self.orc.L.orc_fs_digest(self.h, _p(out))
+        return out.tobytes()
