@@ -415,11 +415,70 @@ class FsChallenger {
     check(flk_fs_verify_pow(t_, nonce, bits, &ok), "verify_pow");
     return ok != 0;
   }
+  flk_fs* raw() { return t_; }
 
  private:
   flk_fs* t_ = nullptr;
 };
 }  // namespace challenger
+
+// ---------------------------------------------------------------------------
+// ring_switch::prove_batched (CF/pcs/ring_switch.rs:3727-3735, 4310-4383) and
+// pcs::compute_combined_basis_and_target (CF/pcs.rs:778-870) for ring-switched
+// claims; the combined basis (the sum of the gamma-baked rs_eq_ind) stays on the
+// device.  x_outers: one vector of log2(len) + 1 coordinates per claim.
+// ---------------------------------------------------------------------------
+struct RingSwitchBatch {
+  std::vector<std::vector<F128>> s_hat_v;
+  std::vector<F128> sumcheck_claims, gammas;
+  DeviceVec<F128> b_combined;
+  F128 target_combined{0, 0};
+};
+
+namespace detail {
+inline RingSwitchBatch ring_switch_call(const DeviceVec<F128>& z, const std::vector<std::vector<F128>>& x_outers,
+                                        challenger::FsChallenger& ch, bool want_basis, bool combine,
+                                        void* stream) {
+  const size_t log_len = log2_exact(z.len()), k = x_outers.size();
+  std::vector<F128> xs;
+  for (const auto& x : x_outers) {
+    if (x.size() != log_len + 1) throw std::invalid_argument("ring switch: x_outer needs log2(len) + 1 coordinates");
+    xs.insert(xs.end(), x.begin(), x.end());
+  }
+  RingSwitchBatch out;
+  std::vector<F128> shv(128 * k);
+  out.sumcheck_claims.resize(k);
+  out.gammas.resize(k);
+  if (want_basis || combine) out.b_combined = DeviceVec<F128>(z.len());
+  if (combine)
+    check(flk_pcs_combined_basis_and_target(ch.raw(), z.data(), log_len, xs.data(), k, shv.data(),
+                                            out.sumcheck_claims.data(), out.gammas.data(), out.b_combined.data(),
+                                            &out.target_combined, stream),
+          "pcs_combined_basis_and_target");
+  else
+    check(flk_rs_prove_batched(ch.raw(), z.data(), log_len, xs.data(), k, shv.data(), out.sumcheck_claims.data(),
+                               out.gammas.data(), want_basis ? out.b_combined.data() : nullptr, stream),
+          "rs_prove_batched");
+  for (size_t j = 0; j < k; ++j) out.s_hat_v.emplace_back(shv.begin() + 128 * j, shv.begin() + 128 * (j + 1));
+  return out;
+}
+}  // namespace detail
+
+namespace ring_switch {
+inline RingSwitchBatch prove_batched(const DeviceVec<F128>& packed_witness,
+                                     const std::vector<std::vector<F128>>& x_outers, challenger::FsChallenger& ch,
+                                     bool want_basis = true, void* stream = nullptr) {
+  return detail::ring_switch_call(packed_witness, x_outers, ch, want_basis, false, stream);
+}
+}  // namespace ring_switch
+
+namespace pcs {
+inline RingSwitchBatch compute_combined_basis_and_target(const DeviceVec<F128>& packed_witness,
+                                                         const std::vector<std::vector<F128>>& x_outers,
+                                                         challenger::FsChallenger& ch, void* stream = nullptr) {
+  return detail::ring_switch_call(packed_witness, x_outers, ch, true, true, stream);
+}
+}  // namespace pcs
 
 // ---------------------------------------------------------------------------
 // flock_prover::r1cs_hashes::blake3 witness generation

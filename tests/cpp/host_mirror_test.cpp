@@ -168,6 +168,36 @@ int main() {
     for (size_t i = 0; proof_ok && i < k; ++i) proof_ok = proof[i] == tree[idx[i]];
     EXPECT(proof_ok, "merkle_multi_proof == the oracle's sibling hashes");
   }
+  // ring switch of three claims and the combined basis and target
+  {
+    const size_t log_len = 9;
+    std::vector<F128> z = rand_f128(0x5151, size_t(1) << log_len);
+    std::vector<std::vector<F128>> xs;
+    for (int k = 0; k < 3; ++k) xs.push_back(rand_f128(0x77 + k, log_len + 1));
+    flock::challenger::FsChallenger t("rs-test");
+    orc_fs* ot = orc_fs_new(reinterpret_cast<const uint8_t*>("rs-test"), 7);
+    auto got = flock::pcs::compute_combined_basis_and_target(DeviceVec<F128>(z), xs, t);
+    std::vector<orc_f128> flat, shv(3 * 128), cl(3), ga(3), b(z.size());
+    for (const auto& x : xs)
+      for (const auto& v : x) flat.push_back(orc_f128{v.lo, v.hi});
+    orc_f128 tgt{0, 0};
+    EXPECT(orc_pcs_combined_basis_and_target(ot, reinterpret_cast<const orc_f128*>(z.data()), log_len, flat.data(), 3,
+                                             shv.data(), cl.data(), ga.data(), b.data(), &tgt) == 0,
+           "oracle combine");
+    bool ok = got.target_combined.lo == tgt.lo && got.target_combined.hi == tgt.hi;
+    for (size_t k = 0; k < 3; ++k) {
+      ok = ok && got.sumcheck_claims[k].lo == cl[k].lo && got.sumcheck_claims[k].hi == cl[k].hi;
+      ok = ok && got.gammas[k].lo == ga[k].lo && got.gammas[k].hi == ga[k].hi;
+      ok = ok && std::memcmp(got.s_hat_v[k].data(), shv.data() + 128 * k, 128 * 16) == 0;
+    }
+    auto gb = got.b_combined.to_host();
+    ok = ok && std::memcmp(gb.data(), b.data(), b.size() * 16) == 0;
+    uint8_t od[32];
+    orc_fs_digest(ot, od);
+    ok = ok && std::memcmp(t.state_digest().data(), od, 32) == 0;
+    orc_fs_free(ot);
+    EXPECT(ok, "compute_combined_basis_and_target == oracle (claims, gammas, s_hat_v, basis, target, transcript)");
+  }
   std::printf("host mirror parity ok\n");
   return 0;
 }
