@@ -85,6 +85,7 @@ SIGNATURES = {
     "flk_rs_fold_1b_rows": (_I, [_P, _P, _SZ, _P, _P]),
     "flk_rs_fold_b128_elems": (_I, [_P, _P, _SZ, _P, _P]),
     "flk_rs_prove_batched": (_I, [_P, _P, _SZ, _P, _SZ, _P, _P, _P, _P, _P]),
+    "flk_ligerito_ood": (_I, [_P, _P, _P, _SZ, _SZ, _P, _P, _P, _P]),
     "flk_pcs_combined_basis_and_target": (_I, [_P, _P, _SZ, _P, _SZ, _P, _P, _P, _P, _P, _P]),
     "flk_zc_lagrange_weights": (_I, [_SZ, _F, _P]),
     "flk_zc_fold_at_z": (_I, [_P, _SZ, _SZ, _P, _P, _P]),

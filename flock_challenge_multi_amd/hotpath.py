@@ -741,6 +741,28 @@ class SumcheckProver:
         self.transcript.append(msg)
         return msg
 
+    def ood(self, challenger, n_samples):
+        """The OOD samples of a recursive level (CF/pcs/ligerito.rs:10569-10590):
+        per sample z = sample_f128_vec(log n), introduce eq(z) with its
+        evaluation, observe (y, u_0, u_2), glue with a sampled beta.  Appends
+        the messages to the transcript as (u_0, u_2) int pairs (they are host
+        values already) and returns the OOD values."""
+        if self.pending_glue is not None:
+            raise RuntimeError("ood before ordinary glue")
+        n = self.f.shape[0]
+        if _check_f128(self.combined_basis, "b") != n or not _pow2(n) or n < 2:
+            raise ValueError("ood: f and b must have equal power-of-two length >= 2")
+        k = max(n_samples, 1)
+        t = F128.from_int(self.t_r)
+        ys, ms = (F128 * k)(), (F128 * (2 * k))()
+        import ctypes
+        nat.call("flk_ligerito_ood", challenger._h, _ptr(self.f), _ptr(self.combined_basis), _log2(n), n_samples,
+                 ctypes.byref(t), ys, ms, _stream())
+        self.t_r = t.to_int()
+        for i in range(n_samples):
+            self.transcript.append((ms[2 * i].to_int(), ms[2 * i + 1].to_int()))
+        return [ys[i].to_int() for i in range(n_samples)]
+
     def introduce_new(self, b_new, h_new):
         if _check_f128(b_new, "b_new") != self.f.shape[0]:
             raise ValueError("introduce_new: length mismatch")

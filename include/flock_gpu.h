@@ -275,6 +275,18 @@ int flk_rs_prove_batched(struct flk_fs *fs, const flk_f128 *packed_witness, size
                          const flk_f128 *x_outers, size_t n_claims, flk_f128 *s_hat_v_out,
                          flk_f128 *claims_out, flk_f128 *gammas_out, flk_f128 *b_combined, void *stream);
 
+/* The out-of-domain samples of a recursive Ligerito level (CF/pcs/ligerito.rs:10569-10590),
+ * on the sumcheck pair (f, b) of 2^log_n device elements (1 <= log_n <= 34) and the
+ * running target *t_r (HOST, in/out).  Per sample s < n_samples (<= 64):
+ * z = sample_f128_vec(log_n); (u_0, u_2, y) = round_msg_and_eval_lsb(f, eq(z))
+ * (introduce_new_with_eval, :9144-9154); observe_f128(y), observe_f128(u_0),
+ * observe_f128(u_2); beta = sample_f128(); b += beta eq(z); *t_r += beta y (glue,
+ * :9213-9256).  ood_values_out[s] = y and msgs_out[2s..2s+2) = (u_0, u_2) are HOST
+ * arrays (the proof's ood_values and sumcheck messages).  eq(z) lives in
+ * stream-ordered scratch; the call synchronises `stream`. */
+int flk_ligerito_ood(struct flk_fs *fs, const flk_f128 *f, flk_f128 *b, size_t log_n, size_t n_samples,
+                     flk_f128 *t_r, flk_f128 *ood_values_out, flk_f128 *msgs_out, void *stream);
+
 /* compute_combined_basis_and_target (CF/pcs.rs:778-870) for ring-switched
  * claims (no packed-direct claims): observe_label("flock-pcs-open-batch-v0"),
  * flk_rs_prove_batched, and target_out (HOST) = sum_k gamma_k claims_out[k].

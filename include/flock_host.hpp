@@ -243,6 +243,20 @@ class SumcheckProver {
     pending_ = false;
   }
 
+  // The OOD samples of a recursive level (CF/pcs/ligerito.rs:10569-10590) on the
+  // transcript `ch` (challenger::FsChallenger): introduce eq(z) with its evaluation,
+  // observe (y, u_0, u_2), glue with a sampled beta.  Returns the OOD values.
+  template <class Challenger>
+  std::vector<F128> ood(Challenger& ch, size_t n_samples) {
+    if (pending_) throw std::logic_error("ood before ordinary glue");
+    std::vector<F128> ys(n_samples), ms(2 * n_samples);
+    check(flk_ligerito_ood(ch.raw(), f_.data(), basis_.data(), log2_exact(f_.len()), n_samples, &t_r_, ys.data(),
+                           ms.data(), stream_),
+          "ligerito_ood");
+    for (size_t i = 0; i < n_samples; ++i) transcript_.push_back(SumcheckMessage{ms[2 * i], ms[2 * i + 1]});
+    return ys;
+  }
+
   const DeviceVec<F128>& f() const { return f_; }
   const DeviceVec<F128>& combined_basis() const { return basis_; }
   F128 target() const { return t_r_; }

@@ -191,6 +191,10 @@ int orc_pcs_combined_basis_and_target(orc_fs *t, const orc_f128 *z, size_t log_l
                                       orc_f128 *b_combined, orc_f128 *target);
 /* The verifier's claim check: sum_i build_claim_weights(z_skip, x_outer_0)[i] s_hat_v[i]. */
 orc_f128 orc_rs_claim_check(orc_f128 z_skip, orc_f128 x_outer_0, const orc_f128 s_hat_v[128]);
+/* The OOD loop of a recursive level (ligerito_ood.c): b and *t_r updated in place;
+ * ood_values[n_samples], msgs[2 n_samples] = (u_0, u_2) per sample. */
+int orc_ligerito_ood(orc_fs *t, const orc_f128 *f, orc_f128 *b, size_t log_n, size_t n_samples, orc_f128 *t_r,
+                     orc_f128 *ood_values, orc_f128 *msgs);
 
 #ifdef __cplusplus
 }

@@ -198,6 +198,32 @@ int main() {
     orc_fs_free(ot);
     EXPECT(ok, "compute_combined_basis_and_target == oracle (claims, gammas, s_hat_v, basis, target, transcript)");
   }
+  // OOD samples on a sumcheck pair
+  {
+    const size_t log_n = 10;
+    std::vector<F128> f = rand_f128(0x0D0D, size_t(1) << log_n), b = rand_f128(0x0E0E, size_t(1) << log_n);
+    auto [sp, m0] = flock::ligerito::SumcheckProver::new_(DeviceVec<F128>(f), DeviceVec<F128>(b), F128{7, 9});
+    (void)m0;
+    flock::challenger::FsChallenger t("ood-test");
+    orc_fs* ot = orc_fs_new(reinterpret_cast<const uint8_t*>("ood-test"), 8);
+    auto ys = sp.ood(t, 2);
+    std::vector<orc_f128> ob(b.size()), oys(2), oms(4);
+    std::memcpy(ob.data(), b.data(), b.size() * 16);
+    orc_f128 ot_r{7, 9};
+    EXPECT(orc_ligerito_ood(ot, reinterpret_cast<const orc_f128*>(f.data()), ob.data(), log_n, 2, &ot_r, oys.data(),
+                            oms.data()) == 0,
+           "oracle ood");
+    auto gb = sp.combined_basis().to_host();
+    bool ok = std::memcmp(gb.data(), ob.data(), ob.size() * 16) == 0 && sp.target().lo == ot_r.lo &&
+              sp.target().hi == ot_r.hi && std::memcmp(ys.data(), oys.data(), 2 * 16) == 0;
+    const auto& tr = sp.transcript();
+    ok = ok && tr.size() == 3 && tr[1].u_0.lo == oms[0].lo && tr[2].u_2.hi == oms[3].hi;
+    uint8_t od[32];
+    orc_fs_digest(ot, od);
+    ok = ok && std::memcmp(t.state_digest().data(), od, 32) == 0;
+    orc_fs_free(ot);
+    EXPECT(ok, "SumcheckProver::ood == oracle (values, messages, basis, target, transcript)");
+  }
   std::printf("host mirror parity ok\n");
   return 0;
 }

@@ -116,6 +116,7 @@ class Oracle:
             "orc_rs_prove_batched": (ctypes.c_int, [P, P, SZ, P, SZ, P, P, P, P]),
             "orc_pcs_combined_basis_and_target": (ctypes.c_int, [P, P, SZ, P, SZ, P, P, P, P, P]),
             "orc_rs_claim_check": (COrcF128, [COrcF128, COrcF128, P]),
+            "orc_ligerito_ood": (ctypes.c_int, [P, P, P, SZ, SZ, P, P, P]),
             "orc_leading_zero_bits_ok": (ctypes.c_int, [P, ctypes.c_uint32]),
             "orc_pow_ok": (ctypes.c_int, [P, U64, ctypes.c_uint32]),
             "orc_pow_grind": (U64, [P, ctypes.c_uint32]),
@@ -453,6 +454,18 @@ class Oracle:
         v = f128_to_ints(shv)
         return [v[128 * j:128 * (j + 1)] for j in range(k)], f128_to_ints(cl), f128_to_ints(ga), b, \
             f128_to_ints(t)[0]
+
+    def ligerito_ood(self, fs, f, b, n_samples, t_r):
+        """The OOD loop: returns (b', T', ood_values, msgs as (u_0, u_2) pairs); b is not modified."""
+        log_n = int(len(f)).bit_length() - 1
+        b2 = b.copy()
+        t = ints_to_f128([t_r])
+        ys = np.zeros(max(n_samples, 1), F128_DT)
+        ms = np.zeros(2 * max(n_samples, 1), F128_DT)
+        assert self.L.orc_ligerito_ood(fs.h, _p(f), _p(b2), log_n, n_samples, _p(t), _p(ys), _p(ms)) == 0
+        m = f128_to_ints(ms)
+        return b2, f128_to_ints(t)[0], f128_to_ints(ys)[:n_samples], [(m[2 * i], m[2 * i + 1])
+                                                                      for i in range(n_samples)]
 
     def rs_claim_check(self, z_skip, x_outer_0, s_hat_v):
         a = ints_to_f128(s_hat_v)
