@@ -167,6 +167,13 @@ def kernel_table(torch, nat, hp, sptr, st):
     add("f3_rs_combine_2x2^25", _time(torch, st, lambda: nat.call(
         "flk_pcs_combined_basis_and_target", fs._h, P(f.data_ptr()), 25, xo, 2, shv, cl, ga, P(b.data_ptr()),
         ctypes.byref(tg), sptr), reps=3), 32 * N)
+    # one OOD sample at the m = 32 L1 sumcheck length 2^19 (eq(z) built, message + evaluation,
+    # glue; bytes: f and b read, b written)
+    n1 = 1 << 19
+    tr, ys, ms = nat.F128(), (nat.F128 * 1)(), (nat.F128 * 2)()
+    add("f3_ood_1x2^19", _time(torch, st, lambda: nat.call(
+        "flk_ligerito_ood", fs._h, P(f.data_ptr()), P(b.data_ptr()), 19, 1, ctypes.byref(tr), ys, ms, sptr),
+        reps=5), 48 * n1)
     del f, b, fo, bo
     # a8 / a9 zerocheck, a10 lincheck on 2^32-bit witness columns
     nb = (1 << m) // 8
