@@ -47,6 +47,11 @@ inline int log2_exact(size_t n) {
 int partials_alloc(uint4** parts, unsigned grid, int n_out, uint4* out, hipStream_t st);
 int partials_sum_free(uint4* parts, unsigned grid, int n_out, uint4* out, hipStream_t st);
 
+// fold_1b_rows against the split eq tensor eq_lo (2^lo_log) x eq_hi (2^hi_log),
+// lo_log >= 8: out128 (device) = sum_i bit_r(z_i) eq_lo[i mod 2^lo_log] eq_hi[i >> lo_log].
+int rs_fold_1b_rows_split(const flk_f128* z, const flk_f128* eq_lo, size_t lo_log, const flk_f128* eq_hi,
+                          size_t hi_log, flk_f128* out128, hipStream_t st);
+
 // Host-side GF(2^128) product (used for set-up constants only).
 flk_f128 host_mul(flk_f128 a, flk_f128 b);
 flk_f128 host_inv(flk_f128 a);

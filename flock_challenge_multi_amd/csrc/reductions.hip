@@ -18,6 +18,7 @@
 // end as per-work-group partials added by a second launch (partials_sum_free).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 
 #include "gf128.hpp"
@@ -239,6 +240,52 @@ __device__ __forceinline__ uint32_t spread_byte4(uint32_t x) {
   return x;
 }
 
+// One tile of 256 elements (z value zv and eq value ev per lane) folded into
+// the lane's accumulator acc (output r = t % 128, half t / 128).  Ends with a
+// barrier, so the caller may reuse the LDS arrays at once.
+__device__ __forceinline__ void f1_tile(uint4 zv, uint4 ev, uint32_t t, uint32_t* zt, uint32_t* nibw, uint4* tabs,
+                                        uint4& acc) {
+  const uint32_t r = t & 127u, halfsel = t >> 7;
+  reinterpret_cast<uint4*>(zt)[t] = zv;  // one 16-byte store per lane (conflict-free)
+  // Table of group g = t/4 holds sums over subsets of its 4 eq values; this lane
+  // writes the 4 entries whose bits 2,3 equal q = t%4.  The group is the
+  // lane's aligned quad, so its eq values arrive by DPP quad broadcasts
+  // (VALU moves; cross-lane shuffles would go through the LDS).
+  const uint4 e0 = quad_bcast4<0>(ev), e1 = quad_bcast4<1>(ev), e2 = quad_bcast4<2>(ev), e3 = quad_bcast4<3>(ev);
+  const uint32_t q = t & 3u;
+  uint4 base = f_zero();
+  if (q & 1u) base = f_add(base, e2);
+  if (q & 2u) base = f_add(base, e3);
+  uint4* tg = tabs + (t >> 2) * 16 + 4 * q;
+  tg[0] = base;
+  tg[1] = f_add(base, e0);
+  tg[2] = f_add(base, e1);
+  tg[3] = f_add(f_add(base, e0), e1);
+  __syncthreads();
+  // Transpose: item (g, qw) packs the nibbles of r = 8qw .. 8qw+7, i.e. byte
+  // qw & 3 of word qw >> 2 of the group's four elements, bit b of element k
+  // at bit 4b + k.
+#pragma unroll
+  for (int it = 0; it < 4; ++it) {
+    const uint32_t item = (uint32_t)it * F1_WG + t, g = item >> 4, qw = item & 15u;
+    const uint32_t w = qw >> 2, sh = 8u * (qw & 3u);
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v |= spread_byte4(zt[(4 * g + k) * 4 + w] >> sh) << k;
+    nibw[item] = v;
+  }
+  __syncthreads();
+  const uint32_t qr = r >> 3, shr = 4u * (r & 7u);
+#pragma unroll 4
+  for (uint32_t g = halfsel * 32; g < halfsel * 32 + 32; g += 2) {
+    const uint4 x0 = tabs[g * 16 + ((nibw[g * 16 + qr] >> shr) & 15u)];
+    const uint4 x1 = tabs[(g + 1) * 16 + ((nibw[(g + 1) * 16 + qr] >> shr) & 15u)];
+    acc = make_uint4(xor3(acc.x, x0.x, x1.x), xor3(acc.y, x0.y, x1.y), xor3(acc.z, x0.z, x1.z),
+                     xor3(acc.w, x0.w, x1.w));
+  }
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_kernel(const uint4* __restrict__ zp,
                                                                const uint4* __restrict__ eq, uint64_t n,
                                                                uint4* __restrict__ out) {
@@ -257,48 +304,46 @@ __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_kernel(const uint4* __r
       zv = zp[i];
       ev = eq[i];
     }
-    reinterpret_cast<uint4*>(zt)[t] = zv;  // one 16-byte store per lane (conflict-free)
-    // Table of group g = t/4 holds sums over subsets of its 4 eq values; this lane
-    // writes the 4 entries whose bits 2,3 equal q = t%4.  The group is the
-    // lane's aligned quad, so its eq values arrive by DPP quad broadcasts
-    // (VALU moves; cross-lane shuffles would go through the LDS).
-    const uint4 e0 = quad_bcast4<0>(ev), e1 = quad_bcast4<1>(ev), e2 = quad_bcast4<2>(ev), e3 = quad_bcast4<3>(ev);
-    const uint32_t q = t & 3u;
-    uint4 base = f_zero();
-    if (q & 1u) base = f_add(base, e2);
-    if (q & 2u) base = f_add(base, e3);
-    uint4* tg = tabs + (t >> 2) * 16 + 4 * q;
-    tg[0] = base;
-    tg[1] = f_add(base, e0);
-    tg[2] = f_add(base, e1);
-    tg[3] = f_add(f_add(base, e0), e1);
-    __syncthreads();
-    // Transpose: item (g, qw) packs the nibbles of r = 8qw .. 8qw+7, i.e. byte
-    // qw & 3 of word qw >> 2 of the group's four elements, bit b of element k
-    // at bit 4b + k.
-#pragma unroll
-    for (int it = 0; it < 4; ++it) {
-      const uint32_t item = (uint32_t)it * F1_WG + t, g = item >> 4, qw = item & 15u;
-      const uint32_t w = qw >> 2, sh = 8u * (qw & 3u);
-      uint32_t v = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) v |= spread_byte4(zt[(4 * g + k) * 4 + w] >> sh) << k;
-      nibw[item] = v;
-    }
-    __syncthreads();
-    const uint32_t qr = r >> 3, shr = 4u * (r & 7u);
-#pragma unroll 4
-    for (uint32_t g = halfsel * 32; g < halfsel * 32 + 32; g += 2) {
-      const uint4 x0 = tabs[g * 16 + ((nibw[g * 16 + qr] >> shr) & 15u)];
-      const uint4 x1 = tabs[(g + 1) * 16 + ((nibw[(g + 1) * 16 + qr] >> shr) & 15u)];
-      acc = make_uint4(xor3(acc.x, x0.x, x1.x), xor3(acc.y, x0.y, x1.y), xor3(acc.z, x0.z, x1.z),
-                       xor3(acc.w, x0.w, x1.w));
-    }
-    __syncthreads();
+    f1_tile(zv, ev, t, zt, nibw, tabs, acc);
   }
   if (halfsel == 1) red[r] = acc;
   __syncthreads();
   if (halfsel == 0) out[(size_t)r * gridDim.x + blockIdx.x] = f_add(acc, red[r]);
+}
+
+// fold_1b_rows against a split eq tensor eq[i] = eq_lo[i mod 2^lo_log] *
+// eq_hi[i >> lo_log] (the factorisation of build_eq_split, CF/pcs/ring_switch.rs:
+// 331-357): each block of 2^lo_log elements is folded against eq_lo (read from
+// L2, 2^lo_log * 16 bytes), and the block's 128 partial sums are multiplied by
+// eq_hi[block] (a 16-entry comb table of it per block) before they are added,
+// so the 2^n-element tensor is never written or read.  lo_log >= 8, n a
+// multiple of 2^lo_log.
+__global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_split_kernel(const uint4* __restrict__ zp,
+                                                                     const uint4* __restrict__ eq_lo,
+                                                                     uint32_t lo_log,
+                                                                     const uint4* __restrict__ eq_hi,
+                                                                     uint64_t n_hi, uint4* __restrict__ out) {
+  __shared__ __align__(16) uint32_t zt[F1_WG * 4];
+  __shared__ uint32_t nibw[(F1_WG / 4) * 16];
+  __shared__ uint4 tabs[(F1_WG / 4) * 16];
+  __shared__ uint4 red[128];
+  __shared__ uint4 etab[16];
+  const uint32_t t = threadIdx.x;
+  const uint32_t r = t & 127u, halfsel = t >> 7;
+  const uint32_t tiles = 1u << (lo_log - 8);
+  uint4 tot = f_zero();
+  for (uint64_t blk = blockIdx.x; blk < n_hi; blk += gridDim.x) {
+    uint4 acc = f_zero();
+    const uint4* zb = zp + (blk << lo_log);
+    for (uint32_t tt = 0; tt < tiles; ++tt) f1_tile(zb[tt * F1_WG + t], eq_lo[tt * F1_WG + t], t, zt, nibw, tabs, acc);
+    if (t < 16) etab[t] = comb_entry(eq_hi[blk], t);
+    __syncthreads();
+    tot = f_add(tot, comb_mul<1>(acc, etab));
+    __syncthreads();
+  }
+  if (halfsel == 1) red[r] = tot;
+  __syncthreads();
+  if (halfsel == 0) out[(size_t)r * gridDim.x + blockIdx.x] = f_add(tot, red[r]);
 }
 
 // ---------------------------------------------------------------------------
@@ -401,6 +446,22 @@ static unsigned grid_for(uint64_t items, int wg, unsigned cap) {
 
 static inline uint4 u4(flk_f128 a) {
   return make_uint4((uint32_t)a.lo, (uint32_t)(a.lo >> 32), (uint32_t)a.hi, (uint32_t)(a.hi >> 32));
+}
+
+int rs_fold_1b_rows_split(const flk_f128* z, const flk_f128* eq_lo, size_t lo_log, const flk_f128* eq_hi, size_t hi_log,
+                          flk_f128* out128, hipStream_t st) {
+  if (lo_log < 8 || lo_log + hi_log > 40) return set_error(FLK_E_SHAPE, "rs_fold_1b_rows_split: lo_log=%zu", lo_log);
+  const uint64_t n_hi = uint64_t(1) << hi_log;
+  const unsigned grid = (unsigned)std::min<uint64_t>(n_hi, 2048);
+  uint4* o = reinterpret_cast<uint4*>(out128);
+  uint4* parts = nullptr;
+  int rc = partials_alloc(&parts, grid, 128, o, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(rs_fold_1b_rows_split_kernel, dim3(grid), dim3(F1_WG), 0, st, reinterpret_cast<const uint4*>(z),
+                     reinterpret_cast<const uint4*>(eq_lo), (uint32_t)lo_log, reinterpret_cast<const uint4*>(eq_hi),
+                     n_hi, parts);
+  FLK_LAUNCH_CHECK("rs_fold_1b_rows_split_kernel");
+  return partials_sum_free(parts, grid, 128, o, st);
 }
 
 }  // namespace flk

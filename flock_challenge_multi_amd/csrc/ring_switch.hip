@@ -6,8 +6,9 @@
 //                                       ring-switched claims
 //
 // The large work is on the device: each claim's suffix tensor eq(x_outer[1..])
-// (flk_build_eq), its s_hat_v = fold_1b_rows(z, suffix) (flk_rs_fold_1b_rows),
-// and one fused pass that writes the combined basis
+// (flk_build_eq; from 2^12 elements on only its two split factors), its
+// s_hat_v = fold_1b_rows(z, suffix), and one fused pass that writes the
+// combined basis
 //   b[i] = sum_k sum_b bit_b(suffix_k[i]) * gamma_k eq(r''_k)[b]
 // reading every claim's suffix tensor once (nibble tables of gamma_k eq(r''_k)
 // per claim in LDS).  The reference bakes gamma_k into eq(r''_k) before its
@@ -55,6 +56,74 @@ __global__ void __launch_bounds__(RC_WG) rs_combine_kernel(const uint4* __restri
                        acc.z ^ xor3(r0.z, r1.z, r2.z) ^ r3.z, acc.w ^ xor3(r0.w, r1.w, r2.w) ^ r3.w);
     }
     out[i] = acc;
+  }
+}
+
+// The same sum over the split suffix tensor suffix_k[i] = lo_k[i mod B] * hi_k[i / B],
+// B = 2^CS_LO_LOG (build_eq_split, CF/pcs/ring_switch.rs:331-357).  For a fixed
+// e = hi_k[j], v -> fold(v * e) is GF(2)-linear in the bits of v, so block j
+// folds lo_k against the 128-entry table T[c] = fold(x^c * e) (the column
+// composition of compose_block_cols, :2130-2160): no suffix tensor is written or
+// read, lo_k (64 KiB) stays in L2.  Each lane owns 16 elements of the block.
+constexpr uint32_t CS_LO_LOG = 12;
+constexpr uint32_t CS_PER_LANE = (1u << CS_LO_LOG) / RC_WG;
+
+__device__ __forceinline__ uint4 fold_nib(uint4 v, const uint4* t) {
+  const uint4 r0 = nibtab8_sum(v.x, t), r1 = nibtab8_sum(v.y, t + 128);
+  const uint4 r2 = nibtab8_sum(v.z, t + 256), r3 = nibtab8_sum(v.w, t + 384);
+  return make_uint4(xor3(r0.x, r1.x, r2.x) ^ r3.x, xor3(r0.y, r1.y, r2.y) ^ r3.y, xor3(r0.z, r1.z, r2.z) ^ r3.z,
+                    xor3(r0.w, r1.w, r2.w) ^ r3.w);
+}
+
+__global__ void __launch_bounds__(RC_WG) rs_combine_split_kernel(const uint4* __restrict__ lo, const uint4* __restrict__ hi,
+                                                                uint64_t n_hi, uint32_t n_claims,
+                                                                const uint4* __restrict__ eqg, uint32_t accumulate,
+                                                                uint4* __restrict__ out) {
+  __shared__ uint4 gtab[RC_MAXC * 512];  // nibble tables of gamma_k eq(r''_k)
+  __shared__ uint4 col[128];             // T[c] = fold(x^c * e) of the current (claim, block)
+  __shared__ uint4 tnib[512];            // nibble tables of T
+  const uint32_t t = threadIdx.x;
+  for (uint32_t e = t; e < n_claims * 512u; e += RC_WG) {
+    const uint32_t k = e >> 9, np = (e >> 4) & 31u, v = e & 15u;
+    uint4 acc = f_zero();
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if ((v >> j) & 1u) acc = f_add(acc, eqg[128 * k + 4 * np + j]);
+    gtab[e] = acc;
+  }
+  __syncthreads();
+  const uint64_t B = uint64_t(1) << CS_LO_LOG;
+  for (uint64_t blk = blockIdx.x; blk < n_hi; blk += gridDim.x) {
+    uint4 acc[CS_PER_LANE];
+#pragma unroll
+    for (uint32_t q = 0; q < CS_PER_LANE; ++q) acc[q] = accumulate ? out[blk * B + t + RC_WG * q] : f_zero();
+    for (uint32_t k = 0; k < n_claims; ++k) {
+      if (t < 128) {
+        uint4 v = hi[k * n_hi + blk];  // x^t * e, in steps of at most 31 doublings
+        for (uint32_t s = t; s; ) {
+          const uint32_t d = s < 31u ? s : 31u;
+          v = f_mul_xs(v, d);
+          s -= d;
+        }
+        col[t] = fold_nib(v, gtab + 512 * k);
+      }
+      __syncthreads();
+      for (uint32_t e = t; e < 512u; e += RC_WG) {
+        const uint32_t np = e >> 4, v = e & 15u;
+        uint4 a = f_zero();
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if ((v >> j) & 1u) a = f_add(a, col[4 * np + j]);
+        tnib[e] = a;
+      }
+      __syncthreads();
+      const uint4* lk = lo + k * B;
+#pragma unroll
+      for (uint32_t q = 0; q < CS_PER_LANE; ++q) acc[q] = f_add(acc[q], fold_nib(lk[t + RC_WG * q], tnib));
+      __syncthreads();
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < CS_PER_LANE; ++q) out[blk * B + t + RC_WG * q] = acc[q];
   }
 }
 
@@ -109,17 +178,23 @@ int prove_batched(flk_fs* fs, const flk_f128* z, size_t log_len, const flk_f128*
                   flk_f128* s_hat_v_out, flk_f128* claims_out, flk_f128* gammas_out, flk_f128* b_combined,
                   hipStream_t st) {
   const size_t n = size_t(1) << log_len, d = log_len + 1;  // x_outer has log_len + 1 coordinates
+  // From 2^12 elements on, the suffix tensor stays split: per claim eq_lo
+  // (2^12 elements) and eq_hi (2^(log_len - 12)) instead of 2^log_len.
+  const bool split = log_len >= CS_LO_LOG;
+  const size_t hi_log = split ? log_len - CS_LO_LOG : 0, n_hi = size_t(1) << hi_log;
+  const size_t per_claim = split ? (size_t(1) << CS_LO_LOG) + n_hi : n;
 
-  // Workspace: every claim's suffix tensor, its s_hat_v and its gamma-scaled eq(r'').
+  // Workspace: every claim's suffix tensor (or its two factors), its s_hat_v and
+  // its gamma-scaled eq(r'').
   uint4* ws = nullptr;
-  const size_t ws_elems = n_claims * (n + 256);
+  const size_t ws_elems = n_claims * (per_claim + 256);
   hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&ws), ws_elems * sizeof(uint4), st);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return set_error(FLK_E_NOMEM, "rs_prove_batched: %zu-byte workspace allocation failed", ws_elems * sizeof(uint4));
   }
-  uint4* sfx = ws;
-  uint4* shv = ws + n_claims * n;
+  uint4* sfx = ws;  // dense: claim k at k * n; split: lo_k at k * B, then every hi_k at n_claims * B + k * n_hi
+  uint4* shv = ws + n_claims * per_claim;
   uint4* eqg_dev = shv + n_claims * 128;
   std::vector<flk_f128> eqg(n_claims * 128);
   int rc = FLK_OK;
@@ -131,10 +206,21 @@ int prove_batched(flk_fs* fs, const flk_f128* z, size_t log_len, const flk_f128*
   };
 
   // 1. Suffix tensors and s_hat_v (independent of the transcript).
+  const size_t B = size_t(1) << CS_LO_LOG;
+  uint4* his = sfx + n_claims * B;
   for (size_t k = 0; k < n_claims && rc == FLK_OK; ++k) {
-    flk_f128* sk = reinterpret_cast<flk_f128*>(sfx + k * n);
-    rc = flk_build_eq(x_outers + k * d + 1, log_len, sk, st);
-    if (rc == FLK_OK) rc = flk_rs_fold_1b_rows(z, sk, n, reinterpret_cast<flk_f128*>(shv + 128 * k), st);
+    flk_f128* out128 = reinterpret_cast<flk_f128*>(shv + 128 * k);
+    if (split) {
+      flk_f128* lo = reinterpret_cast<flk_f128*>(sfx + k * B);
+      flk_f128* hi = reinterpret_cast<flk_f128*>(his + k * n_hi);
+      rc = flk_build_eq(x_outers + k * d + 1, CS_LO_LOG, lo, st);
+      if (rc == FLK_OK) rc = flk_build_eq(x_outers + k * d + 1 + CS_LO_LOG, hi_log, hi, st);
+      if (rc == FLK_OK) rc = rs_fold_1b_rows_split(z, lo, CS_LO_LOG, hi, hi_log, out128, st);
+    } else {
+      flk_f128* sk = reinterpret_cast<flk_f128*>(sfx + k * n);
+      rc = flk_build_eq(x_outers + k * d + 1, log_len, sk, st);
+      if (rc == FLK_OK) rc = flk_rs_fold_1b_rows(z, sk, n, out128, st);
+    }
   }
   if (rc != FLK_OK) return finish(rc);
   e = hipMemcpyAsync(s_hat_v_out, shv, n_claims * 128 * sizeof(uint4), hipMemcpyDeviceToHost, st);
@@ -161,12 +247,20 @@ int prove_batched(flk_fs* fs, const flk_f128* z, size_t log_len, const flk_f128*
   // 4. b = sum_k fold_b128_elems(suffix_k, gamma_k eq(r''_k)), RC_MAXC claims per pass.
   e = hipMemcpyAsync(eqg_dev, eqg.data(), eqg.size() * sizeof(flk_f128), hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return finish(device_failure(e, "rs_prove_batched: eq upload"));
-  const unsigned grid = (unsigned)std::min<uint64_t>((n + RC_WG - 1) / RC_WG, 1024);
+  const unsigned grid = split ? (unsigned)std::min<uint64_t>(n_hi, 2048)
+                             : (unsigned)std::min<uint64_t>((n + RC_WG - 1) / RC_WG, 1024);
   for (size_t k0 = 0; k0 < n_claims; k0 += RC_MAXC) {
     const uint32_t nc = (uint32_t)std::min<size_t>(RC_MAXC, n_claims - k0);
-    hipLaunchKernelGGL(rs_combine_kernel, dim3(grid), dim3(RC_WG), 0, st, sfx + k0 * n, (uint64_t)n, nc,
-                       eqg_dev + 128 * k0, (uint32_t)(k0 != 0), reinterpret_cast<uint4*>(b_combined));
-    FLK_LAUNCH_CHECK("rs_combine_kernel");
+    if (split) {
+      hipLaunchKernelGGL(rs_combine_split_kernel, dim3(grid), dim3(RC_WG), 0, st, sfx + k0 * B, his + k0 * n_hi,
+                         (uint64_t)n_hi, nc, eqg_dev + 128 * k0, (uint32_t)(k0 != 0),
+                         reinterpret_cast<uint4*>(b_combined));
+      FLK_LAUNCH_CHECK("rs_combine_split_kernel");
+    } else {
+      hipLaunchKernelGGL(rs_combine_kernel, dim3(grid), dim3(RC_WG), 0, st, sfx + k0 * n, (uint64_t)n, nc,
+                         eqg_dev + 128 * k0, (uint32_t)(k0 != 0), reinterpret_cast<uint4*>(b_combined));
+      FLK_LAUNCH_CHECK("rs_combine_kernel");
+    }
   }
   // the host eq table must outlive its upload
   e = hipStreamSynchronize(st);

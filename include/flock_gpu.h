@@ -268,8 +268,9 @@ struct flk_fs;
  * sum_k fold_b128_elems(eq(x_outer_k[1..]), gamma_k eq(r''_k)), the sum of the
  * gamma-baked rs_eq_ind that compute_combined_basis_and_target forms
  * (CF/pcs.rs:865-870 and its b_combined sum).  1 <= n_claims <= 64.  The suffix
- * tensors (n_claims x 2^log_len elements) live in stream-ordered scratch for
- * the call; the call synchronises `stream`.  PaddingSpec only lets the reference
+ * tensors live in stream-ordered scratch for the call (from log_len 12 on only
+ * their split factors, 2^12 + 2^(log_len - 12) elements per claim); the call
+ * synchronises `stream`.  PaddingSpec only lets the reference
  * skip all-zero chunks; the values equal the dense ones computed here. */
 int flk_rs_prove_batched(struct flk_fs *fs, const flk_f128 *packed_witness, size_t log_len,
                          const flk_f128 *x_outers, size_t n_claims, flk_f128 *s_hat_v_out,
