@@ -219,7 +219,26 @@ def kernel_table(torch, nat, hp, sptr, st):
     ms = _time(torch, st, lambda: nat.call("flk_merkle_tree", P(cw.data_ptr()), n_cw * 16, n_pos, nat.FLK_HASH_BLAKE3,
                                             P(tree.data_ptr()), sptr), reps=3)
     add("a4_merkle_m32", ms, 16 * n_cw + 32 * (2 * n_pos - 1), {"blake3_compressions_per_s": (17 * n_pos - 1) / (ms / 1e3)})
-    del msg, cw, tree
+    # the recursive Ligerito prover over this L0 commitment with the m = 32 Fast ladder
+    # (initial_k 6, five recursive levels of 3 folds, rates 1/2 .. 1/64, 218 .. 36 queries,
+    # fold grinds 19 .. 4 bits, one OOD sample per recursive level); host transcript included
+    from flock_challenge_multi_amd import ligerito as lig
+    cfg = lig.ProverConfig(initial_k=6, recursive_ks=[3] * 5, log_inv_rates=[1, 2, 3, 4, 5, 6],
+                           queries=[218, 106, 71, 53, 43, 36], grinding_bits=[0] * 6,
+                           fold_grinding_bits=[19, 14, 11, 8, 6, 4], ood_samples=[0, 1, 1, 1, 1, 1])
+    bb = rnd_f128(n_msg)
+    tgt = int(hp.round_msg_and_eval(msg, bb)[2].cpu().numpy().view("uint64")[0])
+    tgt |= int(hp.round_msg_and_eval(msg, bb)[2].cpu().numpy().view("uint64")[1]) << 64
+    proof = {}
+
+    def open_once():
+        proof["p"] = lig.recursive_prover_with_basis(cfg, msg, bb, tgt, cw, tree, hp.FsChallenger(b"flock-bench"))
+
+    ms = _time(torch, st, open_once, reps=3)
+    add("f3_ligerito_open_m32", ms, 32 * n_msg,
+        {"note": "recursive_prover_with_basis, 2^25 (f, b) + the 2^26 x 1 codeword of this commit; bytes: f and b "
+                 "read once", "yr": len(proof["p"].final_proof.yr)})
+    del msg, cw, tree, bb, proof
     # witness generation of the 2^18 compressions (z, a, b + stripe: 2 GiB written)
     w = witness_leg(torch, nat, hp, sptr, st, 18, 5, 0)
     add("w1_blake3_witness_2^18", w["ms"], w["roofline"]["bytes_per_launch"],

@@ -175,6 +175,7 @@ typedef struct orc_fs orc_fs;
 orc_fs *orc_fs_new(const uint8_t *domain, size_t len);
 void orc_fs_free(orc_fs *t);
 int orc_fs_label(orc_fs *t, const uint8_t *label, size_t len);
+int orc_fs_bytes(orc_fs *t, const uint8_t *b, size_t len);
 int orc_fs_f128(orc_fs *t, orc_f128 v);
 int orc_fs_f128_slice(orc_fs *t, const orc_f128 *v, size_t n);
 int orc_fs_sample(orc_fs *t, orc_f128 *out);

@@ -79,6 +79,12 @@ int orc_fs_label(orc_fs *t, const uint8_t *label, size_t len)
     return fs_put(t, &tag, 1) || fs_u64(t, len) || fs_put(t, label, len);
 }
 
+int orc_fs_bytes(orc_fs *t, const uint8_t *b, size_t len)
+{
+    uint8_t tag = 0x05;
+    return fs_put(t, &tag, 1) || fs_u64(t, len) || fs_put(t, b, len);
+}
+
 int orc_fs_f128(orc_fs *t, orc_f128 v)
 {
     const uint8_t tag[2] = {0x03, 0x01};

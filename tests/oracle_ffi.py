@@ -108,6 +108,7 @@ class Oracle:
             "orc_fs_free": (None, [P]),
             "orc_fs_label": (ctypes.c_int, [P, P, SZ]),
             "orc_fs_f128": (ctypes.c_int, [P, COrcF128]),
+            "orc_fs_bytes": (ctypes.c_int, [P, P, SZ]),
             "orc_fs_f128_slice": (ctypes.c_int, [P, P, SZ]),
             "orc_fs_sample": (ctypes.c_int, [P, P]),
             "orc_fs_sample_vec": (ctypes.c_int, [P, SZ, P]),
@@ -546,3 +547,24 @@ class OracleFs:
         out = np.zeros(32, np.uint8)
         self.orc.L.orc_fs_digest(self.h, _p(out))
         return out.tobytes()
+
+    def observe_bytes(self, data):
+        b = bytes(data)
+        assert self.orc.L.orc_fs_bytes(self.h, b or None, len(b)) == 0
+
+    def verify_pow(self, nonce, bits):
+        """verify_pow (CF/challenger.rs:549-601): check against the digest, then absorb u64le(nonce)."""
+        ok = self.orc.pow_ok(self.state_digest(), nonce, bits)
+        self.observe_bytes(int(nonce).to_bytes(8, "little"))
+        return ok
+
+    def sample_distinct_queries(self, block_len, count):
+        """sample_distinct_queries (CF/pcs/ligerito.rs:9665-9720): lo mod block_len, first
+        occurrence kept, sorted."""
+        seen, out = set(), []
+        while len(out) < count:
+            q = (self.sample_f128() & (2**64 - 1)) % block_len
+            if q not in seen:
+                seen.add(q)
+                out.append(q)
+        return sorted(out)
