@@ -429,6 +429,11 @@ class FsChallenger {
     check(flk_fs_verify_pow(t_, nonce, bits, &ok), "verify_pow");
     return ok != 0;
   }
+  std::vector<uint64_t> sample_distinct_queries(uint64_t block_len, size_t count) {
+    std::vector<uint64_t> q(count);
+    check(flk_fs_sample_distinct_queries(t_, block_len, count, q.data()), "sample_distinct_queries");
+    return q;
+  }
   flk_fs* raw() { return t_; }
 
  private:
@@ -527,5 +532,151 @@ inline Witness generate_witness_with_ab_packed(const DeviceVec<Compression>& blo
 }
 }  // namespace blake3
 }  // namespace r1cs_hashes
+
+// ---------------------------------------------------------------------------
+// recursive_prover_with_basis (CF/pcs/ligerito.rs:9996-10026, body :10185-11040)
+// sequenced over the device entry points, in the transcript order its verifier
+// replays (:11836-12228).  Grinds return the smallest valid nonce.
+// ---------------------------------------------------------------------------
+namespace ligerito {
+struct ProverConfig {
+  size_t initial_k;
+  std::vector<size_t> recursive_ks, log_inv_rates, queries, ood_samples;
+  std::vector<uint32_t> grinding_bits, fold_grinding_bits;
+};
+
+struct OpenedLevel {
+  std::vector<F128> rows;  // n_queries x row_len, query-major
+  std::vector<Hash> merkle_proof;
+};
+
+struct LigeritoProof {
+  Hash initial_root;
+  OpenedLevel initial_proof;
+  std::vector<Hash> recursive_roots;
+  std::vector<OpenedLevel> recursive_proofs;
+  std::vector<F128> yr;
+  OpenedLevel final_proof;
+  std::vector<SumcheckMessage> sumcheck_transcript;
+  std::vector<uint64_t> grinding_nonces, fold_grinding_nonces;
+  std::vector<F128> ood_values;
+};
+
+inline size_t ceil_log2(size_t n) {
+  size_t k = 0;
+  while ((size_t(1) << k) < n) ++k;
+  return k;
+}
+
+inline LigeritoProof recursive_prover_with_basis(const ProverConfig& cfg, DeviceVec<F128> f, DeviceVec<F128> b,
+                                                 F128 target, const DeviceVec<F128>& l0_codeword,
+                                                 const DeviceVec<Hash>& l0_tree, challenger::FsChallenger& ch,
+                                                 void* stream = nullptr) {
+  const size_t r = cfg.recursive_ks.size(), k0 = cfg.initial_k;
+  if (r < 1 || cfg.log_inv_rates.size() != r + 1 || cfg.queries.size() != r + 1 || cfg.ood_samples.size() != r + 1 ||
+      cfg.grinding_bits.size() != r + 1 || cfg.fold_grinding_bits.size() != r + 1 || cfg.ood_samples[0])
+    throw std::invalid_argument("recursive_prover_with_basis: config");
+  if (f.len() != b.len()) throw std::invalid_argument("recursive_prover_with_basis: f and b lengths");
+  const size_t log_n = log2_exact(f.len());
+  size_t n_cur = log_n - k0;
+  const size_t block0 = size_t(1) << (n_cur + cfg.log_inv_rates[0]);
+  if (l0_codeword.len() != block0 << k0) throw std::invalid_argument("recursive_prover_with_basis: L0 codeword");
+  LigeritoProof proof;
+  proof.initial_root = l0_tree.to_host(stream).back();
+  auto observe_msg = [&](const SumcheckMessage& m) {
+    ch.observe_f128(m.u_0);
+    ch.observe_f128(m.u_2);
+  };
+  auto open = [&](const DeviceVec<F128>& cw, const DeviceVec<Hash>& tree, size_t row_len,
+                  const std::vector<uint64_t>& q) {
+    return OpenedLevel{pcs::gather_rows(cw, row_len, q, stream).to_host(stream),
+                       merkle::merkle_multi_proof(tree, cw.len() / row_len, q, stream)};
+  };
+  auto induce = [&](size_t log_cols, const OpenedLevel& op, const std::vector<F128>& v,
+                    const std::vector<uint64_t>& q, const std::vector<F128>& alpha) {
+    DeviceVec<F128> rows(op.rows), basis(size_t(1) << log_cols), sum(1);
+    DeviceVec<uint64_t> qd(q);
+    check(flk_induce_sumcheck_poly(log_cols, rows.data(), q.size(), v.data(), v.size(), qd.data(), alpha.data(),
+                                   alpha.size(), basis.data(), sum.data(), stream),
+          "induce_sumcheck_poly");
+    F128 h = sum.to_host(stream)[0];
+    return std::make_pair(std::move(basis), h);
+  };
+
+  ch.observe_label("flock-ligerito-basis-v0");
+  ch.observe_f128(target);
+  ch.observe_bytes(proof.initial_root.data(), 32);
+  auto started = SumcheckProver::new_(std::move(f), std::move(b), target, stream);
+  SumcheckProver& sp = started.first;
+  observe_msg(started.second);
+  auto fold_rounds = [&](size_t k, size_t level) {
+    std::vector<F128> rs;
+    for (size_t j = 0; j < k; ++j) {
+      const uint32_t fb = cfg.fold_grinding_bits[level];
+      if (fb > j) proof.fold_grinding_nonces.push_back(ch.grind_pow(fb - (uint32_t)j, stream));
+      const F128 ri = ch.sample_f128();
+      rs.push_back(ri);
+      observe_msg(sp.fold(ri));
+    }
+    return rs;
+  };
+  auto introduce = [&](DeviceVec<F128> basis, F128 h) {
+    observe_msg(sp.introduce_new(std::move(basis), h));
+    sp.glue(ch.sample_f128());
+  };
+  auto ood = [&](size_t level) {
+    auto ys = sp.ood(ch, cfg.ood_samples[level]);
+    proof.ood_values.insert(proof.ood_values.end(), ys.begin(), ys.end());
+  };
+
+  const std::vector<F128> r_lane = fold_rounds(k0, 0);
+  size_t k_next = cfg.recursive_ks[0];
+  pcs::ProverData prev = pcs::ligero_commit(sp.f(), n_cur - k_next, k_next, cfg.log_inv_rates[1],
+                                            merkle::HashKind::Blake3, nullptr, stream);
+  size_t prev_k = k_next, prev_block = size_t(1) << (n_cur - k_next + cfg.log_inv_rates[1]);
+  proof.recursive_roots.push_back(prev.merkle_tree.to_host(stream).back());
+  ch.observe_bytes(proof.recursive_roots.back().data(), 32);
+  ood(1);
+  proof.grinding_nonces.push_back(ch.grind_pow(cfg.grinding_bits[0], stream));
+  auto q0 = ch.sample_distinct_queries(block0, cfg.queries[0]);
+  auto a0 = ch.sample_f128_vec(ceil_log2(cfg.queries[0]));
+  proof.initial_proof = open(l0_codeword, l0_tree, size_t(1) << k0, q0);
+  {
+    auto [basis, h] = induce(n_cur, proof.initial_proof, r_lane, q0, a0);
+    introduce(std::move(basis), h);
+  }
+  for (size_t i = 0; i < r; ++i) {
+    const std::vector<F128> level_rs = fold_rounds(cfg.recursive_ks[i], i + 1);
+    n_cur -= cfg.recursive_ks[i];
+    if (i == r - 1) {
+      proof.yr = sp.f().to_host(stream);
+      for (const F128& v : proof.yr) ch.observe_f128(v);
+      proof.grinding_nonces.push_back(ch.grind_pow(cfg.grinding_bits[i + 1], stream));
+      auto ql = ch.sample_distinct_queries(prev_block, cfg.queries[i + 1]);
+      proof.final_proof = open(prev.codeword, prev.merkle_tree, size_t(1) << prev_k, ql);
+      proof.sumcheck_transcript = sp.transcript();
+      return proof;
+    }
+    k_next = cfg.recursive_ks[i + 1];
+    pcs::ProverData nxt = pcs::ligero_commit(sp.f(), n_cur - k_next, k_next, cfg.log_inv_rates[i + 2],
+                                             merkle::HashKind::Blake3, nullptr, stream);
+    proof.recursive_roots.push_back(nxt.merkle_tree.to_host(stream).back());
+    ch.observe_bytes(proof.recursive_roots.back().data(), 32);
+    ood(i + 2);
+    proof.grinding_nonces.push_back(ch.grind_pow(cfg.grinding_bits[i + 1], stream));
+    auto qi = ch.sample_distinct_queries(prev_block, cfg.queries[i + 1]);
+    auto ai = ch.sample_f128_vec(ceil_log2(cfg.queries[i + 1]));
+    proof.recursive_proofs.push_back(open(prev.codeword, prev.merkle_tree, size_t(1) << prev_k, qi));
+    {
+      auto [basis, h] = induce(n_cur, proof.recursive_proofs.back(), level_rs, qi, ai);
+      introduce(std::move(basis), h);
+    }
+    prev = std::move(nxt);
+    prev_k = k_next;
+    prev_block = size_t(1) << (n_cur - k_next + cfg.log_inv_rates[i + 2]);
+  }
+  throw std::logic_error("recursive_prover_with_basis: unreachable");
+}
+}  // namespace ligerito
 
 }  // namespace flock

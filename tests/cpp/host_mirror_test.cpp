@@ -37,7 +37,27 @@ static bool same(const std::vector<F128>& a, const std::vector<F128>& b) {
 
 static orc_f128 O(F128 a) { return orc_f128{a.lo, a.hi}; }
 
-int main() {
+static void dump_f128s(FILE* o, const char* key, const std::vector<F128>& v) {
+  std::fprintf(o, "%s", key);
+  for (const F128& x : v)
+    std::fprintf(o, " %016llx%016llx", (unsigned long long)x.hi, (unsigned long long)x.lo);
+  std::fprintf(o, "\n");
+}
+static void dump_hashes(FILE* o, const char* key, const std::vector<flock::Hash>& v) {
+  std::fprintf(o, "%s", key);
+  for (const auto& h : v) {
+    std::fprintf(o, " ");
+    for (uint8_t c : h) std::fprintf(o, "%02x", c);
+  }
+  std::fprintf(o, "\n");
+}
+static void dump_u64s(FILE* o, const char* key, const std::vector<uint64_t>& v) {
+  std::fprintf(o, "%s", key);
+  for (uint64_t x : v) std::fprintf(o, " %llu", (unsigned long long)x);
+  std::fprintf(o, "\n");
+}
+
+int main(int argc, char** argv) {
   if (!flk_device_available()) {
     std::fprintf(stderr, "no device: %s\n", flk_last_error());
     return 2;
@@ -223,6 +243,45 @@ int main() {
     ok = ok && std::memcmp(t.state_digest().data(), od, 32) == 0;
     orc_fs_free(ot);
     EXPECT(ok, "SumcheckProver::ood == oracle (values, messages, basis, target, transcript)");
+  }
+  // recursive Ligerito prover (two recursive levels); with a path argument the
+  // proof is written out for the Python-side verifier and prover comparison
+  {
+    const size_t log_n = 12, k0 = 3;
+    std::vector<F128> f = rand_f128(7, size_t(1) << log_n), b = rand_f128(8, size_t(1) << log_n);
+    F128 target{0, 0};
+    for (size_t i = 0; i < f.size(); ++i) target = target + f[i] * b[i];
+    auto l0 = flock::pcs::ligero_commit(DeviceVec<F128>(f), log_n - k0, k0, 1, flock::merkle::HashKind::Blake3);
+    flock::ligerito::ProverConfig cfg{k0, {2, 2}, {1, 2, 2}, {8, 6, 5}, {0, 1, 1}, {2, 2, 2}, {3, 2, 1}};
+    flock::challenger::FsChallenger t("flock-test-v0");
+    auto p = flock::ligerito::recursive_prover_with_basis(cfg, DeviceVec<F128>(f), DeviceVec<F128>(b), target,
+                                                          l0.codeword, l0.merkle_tree, t);
+    EXPECT(p.yr.size() == 32 && p.recursive_roots.size() == 2 && p.recursive_proofs.size() == 1 &&
+               p.grinding_nonces.size() == 3 && p.ood_values.size() == 2 && p.sumcheck_transcript.size() == 12,
+           "ligerito proof shape");
+    if (argc > 1) {
+      FILE* o = std::fopen(argv[1], "w");
+      EXPECT(o, "open the proof dump");
+      dump_hashes(o, "initial_root", {p.initial_root});
+      dump_f128s(o, "initial_rows", p.initial_proof.rows);
+      dump_hashes(o, "initial_merkle", p.initial_proof.merkle_proof);
+      dump_hashes(o, "recursive_roots", p.recursive_roots);
+      dump_f128s(o, "recursive_rows", p.recursive_proofs[0].rows);
+      dump_hashes(o, "recursive_merkle", p.recursive_proofs[0].merkle_proof);
+      dump_f128s(o, "yr", p.yr);
+      dump_f128s(o, "final_rows", p.final_proof.rows);
+      dump_hashes(o, "final_merkle", p.final_proof.merkle_proof);
+      std::vector<F128> msgs;
+      for (const auto& m : p.sumcheck_transcript) {
+        msgs.push_back(m.u_0);
+        msgs.push_back(m.u_2);
+      }
+      dump_f128s(o, "sumcheck_transcript", msgs);
+      dump_u64s(o, "grinding_nonces", p.grinding_nonces);
+      dump_u64s(o, "fold_grinding_nonces", p.fold_grinding_nonces);
+      dump_f128s(o, "ood_values", p.ood_values);
+      std::fclose(o);
+    }
   }
   std::printf("host mirror parity ok\n");
   return 0;

@@ -107,3 +107,31 @@ def test_tampered_proofs_rejected(orc, dev):
     b2 = b.copy()
     b2[5]["hi"] ^= np.uint64(1)
     assert not verify(orc, cfg, good, b2, target, root, orc.fs(b"flock-test-v0"))
+
+
+def test_cpp_mirror_matches(orc, dev, tmp_path):
+    """The C++ host mirror's recursive_prover_with_basis (include/flock_host.hpp,
+    run by tests/cpp/host_mirror_test) writes the same proof as the Python one
+    on the same instance, and the restated verifier accepts it."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "host_mirror_test")
+    out = tmp_path / "proof.txt"
+    r = subprocess.run([exe, str(out)], capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stderr
+    fields = {}
+    for line in out.read_text().splitlines():
+        key, *vals = line.split(" ")
+        fields[key] = vals
+    cfg, p, b, target, root = prove(orc, "two_levels", 7)
+    f128s = lambda k: [int(v, 16) for v in fields[k]]
+    hashes = lambda k: [bytes.fromhex(v) for v in fields[k]]
+    assert hashes("initial_root") == [p.initial_root] and hashes("recursive_roots") == p.recursive_roots
+    assert f128s("yr") == p.final_proof.yr and f128s("ood_values") == p.ood_values
+    msgs = f128s("sumcheck_transcript")
+    assert list(zip(msgs[::2], msgs[1::2])) == p.sumcheck_transcript
+    assert [int(v) for v in fields["grinding_nonces"]] == p.grinding_nonces
+    assert [int(v) for v in fields["fold_grinding_nonces"]] == p.fold_grinding_nonces
+    for key, op in (("initial", p.initial_proof), ("recursive", p.recursive_proofs[0]), ("final", p.final_proof)):
+        assert f128s(f"{key}_rows") == f128_to_ints(from_dev(op.opened_rows))
+        assert hashes(f"{key}_merkle") == [bytes(h) for h in op.merkle_proof.cpu().numpy()]
