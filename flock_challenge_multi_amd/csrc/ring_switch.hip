@@ -6,7 +6,7 @@
 //                                       ring-switched claims
 //
 // The large work is on the device: each claim's suffix tensor eq(x_outer[1..])
-// (flk_build_eq; from 2^12 elements on only its two split factors), its
+// (flk_build_eq; from 2^20 elements on only its two split factors), its
 // s_hat_v = fold_1b_rows(z, suffix), and one fused pass that writes the
 // combined basis
 //   b[i] = sum_k sum_b bit_b(suffix_k[i]) * gamma_k eq(r''_k)[b]
@@ -67,6 +67,7 @@ __global__ void __launch_bounds__(RC_WG) rs_combine_kernel(const uint4* __restri
 // read, lo_k (64 KiB) stays in L2.  Each lane owns 16 elements of the block.
 constexpr uint32_t CS_LO_LOG = 12;
 constexpr uint32_t CS_PER_LANE = (1u << CS_LO_LOG) / RC_WG;
+constexpr size_t CS_SPLIT_MIN_LOG = 20;
 
 __device__ __forceinline__ uint4 fold_nib(uint4 v, const uint4* t) {
   const uint4 r0 = nibtab8_sum(v.x, t), r1 = nibtab8_sum(v.y, t + 128);
@@ -178,9 +179,10 @@ int prove_batched(flk_fs* fs, const flk_f128* z, size_t log_len, const flk_f128*
                   flk_f128* s_hat_v_out, flk_f128* claims_out, flk_f128* gammas_out, flk_f128* b_combined,
                   hipStream_t st) {
   const size_t n = size_t(1) << log_len, d = log_len + 1;  // x_outer has log_len + 1 coordinates
-  // From 2^12 elements on, the suffix tensor stays split: per claim eq_lo
-  // (2^12 elements) and eq_hi (2^(log_len - 12)) instead of 2^log_len.
-  const bool split = log_len >= CS_LO_LOG;
+  // From 2^20 elements on (>= 256 blocks of 2^12, one work-group each), the
+  // suffix tensor stays split: per claim eq_lo (2^12 elements) and eq_hi
+  // (2^(log_len - 12)) instead of 2^log_len.
+  const bool split = log_len >= CS_SPLIT_MIN_LOG;
   const size_t hi_log = split ? log_len - CS_LO_LOG : 0, n_hi = size_t(1) << hi_log;
   const size_t per_claim = split ? (size_t(1) << CS_LO_LOG) + n_hi : n;
 

@@ -268,7 +268,7 @@ struct flk_fs;
  * sum_k fold_b128_elems(eq(x_outer_k[1..]), gamma_k eq(r''_k)), the sum of the
  * gamma-baked rs_eq_ind that compute_combined_basis_and_target forms
  * (CF/pcs.rs:865-870 and its b_combined sum).  1 <= n_claims <= 64.  The suffix
- * tensors live in stream-ordered scratch for the call (from log_len 12 on only
+ * tensors live in stream-ordered scratch for the call (from log_len 20 on only
  * their split factors, 2^12 + 2^(log_len - 12) elements per claim); the call
  * synchronises `stream`.  PaddingSpec only lets the reference
  * skip all-zero chunks; the values equal the dense ones computed here. */

@@ -31,7 +31,7 @@ def prefix(t, r):
 
 
 @pytest.mark.parametrize("log_len,claims", [(0, 1), (1, 2), (3, 1), (5, 2), (5, 3), (6, 4), (6, 5),
-                                            (9, 2), (12, 9), (14, 2)])
+                                            (9, 2), (12, 9), (14, 2), (20, 2), (20, 5)])
 def test_combined_basis_matches_oracle(orc, dev, log_len, claims):
     r = random.Random(log_len * 100 + claims)
     z = orc.fill(log_len + 7 * claims, 1 << log_len)
