@@ -219,6 +219,11 @@ __global__ void __launch_bounds__(256) lc_bind_top_kernel(uint4* __restrict__ v,
 // nibble up.  out: work-group partials, out[r * gridDim.x + blockIdx.x].
 // ---------------------------------------------------------------------------
 constexpr int F1_WG = 256;
+// Entries between the 16-entry subset-sum tables of consecutive element groups:
+// the pad of one entry shifts each group's table by one bank group, so the
+// quads' table stores (16 groups per wave) spread over the LDS banks instead of
+// all hitting the same ones; a lookup still reads one 16-entry table.
+constexpr int F1_TS = 17;
 
 // Lane K of the caller's aligned lane quad, broadcast to the quad (DPP).
 template <int K>
@@ -256,7 +261,7 @@ __device__ __forceinline__ void f1_tile(uint4 zv, uint4 ev, uint32_t t, uint32_t
   uint4 base = f_zero();
   if (q & 1u) base = f_add(base, e2);
   if (q & 2u) base = f_add(base, e3);
-  uint4* tg = tabs + (t >> 2) * 16 + 4 * q;
+  uint4* tg = tabs + (t >> 2) * F1_TS + 4 * q;
   tg[0] = base;
   tg[1] = f_add(base, e0);
   tg[2] = f_add(base, e1);
@@ -278,8 +283,8 @@ __device__ __forceinline__ void f1_tile(uint4 zv, uint4 ev, uint32_t t, uint32_t
   const uint32_t qr = r >> 3, shr = 4u * (r & 7u);
 #pragma unroll 4
   for (uint32_t g = halfsel * 32; g < halfsel * 32 + 32; g += 2) {
-    const uint4 x0 = tabs[g * 16 + ((nibw[g * 16 + qr] >> shr) & 15u)];
-    const uint4 x1 = tabs[(g + 1) * 16 + ((nibw[(g + 1) * 16 + qr] >> shr) & 15u)];
+    const uint4 x0 = tabs[g * F1_TS + ((nibw[g * 16 + qr] >> shr) & 15u)];
+    const uint4 x1 = tabs[(g + 1) * F1_TS + ((nibw[(g + 1) * 16 + qr] >> shr) & 15u)];
     acc = make_uint4(xor3(acc.x, x0.x, x1.x), xor3(acc.y, x0.y, x1.y), xor3(acc.z, x0.z, x1.z),
                      xor3(acc.w, x0.w, x1.w));
   }
@@ -291,7 +296,7 @@ __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_kernel(const uint4* __r
                                                                uint4* __restrict__ out) {
   __shared__ __align__(16) uint32_t zt[F1_WG * 4];
   __shared__ uint32_t nibw[(F1_WG / 4) * 16];  // [group][r / 8]: nibbles of r = 8q .. 8q+7
-  __shared__ uint4 tabs[(F1_WG / 4) * 16];
+  __shared__ uint4 tabs[(F1_WG / 4) * F1_TS];
   __shared__ uint4 red[128];
   const uint32_t t = threadIdx.x;
   const uint32_t r = t & 127u, halfsel = t >> 7;
@@ -325,7 +330,7 @@ __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_split_kernel(const uint
                                                                      uint64_t n_hi, uint4* __restrict__ out) {
   __shared__ __align__(16) uint32_t zt[F1_WG * 4];
   __shared__ uint32_t nibw[(F1_WG / 4) * 16];
-  __shared__ uint4 tabs[(F1_WG / 4) * 16];
+  __shared__ uint4 tabs[(F1_WG / 4) * F1_TS];
   __shared__ uint4 red[128];
   __shared__ uint4 etab[16];
   const uint32_t t = threadIdx.x;
