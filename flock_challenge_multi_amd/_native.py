@@ -54,6 +54,8 @@ SIGNATURES = {
     "flk_f128_mul": (_F, [_F, _F]),
     "flk_alloc": (_I, [_SZ, ctypes.POINTER(ctypes.c_void_p)]),
     "flk_free": (_I, [_P]),
+    "flk_alloc_async": (_I, [_SZ, _P, ctypes.POINTER(ctypes.c_void_p)]),
+    "flk_free_async": (_I, [_P, _P]),
     "flk_copy_to_device": (_I, [_P, _P, _SZ, _P]),
     "flk_copy_to_host": (_I, [_P, _P, _SZ, _P]),
     "flk_synchronize": (_I, [_P]),

@@ -308,6 +308,29 @@ int flk_free(void* p) {
   return FLK_OK;
 }
 
+int flk_alloc_async(size_t bytes, void* stream, void** out) {
+  if (!out) return flk::set_error(FLK_E_SHAPE, "flk_alloc_async: null output");
+  *out = nullptr;
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (bytes == 0) return FLK_OK;
+  flk::keep_pool_memory();
+  hipError_t e = hipMallocAsync(out, bytes, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return flk::set_error(FLK_E_NOMEM, "flk_alloc_async(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  return FLK_OK;
+}
+
+int flk_free_async(void* p, void* stream) {
+  if (!p) return FLK_OK;
+  hipError_t e = hipFreeAsync(p, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return flk::device_failure(e, "flk_free_async");
+  return FLK_OK;
+}
+
 int flk_copy_to_device(void* dst, const void* src, size_t bytes, void* stream) {
   int rc = flk::check_enabled();
   if (rc) return rc;

@@ -89,6 +89,14 @@ flk_f128 flk_f128_mul(flk_f128 a, flk_f128 b);
 int flk_alloc(size_t bytes, void **out);
 /* Free a pointer from flk_alloc (NULL is a no-op). */
 int flk_free(void *p);
+/* Stream-ordered pool allocation (hipMallocAsync / hipFreeAsync on `stream`):
+ * the memory may be used by work enqueued on `stream` after the allocation and
+ * returns to the pool when `stream` reaches the free, without a device-wide
+ * synchronisation.  Work on other streams must be ordered against both by the
+ * caller.  For the C++ host mirror's temporaries (sumcheck folds, recursive
+ * commits); no reference counterpart. */
+int flk_alloc_async(size_t bytes, void *stream, void **out);
+int flk_free_async(void *p, void *stream);
 /* Enqueue a host -> device copy on `stream` (src is HOST memory). */
 int flk_copy_to_device(void *dst, const void *src, size_t bytes, void *stream);
 /* Enqueue a device -> host copy on `stream` (dst is HOST memory). */

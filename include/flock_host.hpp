@@ -38,16 +38,19 @@ template <class T>
 class DeviceVec {
  public:
   DeviceVec() = default;
+  // Memory comes from the stream-ordered pool of the default stream (no
+  // device-wide synchronisation on free); work on other streams that uses it
+  // must be ordered against the default stream by the caller.
   explicit DeviceVec(size_t n) : n_(n) {
     void* p = nullptr;
-    check(flk_alloc(n * sizeof(T), &p), "flk_alloc");
+    check(flk_alloc_async(n * sizeof(T), nullptr, &p), "flk_alloc_async");
     p_ = static_cast<T*>(p);
   }
   explicit DeviceVec(const std::vector<T>& host, void* stream = nullptr) : DeviceVec(host.size()) {
     upload(host, stream);
   }
   ~DeviceVec() {
-    if (p_) flk_free(p_);
+    if (p_) flk_free_async(p_, nullptr);
   }
   DeviceVec(DeviceVec&& o) noexcept : p_(o.p_), n_(o.n_) {
     o.p_ = nullptr;
@@ -68,6 +71,14 @@ class DeviceVec {
   void upload(const std::vector<T>& host, void* stream = nullptr) {
     if (host.size() != n_) throw std::invalid_argument("DeviceVec::upload: length mismatch");
     check(flk_copy_to_device(p_, host.data(), n_ * sizeof(T), stream), "flk_copy_to_device");
+  }
+  // The last element (e.g. a Merkle tree's root) without copying the rest.
+  T back(void* stream = nullptr) const {
+    if (!n_) throw std::out_of_range("DeviceVec::back: empty");
+    T h;
+    check(flk_copy_to_host(&h, p_ + (n_ - 1), sizeof(T), stream), "flk_copy_to_host");
+    check(flk_synchronize(stream), "flk_synchronize");
+    return h;
   }
   std::vector<T> to_host(void* stream = nullptr) const {
     std::vector<T> h(n_);
@@ -582,7 +593,7 @@ inline LigeritoProof recursive_prover_with_basis(const ProverConfig& cfg, Device
   const size_t block0 = size_t(1) << (n_cur + cfg.log_inv_rates[0]);
   if (l0_codeword.len() != block0 << k0) throw std::invalid_argument("recursive_prover_with_basis: L0 codeword");
   LigeritoProof proof;
-  proof.initial_root = l0_tree.to_host(stream).back();
+  proof.initial_root = l0_tree.back(stream);
   auto observe_msg = [&](const SumcheckMessage& m) {
     ch.observe_f128(m.u_0);
     ch.observe_f128(m.u_2);
@@ -634,7 +645,7 @@ inline LigeritoProof recursive_prover_with_basis(const ProverConfig& cfg, Device
   pcs::ProverData prev = pcs::ligero_commit(sp.f(), n_cur - k_next, k_next, cfg.log_inv_rates[1],
                                             merkle::HashKind::Blake3, nullptr, stream);
   size_t prev_k = k_next, prev_block = size_t(1) << (n_cur - k_next + cfg.log_inv_rates[1]);
-  proof.recursive_roots.push_back(prev.merkle_tree.to_host(stream).back());
+  proof.recursive_roots.push_back(prev.merkle_tree.back(stream));
   ch.observe_bytes(proof.recursive_roots.back().data(), 32);
   ood(1);
   proof.grinding_nonces.push_back(ch.grind_pow(cfg.grinding_bits[0], stream));
@@ -660,7 +671,7 @@ inline LigeritoProof recursive_prover_with_basis(const ProverConfig& cfg, Device
     k_next = cfg.recursive_ks[i + 1];
     pcs::ProverData nxt = pcs::ligero_commit(sp.f(), n_cur - k_next, k_next, cfg.log_inv_rates[i + 2],
                                              merkle::HashKind::Blake3, nullptr, stream);
-    proof.recursive_roots.push_back(nxt.merkle_tree.to_host(stream).back());
+    proof.recursive_roots.push_back(nxt.merkle_tree.back(stream));
     ch.observe_bytes(proof.recursive_roots.back().data(), 32);
     ood(i + 2);
     proof.grinding_nonces.push_back(ch.grind_pow(cfg.grinding_bits[i + 1], stream));

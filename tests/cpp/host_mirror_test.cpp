@@ -2,6 +2,8 @@
 // oracle, written the way the reference's own tests read
 // (CF/pcs/commit.rs:1838-1877 commit_matches_full_ntt_oracle; the SumcheckProver
 // fold/introduce/glue sequence of CF/pcs/ligerito.rs:8906-9275).  Exit code 0 = pass.
+#include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -57,7 +59,41 @@ static void dump_u64s(FILE* o, const char* key, const std::vector<uint64_t>& v) 
   std::fprintf(o, "\n");
 }
 
+// The recursive Ligerito open with the m = 32 Fast ladder on a 2^25-element
+// (f, b) pair and its L0 commitment: prints the median wall time of 5 opens.
+static int bench_open_m32() {
+  const size_t log_n = 25, k0 = 6, n = size_t(1) << log_n;
+  DeviceVec<F128> f(rand_f128(0xF00D, n)), b(rand_f128(0xBEEF, n));
+  auto l0 = flock::pcs::ligero_commit(f, log_n - k0, k0, 1, flock::merkle::HashKind::Blake3);
+  DeviceVec<F128> out3(3);
+  flock::check(flk_sc_round_msg_and_eval(f.data(), b.data(), n, out3.data(), nullptr), "round_msg_and_eval");
+  const F128 target = out3.to_host()[2];
+  flock::ligerito::ProverConfig cfg{k0, {3, 3, 3, 3, 3}, {1, 2, 3, 4, 5, 6}, {218, 106, 71, 53, 43, 36},
+                                    {0, 1, 1, 1, 1, 1}, {0, 0, 0, 0, 0, 0}, {19, 14, 11, 8, 6, 4}};
+  std::vector<double> ms;
+  for (int it = 0; it < 6; ++it) {
+    DeviceVec<F128> fc(n), bc(n);
+    flock::check(flk_copy_to_device(fc.data(), f.to_host().data(), n * 16, nullptr), "copy");
+    flock::check(flk_copy_to_device(bc.data(), b.to_host().data(), n * 16, nullptr), "copy");
+    flock::check(flk_synchronize(nullptr), "sync");
+    flock::challenger::FsChallenger t("flock-bench");
+    const auto t0 = std::chrono::steady_clock::now();
+    auto p = flock::ligerito::recursive_prover_with_basis(cfg, std::move(fc), std::move(bc), target, l0.codeword,
+                                                          l0.merkle_tree, t);
+    flock::check(flk_synchronize(nullptr), "sync");
+    const double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (it > 0) ms.push_back(dt);  // the first open warms the twiddle caches
+    if (p.yr.size() != 16) return 1;
+    std::printf("open %d: %.3f ms\n", it, dt);
+    std::fflush(stdout);
+  }
+  std::sort(ms.begin(), ms.end());
+  std::printf("ligerito open m32 (C++ host, median of %zu): %.3f ms\n", ms.size(), ms[ms.size() / 2]);
+  return 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc > 1 && std::strcmp(argv[1], "--bench-open-m32") == 0) return bench_open_m32();
   if (!flk_device_available()) {
     std::fprintf(stderr, "no device: %s\n", flk_last_error());
     return 2;
