@@ -21,7 +21,9 @@ BLAKE3 proof-of-work hash rate and grind latency at 19 bits.
 
 `ms_per_step` is the wall time of `steps` commits queued back to back between
 two device synchronisations; p50/p99 come from a second loop of the same
-commits, each bracketed by HIP events.
+commits, each bracketed by HIP events.  `graph_replay` times the same commit
+replayed from a captured HIP graph (same kernels, root checked against the
+eager one); it is reported beside `value`, not in it.
 """
 import argparse
 import ctypes
@@ -434,6 +436,34 @@ def main():
     torch.cuda.synchronize()
     lat_ms = [s.elapsed_time(e) for s, e in evs]
 
+    # ---- the same commit replayed from a HIP graph (the same kernels; only the
+    # launch path differs), reported beside `value`, root checked against the eager one
+    graph = None
+    try:
+        eager_root = bytes(tree[-1].cpu().numpy().tobytes())
+        tree.zero_()
+        gr = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(gr):
+            cs = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+            nat.call("flk_commit", ctypes.c_void_p(msg.data_ptr()), LOG_COLS, LOG_LANES, LOG_RATE,
+                     nat.FLK_HASH_BLAKE3, ctypes.c_void_p(cw.data_ptr()), ctypes.c_void_p(tree.data_ptr()), None, cs)
+        for _ in range(max(args.warmup, 1)):
+            gr.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            gr.replay()
+        torch.cuda.synchronize()
+        gms = (time.perf_counter() - t0) * 1e3 / args.steps
+        graph = {"ms_per_step": gms, "compressions_per_s": compressions / (gms / 1e3),
+                 "root_matches_eager": bytes(tree[-1].cpu().numpy().tobytes()) == eager_root}
+        del gr
+    except Exception as ex:  # reported, never replaces the eager figure
+        graph = {"error": str(ex)}
+    finally:  # the checks below read the eager commit's codeword and tree
+        commit_once()
+        torch.cuda.synchronize()
+
     # ---- dominant kernel: the RS encode (NTT passes), timed alone ------------
     enc = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     for s, e in enc:
@@ -579,6 +609,7 @@ def main():
         },
         "merkle_roofline": {"achieved": mer_bytes / (mer_ms / 1e3) / 1e9, "unit": "GB/s",
                             "compressions_per_s": (n_pos * 17 - 1) / (mer_ms / 1e3)},
+        "graph_replay": graph,
         "m32_l0_commit": m32,
         "witness": witness,
         "pow": pow_,
