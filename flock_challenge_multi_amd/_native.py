@@ -111,6 +111,19 @@ SIGNATURES = {
     "flk_fs_sample_distinct_queries": (_I, [_P, ctypes.c_uint64, _SZ, _P]),
     "flk_gather_rows": (_I, [_P, _SZ, _SZ, _P, _SZ, _P, _P]),
     "flk_merkle_multi_proof": (_I, [_P, _SZ, _P, _SZ, _P, _SZ, ctypes.POINTER(_SZ), _P]),
+    "flk_copy_device_to_device": (_I, [_P, _P, _SZ, _P]),
+    "flk_blake3_r1cs_new": (_I, [_SZ, ctypes.POINTER(ctypes.c_void_p)]),
+    "flk_r1cs_free": (None, [_P]),
+    "flk_r1cs_shape": (_I, [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ), ctypes.POINTER(_SZ),
+                            ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
+    "flk_r1cs_statement_digest": (_I, [_P, _P]),
+    "flk_lincheck_comb": (_I, [_P, _F, _P, _SZ, _F, _F, _P, _P]),
+    "flk_zerocheck_prove": (_I, [_P, _P, _P, _P, _SZ, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "flk_lincheck_prove": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "flk_blake3_setup_new": (_I, [_SZ, _I, ctypes.POINTER(ctypes.c_void_p)]),
+    "flk_blake3_setup_free": (None, [_P]),
+    "flk_blake3_setup_info": (_I, [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ), ctypes.POINTER(_SZ), _P]),
+    "flk_blake3_prove": (_I, [_P, _P, _SZ, _P, _SZ, _P, _SZ, ctypes.POINTER(_SZ), _P, _P]),
 }
 
 _lib = None

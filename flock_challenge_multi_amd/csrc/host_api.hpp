@@ -1,0 +1,33 @@
+// Host-side helpers of the device library for the protocol code compiled
+// without the HIP headers (the .cpp files): status reporting, host GF(2^128)
+// arithmetic and the zerocheck constants (CF = crates/flock-core/src/).
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/flock_gpu.h"
+
+namespace flk {
+
+int set_error(int code, const char* fmt, ...);
+int check_enabled();
+flk_f128 host_mul(flk_f128 a, flk_f128 b);
+flk_f128 host_inv(flk_f128 a);
+
+// phi8: GF(2^8) -> GF(2^128), fixed by phi8(0x02) (CF/field/phi8.rs:19-31,1047).
+flk_f128 phi8_host(uint8_t v);
+
+// L_i(z) for i < 2^k_skip on the nodes phi8(base .. base + 2^k_skip - 1):
+// base 0 is the skip domain S (lagrange_weights_naive, CF/zerocheck/multilinear.rs:293),
+// base 2^k_skip the extension domain Lambda (lagrange_weights_lambda_naive, :399-405).
+// The node-only denominators are cached per (k_skip, base).  k_skip <= 7.
+int lagrange_weights_nodes(size_t k_skip, flk_f128 z, size_t base, flk_f128* out);
+
+// r[k_skip .. k_skip + 7) of the zerocheck: small_challenges_ghash (phi8 of 0xF7,
+// 0x53, 0xB5) then medium_challenges_ghash (x^(2^i) / (1 + x^(2^i)), i < 4),
+// CF/zerocheck/univariate_skip_optimized.rs:126-232.
+void zc_fixed_challenges(flk_f128 out[7]);
+
+inline flk_f128 fadd(flk_f128 a, flk_f128 b) { return flk_f128{a.lo ^ b.lo, a.hi ^ b.hi}; }
+
+}  // namespace flk

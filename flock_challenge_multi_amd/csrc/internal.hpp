@@ -55,5 +55,8 @@ int rs_fold_1b_rows_split(const flk_f128* z, const flk_f128* eq_lo, size_t lo_lo
 // Host-side GF(2^128) product (used for set-up constants only).
 flk_f128 host_mul(flk_f128 a, flk_f128 b);
 flk_f128 host_inv(flk_f128 a);
+flk_f128 phi8_host(uint8_t v);
+int lagrange_weights_nodes(size_t k_skip, flk_f128 z, size_t base, flk_f128* out);
+void zc_fixed_challenges(flk_f128 out[7]);
 
 }  // namespace flk

@@ -1,0 +1,588 @@
+// The BLAKE3 R1CS prover sequenced over the device entry points
+// (CF = crates/flock-core/src/, CP = crates/flock-prover/src/):
+//
+//   flk_zerocheck_prove   zerocheck::prove_packed_padded (CF/zerocheck.rs:384-535,
+//                         body :539-1620; transcript order of verify :1642-1790)
+//   flk_lincheck_prove    lincheck::prove_padded (CF/lincheck.rs:5345-5878)
+//   flk_blake3_setup_*    Blake3Setup::new (CP/r1cs_hashes/blake3.rs:4706-4788) with
+//                         the Ligerito ladder of (m, profile) (CF/pcs/ligerito.rs:256-345)
+//   flk_blake3_prove      witness + pcs::commit + prover::prove_ligerito
+//                         (CP/prover.rs:373-472) + R1csProofBundleLigerito::to_bytes
+//                         (CP/proof_io.rs:160-198)
+//
+// Every data-parallel step runs on the device on the caller's stream; the
+// host keeps the Fiat-Shamir transcript and the O(1)-sized protocol algebra
+// (Lagrange weights over 64 nodes, the 64-point interpolation of round 1's C
+// message, the final lincheck combination).  Each round reads its message
+// back (a few F128) before the next challenge can be sampled; that read-back
+// is the only synchronisation.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/flock_gpu.h"
+#include "../../include/flock_host.hpp"
+#include "host_api.hpp"
+#include "r1cs.hpp"
+
+using flock::DeviceVec;
+using flock::F128;
+using flock::Hash;
+
+namespace {
+
+constexpr size_t K_SKIP = flk::r1cs::K_SKIP;
+constexpr size_t K_LOG = flk::r1cs::K_LOG;
+constexpr size_t N_INNER = 7;  // the fixed small + medium eq coordinates of round 1
+
+void chk(int rc, const char* where) {
+  if (rc != FLK_OK) throw flock::GpuError(rc, where);
+}
+
+struct Fs {
+  flk_fs* t;
+  void label(const char* l) { chk(flk_fs_observe_label(t, reinterpret_cast<const uint8_t*>(l), std::strlen(l)), "observe_label"); }
+  void f128(F128 v) { chk(flk_fs_observe_f128(t, v), "observe_f128"); }
+  void slice(const std::vector<F128>& v) { chk(flk_fs_observe_f128_slice(t, v.data(), v.size()), "observe_f128_slice"); }
+  void bytes(const uint8_t* b, size_t n) { chk(flk_fs_observe_bytes(t, b, n), "observe_bytes"); }
+  F128 sample() {
+    F128 v;
+    chk(flk_fs_sample_f128(t, &v), "sample_f128");
+    return v;
+  }
+  std::vector<F128> sample_vec(size_t n) {
+    std::vector<F128> v(n);
+    chk(flk_fs_sample_f128_vec(t, n, v.data()), "sample_f128_vec");
+    return v;
+  }
+};
+
+F128 inner(const std::vector<F128>& a, const std::vector<F128>& b) {
+  F128 acc{0, 0};
+  for (size_t i = 0; i < a.size(); ++i) acc = flk::fadd(acc, flk::host_mul(a[i], b[i]));
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// Zerocheck
+// ---------------------------------------------------------------------------
+struct ZcProof {
+  std::vector<F128> round1_ab, round1_c;
+  std::vector<std::array<F128, 2>> rounds;  // (G(1), G(inf)) per multilinear round
+  F128 final_a, final_b, final_c;
+};
+struct ZcClaim {
+  F128 z;
+  std::vector<F128> mlv, r_rest;  // m - 6 each
+};
+
+void zerocheck_prove(Fs& ch, const uint8_t* a_bits, const uint8_t* b_bits, const uint8_t* c_bits, size_t m,
+                     ZcProof& pf, ZcClaim& cl, void* st) {
+  const size_t n_mlv = m - K_SKIP;
+  ch.label("flock-zerocheck-v0");
+  // r = [r_skip (6), small (3), medium (4), r_outer (m - 13)] (CF/zerocheck.rs:576-586).
+  std::vector<F128> r = ch.sample_vec(K_SKIP);
+  std::vector<F128> r_outer = ch.sample_vec(m - K_SKIP - N_INNER);
+  F128 fixed[N_INNER];
+  flk::zc_fixed_challenges(fixed);
+  for (const F128& f : fixed) r.push_back(f);
+  r.insert(r.end(), r_outer.begin(), r_outer.end());
+
+  // Round 1 in the naive convention (the C_s factor is part of round1_naive).
+  DeviceVec<F128> r1(128, st);
+  chk(flk_zc_round1(a_bits, b_bits, c_bits, m, r_outer.data(), r1.data(), r1.data() + 64, st), "zc_round1");
+  const std::vector<F128> h = r1.to_host();
+  pf.round1_ab.assign(h.begin(), h.begin() + 64);
+  pf.round1_c.assign(h.begin() + 64, h.end());
+  ch.slice(pf.round1_ab);
+  ch.slice(pf.round1_c);
+  const F128 z = ch.sample();
+
+  // c_eval = P^C(z) by interpolation on Lambda (interpolate_at_z_on_lambda, multilinear.rs:410-428).
+  std::vector<F128> lam(64), ws(64);
+  chk(flk::lagrange_weights_nodes(K_SKIP, z, 64, lam.data()), "lagrange_weights(Lambda)");
+  pf.final_c = inner(lam, pf.round1_c);
+
+  // Fold at z on S, then the multilinear rounds binding the low variable first
+  // (fold_at_z_naive, round_pair_naive with r_next = [1, r[k_skip + i + 1 ..]],
+  // fold_in_place_pair; multilinear.rs:575-640, 3194-3213).
+  chk(flk::lagrange_weights_nodes(K_SKIP, z, 0, ws.data()), "lagrange_weights(S)");
+  DeviceVec<F128> wd(ws, st);
+  size_t n = size_t(1) << n_mlv;
+  DeviceVec<F128> am(n, st), bm(n, st), msg(2, st);
+  chk(flk_zc_fold_at_z(a_bits, m, K_SKIP, wd.data(), am.data(), st), "zc_fold_at_z(a)");
+  chk(flk_zc_fold_at_z(b_bits, m, K_SKIP, wd.data(), bm.data(), st), "zc_fold_at_z(b)");
+  std::vector<F128> rhos;
+  auto fold = [&](F128 rho) {
+    DeviceVec<F128> an(n / 2, st), bn(n / 2, st);
+    chk(flk_partial_eval_lsb(am.data(), n, &rho, 1, an.data(), st), "zc fold(a)");
+    chk(flk_partial_eval_lsb(bm.data(), n, &rho, 1, bn.data(), st), "zc fold(b)");
+    am = std::move(an);
+    bm = std::move(bn);
+    n /= 2;
+  };
+  for (size_t i = 0; i < n_mlv; ++i) {
+    if (i) fold(rhos.back());
+    const size_t d = n_mlv - i - 1;  // eq over r[k_skip + i + 1 .. m)
+    DeviceVec<F128> eq(size_t(1) << d, st);
+    chk(flk_build_eq(r.data() + K_SKIP + i + 1, d, eq.data(), st), "build_eq");
+    chk(flk_zc_round_pair(am.data(), bm.data(), n, eq.data(), msg.data(), st), "zc_round_pair");
+    const std::vector<F128> g = msg.to_host();
+    pf.rounds.push_back({g[0], g[1]});
+    ch.f128(g[0]);
+    ch.f128(g[1]);
+    rhos.push_back(ch.sample());
+  }
+  fold(rhos.back());
+  pf.final_a = am.to_host()[0];
+  pf.final_b = bm.to_host()[0];
+  ch.f128(pf.final_a);
+  ch.f128(pf.final_b);
+  cl.z = z;
+  cl.mlv = rhos;
+  cl.r_rest.assign(r.begin() + K_SKIP, r.end());
+}
+
+// ---------------------------------------------------------------------------
+// Lincheck
+// ---------------------------------------------------------------------------
+struct LcProof {
+  std::vector<std::array<F128, 2>> rounds;  // (q(1), q(inf)) per round
+  std::vector<F128> z_partial;              // 2^k_skip
+};
+struct LcClaim {
+  F128 r_inner_skip;
+  std::vector<F128> r_inner_rest;
+  F128 w;
+};
+
+// x_ab = z_skip || x_inner_rest (k_log - k_skip) || x_outer (m - k_log).
+void lincheck_prove(Fs& ch, const flk_r1cs* r1cs, const uint8_t* z_stripe, size_t m, const F128* x_ab, LcProof& pf,
+                    LcClaim& cl, void* st) {
+  const size_t k = size_t(1) << K_LOG, rest = K_LOG - K_SKIP, n_outer = m - K_LOG;
+  ch.label("flock-lincheck-v0");
+  const F128 alpha = ch.sample();
+  const F128 beta = ch.sample();  // the constant-wire pin, sampled right after alpha
+  DeviceVec<F128> comb(k, st), zv(k, st), msg(2, st);
+  chk(flk_lincheck_comb(r1cs, x_ab[0], x_ab + 1, rest, alpha, beta, comb.data(), st), "lincheck_comb");
+  {
+    DeviceVec<F128> eq_outer(size_t(1) << n_outer, st);
+    chk(flk_build_eq(x_ab + 1 + rest, n_outer, eq_outer.data(), st), "build_eq(x_outer)");
+    chk(flk_lc_partial_fold(z_stripe, m, K_LOG, eq_outer.data(), zv.data(), st), "lc_partial_fold");
+  }
+  size_t n = k;
+  std::vector<F128> rr;
+  for (size_t t = 0; t < rest; ++t) {
+    chk(flk_lc_round_msg_msb(comb.data(), zv.data(), n, msg.data(), st), "lc_round_msg_msb");
+    const std::vector<F128> q = msg.to_host();
+    ch.f128(q[0]);
+    ch.f128(q[1]);
+    const F128 ri = ch.sample();
+    pf.rounds.push_back({q[0], q[1]});
+    rr.push_back(ri);
+    chk(flk_lc_bind_top(comb.data(), n, ri, st), "lc_bind_top(comb)");
+    chk(flk_lc_bind_top(zv.data(), n, ri, st), "lc_bind_top(z)");
+    n /= 2;
+  }
+  pf.z_partial.resize(n);
+  chk(flk_copy_to_host(pf.z_partial.data(), zv.data(), n * sizeof(F128), st), "copy z_partial");
+  chk(flk_synchronize(st), "synchronize");
+  ch.slice(pf.z_partial);
+  cl.r_inner_skip = ch.sample();
+  std::vector<F128> lam(size_t(1) << K_SKIP);
+  chk(flk::lagrange_weights_nodes(K_SKIP, cl.r_inner_skip, 0, lam.data()), "lagrange_weights");
+  cl.w = inner(lam, pf.z_partial);
+  cl.r_inner_rest.assign(rr.rbegin(), rr.rend());  // LSB-first order of x_inner_rest
+}
+
+// ---------------------------------------------------------------------------
+// Ligerito ladders (generated data; CF/pcs/ligerito.rs:963-1013)
+// ---------------------------------------------------------------------------
+struct Level {
+  uint32_t log_inv_rate, log_msg_cols, log_num_interleaved, k_recursive, queries, grinding_bits,
+      fold_grinding_bits, ood_samples;
+};
+struct Ladder {
+  uint32_t m, profile, log_n, initial_k, yr_log_n, n_levels;
+  Level levels[8];
+};
+const Ladder LADDERS[] = {
+#include "ligerito_ladders.inc"
+};
+
+const Ladder* find_ladder(size_t m, int profile) {
+  for (const Ladder& l : LADDERS)
+    if (l.m == m && (int)l.profile == profile) return &l;
+  return nullptr;
+}
+
+flock::ligerito::ProverConfig prover_config(const Ladder& l) {
+  flock::ligerito::ProverConfig c;
+  c.initial_k = l.initial_k;
+  for (uint32_t i = 0; i < l.n_levels; ++i) {
+    const Level& v = l.levels[i];
+    if (i) c.recursive_ks.push_back(v.k_recursive);
+    c.log_inv_rates.push_back(v.log_inv_rate);
+    c.queries.push_back(v.queries);
+    c.ood_samples.push_back(v.ood_samples);
+    c.grinding_bits.push_back(v.grinding_bits);
+    c.fold_grinding_bits.push_back(v.fold_grinding_bits);
+  }
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// bincode fixint encoding of R1csProofBundleLigerito (CP/proof_io.rs:160-198,
+// 804-941): u64le lengths, F128 = lo || hi, Hash = 32 raw bytes, unit enums
+// as u32le variant indices, structs as field concatenation.
+// ---------------------------------------------------------------------------
+struct Writer {
+  std::vector<uint8_t> b;
+  void raw(const void* p, size_t n) {
+    const uint8_t* q = static_cast<const uint8_t*>(p);
+    b.insert(b.end(), q, q + n);
+  }
+  void u64(uint64_t v) {
+    for (int i = 0; i < 8; ++i) b.push_back(uint8_t(v >> (8 * i)));
+  }
+  void u32(uint32_t v) {
+    for (int i = 0; i < 4; ++i) b.push_back(uint8_t(v >> (8 * i)));
+  }
+  void f128(F128 v) {
+    u64(v.lo);
+    u64(v.hi);
+  }
+  void f128_vec(const F128* v, size_t n) {
+    u64(n);
+    for (size_t i = 0; i < n; ++i) f128(v[i]);
+  }
+  void f128_vec(const std::vector<F128>& v) { f128_vec(v.data(), v.size()); }
+  void pairs(const std::vector<std::array<F128, 2>>& v) {
+    u64(v.size());
+    for (const auto& p : v) {
+      f128(p[0]);
+      f128(p[1]);
+    }
+  }
+  void hashes(const std::vector<Hash>& v) {
+    u64(v.size());
+    for (const Hash& h : v) raw(h.data(), 32);
+  }
+  void u64_vec(const std::vector<uint64_t>& v) {
+    u64(v.size());
+    for (uint64_t x : v) u64(x);
+  }
+  // Vec<Vec<F128>> of n_rows rows of row_len elements (query-major flat rows).
+  void rows(const std::vector<F128>& flat, size_t row_len) {
+    const size_t n_rows = row_len ? flat.size() / row_len : 0;
+    u64(n_rows);
+    for (size_t q = 0; q < n_rows; ++q) f128_vec(flat.data() + q * row_len, row_len);
+  }
+};
+
+}  // namespace
+
+struct flk_blake3_setup {
+  size_t n_blocks_log, m;
+  int profile;
+  const Ladder* ladder;
+  flk_r1cs* r1cs;
+  uint8_t digest[32];
+};
+
+namespace {
+
+// Bytes of device memory one prove holds at its peak (the zerocheck rounds):
+// z, a, b and the stripe (2^(m-3) bytes each), the codeword (2^(m-2)), the
+// tree (64 * 2^(m-12)), the folded a, b tables and their halves (3 * 2^(m-2)),
+// plus 1/8 headroom for the smaller tables.
+size_t prove_footprint(size_t m) {
+  const size_t z = size_t(1) << (m - 3);
+  const size_t base = 4 * z + 2 * z + 64 * (size_t(1) << (m - 12)) + 6 * z;
+  return base + base / 8;
+}
+
+int guarded(const char* where, int (*fn)(void*), void* ctx) {
+  try {
+    return fn(ctx);
+  } catch (const flock::GpuError& e) {
+    return e.code;
+  } catch (const std::invalid_argument& e) {
+    return flk::set_error(FLK_E_SHAPE, "%s: %s", where, e.what());
+  } catch (const std::bad_alloc&) {
+    return flk::set_error(FLK_E_NOMEM, "%s: out of host memory", where);
+  } catch (const std::exception& e) {
+    return flk::set_error(FLK_E_SHAPE, "%s: %s", where, e.what());
+  }
+}
+
+struct ProveCall {
+  const flk_blake3_setup* s;
+  const flk_compression* blocks;
+  size_t n_blocks;
+  const uint8_t* domain;
+  size_t domain_len;
+  uint8_t* out;
+  size_t cap;
+  size_t* out_len;
+  double* stage_s;
+  void* st;
+};
+
+int prove_impl(void* vp) {
+  using clk = std::chrono::steady_clock;
+  ProveCall& c = *static_cast<ProveCall*>(vp);
+  const flk_blake3_setup& s = *c.s;
+  void* st = c.st;
+  const size_t m = s.m, nbl = s.n_blocks_log;
+  const size_t n_f128 = size_t(1) << (m - 7);
+  auto t0 = clk::now();
+  double stages[6] = {0, 0, 0, 0, 0, 0};
+  auto lap = [&](int i) {
+    const auto t1 = clk::now();
+    stages[i] = std::chrono::duration<double>(t1 - t0).count();
+    t0 = t1;
+  };
+
+  // Witness (blake3.rs:4617-4647) and the PCS commit (CF/pcs/commit.rs:187-227).
+  DeviceVec<F128> z(n_f128, st);
+  DeviceVec<F128> a(n_f128, st), b(n_f128, st);
+  DeviceVec<uint8_t> stripe(size_t(2048) << nbl, st);
+  chk(flk_blake3_witness(c.blocks, c.n_blocks, nbl, z.data(), a.data(), b.data(), stripe.data(), st), "witness");
+  const size_t log_cols = m - 7 - 6, n_pos = size_t(1) << (log_cols + 1);
+  DeviceVec<F128> codeword(n_pos << 6, st);
+  DeviceVec<Hash> tree(2 * n_pos - 1, st);
+  Hash root;
+  chk(flk_commit(z.data(), log_cols, 6, 1, FLK_HASH_BLAKE3, codeword.data(), tree.data()->data(), root.data(), st),
+      "commit");
+  lap(0);
+
+  flock::challenger::FsChallenger fsc(std::string(reinterpret_cast<const char*>(c.domain), c.domain_len));
+  Fs ch{fsc.raw()};
+  // bind_statement (CF/proof.rs:51-59).
+  ch.label("flock-r1cs-v0");
+  ch.bytes(s.digest, 32);
+  ch.bytes(root.data(), 32);
+
+  ZcProof zp;
+  ZcClaim zc;
+  zerocheck_prove(ch, reinterpret_cast<const uint8_t*>(a.data()), reinterpret_cast<const uint8_t*>(b.data()),
+                  reinterpret_cast<const uint8_t*>(z.data()), m, zp, zc, st);
+  a = DeviceVec<F128>();
+  b = DeviceVec<F128>();
+  lap(1);
+
+  // x_ab = (z, mlv[..8], mlv[8..]) (BlockR1cs::x_ab_from_mlv, RowMajor; CF/r1cs.rs:241-270).
+  std::vector<F128> x_ab;
+  x_ab.push_back(zc.z);
+  x_ab.insert(x_ab.end(), zc.mlv.begin(), zc.mlv.end());
+  LcProof lp;
+  LcClaim lc;
+  lincheck_prove(ch, s.r1cs, stripe.data(), m, x_ab.data(), lp, lc, st);
+  stripe = DeviceVec<uint8_t>();
+  lap(2);
+
+  // The two z-claims, address-ordered for the PCS (ab_claim_point / c_claim_point,
+  // CF/r1cs.rs:272-330): x_full = x_inner_rest || x_outer.
+  const size_t rest = K_LOG - K_SKIP;
+  std::vector<F128> x_full_ab = lc.r_inner_rest;
+  x_full_ab.insert(x_full_ab.end(), zc.mlv.begin() + rest, zc.mlv.end());
+  std::vector<std::vector<F128>> x_fulls{x_full_ab, zc.r_rest};
+  flock::RingSwitchBatch rs = flock::pcs::compute_combined_basis_and_target(z, x_fulls, fsc, st);
+  lap(3);
+
+  const flock::ligerito::ProverConfig cfg = prover_config(*s.ladder);
+  flock::ligerito::LigeritoProof lig = flock::ligerito::recursive_prover_with_basis(
+      cfg, std::move(z), std::move(rs.b_combined), rs.target_combined, codeword, tree, fsc, st);
+  lap(4);
+
+  Writer w;
+  w.b.reserve(size_t(1) << 19);
+  w.raw("FLOCK", 5);
+  w.b.push_back(4);  // VERSION
+  w.b.push_back(2);  // flavor: R1cs Ligerito
+  // Commitment { root, PcsParams { m, log_inv_rate, log_batch_size, profile, merkle_hash } }
+  w.raw(root.data(), 32);
+  w.u64(m);
+  w.u64(s.ladder->levels[0].log_inv_rate);
+  w.u64(6);
+  w.u32(uint32_t(s.profile));
+  w.u32(1);  // HashKind::Blake3
+  // ZerocheckProof
+  w.f128_vec(zp.round1_ab);
+  w.f128_vec(zp.round1_c);
+  w.pairs(zp.rounds);
+  w.f128(zp.final_a);
+  w.f128(zp.final_b);
+  w.f128(zp.final_c);
+  // LincheckProof
+  w.pairs(lp.rounds);
+  w.f128_vec(lp.z_partial);
+  // BatchOpeningProofLigerito { ring_switches, ligerito }
+  w.u64(rs.s_hat_v.size());
+  for (const auto& v : rs.s_hat_v) w.f128_vec(v);
+  w.raw(lig.initial_root.data(), 32);
+  w.rows(lig.initial_proof.rows, size_t(1) << cfg.initial_k);
+  w.hashes(lig.initial_proof.merkle_proof);
+  w.hashes(lig.recursive_roots);
+  w.u64(lig.recursive_proofs.size());
+  for (size_t i = 0; i < lig.recursive_proofs.size(); ++i) {
+    w.rows(lig.recursive_proofs[i].rows, size_t(1) << cfg.recursive_ks[i]);
+    w.hashes(lig.recursive_proofs[i].merkle_proof);
+  }
+  w.f128_vec(lig.yr);
+  w.rows(lig.final_proof.rows, size_t(1) << cfg.recursive_ks.back());
+  w.hashes(lig.final_proof.merkle_proof);
+  w.u64(lig.sumcheck_transcript.size());
+  for (const auto& msg : lig.sumcheck_transcript) {
+    w.f128(msg.u_0);
+    w.f128(msg.u_2);
+  }
+  w.u64_vec(lig.grinding_nonces);
+  w.f128_vec(lig.ood_values);
+  w.u64_vec(lig.fold_grinding_nonces);
+  lap(5);
+
+  if (c.out_len) *c.out_len = w.b.size();
+  if (c.stage_s) std::copy(stages, stages + 6, c.stage_s);
+  if (w.b.size() > c.cap)
+    return flk::set_error(FLK_E_SHAPE, "blake3_prove: proof of %zu bytes exceeds the %zu-byte buffer", w.b.size(),
+                          c.cap);
+  std::memcpy(c.out, w.b.data(), w.b.size());
+  return FLK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int flk_zerocheck_prove(flk_fs* fs, const uint8_t* a_bits, const uint8_t* b_bits, const uint8_t* c_bits, size_t m,
+                        flk_f128* round1_ab, flk_f128* round1_c, flk_f128* rounds, flk_f128* finals,
+                        flk_f128* z_out, flk_f128* mlv_out, flk_f128* r_rest_out, void* stream) {
+  if (int rc = flk::check_enabled()) return rc;
+  if (!fs || !a_bits || !b_bits || !c_bits || !round1_ab || !round1_c || !rounds || !finals || !z_out || !mlv_out ||
+      !r_rest_out)
+    return flk::set_error(FLK_E_SHAPE, "zerocheck_prove: null pointer");
+  if (m < K_SKIP + N_INNER || m > 40) return flk::set_error(FLK_E_SHAPE, "zerocheck_prove: m=%zu outside 13..40", m);
+  struct Ctx {
+    flk_fs* fs;
+    const uint8_t *a, *b, *c;
+    size_t m;
+    flk_f128 *r1ab, *r1c, *rounds, *finals, *z, *mlv, *rr;
+    void* st;
+  } ctx{fs, a_bits, b_bits, c_bits, m, round1_ab, round1_c, rounds, finals, z_out, mlv_out, r_rest_out, stream};
+  return guarded("zerocheck_prove", [](void* p) -> int {
+    Ctx& c = *static_cast<Ctx*>(p);
+    Fs ch{c.fs};
+    ZcProof pf;
+    ZcClaim cl;
+    zerocheck_prove(ch, c.a, c.b, c.c, c.m, pf, cl, c.st);
+    std::copy(pf.round1_ab.begin(), pf.round1_ab.end(), c.r1ab);
+    std::copy(pf.round1_c.begin(), pf.round1_c.end(), c.r1c);
+    for (size_t i = 0; i < pf.rounds.size(); ++i) {
+      c.rounds[2 * i] = pf.rounds[i][0];
+      c.rounds[2 * i + 1] = pf.rounds[i][1];
+    }
+    c.finals[0] = pf.final_a;
+    c.finals[1] = pf.final_b;
+    c.finals[2] = pf.final_c;
+    *c.z = cl.z;
+    std::copy(cl.mlv.begin(), cl.mlv.end(), c.mlv);
+    std::copy(cl.r_rest.begin(), cl.r_rest.end(), c.rr);
+    return FLK_OK;
+  }, &ctx);
+}
+
+int flk_lincheck_prove(flk_fs* fs, const flk_r1cs* r1cs, const uint8_t* z_stripe, const flk_f128* x_ab,
+                       flk_f128* rounds, flk_f128* z_partial, flk_f128* r_inner_skip, flk_f128* r_inner_rest,
+                       flk_f128* w, void* stream) {
+  if (int rc = flk::check_enabled()) return rc;
+  if (!fs || !r1cs || !z_stripe || !x_ab || !rounds || !z_partial || !r_inner_skip || !r_inner_rest || !w)
+    return flk::set_error(FLK_E_SHAPE, "lincheck_prove: null pointer");
+  size_t m = 0;
+  if (int rc = flk_r1cs_shape(r1cs, &m, nullptr, nullptr, nullptr, nullptr)) return rc;
+  struct Ctx {
+    flk_fs* fs;
+    const flk_r1cs* r;
+    const uint8_t* z;
+    const flk_f128* x;
+    size_t m;
+    flk_f128 *rounds, *zp, *rs, *rr, *w;
+    void* st;
+  } ctx{fs, r1cs, z_stripe, x_ab, m, rounds, z_partial, r_inner_skip, r_inner_rest, w, stream};
+  return guarded("lincheck_prove", [](void* p) -> int {
+    Ctx& c = *static_cast<Ctx*>(p);
+    Fs ch{c.fs};
+    LcProof pf;
+    LcClaim cl;
+    lincheck_prove(ch, c.r, c.z, c.m, c.x, pf, cl, c.st);
+    for (size_t i = 0; i < pf.rounds.size(); ++i) {
+      c.rounds[2 * i] = pf.rounds[i][0];
+      c.rounds[2 * i + 1] = pf.rounds[i][1];
+    }
+    std::copy(pf.z_partial.begin(), pf.z_partial.end(), c.zp);
+    *c.rs = cl.r_inner_skip;
+    std::copy(cl.r_inner_rest.begin(), cl.r_inner_rest.end(), c.rr);
+    *c.w = cl.w;
+    return FLK_OK;
+  }, &ctx);
+}
+
+int flk_blake3_setup_new(size_t n_blocks, int profile, flk_blake3_setup** out) {
+  if (int rc = flk::check_enabled()) return rc;
+  if (!out) return flk::set_error(FLK_E_SHAPE, "blake3_setup_new: null pointer");
+  if (n_blocks < 1) return flk::set_error(FLK_E_SHAPE, "blake3_setup_new: n_blocks must be >= 1");
+  size_t nbl = 3;  // min_n_blocks_log: ceil(log2(max(n, 8))) (blake3.rs:1854-1858)
+  while ((size_t(1) << nbl) < n_blocks) ++nbl;
+  const size_t m = K_LOG + nbl;
+  const Ladder* l = find_ladder(m, profile);
+  if (!l || l->initial_k != 6)
+    return flk::set_error(FLK_E_SHAPE, "blake3_setup_new: no Ligerito ladder for m=%zu profile=%d (m = 22..35)", m,
+                          profile);
+  flk_r1cs* r = nullptr;
+  if (int rc = flk_blake3_r1cs_new(nbl, &r)) return rc;
+  flk_blake3_setup* s = new (std::nothrow) flk_blake3_setup{nbl, m, profile, l, r, {}};
+  if (!s) {
+    flk_r1cs_free(r);
+    return flk::set_error(FLK_E_NOMEM, "blake3_setup_new: out of host memory");
+  }
+  flk::r1cs::statement_digest(m, s->digest);
+  *out = s;
+  return FLK_OK;
+}
+
+void flk_blake3_setup_free(flk_blake3_setup* s) {
+  if (!s) return;
+  flk_r1cs_free(s->r1cs);
+  delete s;
+}
+
+int flk_blake3_setup_info(const flk_blake3_setup* s, size_t* m, size_t* n_blocks_log, size_t* device_bytes,
+                          uint8_t* digest_out) {
+  if (!s) return flk::set_error(FLK_E_SHAPE, "blake3_setup_info: null pointer");
+  if (m) *m = s->m;
+  if (n_blocks_log) *n_blocks_log = s->n_blocks_log;
+  if (device_bytes) *device_bytes = prove_footprint(s->m);
+  if (digest_out) std::memcpy(digest_out, s->digest, 32);
+  return FLK_OK;
+}
+
+int flk_blake3_prove(const flk_blake3_setup* s, const flk_compression* blocks, size_t n_blocks, const uint8_t* domain,
+                     size_t domain_len, uint8_t* proof_out, size_t cap, size_t* proof_len, double* stage_seconds,
+                     void* stream) {
+  if (int rc = flk::check_enabled()) return rc;
+  if (!s || (n_blocks && !blocks) || (domain_len && !domain) || !proof_out)
+    return flk::set_error(FLK_E_SHAPE, "blake3_prove: null pointer");
+  if (n_blocks > (size_t(1) << s->n_blocks_log))
+    return flk::set_error(FLK_E_SHAPE, "blake3_prove: %zu compressions for %zu slots", n_blocks,
+                          size_t(1) << s->n_blocks_log);
+  ProveCall c{s, blocks, n_blocks, domain, domain_len, proof_out, cap, proof_len, stage_seconds, stream};
+  return guarded("blake3_prove", prove_impl, &c);
+}
+
+}  // extern "C"

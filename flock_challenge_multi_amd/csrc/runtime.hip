@@ -106,6 +106,63 @@ flk_f128 host_inv(flk_f128 a) {
   return result;
 }
 
+flk_f128 phi8_host(uint8_t v) {
+  const flk_f128 g{0x6b8330483c2e9849ull, 0x0dcb364640a222feull};  // phi8(0x02), CF/field/phi8.rs:28-31
+  flk_f128 acc{0, 0}, p{1, 0};
+  for (int i = 0; i < 8; ++i) {
+    if ((v >> i) & 1) acc = flk_f128{acc.lo ^ p.lo, acc.hi ^ p.hi};
+    p = host_mul(p, g);
+  }
+  return acc;
+}
+
+// L_i(z) = prod_{j != i} (z + s_j) / prod_{j != i} (s_i + s_j), s_j = phi8(base + j).
+// The denominators depend on the nodes only and are inverted once per
+// (k_skip, base); the numerators are prefix / suffix products of (z + s_j).
+int lagrange_weights_nodes(size_t k_skip, flk_f128 z, size_t base, flk_f128* out) {
+  if (k_skip > 7 || base + (size_t(1) << k_skip) > 256 || !out)
+    return set_error(FLK_E_SHAPE, "lagrange_weights: k_skip=%zu base=%zu", k_skip, base);
+  static std::mutex mu;
+  static std::map<std::pair<size_t, size_t>, std::pair<std::vector<flk_f128>, std::vector<flk_f128>>> cache;
+  const size_t ell = size_t(1) << k_skip;
+  const std::vector<flk_f128>* nodes;
+  const std::vector<flk_f128>* den_inv;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto& e = cache[{k_skip, base}];
+    if (e.first.empty()) {
+      e.first.resize(ell);
+      e.second.resize(ell);
+      for (size_t j = 0; j < ell; ++j) e.first[j] = phi8_host((uint8_t)(base + j));
+      for (size_t i = 0; i < ell; ++i) {
+        flk_f128 den{1, 0};
+        for (size_t j = 0; j < ell; ++j)
+          if (j != i) den = host_mul(den, flk_f128{e.first[i].lo ^ e.first[j].lo, e.first[i].hi ^ e.first[j].hi});
+        e.second[i] = host_inv(den);
+      }
+    }
+    nodes = &e.first;
+    den_inv = &e.second;
+  }
+  std::vector<flk_f128> t(ell), pre(ell + 1), suf(ell + 1);
+  for (size_t j = 0; j < ell; ++j) t[j] = flk_f128{z.lo ^ (*nodes)[j].lo, z.hi ^ (*nodes)[j].hi};
+  pre[0] = flk_f128{1, 0};
+  for (size_t j = 0; j < ell; ++j) pre[j + 1] = host_mul(pre[j], t[j]);
+  suf[ell] = flk_f128{1, 0};
+  for (size_t j = ell; j-- > 0;) suf[j] = host_mul(suf[j + 1], t[j]);
+  for (size_t i = 0; i < ell; ++i) out[i] = host_mul(host_mul(pre[i], suf[i + 1]), (*den_inv)[i]);
+  return FLK_OK;
+}
+
+void zc_fixed_challenges(flk_f128 out[7]) {
+  static const uint8_t small[3] = {0xF7, 0x53, 0xB5};
+  for (int i = 0; i < 3; ++i) out[i] = phi8_host(small[i]);
+  for (int i = 0; i < 4; ++i) {
+    const flk_f128 g{uint64_t(1) << (1u << i), 0};
+    out[3 + i] = host_mul(g, host_inv(flk_f128{g.lo ^ 1u, 0}));
+  }
+}
+
 // ---- twiddle tables ---------------------------------------------------------
 // Row i of the normalised subspace evaluations: evals[i][k] = Ŵ_i(x^{i+k})
 // (CF/ntt/additive_ntt_f128.rs:54-84); twiddle(l, b) = sum_j bit_j(b) evals[dim-l-1][1+j]
@@ -351,6 +408,16 @@ int flk_copy_to_host(void* dst, const void* src, size_t bytes, void* stream) {
   return FLK_OK;
 }
 
+int flk_copy_device_to_device(void* dst, const void* src, size_t bytes, void* stream) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!bytes) return FLK_OK;
+  if (!dst || !src) return flk::set_error(FLK_E_SHAPE, "flk_copy_device_to_device: null pointer");
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return flk::device_failure(e, "flk_copy_device_to_device");
+  return FLK_OK;
+}
+
 int flk_synchronize(void* stream) {
   hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return flk::device_failure(e, "flk_synchronize");
@@ -358,32 +425,10 @@ int flk_synchronize(void* stream) {
 }
 
 int flk_zc_lagrange_weights(size_t k_skip, flk_f128 z, flk_f128* out) {
-  // L_i(z) = prod_{j != i} (z + phi8(j)) / (phi8(i) + phi8(j)) on the nodes
-  // phi8(0 .. 2^k_skip - 1) (CF/zerocheck/multilinear.rs:285-330).  phi8 is the
-  // embedding GF(2^8) -> GF(2^128) fixed by the image of x (0x02).
-  if (k_skip > 8 || !out) return flk::set_error(FLK_E_SHAPE, "zc_lagrange_weights: k_skip=%zu", k_skip);
-  const flk_f128 g{0x6b8330483c2e9849ull, 0x0dcb364640a222feull};
-  flk_f128 pw[8];
-  pw[0] = flk_f128{1, 0};
-  for (int i = 1; i < 8; ++i) pw[i] = flk::host_mul(pw[i - 1], g);
-  const size_t ell = size_t(1) << k_skip;
-  std::vector<flk_f128> phi(ell);
-  for (size_t v = 0; v < ell; ++v) {
-    flk_f128 acc{0, 0};
-    for (int i = 0; i < 8; ++i)
-      if ((v >> i) & 1) acc = flk_f128{acc.lo ^ pw[i].lo, acc.hi ^ pw[i].hi};
-    phi[v] = acc;
-  }
-  for (size_t i = 0; i < ell; ++i) {
-    flk_f128 num{1, 0}, den{1, 0};
-    for (size_t j = 0; j < ell; ++j) {
-      if (j == i) continue;
-      num = flk::host_mul(num, flk_f128{z.lo ^ phi[j].lo, z.hi ^ phi[j].hi});
-      den = flk::host_mul(den, flk_f128{phi[i].lo ^ phi[j].lo, phi[i].hi ^ phi[j].hi});
-    }
-    out[i] = flk::host_mul(num, flk::host_inv(den));
-  }
-  return FLK_OK;
+  // lagrange_weights_naive (CF/zerocheck/multilinear.rs:285-330) on the skip
+  // domain S = phi8(0 .. 2^k_skip - 1).
+  if (k_skip > 7 || !out) return flk::set_error(FLK_E_SHAPE, "zc_lagrange_weights: k_skip=%zu", k_skip);
+  return flk::lagrange_weights_nodes(k_skip, z, 0, out);
 }
 
 int flk_ntt_twiddles(size_t dim, flk_f128* out) {

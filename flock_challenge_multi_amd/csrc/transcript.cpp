@@ -22,9 +22,8 @@
 //   verify_pow(n, bits)    the PoW check against state_digest (n == 0 when
 //                          bits == 0), then observe_bytes(u64le(n))
 //
-// BLAKE3 here is written from the BLAKE3 specification (chunk chaining,
-// parent merges on a CV stack, root output blocks indexed by the counter);
-// the device kernels have their own copy (blake3_dev.hpp).  The SHA-256
+// BLAKE3 comes from blake3_host.hpp (the specification's compression, chunk
+// chaining and root output blocks); this file keeps the incremental CV stack.  The SHA-256
 // transcript of the reference is not provided: the prover's BLAKE3 profile
 // uses this one.
 #include <algorithm>
@@ -34,6 +33,7 @@
 #include <vector>
 
 #include "../../include/flock_gpu.h"
+#include "blake3_host.hpp"
 
 namespace flk {
 int set_error(int code, const char* fmt, ...);
@@ -42,87 +42,15 @@ int check_enabled();
 
 namespace {
 
-constexpr uint32_t IV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
-                            0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
-constexpr int PERM[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
-enum : uint32_t { CHUNK_START = 1, CHUNK_END = 2, PARENT = 4, ROOT = 8 };
-constexpr size_t CHUNK_LEN = 1024, BLOCK_LEN = 64;
-
-inline uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
-
-inline void g(uint32_t* s, int a, int b, int c, int d, uint32_t x, uint32_t y) {
-  s[a] = s[a] + s[b] + x;
-  s[d] = rotr(s[d] ^ s[a], 16);
-  s[c] = s[c] + s[d];
-  s[b] = rotr(s[b] ^ s[c], 12);
-  s[a] = s[a] + s[b] + y;
-  s[d] = rotr(s[d] ^ s[a], 8);
-  s[c] = s[c] + s[d];
-  s[b] = rotr(s[b] ^ s[c], 7);
-}
-
-// The full 16-word output of the compression function.
-void compress(const uint32_t cv[8], const uint32_t m_in[16], uint64_t counter, uint32_t block_len,
-              uint32_t flags, uint32_t out[16]) {
-  uint32_t s[16] = {cv[0], cv[1], cv[2], cv[3], cv[4], cv[5], cv[6], cv[7],
-                    IV[0], IV[1], IV[2], IV[3], (uint32_t)counter, (uint32_t)(counter >> 32), block_len, flags};
-  uint32_t m[16];
-  std::memcpy(m, m_in, sizeof m);
-  for (int r = 0; r < 7; ++r) {
-    g(s, 0, 4, 8, 12, m[0], m[1]);
-    g(s, 1, 5, 9, 13, m[2], m[3]);
-    g(s, 2, 6, 10, 14, m[4], m[5]);
-    g(s, 3, 7, 11, 15, m[6], m[7]);
-    g(s, 0, 5, 10, 15, m[8], m[9]);
-    g(s, 1, 6, 11, 12, m[10], m[11]);
-    g(s, 2, 7, 8, 13, m[12], m[13]);
-    g(s, 3, 4, 9, 14, m[14], m[15]);
-    uint32_t p[16];
-    for (int i = 0; i < 16; ++i) p[i] = m[PERM[i]];
-    std::memcpy(m, p, sizeof m);
-  }
-  for (int i = 0; i < 8; ++i) {
-    out[i] = s[i] ^ s[i + 8];
-    out[i + 8] = s[i + 8] ^ cv[i];
-  }
-}
-
-void words_le(const uint8_t* b, uint32_t w[16]) {
-  for (int i = 0; i < 16; ++i)
-    w[i] = (uint32_t)b[4 * i] | ((uint32_t)b[4 * i + 1] << 8) | ((uint32_t)b[4 * i + 2] << 16) |
-           ((uint32_t)b[4 * i + 3] << 24);
-}
-
-// The last compression of a node, kept unevaluated: its chaining value, or
-// (at the root) the output blocks read by counter.
-struct Output {
-  uint32_t cv[8];
-  uint32_t block[16];
-  uint64_t counter;
-  uint32_t block_len, flags;
-  void chaining_value(uint32_t out[8]) const {
-    uint32_t o[16];
-    compress(cv, block, counter, block_len, flags, o);
-    std::memcpy(out, o, 32);
-  }
-  void root_bytes(uint64_t offset, uint8_t* out, size_t len) const {
-    uint64_t blk = offset / 64;
-    size_t skip = (size_t)(offset % 64);
-    while (len) {
-      uint32_t o[16];
-      compress(cv, block, blk, block_len, flags | ROOT, o);
-      uint8_t bytes[64];
-      for (int i = 0; i < 16; ++i)
-        for (int j = 0; j < 4; ++j) bytes[4 * i + j] = (uint8_t)(o[i] >> (8 * j));
-      const size_t take = 64 - skip < len ? 64 - skip : len;
-      std::memcpy(out, bytes + skip, take);
-      out += take;
-      len -= take;
-      skip = 0;
-      ++blk;
-    }
-  }
-};
+using flk::b3::BLOCK_LEN;
+using flk::b3::CHUNK_END;
+using flk::b3::CHUNK_LEN;
+using flk::b3::CHUNK_START;
+using flk::b3::compress;
+using flk::b3::IV;
+using flk::b3::Output;
+using flk::b3::parent_output;
+using flk::b3::words_le;
 
 struct ChunkState {
   uint32_t cv[8];
@@ -166,17 +94,6 @@ struct ChunkState {
     return o;
   }
 };
-
-Output parent_output(const uint32_t l[8], const uint32_t r[8]) {
-  Output o;
-  std::memcpy(o.cv, IV, 32);
-  std::memcpy(o.block, l, 32);
-  std::memcpy(o.block + 8, r, 32);
-  o.counter = 0;
-  o.block_len = BLOCK_LEN;
-  o.flags = PARENT;
-  return o;
-}
 
 struct Hasher {
   ChunkState chunk;

@@ -109,16 +109,6 @@ static void g8_ifft(uint8_t* v, size_t n, const uint8_t* tw, size_t idx) {
   }
 }
 
-static flk_f128 phi8_host(uint8_t v) {
-  const flk_f128 g{0x6b8330483c2e9849ull, 0x0dcb364640a222feull};  // phi8(0x02), CF/field/phi8.rs:28-31
-  flk_f128 acc{0, 0}, p{1, 0};
-  for (int i = 0; i < 8; ++i) {
-    if ((v >> i) & 1) acc = flk_f128{acc.lo ^ p.lo, acc.hi ^ p.hi};
-    p = host_mul(p, g);
-  }
-  return acc;
-}
-
 static void build_round1_tables(Round1Tables& T) {
   uint8_t twS[64], twL[64];
   g8_twiddles(6, 0, twS);

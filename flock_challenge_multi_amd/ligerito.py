@@ -48,8 +48,9 @@ class ProverConfig:
                 raise ValueError(f"ProverConfig: {name} needs {r + 1} entries")
         if self.ood_samples[0]:
             raise ValueError("ProverConfig: L0 takes no OOD samples")
-        if log_n - self.initial_k - sum(self.recursive_ks) < 0:
-            raise ValueError("ProverConfig: more folds than variables")
+        if log_n - self.initial_k - sum(self.recursive_ks) < 1:
+            # every fold needs two elements per output, and yr keeps at least one variable
+            raise ValueError("ProverConfig: the folds leave no variable for yr")
 
 
 @dataclass

@@ -101,6 +101,8 @@ int flk_free_async(void *p, void *stream);
 int flk_copy_to_device(void *dst, const void *src, size_t bytes, void *stream);
 /* Enqueue a device -> host copy on `stream` (dst is HOST memory). */
 int flk_copy_to_host(void *dst, const void *src, size_t bytes, void *stream);
+/* Enqueue a device -> device copy on `stream`. */
+int flk_copy_device_to_device(void *dst, const void *src, size_t bytes, void *stream);
 /* Wait for all work enqueued on `stream`. */
 int flk_synchronize(void *stream);
 
@@ -433,6 +435,86 @@ int flk_gather_rows(const flk_f128 *codeword, size_t n_rows, size_t row_len, con
  * HOST array (any order, repeats allowed); synchronises `stream`. */
 int flk_merkle_multi_proof(const uint8_t *tree, size_t num_leaves, const uint64_t *positions, size_t n,
                            uint8_t *proof_out, size_t max_hashes, size_t *n_hashes_out, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* The BLAKE3 R1CS and its lincheck circuit (CP/r1cs_hashes/blake3.rs,       */
+/* CF/r1cs.rs, CF/lincheck.rs).  A handle holds the block-diagonal instance  */
+/* of 2^n_blocks_log compressions (m = 14 + n_blocks_log, k_log = 14,        */
+/* k_skip = 6, C_0 = I) and the column-major (A_0, B_0) in HBM (one ~42 MB    */
+/* copy per device, shared by every handle).                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct flk_r1cs flk_r1cs;
+
+/* build_block_r1cs(n_blocks_log) (blake3.rs:1531-1545), 3 <= n_blocks_log <= 26,
+ * with its CscCircuit (CF/lincheck.rs:1505-1560) uploaded to the current device. */
+int flk_blake3_r1cs_new(size_t n_blocks_log, flk_r1cs **out);
+void flk_r1cs_free(flk_r1cs *r);
+/* m, k_log, k_skip and the nonzero counts of A_0 and B_0 (any pointer may be NULL). */
+int flk_r1cs_shape(const flk_r1cs *r, size_t *m, size_t *k_log, size_t *k_skip, size_t *nnz_a, size_t *nnz_b);
+/* BlockR1cs::statement_digest (CF/r1cs.rs:334-385) into HOST out; cached per m. */
+int flk_r1cs_statement_digest(const flk_r1cs *r, uint8_t out[32]);
+/* The lincheck comb vector (CF/lincheck.rs:5590-5660 and the verifier's
+ * :5945-5965): comb_out (device, 2^14) = circuit.fold_alpha_batched(alpha,
+ * build_quirky_eq_table(z_skip, x_inner_rest, 6)) with beta added at the pinned
+ * constant column 512.  x_inner_rest is a HOST array of n_rest = 8 coordinates. */
+int flk_lincheck_comb(const flk_r1cs *r, flk_f128 z_skip, const flk_f128 *x_inner_rest, size_t n_rest,
+                      flk_f128 alpha, flk_f128 beta, flk_f128 *comb_out, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Protocol provers sequenced over the device entry points; the transcript   */
+/* stays on the host (every output below is HOST memory unless stated).     */
+/* ------------------------------------------------------------------------ */
+
+/* zerocheck::prove_packed_padded (CF/zerocheck.rs:384-535; transcript order of
+ * verify, :1642-1790) for a(y) b(y) + c(y) = 0 over 2^m points, a_bits, b_bits,
+ * c_bits device bit vectors (LSB-first, 8-byte aligned), 13 <= m <= 40.
+ * Outputs: round1_ab[64], round1_c[64] (naive convention), rounds[2 (m - 6)]
+ * = (G(1), G(inf)) per multilinear round, finals = {final_a, final_b,
+ * final_c}; the claim: *z_out, mlv_out[m - 6] (the rho challenges), r_rest_out[m - 6]. */
+int flk_zerocheck_prove(flk_fs *fs, const uint8_t *a_bits, const uint8_t *b_bits, const uint8_t *c_bits, size_t m,
+                        flk_f128 *round1_ab, flk_f128 *round1_c, flk_f128 *rounds, flk_f128 *finals,
+                        flk_f128 *z_out, flk_f128 *mlv_out, flk_f128 *r_rest_out, void *stream);
+
+/* lincheck::prove_padded (CF/lincheck.rs:5345-5878) with the handle's circuit and
+ * constant pin: z_stripe is the device lincheck stripe of the witness
+ * (flk_blake3_witness's z_lincheck); x_ab (HOST) = z_skip, x_inner_rest[8],
+ * x_outer[m - 14].  Outputs: rounds[16] = (q(1), q(inf)) per round, z_partial[64],
+ * the claim *r_inner_skip, r_inner_rest[8] (LSB-first), *w. */
+int flk_lincheck_prove(flk_fs *fs, const flk_r1cs *r, const uint8_t *z_stripe, const flk_f128 *x_ab,
+                       flk_f128 *rounds, flk_f128 *z_partial, flk_f128 *r_inner_skip, flk_f128 *r_inner_rest,
+                       flk_f128 *w, void *stream);
+
+/* Blake3Setup (CP/r1cs_hashes/blake3.rs:4706-4788) for n_blocks compressions:
+ * n_blocks_log = ceil(log2(max(n_blocks, 8))), m = 14 + n_blocks_log, PcsParams
+ * {m, log_inv_rate of the profile's L0, log_batch_size 6, profile, BLAKE3
+ * Merkle}, the Ligerito ladder of (m, profile) (profile 0 fast, 1 slim,
+ * 2 secure; ladders exist for m = 22..35, CF/pcs/ligerito.rs:256-345) and the
+ * statement digest.  Fast is what the reference's benchmark proves with. */
+typedef struct flk_blake3_setup flk_blake3_setup;
+int flk_blake3_setup_new(size_t n_blocks, int profile, flk_blake3_setup **out);
+void flk_blake3_setup_free(flk_blake3_setup *s);
+/* m, n_blocks_log, the device bytes one prove holds at its peak, and the
+ * 32-byte statement digest (HOST; any pointer may be NULL). */
+int flk_blake3_setup_info(const flk_blake3_setup *s, size_t *m, size_t *n_blocks_log, size_t *device_bytes,
+                          uint8_t *digest_out);
+
+/* One proof: generate_witness_with_ab_packed_and_lincheck of the device
+ * compressions (padding slots are the all-zero compression), pcs::commit,
+ * prover::prove_ligerito (CP/prover.rs:373-472: bind_statement, zerocheck,
+ * lincheck, ring switch + claim combination, recursive Ligerito open) on a
+ * transcript FsChallenger::with_hash(domain, Blake3), serialised as
+ * R1csProofBundleLigerito::to_bytes (CP/proof_io.rs:160-198: "FLOCK", version
+ * 4, flavor 2, bincode fixint payload) into proof_out (HOST, cap bytes).
+ * *proof_len receives the size (also when cap is too small: FLK_E_SHAPE).
+ * stage_seconds (HOST, 6 entries, may be NULL) receives the wall time of
+ * witness + commit, zerocheck, lincheck, ring switch + combination, Ligerito
+ * open and serialisation.  All device work runs on `stream`; concurrent calls
+ * on different streams share nothing but the setup (read-only).  Proof-of-work
+ * nonces are the smallest valid ones (grind_pow), so the bytes are a function
+ * of the inputs. */
+int flk_blake3_prove(const flk_blake3_setup *s, const flk_compression *blocks, size_t n_blocks, const uint8_t *domain,
+                     size_t domain_len, uint8_t *proof_out, size_t cap, size_t *proof_len, double *stage_seconds,
+                     void *stream);
 
 #ifdef __cplusplus
 }

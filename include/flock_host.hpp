@@ -33,32 +33,36 @@ inline F128 operator+(F128 a, F128 b) { return F128{a.lo ^ b.lo, a.hi ^ b.hi}; }
 inline F128 operator*(F128 a, F128 b) { return flk_f128_mul(a, b); }
 inline bool operator==(F128 a, F128 b) { return a.lo == b.lo && a.hi == b.hi; }
 
-// Device buffer of n elements of T.
+// Device buffer of n elements of T in stream-ordered pool memory of one
+// stream: the allocation, uploads, read-backs and the free are all ordered on
+// that stream, so work the caller enqueues on the same stream needs no further
+// synchronisation.  Work on another stream that touches the buffer must be
+// ordered against this one by the caller.  Uploads and read-backs go through
+// pageable host memory and synchronise the stream before returning, so the
+// host vector may be released as soon as they return.
 template <class T>
 class DeviceVec {
  public:
   DeviceVec() = default;
-  // Memory comes from the stream-ordered pool of the default stream (no
-  // device-wide synchronisation on free); work on other streams that uses it
-  // must be ordered against the default stream by the caller.
-  explicit DeviceVec(size_t n) : n_(n) {
+  explicit DeviceVec(size_t n, void* stream = nullptr) : n_(n), stream_(stream) {
     void* p = nullptr;
-    check(flk_alloc_async(n * sizeof(T), nullptr, &p), "flk_alloc_async");
+    check(flk_alloc_async(n * sizeof(T), stream_, &p), "flk_alloc_async");
     p_ = static_cast<T*>(p);
   }
-  explicit DeviceVec(const std::vector<T>& host, void* stream = nullptr) : DeviceVec(host.size()) {
-    upload(host, stream);
+  explicit DeviceVec(const std::vector<T>& host, void* stream = nullptr) : DeviceVec(host.size(), stream) {
+    upload(host);
   }
   ~DeviceVec() {
-    if (p_) flk_free_async(p_, nullptr);
+    if (p_) flk_free_async(p_, stream_);
   }
-  DeviceVec(DeviceVec&& o) noexcept : p_(o.p_), n_(o.n_) {
+  DeviceVec(DeviceVec&& o) noexcept : p_(o.p_), n_(o.n_), stream_(o.stream_) {
     o.p_ = nullptr;
     o.n_ = 0;
   }
   DeviceVec& operator=(DeviceVec&& o) noexcept {
     std::swap(p_, o.p_);
     std::swap(n_, o.n_);
+    std::swap(stream_, o.stream_);
     return *this;
   }
   DeviceVec(const DeviceVec&) = delete;
@@ -67,29 +71,33 @@ class DeviceVec {
   T* data() { return p_; }
   const T* data() const { return p_; }
   size_t len() const { return n_; }
+  void* stream() const { return stream_; }
 
-  void upload(const std::vector<T>& host, void* stream = nullptr) {
+  void upload(const std::vector<T>& host) {
     if (host.size() != n_) throw std::invalid_argument("DeviceVec::upload: length mismatch");
-    check(flk_copy_to_device(p_, host.data(), n_ * sizeof(T), stream), "flk_copy_to_device");
+    if (!n_) return;
+    check(flk_copy_to_device(p_, host.data(), n_ * sizeof(T), stream_), "flk_copy_to_device");
+    check(flk_synchronize(stream_), "flk_synchronize");
   }
   // The last element (e.g. a Merkle tree's root) without copying the rest.
-  T back(void* stream = nullptr) const {
+  T back() const {
     if (!n_) throw std::out_of_range("DeviceVec::back: empty");
     T h;
-    check(flk_copy_to_host(&h, p_ + (n_ - 1), sizeof(T), stream), "flk_copy_to_host");
-    check(flk_synchronize(stream), "flk_synchronize");
+    check(flk_copy_to_host(&h, p_ + (n_ - 1), sizeof(T), stream_), "flk_copy_to_host");
+    check(flk_synchronize(stream_), "flk_synchronize");
     return h;
   }
-  std::vector<T> to_host(void* stream = nullptr) const {
+  std::vector<T> to_host() const {
     std::vector<T> h(n_);
-    check(flk_copy_to_host(h.data(), p_, n_ * sizeof(T), stream), "flk_copy_to_host");
-    check(flk_synchronize(stream), "flk_synchronize");
+    if (n_) check(flk_copy_to_host(h.data(), p_, n_ * sizeof(T), stream_), "flk_copy_to_host");
+    check(flk_synchronize(stream_), "flk_synchronize");
     return h;
   }
 
  private:
   T* p_ = nullptr;
   size_t n_ = 0;
+  void* stream_ = nullptr;
 };
 
 inline bool is_pow2(size_t n) { return n && !(n & (n - 1)); }
@@ -144,7 +152,7 @@ enum class HashKind { Sha256 = FLK_HASH_SHA256, Blake3 = FLK_HASH_BLAKE3 };
 
 inline DeviceVec<Hash> merkle_tree(const uint8_t* data, size_t data_len, size_t num_leaves, HashKind kind,
                                    void* stream = nullptr) {
-  DeviceVec<Hash> tree(2 * num_leaves - 1);
+  DeviceVec<Hash> tree(2 * num_leaves - 1, stream);
   check(flk_merkle_tree(data, data_len, num_leaves, static_cast<int>(kind), tree.data()->data(), stream),
         "merkle_tree");
   return tree;
@@ -187,7 +195,7 @@ inline ProverData ligero_commit(const DeviceVec<F128>& poly, size_t log_msg_cols
   if (poly.len() != size_t(1) << (log_msg_cols + log_num_interleaved))
     throw std::invalid_argument("ligero_commit: poly length");
   const size_t n_pos = size_t(1) << (log_msg_cols + log_inv_rate);
-  ProverData pd{DeviceVec<F128>(n_pos << log_num_interleaved), DeviceVec<Hash>(2 * n_pos - 1)};
+  ProverData pd{DeviceVec<F128>(n_pos << log_num_interleaved, stream), DeviceVec<Hash>(2 * n_pos - 1, stream)};
   check(flk_commit(poly.data(), log_msg_cols, log_num_interleaved, log_inv_rate, static_cast<int>(kind),
                    pd.codeword.data(), pd.merkle_tree.data()->data(), root_out ? root_out->data() : nullptr, stream),
         "commit");
@@ -227,7 +235,7 @@ class SumcheckProver {
   SumcheckMessage fold(F128 r) {
     if (pending_) throw std::logic_error("fold before ordinary glue");
     const size_t n = f_.len();
-    DeviceVec<F128> nf(n / 2), nb(n / 2);
+    DeviceVec<F128> nf(n / 2, stream_), nb(n / 2, stream_);
     check(flk_sc_fold_and_msg(f_.data(), basis_.data(), n, r, nf.data(), nb.data(), msg_.data(), stream_),
           "fold_and_msg");
     f_ = std::move(nf);
@@ -275,13 +283,13 @@ class SumcheckProver {
 
  private:
   SumcheckProver(DeviceVec<F128> f, DeviceVec<F128> b, F128 h, void* stream)
-      : f_(std::move(f)), basis_(std::move(b)), msg_(2), t_r_(h), stream_(stream) {}
+      : f_(std::move(f)), basis_(std::move(b)), msg_(2, stream), t_r_(h), stream_(stream) {}
   SumcheckMessage message_of(const DeviceVec<F128>& f, const DeviceVec<F128>& b) {
     check(flk_sc_round_msg(f.data(), b.data(), f.len(), msg_.data(), stream_), "round_msg");
     return read_msg();
   }
   SumcheckMessage read_msg() {
-    std::vector<F128> h = msg_.to_host(stream_);
+    std::vector<F128> h = msg_.to_host();
     return SumcheckMessage{h[0], h[1]};
   }
   DeviceVec<F128> f_, basis_, msg_, b_new_;
@@ -298,7 +306,7 @@ class SumcheckProver {
 namespace lincheck {
 inline DeviceVec<F128> partial_fold_packed_z(const DeviceVec<uint8_t>& z_packed, size_t m, size_t k_log,
                                              const DeviceVec<F128>& eq_outer, void* stream = nullptr) {
-  DeviceVec<F128> out(size_t(1) << k_log);
+  DeviceVec<F128> out(size_t(1) << k_log, stream);
   check(flk_lc_partial_fold(z_packed.data(), m, k_log, eq_outer.data(), out.data(), stream), "partial_fold");
   return out;
 }
@@ -307,7 +315,7 @@ inline DeviceVec<F128> partial_fold_packed_z(const DeviceVec<uint8_t>& z_packed,
 namespace ring_switch {
 inline DeviceVec<F128> fold_1b_rows(const DeviceVec<F128>& packed_witness, const DeviceVec<F128>& suffix_tensor,
                                     void* stream = nullptr) {
-  DeviceVec<F128> out(128);
+  DeviceVec<F128> out(128, stream);
   check(flk_rs_fold_1b_rows(packed_witness.data(), suffix_tensor.data(), packed_witness.len(), out.data(), stream),
         "fold_1b_rows");
   return out;
@@ -315,7 +323,7 @@ inline DeviceVec<F128> fold_1b_rows(const DeviceVec<F128>& packed_witness, const
 inline DeviceVec<F128> fold_b128_elems(const DeviceVec<F128>& suffix_tensor, const DeviceVec<F128>& eq_r_dprime,
                                        void* stream = nullptr) {
   if (eq_r_dprime.len() != 128) throw std::invalid_argument("fold_b128_elems: eq_r_dprime must have 128 entries");
-  DeviceVec<F128> out(suffix_tensor.len());
+  DeviceVec<F128> out(suffix_tensor.len(), stream);
   check(flk_rs_fold_b128_elems(suffix_tensor.data(), eq_r_dprime.data(), suffix_tensor.len(), out.data(), stream),
         "fold_b128_elems");
   return out;
@@ -332,9 +340,9 @@ inline std::pair<std::vector<F128>, std::vector<F128>> round1(const DeviceVec<ui
                                                               const DeviceVec<uint8_t>& c, size_t m,
                                                               const std::vector<F128>& r_outer,
                                                               void* stream = nullptr) {
-  DeviceVec<F128> out(128);
+  DeviceVec<F128> out(128, stream);
   check(flk_zc_round1(a.data(), b.data(), c.data(), m, r_outer.data(), out.data(), out.data() + 64, stream), "round1");
-  std::vector<F128> h = out.to_host(stream);
+  std::vector<F128> h = out.to_host();
   return {std::vector<F128>(h.begin(), h.begin() + 64), std::vector<F128>(h.begin() + 64, h.end())};
 }
 }  // namespace zerocheck
@@ -362,7 +370,7 @@ inline std::vector<Hash> merkle_multi_proof(const DeviceVec<Hash>& tree, size_t 
   size_t depth = 0;
   while ((size_t(1) << depth) < num_leaves) ++depth;
   const size_t cap = positions.size() * (depth ? depth : 1);
-  DeviceVec<Hash> tmp(cap ? cap : 1);
+  DeviceVec<Hash> tmp(cap ? cap : 1, stream);
   size_t k = 0;
   check(flk_merkle_multi_proof(reinterpret_cast<const uint8_t*>(tree.data()), num_leaves, positions.data(),
                                positions.size(), reinterpret_cast<uint8_t*>(tmp.data()), cap, &k, stream),
@@ -381,7 +389,8 @@ namespace pcs {
 inline DeviceVec<F128> gather_rows(const DeviceVec<F128>& codeword, size_t row_len,
                                    const std::vector<uint64_t>& positions, void* stream = nullptr) {
   if (row_len == 0 || codeword.len() % row_len) throw std::invalid_argument("gather_rows: row_len");
-  DeviceVec<F128> out(positions.size() * row_len ? positions.size() * row_len : 1);
+  const size_t n_out = positions.size() * row_len;
+  DeviceVec<F128> out(n_out ? n_out : 1, stream);
   check(flk_gather_rows(codeword.data(), codeword.len() / row_len, row_len, positions.data(), positions.size(),
                         out.data(), stream),
         "gather_rows");
@@ -479,7 +488,7 @@ inline RingSwitchBatch ring_switch_call(const DeviceVec<F128>& z, const std::vec
   std::vector<F128> shv(128 * k);
   out.sumcheck_claims.resize(k);
   out.gammas.resize(k);
-  if (want_basis || combine) out.b_combined = DeviceVec<F128>(z.len());
+  if (want_basis || combine) out.b_combined = DeviceVec<F128>(z.len(), stream);
   if (combine)
     check(flk_pcs_combined_basis_and_target(ch.raw(), z.data(), log_len, xs.data(), k, shv.data(),
                                             out.sumcheck_claims.data(), out.gammas.data(), out.b_combined.data(),
@@ -530,8 +539,9 @@ inline Witness generate_witness_with_ab_packed_and_lincheck(const DeviceVec<Comp
     throw std::invalid_argument("generate_witness: n_blocks_log must be in 3..30");
   const size_t n_total = size_t(1) << n_blocks_log;
   if (blocks.len() > n_total) throw std::invalid_argument("generate_witness: more compressions than slots");
-  Witness w{DeviceVec<F128>(n_total * 128), DeviceVec<F128>(n_total * 128), DeviceVec<F128>(n_total * 128),
-            lincheck ? DeviceVec<uint8_t>(n_total * 2048) : DeviceVec<uint8_t>()};
+  Witness w{DeviceVec<F128>(n_total * 128, stream), DeviceVec<F128>(n_total * 128, stream),
+            DeviceVec<F128>(n_total * 128, stream),
+            lincheck ? DeviceVec<uint8_t>(n_total * 2048, stream) : DeviceVec<uint8_t>()};
   check(flk_blake3_witness(blocks.data(), blocks.len(), n_blocks_log, w.z.data(), w.a.data(), w.b.data(),
                            lincheck ? w.z_lincheck.data() : nullptr, stream),
         "generate_witness");
@@ -589,28 +599,32 @@ inline LigeritoProof recursive_prover_with_basis(const ProverConfig& cfg, Device
     throw std::invalid_argument("recursive_prover_with_basis: config");
   if (f.len() != b.len()) throw std::invalid_argument("recursive_prover_with_basis: f and b lengths");
   const size_t log_n = log2_exact(f.len());
+  size_t folds = k0;
+  for (size_t k : cfg.recursive_ks) folds += k;
+  // Every fold needs two elements per output and the final yr at least one variable.
+  if (folds + 1 > log_n) throw std::invalid_argument("recursive_prover_with_basis: the config folds all variables");
   size_t n_cur = log_n - k0;
   const size_t block0 = size_t(1) << (n_cur + cfg.log_inv_rates[0]);
   if (l0_codeword.len() != block0 << k0) throw std::invalid_argument("recursive_prover_with_basis: L0 codeword");
   LigeritoProof proof;
-  proof.initial_root = l0_tree.back(stream);
+  proof.initial_root = l0_tree.back();
   auto observe_msg = [&](const SumcheckMessage& m) {
     ch.observe_f128(m.u_0);
     ch.observe_f128(m.u_2);
   };
   auto open = [&](const DeviceVec<F128>& cw, const DeviceVec<Hash>& tree, size_t row_len,
                   const std::vector<uint64_t>& q) {
-    return OpenedLevel{pcs::gather_rows(cw, row_len, q, stream).to_host(stream),
+    return OpenedLevel{pcs::gather_rows(cw, row_len, q, stream).to_host(),
                        merkle::merkle_multi_proof(tree, cw.len() / row_len, q, stream)};
   };
   auto induce = [&](size_t log_cols, const OpenedLevel& op, const std::vector<F128>& v,
                     const std::vector<uint64_t>& q, const std::vector<F128>& alpha) {
-    DeviceVec<F128> rows(op.rows), basis(size_t(1) << log_cols), sum(1);
-    DeviceVec<uint64_t> qd(q);
+    DeviceVec<F128> rows(op.rows, stream), basis(size_t(1) << log_cols, stream), sum(1, stream);
+    DeviceVec<uint64_t> qd(q, stream);
     check(flk_induce_sumcheck_poly(log_cols, rows.data(), q.size(), v.data(), v.size(), qd.data(), alpha.data(),
                                    alpha.size(), basis.data(), sum.data(), stream),
           "induce_sumcheck_poly");
-    F128 h = sum.to_host(stream)[0];
+    F128 h = sum.to_host()[0];
     return std::make_pair(std::move(basis), h);
   };
 
@@ -645,7 +659,7 @@ inline LigeritoProof recursive_prover_with_basis(const ProverConfig& cfg, Device
   pcs::ProverData prev = pcs::ligero_commit(sp.f(), n_cur - k_next, k_next, cfg.log_inv_rates[1],
                                             merkle::HashKind::Blake3, nullptr, stream);
   size_t prev_k = k_next, prev_block = size_t(1) << (n_cur - k_next + cfg.log_inv_rates[1]);
-  proof.recursive_roots.push_back(prev.merkle_tree.back(stream));
+  proof.recursive_roots.push_back(prev.merkle_tree.back());
   ch.observe_bytes(proof.recursive_roots.back().data(), 32);
   ood(1);
   proof.grinding_nonces.push_back(ch.grind_pow(cfg.grinding_bits[0], stream));
@@ -660,7 +674,7 @@ inline LigeritoProof recursive_prover_with_basis(const ProverConfig& cfg, Device
     const std::vector<F128> level_rs = fold_rounds(cfg.recursive_ks[i], i + 1);
     n_cur -= cfg.recursive_ks[i];
     if (i == r - 1) {
-      proof.yr = sp.f().to_host(stream);
+      proof.yr = sp.f().to_host();
       for (const F128& v : proof.yr) ch.observe_f128(v);
       proof.grinding_nonces.push_back(ch.grind_pow(cfg.grinding_bits[i + 1], stream));
       auto ql = ch.sample_distinct_queries(prev_block, cfg.queries[i + 1]);
@@ -671,7 +685,7 @@ inline LigeritoProof recursive_prover_with_basis(const ProverConfig& cfg, Device
     k_next = cfg.recursive_ks[i + 1];
     pcs::ProverData nxt = pcs::ligero_commit(sp.f(), n_cur - k_next, k_next, cfg.log_inv_rates[i + 2],
                                              merkle::HashKind::Blake3, nullptr, stream);
-    proof.recursive_roots.push_back(nxt.merkle_tree.back(stream));
+    proof.recursive_roots.push_back(nxt.merkle_tree.back());
     ch.observe_bytes(proof.recursive_roots.back().data(), 32);
     ood(i + 2);
     proof.grinding_nonces.push_back(ch.grind_pow(cfg.grinding_bits[i + 1], stream));

@@ -73,8 +73,8 @@ static int bench_open_m32() {
   std::vector<double> ms;
   for (int it = 0; it < 6; ++it) {
     DeviceVec<F128> fc(n), bc(n);
-    flock::check(flk_copy_to_device(fc.data(), f.to_host().data(), n * 16, nullptr), "copy");
-    flock::check(flk_copy_to_device(bc.data(), b.to_host().data(), n * 16, nullptr), "copy");
+    flock::check(flk_copy_device_to_device(fc.data(), f.data(), n * 16, nullptr), "copy");
+    flock::check(flk_copy_device_to_device(bc.data(), b.data(), n * 16, nullptr), "copy");
     flock::check(flk_synchronize(nullptr), "sync");
     flock::challenger::FsChallenger t("flock-bench");
     const auto t0 = std::chrono::steady_clock::now();
