@@ -113,6 +113,9 @@ void orc_rs_fold_b128_elems(const orc_f128 *suffix, const orc_f128 eq_r[128], si
 int orc_zc_lagrange_weights(size_t k_skip, orc_f128 z, orc_f128 *out);
 int orc_zc_fold_at_z(const uint8_t *bits, size_t m, size_t k_skip, const orc_f128 *weights, orc_f128 *out);
 void orc_zc_round_pair(const orc_f128 *a, const orc_f128 *b, size_t n, const orc_f128 *eq_rest, orc_f128 out[2]);
+/* out[c] = XOR of w[r] over the nonzeros (r, c) of a CSR matrix (transpose times w). */
+void orc_sparse_fold(const uint32_t *ptr, const uint32_t *cols, size_t n_rows, const orc_f128 *w, orc_f128 *out,
+                     size_t n_cols);
 
 /* ---- GF(2^8) additive NTT and zerocheck round 1 (zerocheck_round1.c) ---- */
 int orc_gf8_twiddles(size_t k, uint8_t beta, uint8_t *tw);

@@ -133,3 +133,18 @@ void orc_zc_round_pair(const orc_f128 *a, const orc_f128 *b, size_t n, const orc
     out[0] = g1;
     out[1] = ginf;
 }
+
+/* out[c] ^= w[r] for every nonzero (r, c) of a CSR matrix with n_rows rows
+ * (row r's columns are cols[ptr[r] .. ptr[r+1])): the transpose-times-vector
+ * over GF(2^128) that lincheck's fold_alpha_batched applies to A_0 and B_0
+ * (CF/lincheck.rs:4683-4720).  out (n_cols entries) is zeroed first. */
+void orc_sparse_fold(const uint32_t *ptr, const uint32_t *cols, size_t n_rows, const orc_f128 *w, orc_f128 *out,
+                     size_t n_cols)
+{
+    memset(out, 0, n_cols * sizeof(orc_f128));
+    for (size_t r = 0; r < n_rows; ++r)
+        for (uint32_t j = ptr[r]; j < ptr[r + 1]; ++j) {
+            out[cols[j]].lo ^= w[r].lo;
+            out[cols[j]].hi ^= w[r].hi;
+        }
+}

@@ -114,6 +114,7 @@ class Oracle:
             "orc_fs_sample_vec": (ctypes.c_int, [P, SZ, P]),
             "orc_fs_digest": (None, [P, P]),
             "orc_rs_transpose": (None, [P, P]),
+            "orc_sparse_fold": (None, [P, P, SZ, P, P, SZ]),
             "orc_rs_prove_batched": (ctypes.c_int, [P, P, SZ, P, SZ, P, P, P, P]),
             "orc_pcs_combined_basis_and_target": (ctypes.c_int, [P, P, SZ, P, SZ, P, P, P, P, P]),
             "orc_rs_claim_check": (COrcF128, [COrcF128, COrcF128, P]),
@@ -422,6 +423,14 @@ class Oracle:
     # ---- ring switch proving and claim combination (ring_switch.c) ----
     def fs(self, domain):
         return OracleFs(self, domain)
+
+    def sparse_fold(self, ptr, cols, w, n_cols):
+        """out[c] = XOR of w[r] over the nonzeros (r, c) of the CSR matrix (ptr, cols)."""
+        p = np.ascontiguousarray(ptr, dtype=np.uint32)
+        c = np.ascontiguousarray(cols, dtype=np.uint32)
+        out = np.zeros(n_cols, dtype=F128_DT)
+        self.L.orc_sparse_fold(_p(p), _p(c), len(p) - 1, _p(w), _p(out), n_cols)
+        return out
 
     def rs_transpose(self, s_hat_v):
         a = ints_to_f128(s_hat_v) if not isinstance(s_hat_v, np.ndarray) else s_hat_v
