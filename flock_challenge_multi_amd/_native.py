@@ -117,6 +117,7 @@ SIGNATURES = {
     "flk_r1cs_shape": (_I, [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ), ctypes.POINTER(_SZ),
                             ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
     "flk_r1cs_statement_digest": (_I, [_P, _P]),
+    "flk_blake3_statement_digest": (_I, [_SZ, _P]),
     "flk_lincheck_comb": (_I, [_P, _F, _P, _SZ, _F, _F, _P, _P]),
     "flk_zerocheck_prove": (_I, [_P, _P, _P, _P, _SZ, _P, _P, _P, _P, _P, _P, _P, _P]),
     "flk_lincheck_prove": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),

@@ -533,6 +533,13 @@ int flk_lincheck_prove(flk_fs* fs, const flk_r1cs* r1cs, const uint8_t* z_stripe
   }, &ctx);
 }
 
+int flk_blake3_statement_digest(size_t m, uint8_t* out) {
+  if (!out) return flk::set_error(FLK_E_SHAPE, "blake3_statement_digest: null pointer");
+  if (m < K_LOG + 3 || m > K_LOG + 26) return flk::set_error(FLK_E_SHAPE, "blake3_statement_digest: m=%zu", m);
+  flk::r1cs::statement_digest(m, out);
+  return FLK_OK;
+}
+
 int flk_blake3_setup_new(size_t n_blocks, int profile, flk_blake3_setup** out) {
   if (int rc = flk::check_enabled()) return rc;
   if (!out) return flk::set_error(FLK_E_SHAPE, "blake3_setup_new: null pointer");

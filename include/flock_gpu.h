@@ -453,6 +453,9 @@ void flk_r1cs_free(flk_r1cs *r);
 int flk_r1cs_shape(const flk_r1cs *r, size_t *m, size_t *k_log, size_t *k_skip, size_t *nnz_a, size_t *nnz_b);
 /* BlockR1cs::statement_digest (CF/r1cs.rs:334-385) into HOST out; cached per m. */
 int flk_r1cs_statement_digest(const flk_r1cs *r, uint8_t out[32]);
+/* The same digest for m = 14 + n_blocks_log without a handle or a device
+ * (host only; 17 <= m <= 40). */
+int flk_blake3_statement_digest(size_t m, uint8_t out[32]);
 /* The lincheck comb vector (CF/lincheck.rs:5590-5660 and the verifier's
  * :5945-5965): comb_out (device, 2^14) = circuit.fold_alpha_batched(alpha,
  * build_quirky_eq_table(z_skip, x_inner_rest, 6)) with beta added at the pinned

@@ -143,6 +143,77 @@ def parse_bundle(b):
     return out
 
 
+def serialize_bundle(b):
+    """The inverse of parse_bundle: bincode fixint of R1csProofBundleLigerito with the
+    FLOCK header (written field by field from the struct order of CF/proof.rs:17-22,
+    CF/zerocheck.rs:311-324, CF/lincheck.rs:2002-2015, CF/pcs.rs:42-47,
+    CF/pcs/ligerito.rs:1021-1048)."""
+    out = bytearray(b"FLOCK\x04\x02")
+    u64 = lambda v: out.extend(int(v).to_bytes(8, "little"))  # noqa: E731
+    f = lambda v: out.extend(int(v).to_bytes(16, "little"))  # noqa: E731
+
+    def fvec(v):
+        u64(len(v))
+        for x in v:
+            f(x)
+
+    def pairs(v):
+        u64(len(v))
+        for x, y in v:
+            f(x)
+            f(y)
+
+    def hashes(v):
+        u64(len(v))
+        for h in v:
+            out.extend(h)
+
+    def rows(v):
+        u64(len(v))
+        for r in v:
+            u64(len(r))
+            out.extend(np.ascontiguousarray(r, dtype=F128_DT).tobytes())
+
+    out.extend(b["root"])
+    u64(b["m"])
+    u64(b["log_inv_rate"])
+    u64(b["log_batch_size"])
+    out.extend(int(b["profile"]).to_bytes(4, "little"))
+    out.extend(int(b["merkle_hash"]).to_bytes(4, "little"))
+    zc, lc, lig = b["zc"], b["lc"], b["lig"]
+    fvec(zc["round1_ab"])
+    fvec(zc["round1_c"])
+    pairs(zc["rounds"])
+    for k in ("final_a", "final_b", "final_c"):
+        f(zc[k])
+    pairs(lc["rounds"])
+    fvec(lc["z_partial"])
+    u64(len(b["s_hat_v"]))
+    for v in b["s_hat_v"]:
+        fvec(v)
+    out.extend(lig["initial_root"])
+    rows(lig["initial_proof"][0])
+    hashes(lig["initial_proof"][1])
+    hashes(lig["recursive_roots"])
+    u64(len(lig["recursive_proofs"]))
+    for rw, hs in lig["recursive_proofs"]:
+        rows(rw)
+        hashes(hs)
+    fvec(lig["yr"])
+    rows(lig["final_proof"][0])
+    hashes(lig["final_proof"][1])
+    pairs(lig["sumcheck_transcript"])
+    for k in ("grinding_nonces",):
+        u64(len(lig[k]))
+        for x in lig[k]:
+            u64(x)
+    fvec(lig["ood_values"])
+    u64(len(lig["fold_grinding_nonces"]))
+    for x in lig["fold_grinding_nonces"]:
+        u64(x)
+    return bytes(out)
+
+
 # ---------------------------------------------------------------------------
 # Field helpers
 # ---------------------------------------------------------------------------
