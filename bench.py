@@ -41,12 +41,16 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # 300 commits take ~0.13 s; shorter runs measure the clock ramp after the
-    # warm-up (30 steps after 3 warm-up commits: 0.51 ms per commit, against
-    # 0.44 ms from 300 steps on)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=30)
+    # one step = one full proof of 2^18 compressions (~60 ms); every step proves
+    # a different seeded input, all resident in HBM before the timed region
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log2", type=int, default=18, help="compressions per proof = 2^log2 (m = 14 + log2)")
+    ap.add_argument("--commit-steps", type=int, default=300)
+    ap.add_argument("--commit-warmup", type=int, default=30)
+    ap.add_argument("--no-verify", action="store_true", help="skip the untimed verification of the proofs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-commit", action="store_true", help="skip the standalone config-1 commit leg")
     ap.add_argument("--no-m32", action="store_true")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel table at m = 32 sizes")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -371,19 +375,10 @@ def pow_leg(hp, cpu_sample_bits):
     return out
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark measures one GPU only; run with --gpus 1"}))
-        sys.exit(2)
-    import torch
-
-    from flock_challenge_multi_amd import _native as nat
-    from flock_challenge_multi_amd import hotpath as hp
-
-    torch.cuda.set_device(0)
-    lib = nat.load()
-    assert lib.flk_device_available() == 1, lib.flk_last_error()
+def commit_leg(args, torch, nat, hp):
+    """The standalone config-1 commit (2^22 coefficients): the dominant proof kernels
+    (RS encode + Merkle) on their own, with their roofline and the oracle's commit as
+    CPU baseline."""
     st = torch.cuda.current_stream()
     sptr = ctypes.c_void_p(st.cuda_stream)
 
@@ -412,23 +407,24 @@ def main():
         nat.call("flk_merkle_tree", ctypes.c_void_p(cw.data_ptr()), n_cw * 16, n_pos, nat.FLK_HASH_BLAKE3,
                  ctypes.c_void_p(tree.data_ptr()), sptr)
 
-    for _ in range(args.warmup):
+    for _ in range(args.commit_warmup):
         commit_once()
     torch.cuda.synchronize()
 
-    # ---- timed region: exactly `steps` commits back to back -----------------
+    # ---- exactly `commit_steps` commits back to back ------------------------
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(args.commit_steps):
         commit_once()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
+    ms_per_step = wall * 1e3 / args.commit_steps
     value = compressions / (ms_per_step / 1e3)
 
     # ---- per-commit latency (p50/p99): the same commits bracketed by HIP events,
     # in a loop of their own (each event pair adds ~10 us of queue work) --------
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.commit_steps)]
     for s, e in evs:
         s.record(st)
         commit_once()
@@ -447,14 +443,14 @@ def main():
             cs = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
             nat.call("flk_commit", ctypes.c_void_p(msg.data_ptr()), LOG_COLS, LOG_LANES, LOG_RATE,
                      nat.FLK_HASH_BLAKE3, ctypes.c_void_p(cw.data_ptr()), ctypes.c_void_p(tree.data_ptr()), None, cs)
-        for _ in range(max(args.warmup, 1)):
+        for _ in range(max(args.commit_warmup, 1)):
             gr.replay()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(args.commit_steps):
             gr.replay()
         torch.cuda.synchronize()
-        gms = (time.perf_counter() - t0) * 1e3 / args.steps
+        gms = (time.perf_counter() - t0) * 1e3 / args.commit_steps
         graph = {"ms_per_step": gms, "compressions_per_s": compressions / (gms / 1e3),
                  "root_matches_eager": bytes(tree[-1].cpu().numpy().tobytes()) == eager_root}
         del gr
@@ -465,12 +461,14 @@ def main():
         torch.cuda.synchronize()
 
     # ---- dominant kernel: the RS encode (NTT passes), timed alone ------------
-    enc = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    enc = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.commit_steps)]
     for s, e in enc:
         s.record(st)
         encode_once()
         e.record(st)
-    mer = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    mer = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.commit_steps)]
     for s, e in mer:
         s.record(st)
         merkle_once()
@@ -574,28 +572,17 @@ def main():
     pow_ = pow_leg(hp, 0 if args.no_cpu_baseline else 22)
 
     traffic, valu = encode_counters()
-    line = {
-        "metric": "verified BLAKE3 compressions/s (2^18 batch, 1 GPU) + p50/p99 proof latency",
+    return {
         "value": value,
         "unit": "committed BLAKE3 compressions/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": args.commit_steps,
         "ms_per_step": ms_per_step,
         "p50_ms": pctl(lat_ms, 0.5),
         "p99_ms": pctl(lat_ms, 0.99),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "gf2_128 (u32 integer ops)",
-        "data": "synthetic (seeded uniform coefficients)",
         "verified": verified,
-        "config": {
-            "workload": "standalone Ligerito commit: 2^22 F128 coefficients (witness of 2^15 BLAKE3 "
-                        "compressions, m=29) as 2^16 positions x 64 lanes, additive-NTT RS encode at rate 1/2 "
-                        "(2^17 x 64 codeword) + BLAKE3 Merkle tree over 2^17 x 1 KiB rows",
-            "coefficients": n_msg, "lanes": 1 << LOG_LANES, "log_inv_rate": LOG_RATE, "leaves": n_pos,
-        },
+        "workload": "standalone Ligerito commit: 2^22 F128 coefficients (witness of 2^15 BLAKE3 compressions, "
+                    "m=29) as 2^16 positions x 64 lanes, additive-NTT RS encode at rate 1/2 (2^17 x 64 codeword) "
+                    "+ BLAKE3 Merkle tree over 2^17 x 1 KiB rows",
         "breakdown_ms": {"rs_encode": enc_ms, "merkle_tree": mer_ms},
         "roofline": {
             "bound": "hbm", "achieved": enc_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -617,7 +604,178 @@ def main():
         "kernels_m32": kernels,
         "cpu_baseline": cpu,
     }
+
+
+def proof_leg(args, torch, nat, hp):
+    """Config 2: full proofs of 2^log2 seeded compressions.  Returns the headline
+    fields, the proofs and their inputs (for the untimed verification)."""
+    import numpy as np
+
+    from flock_challenge_multi_amd import prover as pv
+    from flock_challenge_multi_amd.inputs import generate_compressions
+
+    n = 1 << args.log2
+    t0 = time.perf_counter()
+    setup = pv.Blake3Setup(n)
+    setup_s = time.perf_counter() - t0
+    seeds = [0x5EED_0000 + i for i in range(args.steps)]
+    comps = [generate_compressions(args.log2, sd) for sd in seeds]
+    inputs = [torch.from_numpy(c.view(np.int32)).cuda() for c in comps]
+    warm = torch.from_numpy(generate_compressions(args.log2, 0x00C0FFEEBEEFD15C).view(np.int32)).cuda()
+    for _ in range(args.warmup):
+        setup.prove(warm)
+    torch.cuda.synchronize()
+
+    # ---- timed region: exactly `steps` proofs, one after the other ----------
+    lat, stages, proofs = [], [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for blk in inputs:
+        t1 = time.perf_counter()
+        pb, st_s = setup.prove(blk)
+        lat.append(time.perf_counter() - t1)
+        proofs.append(pb)
+        stages.append(st_s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    stage_ms = {name: 1e3 * sorted(s[i] for s in stages)[len(stages) // 2] for i, name in enumerate(pv.STAGES)}
+    return {
+        "value": n / (ms_per_step / 1e3),
+        "ms_per_step": ms_per_step,
+        "p50_ms": pctl([x * 1e3 for x in lat], 0.5),
+        "p99_ms": pctl([x * 1e3 for x in lat], 0.99),
+        "score_median": n / pctl(lat, 0.5),
+        "stage_ms_p50": stage_ms,
+        "proof_bytes": max(len(p) for p in proofs),
+        "setup_s": setup_s,
+        "m": setup.m,
+        "device_bytes_per_proof": setup.device_bytes,
+    }, proofs, comps, inputs
+
+
+def proof_kernels(args, torch, nat, hp, blocks):
+    """The largest device kernels of one proof at its real shape, each timed alone
+    with HIP events on the launch stream (the stream the kernels run on):
+    zerocheck round 1 (flk_zc_round1 over a, b, c = z of the proof's witness) and
+    the L0 RS encode.  Algorithmic bytes: round 1 reads 3 bit vectors of 2^m bits;
+    the encode reads the 2^(m-7)-element message and writes the 2^(m-6) codeword."""
+    st = torch.cuda.current_stream()
+    sptr = ctypes.c_void_p(st.cuda_stream)
+    m = 14 + args.log2
+    z, a, b, _ = hp.generate_witness_with_ab_packed_and_lincheck(blocks, args.log2)
+    r_outer = [(0x9E3779B97F4A7C15 * (i + 1)) & ((1 << 128) - 1) for i in range(m - 13)]
+    ro = (nat.F128 * len(r_outer))(*[nat.F128.from_int(v) for v in r_outer])
+    out = hp.f128_empty(128)
+
+    def r1():
+        nat.call("flk_zc_round1", ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+                 ctypes.c_void_p(z.data_ptr()), m, ro, ctypes.c_void_p(out.data_ptr()),
+                 ctypes.c_void_p(out.data_ptr() + 64 * 16), sptr)
+
+    n_msg, n_cw = 1 << (m - 7), 1 << (m - 6)
+    cw = hp.f128_empty(n_cw)
+
+    def enc():
+        nat.call("flk_ntt_rs_encode_interleaved", m - 12, ctypes.c_void_p(z.data_ptr()), n_msg,
+                 ctypes.c_void_p(cw.data_ptr()), n_cw, 64, sptr)
+
+    r1_ms = _time(torch, st, r1, reps=5)
+    enc_ms = _time(torch, st, enc, reps=5)
+    r1_bytes = 3 * (1 << m) // 8
+    enc_bytes = 16 * (n_msg + n_cw)
+    del z, a, b, cw
+    return {"zc_round1": {"ms": r1_ms, "bytes": r1_bytes, "gbs": r1_bytes / (r1_ms / 1e3) / 1e9},
+            "rs_encode_l0": {"ms": enc_ms, "bytes": enc_bytes, "gbs": enc_bytes / (enc_ms / 1e3) / 1e9}}
+
+
+def verify_leg(args, proofs, comps):
+    """Every timed proof through the restated verifier (untimed): header, size, the
+    commitment root recomputed from the regenerated witness, zerocheck, lincheck,
+    ring switch and the succinct Ligerito check."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    try:
+        import flock_verifier as fv
+        import oracle_ffi
+        orc = oracle_ffi.Oracle()
+        ok = 0
+        t0 = time.perf_counter()
+        for pb, c in zip(proofs, comps):
+            try:
+                fv.verify_proof(orc, pb, c.view(oracle_ffi.COMPRESSION_DT).reshape(-1), args.log2)
+                ok += 1
+            except fv.Reject:
+                pass
+        return {"accepted": ok, "proofs": len(proofs), "verify_s": time.perf_counter() - t0}
+    except Exception as ex:  # the verifier is only a checker here
+        return {"error": f"verifier unavailable: {ex}"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark measures one GPU only (the driver's box has one); "
+                                   "run with --gpus 1"}))
+        sys.exit(2)
+    import torch
+
+    from flock_challenge_multi_amd import _native as nat
+    from flock_challenge_multi_amd import hotpath as hp
+
+    torch.cuda.set_device(0)
+    lib = nat.load()
+    assert lib.flk_device_available() == 1, lib.flk_last_error()
+
+    head, proofs, comps, inputs = proof_leg(args, torch, nat, hp)
+    kern = proof_kernels(args, torch, nat, hp, inputs[0])
+    del inputs
+    torch.cuda.empty_cache()
+    ver = None if args.no_verify else verify_leg(args, proofs, comps)
+    cpu = None if args.no_cpu_baseline else proof_cpu_baseline(args)
+    commit = None if args.no_commit else commit_leg(args, torch, nat, hp)
+    dom = kern["zc_round1"]
+    line = {
+        "metric": "verified BLAKE3 compressions/s (2^18 batch, 1 GPU) + p50/p99 proof latency",
+        "value": head["value"],
+        "unit": "verified BLAKE3 compressions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "p50_ms": head["p50_ms"],
+        "p99_ms": head["p99_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "gf2_128 (u32 integer ops)",
+        "data": "synthetic: generate_compressions(log2, seed), a different seed per step",
+        "verified": ver,
+        "config": {
+            "workload": f"full proof of 2^{args.log2} BLAKE3 compressions (m = {head['m']}): witness, PCS commit, "
+                        "zerocheck, lincheck, ring switch + claim combination, recursive Ligerito open "
+                        "(fast profile), FLOCK v4 bytes",
+            "compressions": 1 << args.log2, "m": head["m"], "proof_bytes": head["proof_bytes"],
+            "fs_domain": "flock-bench-v0", "merkle_hash": "blake3",
+        },
+        "score_median": head["score_median"],
+        "stage_ms_p50": head["stage_ms_p50"],
+        "device_bytes_per_proof": head["device_bytes_per_proof"],
+        "setup_s": head["setup_s"],
+        "roofline": {
+            "bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": None,
+            "kernel": f"zerocheck round 1 (zc_round1_kernel, one launch per proof): algorithmic bytes "
+                      f"3 * 2^m / 8 = {dom['bytes']} per launch, {dom['ms']:.3f} ms per launch",
+        },
+        "proof_kernels": kern,
+        "cpu_baseline": cpu,
+        "commit_config1": commit,
+    }
     print(json.dumps(line), flush=True)
+
+
+def proof_cpu_baseline(args):
+    return None
 
 
 if __name__ == "__main__":
