@@ -77,6 +77,7 @@ SIGNATURES = {
     "flk_build_eq": (_I, [_P, _SZ, _P, _P]),
     "flk_sc_round_msg": (_I, [_P, _P, _SZ, _P, _P]),
     "flk_sc_round_msg_and_eval": (_I, [_P, _P, _SZ, _P, _P]),
+    "flk_sc_round_msg_noalloc": (_I, [_P, _P, _SZ, _P, _P]),
     "flk_sc_fold_and_msg": (_I, [_P, _P, _SZ, _F, _P, _P, _P, _P]),
     "flk_sc_glue": (_I, [_P, _P, _SZ, _F, _P]),
     "flk_partial_eval_lsb": (_I, [_P, _SZ, _P, _SZ, _P, _P]),

@@ -46,6 +46,9 @@ inline int log2_exact(size_t n) {
 // With grid == 1 the partials are out itself (no scratch, no second launch).
 int partials_alloc(uint4** parts, unsigned grid, int n_out, uint4* out, hipStream_t st);
 int partials_sum_free(uint4* parts, unsigned grid, int n_out, uint4* out, hipStream_t st);
+// out[k] = XOR of parts[k * grid .. k * grid + grid) for k < n_out, without freeing
+// parts (caller-owned scratch).
+int partials_sum(const uint4* parts, unsigned grid, int n_out, uint4* out, hipStream_t st);
 
 // fold_1b_rows against the split eq tensor eq_lo (2^lo_log) x eq_hi (2^hi_log),
 // lo_log >= 8: out128 (device) = sum_i bit_r(z_i) eq_lo[i mod 2^lo_log] eq_hi[i >> lo_log].

@@ -307,6 +307,13 @@ int partials_alloc(uint4** parts, unsigned grid, int n_out, uint4* out, hipStrea
   return FLK_OK;
 }
 
+int partials_sum(const uint4* parts, unsigned grid, int n_out, uint4* out, hipStream_t st) {
+  hipLaunchKernelGGL(partials_sum_kernel, dim3((unsigned)n_out), dim3(PSUM_WG), 0, st, parts, (uint32_t)grid, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return device_failure(e, "partials_sum_kernel");
+  return FLK_OK;
+}
+
 int partials_sum_free(uint4* parts, unsigned grid, int n_out, uint4* out, hipStream_t st) {
   if (parts == out) return FLK_OK;  // one work-group wrote out directly
   hipLaunchKernelGGL(partials_sum_kernel, dim3((unsigned)n_out), dim3(PSUM_WG), 0, st, parts, (uint32_t)grid, out);

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <mutex>
 
 #include "gf128.hpp"
 #include "internal.hpp"
@@ -216,6 +217,41 @@ __global__ void __launch_bounds__(256) fold_pass_kernel(const uint4* __restrict_
   out[i] = x[0];
 }
 
+// Scratch of the allocation-free round message (flk_sc_round_msg_noalloc): the
+// work-group partials of at most NOALLOC_GRID work-groups for 2 outputs
+// (128 KiB per device, held until the process ends).  Host code never uses the
+// symbol's name as an address: noalloc_scratch() resolves each device's copy
+// once with hipGetSymbolAddress and every launch receives that pointer.
+constexpr unsigned NOALLOC_GRID = 4096;
+static __device__ __align__(256) uint4 scratch[2 * NOALLOC_GRID];
+
+static uint4* noalloc_scratch(int* status) {
+  static std::mutex mu;
+  static uint4* per_dev[64] = {};
+  *status = FLK_OK;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    *status = device_failure(e, "hipGetDevice");
+    return nullptr;
+  }
+  if (dev < 0 || dev >= 64) {
+    *status = set_error(FLK_E_SHAPE, "round_msg_noalloc: device index %d out of range", dev);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  if (!per_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+      *status = device_failure(e, "hipGetSymbolAddress(scratch)");
+      return nullptr;
+    }
+    per_dev[dev] = static_cast<uint4*>(p);
+  }
+  return per_dev[dev];
+}
+
 static inline uint4 to_u4(flk_f128 a) {
   return make_uint4((uint32_t)a.lo, (uint32_t)(a.lo >> 32), (uint32_t)a.hi, (uint32_t)(a.hi >> 32));
 }
@@ -241,6 +277,36 @@ int flk_sc_round_msg(const flk_f128* f, const flk_f128* b, size_t n, flk_f128* o
                      reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, parts);
   FLK_LAUNCH_CHECK("round_msg_kernel");
   return flk::partials_sum_free(parts, grid, 2, o, st);
+}
+
+int flk_sc_round_msg_noalloc(const flk_f128* f, const flk_f128* b, size_t n, flk_f128* out2, void* stream) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!f || !b || !out2) return flk::set_error(FLK_E_SHAPE, "round_msg_noalloc: null pointer");
+  if (!flk::is_pow2(n) || n < 2 || n > (size_t(1) << 40))
+    return flk::set_error(FLK_E_SHAPE, "round_msg_noalloc: n=%zu must be a power of two in [2, 2^40]", n);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  uint4* o = reinterpret_cast<uint4*>(out2);
+  const uint64_t pairs = n / 2;
+  uint64_t g = (pairs + SC_WG - 1) / SC_WG;
+  const unsigned grid = (unsigned)(g > flk::NOALLOC_GRID ? flk::NOALLOC_GRID : g);
+  uint4* parts = o;  // one work-group writes out directly
+  if (grid > 1) {
+    if (grid > flk::NOALLOC_GRID)
+      return flk::set_error(FLK_E_SHAPE, "round_msg_noalloc: %u partials exceed the %u-entry scratch", grid,
+                            flk::NOALLOC_GRID);
+    parts = flk::noalloc_scratch(&rc);
+    if (!parts) return rc;
+  }
+  hipLaunchKernelGGL(flk::round_msg_kernel<false>, dim3(grid), dim3(SC_WG), 0, st,
+                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, parts);
+  FLK_LAUNCH_CHECK("round_msg_kernel(noalloc)");
+  if (grid > 1) {
+    // every partial [k * grid, k * grid + grid) was just stored by its work-group,
+    // so nothing in the scratch has to start at zero
+    if ((rc = flk::partials_sum(parts, grid, 2, o, st))) return rc;
+  }
+  return FLK_OK;
 }
 
 int flk_sc_round_msg_and_eval(const flk_f128* f, const flk_f128* b, size_t n, flk_f128* out3, void* stream) {

@@ -204,6 +204,17 @@ int flk_build_eq(const flk_f128 *point, size_t d, flk_f128 *out, void *stream);
 /* round_msg_lsb (:4958-5019): u_0 = sum f[2j] b[2j], u_2 = sum (f[2j]+f[2j+1])(b[2j]+b[2j+1]). */
 int flk_sc_round_msg(const flk_f128 *f, const flk_f128 *b, size_t n, flk_f128 *out2, void *stream);
 
+/* flk_sc_round_msg without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The work-group partials go to a
+ * file-scope device array of 2 x 4096 entries (128 KiB per device, held until
+ * the process ends) instead of stream-ordered pool memory; the grid is capped
+ * at 4096 work-groups, so every n in [2, 2^40] fits, and larger n is rejected
+ * with FLK_E_SHAPE.  All calls on one GPU share that array: calls that overlap
+ * on one GPU (other streams, other threads) race and must be serialised by the
+ * caller.  Nothing in the array needs to start at zero. */
+int flk_sc_round_msg_noalloc(const flk_f128 *f, const flk_f128 *b, size_t n, flk_f128 *out2, void *stream);
+
 /* round_msg_and_eval_lsb (:5020-5075): out3 = {u_0, u_2, sum f.b}. */
 int flk_sc_round_msg_and_eval(const flk_f128 *f, const flk_f128 *b, size_t n, flk_f128 *out3,
                               void *stream);
