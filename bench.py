@@ -1,29 +1,25 @@
-"""Benchmark of the Flock hot path on one MI355X.
+"""Benchmark of the Flock BLAKE3 R1CS prover on one MI355X.
 
-Workload (one "step"): the standalone Ligerito L0 commit of 2^22 uniform
-GF(2^128) coefficients — the packed witness of 2^15 BLAKE3 compressions
-(m = 29) laid out as 2^16 positions x 64 lanes, RS-encoded at rate 1/2 with the
-additive NTT (2^17 x 64 codeword, 128 MiB) and Merkle-hashed with BLAKE3 over
-the 2^17 codeword rows (1 KiB leaves).  Inputs are resident in HBM before the
-timed region; the codeword and tree stay on the device.
+Workload (one "step"): one full proof of 2^18 seeded BLAKE3 compressions
+(m = 32) through the C ABI entry point flk_blake3_prove: the witness (z, a, b
+and the lincheck stripe), the PCS commit (RS rate 1/2, 64 lanes, BLAKE3 Merkle),
+bind_statement, zerocheck, lincheck, the ring switch and claim combination, the
+recursive Ligerito opening (fast profile) and the FLOCK v4 proof bytes.  Every
+step proves a different seeded input; all inputs are resident in HBM before the
+timed region.
 
-`value` = committed compressions per second = 2^15 / seconds per commit
-(128 packed coefficients carry one compression's 2^14 witness bits).
-The line also carries p50/p99 per-commit latency, the roofline of the
-dominant kernel (NTT passes, timed with HIP events on the launch stream), the
-m = 32 L0 commit (the 2^18-compression batch's commit, 1 GiB codeword) and a
-CPU baseline: the oracle's commit (C with PCLMULQDQ, threaded over up to 16 host
-cores, with its 1-thread figure beside it) on the same workload.  `witness`
-times the device BLAKE3 R1CS witness generation (z, a, b and the lincheck
-stripe) of 2^15 synthetic compressions, with its HBM roofline and the oracle's
-threaded witness on the same batch as its CPU baseline; `pow` the device
-BLAKE3 proof-of-work hash rate and grind latency at 19 bits.
+`value` = 2^18 / (wall seconds of `steps` proofs / steps): compressions proved per
+second, whole job.  p50/p99 are the per-proof latencies of the same loop, and
+`stage_ms_p50` the prover's own stage split.  `verified`: every timed proof is then
+checked (untimed) by the restated verifier in tests/, with the commitment root
+recomputed on the CPU from the regenerated witness.
 
-`ms_per_step` is the wall time of `steps` commits queued back to back between
-two device synchronisations; p50/p99 come from a second loop of the same
-commits, each bracketed by HIP events.  `graph_replay` times the same commit
-replayed from a captured HIP graph (same kernels, root checked against the
-eager one); it is reported beside `value`, not in it.
+`roofline` is that of zerocheck round 1 (flk_zc_round1, the proof's longest
+kernel), timed alone with HIP events on its launch stream; `proof_kernels` adds
+the L0 RS encode.  `cpu_baseline` is the oracle's CPU prover (tests/oracle_prover.py)
+on one proof of 2^15 compressions, whose bytes must equal the device proof of the
+same input.  `commit_config1` is the standalone L0 commit of 2^15 compressions
+(m = 29) with its NTT roofline, witness and proof-of-work figures.
 """
 import argparse
 import ctypes
@@ -36,6 +32,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+# HBM bytes per zerocheck round-1 launch at m = 32 from two rocprofv3 --pmc passes
+# (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB device copy: read = 2.00 x FETCH_SIZE,
+# write = 1.00 x WRITE_SIZE (profiles/proof_m32.md)
+ZC_ROUND1_TRAFFIC_M32 = 1_643_786_880
 
 
 def parse():
@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--commit-warmup", type=int, default=30)
     ap.add_argument("--no-verify", action="store_true", help="skip the untimed verification of the proofs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-log2", type=int, default=15, help="compressions of the CPU baseline's proof = 2^cpu_log2")
     ap.add_argument("--no-commit", action="store_true", help="skip the standalone config-1 commit leg")
     ap.add_argument("--no-m32", action="store_true")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel table at m = 32 sizes")
@@ -498,17 +499,6 @@ def commit_leg(args, torch, nat, hp):
                 "note": "host call to synchronised result, median of 20"}
     del rows, proof
 
-    # ---- optional calibration copy: a plain kernel of known traffic ------------
-    if args.calib_copy_mib:
-        nb = args.calib_copy_mib << 20
-        src = torch.empty(nb, dtype=torch.uint8, device="cuda")
-        dst = torch.empty(nb, dtype=torch.uint8, device="cuda")
-        src.fill_(1)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-
     # ---- verification of the timed output against the oracle (untimed) ------
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     verified = None
@@ -689,6 +679,19 @@ def proof_kernels(args, torch, nat, hp, blocks):
             "rs_encode_l0": {"ms": enc_ms, "bytes": enc_bytes, "gbs": enc_bytes / (enc_ms / 1e3) / 1e9}}
 
 
+def calib_copy(torch, mib):
+    """One device-to-device copy of `mib` MiB: a plain kernel of known traffic whose
+    FETCH_SIZE / WRITE_SIZE readings scale the counters of the proof's kernels."""
+    nb = mib << 20
+    src = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    dst = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    src.fill_(1)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def verify_leg(args, proofs, comps):
     """Every timed proof through the restated verifier (untimed): header, size, the
     commitment root recomputed from the regenerated witness, zerocheck, lincheck,
@@ -727,13 +730,31 @@ def main():
     assert lib.flk_device_available() == 1, lib.flk_last_error()
 
     head, proofs, comps, inputs = proof_leg(args, torch, nat, hp)
-    kern = proof_kernels(args, torch, nat, hp, inputs[0])
+    kern = None if args.no_kernels else proof_kernels(args, torch, nat, hp, inputs[0])
     del inputs
+    if args.calib_copy_mib:
+        calib_copy(torch, args.calib_copy_mib)
     torch.cuda.empty_cache()
     ver = None if args.no_verify else verify_leg(args, proofs, comps)
-    cpu = None if args.no_cpu_baseline else proof_cpu_baseline(args)
+    cpu = None
+    if not args.no_cpu_baseline:
+        import numpy as np
+
+        from flock_challenge_multi_amd import prover as pv
+
+        def device_prove(c):
+            s = pv.Blake3Setup(len(c))
+            return s.prove(torch.from_numpy(np.ascontiguousarray(c).view(np.int32)).cuda())[0]
+
+        cpu = proof_cpu_baseline(args, device_prove)
     commit = None if args.no_commit else commit_leg(args, torch, nat, hp)
-    dom = kern["zc_round1"]
+    dom = kern["zc_round1"] if kern else None
+    roofline = None if dom is None else {
+        "bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": ZC_ROUND1_TRAFFIC_M32 if head["m"] == 32 else None,
+        "kernel": f"zerocheck round 1 (zc_round1_kernel, one launch per proof): algorithmic bytes "
+                  f"3 * 2^m / 8 = {dom['bytes']} per launch, {dom['ms']:.3f} ms per launch",
+    }
     line = {
         "metric": "verified BLAKE3 compressions/s (2^18 batch, 1 GPU) + p50/p99 proof latency",
         "value": head["value"],
@@ -761,12 +782,7 @@ def main():
         "stage_ms_p50": head["stage_ms_p50"],
         "device_bytes_per_proof": head["device_bytes_per_proof"],
         "setup_s": head["setup_s"],
-        "roofline": {
-            "bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": None,
-            "kernel": f"zerocheck round 1 (zc_round1_kernel, one launch per proof): algorithmic bytes "
-                      f"3 * 2^m / 8 = {dom['bytes']} per launch, {dom['ms']:.3f} ms per launch",
-        },
+        "roofline": roofline,
         "proof_kernels": kern,
         "cpu_baseline": cpu,
         "commit_config1": commit,
@@ -774,8 +790,34 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def proof_cpu_baseline(args):
-    return None
+def proof_cpu_baseline(args, device_prove):
+    """CPU baseline: the oracle's CPU prover (tests/oracle_prover.py over the C
+    restatements in oracle/; witness, commit and zerocheck round 1 threaded over up
+    to 16 host cores, the other stages on one) on a bounded sample of the workload,
+    one full proof of 2^cpu_log2 seeded compressions (m = 14 + cpu_log2; the
+    statement digest is per-m set-up on both sides and stays outside the timing).
+    value = 2^cpu_log2 / proof seconds.  The CPU proof's bytes are compared with the
+    device proof of the same input (`device_proof_identical`)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import flock_verifier as fv
+    import oracle_ffi
+    import oracle_prover as op
+    from flock_challenge_multi_amd.inputs import generate_compressions
+
+    orc = oracle_ffi.Oracle()
+    log2 = args.cpu_log2
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    comps = generate_compressions(log2, 0xC0DE_0000 + log2)
+    digest = fv.statement_digest(orc, 14 + log2)
+    t0 = time.perf_counter()
+    proof, stages = op.prove(orc, comps.view(oracle_ffi.COMPRESSION_DT).reshape(-1), log2, threads=threads,
+                             digest=digest)
+    secs = time.perf_counter() - t0
+    return {"value": (1 << log2) / secs, "unit": "verified BLAKE3 compressions/s", "cores": threads, "kind": "port",
+            "sample": f"one full proof of 2^{log2} compressions (m = {14 + log2}, generate_compressions({log2}, "
+                      f"{0xC0DE_0000 + log2:#x})) by the oracle's CPU prover: {secs:.2f} s; witness, commit and "
+                      f"zerocheck round 1 on {threads} threads, the other stages on 1",
+            "stage_s": stages, "device_proof_identical": device_prove(comps) == proof}
 
 
 if __name__ == "__main__":

@@ -125,6 +125,7 @@ SIGNATURES = {
     "flk_blake3_setup_new": (_I, [_SZ, _I, ctypes.POINTER(ctypes.c_void_p)]),
     "flk_blake3_setup_free": (None, [_P]),
     "flk_blake3_setup_info": (_I, [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ), ctypes.POINTER(_SZ), _P]),
+    "flk_blake3_proof_bound": (_I, [_P, ctypes.POINTER(_SZ)]),
     "flk_blake3_prove": (_I, [_P, _P, _SZ, _P, _SZ, _P, _SZ, ctypes.POINTER(_SZ), _P, _P]),
 }
 

@@ -298,12 +298,12 @@ struct flk_blake3_setup {
 namespace {
 
 // Bytes of device memory one prove holds at its peak (the zerocheck rounds):
-// z, a, b and the stripe (2^(m-3) bytes each), the codeword (2^(m-2)), the
-// tree (64 * 2^(m-12)), the folded a, b tables and their halves (3 * 2^(m-2)),
-// plus 1/8 headroom for the smaller tables.
-size_t prove_footprint(size_t m) {
+// z, a, b and the stripe (2^(m-3) bytes each), the L0 codeword (2^(m-3+rate))
+// and tree (64 * 2^(m-13+rate)), the folded a, b tables and their halves
+// (3 * 2^(m-2)), plus 1/8 headroom for the smaller tables.
+size_t prove_footprint(size_t m, size_t rate) {
   const size_t z = size_t(1) << (m - 3);
-  const size_t base = 4 * z + 2 * z + 64 * (size_t(1) << (m - 12)) + 6 * z;
+  const size_t base = 4 * z + (z << rate) + 64 * (size_t(1) << (m - 13 + rate)) + 6 * z;
   return base + base / 8;
 }
 
@@ -354,11 +354,14 @@ int prove_impl(void* vp) {
   DeviceVec<F128> a(n_f128, st), b(n_f128, st);
   DeviceVec<uint8_t> stripe(size_t(2048) << nbl, st);
   chk(flk_blake3_witness(c.blocks, c.n_blocks, nbl, z.data(), a.data(), b.data(), stripe.data(), st), "witness");
-  const size_t log_cols = m - 7 - 6, n_pos = size_t(1) << (log_cols + 1);
+  // PcsParams { log_inv_rate: the profile's, log_batch_size: 6 } (blake3.rs:4721-4775)
+  const size_t rate = s.ladder->levels[0].log_inv_rate;
+  const size_t log_cols = m - 7 - 6, n_pos = size_t(1) << (log_cols + rate);
   DeviceVec<F128> codeword(n_pos << 6, st);
   DeviceVec<Hash> tree(2 * n_pos - 1, st);
   Hash root;
-  chk(flk_commit(z.data(), log_cols, 6, 1, FLK_HASH_BLAKE3, codeword.data(), tree.data()->data(), root.data(), st),
+  chk(flk_commit(z.data(), log_cols, 6, rate, FLK_HASH_BLAKE3, codeword.data(), tree.data()->data(), root.data(),
+                 st),
       "commit");
   lap(0);
 
@@ -574,8 +577,30 @@ int flk_blake3_setup_info(const flk_blake3_setup* s, size_t* m, size_t* n_blocks
   if (!s) return flk::set_error(FLK_E_SHAPE, "blake3_setup_info: null pointer");
   if (m) *m = s->m;
   if (n_blocks_log) *n_blocks_log = s->n_blocks_log;
-  if (device_bytes) *device_bytes = prove_footprint(s->m);
+  if (device_bytes) *device_bytes = prove_footprint(s->m, s->ladder->levels[0].log_inv_rate);
   if (digest_out) std::memcpy(digest_out, s->digest, 32);
+  return FLK_OK;
+}
+
+int flk_blake3_proof_bound(const flk_blake3_setup* s, size_t* bytes) {
+  if (!s || !bytes) return flk::set_error(FLK_E_SHAPE, "blake3_proof_bound: null pointer");
+  const Ladder& l = *s->ladder;
+  // Fixed part: header, commitment, zerocheck (2 x 64 + 2 (m - 6) + 3 values),
+  // lincheck (16 + 64 values), two ring switches (2 x 128 values), length words.
+  size_t b = 4096 + 16 * (128 + 2 * (s->m - 6) + 3 + 16 + 64 + 256);
+  size_t msgs = 1 + l.initial_k, nonces = 0, queries = 0, row = 1, depth = 1;
+  for (uint32_t i = 0; i < l.n_levels; ++i) {
+    const Level& v = l.levels[i];
+    row = std::max(row, size_t(1) << v.log_num_interleaved);
+    depth = std::max(depth, size_t(v.log_msg_cols + v.log_inv_rate + 1));
+    queries += v.queries;
+    msgs += v.k_recursive + v.ood_samples + 1;
+    nonces += 1 + v.fold_grinding_bits;
+  }
+  // opened rows (length word + row per query) and multi-proof hashes at every level
+  b += 16 * l.n_levels + queries * (8 + 16 * row) + queries * depth * 32;
+  b += 16 * (size_t(1) << l.yr_log_n) + 32 * msgs + 8 * nonces + 16 * 64 + 32 * (l.n_levels + 1);
+  *bytes = b;
   return FLK_OK;
 }
 

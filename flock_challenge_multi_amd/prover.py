@@ -18,6 +18,7 @@ errors raise ValueError, device failures FlockGpuError.
 """
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _native as nat
@@ -133,6 +134,9 @@ class Blake3Setup:
         self.m, self.n_blocks_log, self.device_bytes = m.value, nbl.value, dev.value
         self.statement_digest = bytes(dig)
         self.profile = profile
+        bound = ctypes.c_size_t()
+        nat.call("flk_blake3_proof_bound", h, ctypes.byref(bound))
+        self._cap = bound.value
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -149,13 +153,13 @@ class Blake3Setup:
             raise ValueError("blocks must be a contiguous CUDA 32-bit tensor of shape (n, 28)")
         if blocks.shape[0] > (1 << self.n_blocks_log):
             raise ValueError("more compressions than the setup's slots")
-        cap = cap or (1 << 20) + (64 << (self.m - 12))
-        buf = (ctypes.c_uint8 * cap)()
+        cap = cap or self._cap
         plen = ctypes.c_size_t()
         stages = (ctypes.c_double * 6)()
         d = bytes(domain)
         dbuf = (ctypes.c_uint8 * max(len(d), 1)).from_buffer_copy(d or b"\0")
         st = ctypes.c_void_p(stream.cuda_stream) if stream is not None else hp._stream()
-        nat.call("flk_blake3_prove", self._h, hp._ptr(blocks), blocks.shape[0], dbuf, len(d), buf, cap,
-                 ctypes.byref(plen), stages, st)
-        return bytes(buf[:plen.value]), list(stages)
+        buf = np.empty(cap, np.uint8)  # not zero-filled: the library writes plen bytes
+        nat.call("flk_blake3_prove", self._h, hp._ptr(blocks), blocks.shape[0], dbuf, len(d),
+                 buf.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(plen), stages, st)
+        return buf[:plen.value].tobytes(), list(stages)

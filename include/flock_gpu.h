@@ -512,6 +512,13 @@ void flk_blake3_setup_free(flk_blake3_setup *s);
 int flk_blake3_setup_info(const flk_blake3_setup *s, size_t *m, size_t *n_blocks_log, size_t *device_bytes,
                           uint8_t *digest_out);
 
+/* An upper bound on the bytes flk_blake3_prove writes for setup s (its ladder's
+ * opened rows, Merkle multi-proofs, yr, messages and nonces plus the fixed
+ * fields): a proof_out buffer of this size never fails for room.  Host-only.
+ * No counterpart in the reference, whose to_bytes returns a Vec<u8>
+ * (CP/proof_io.rs:160-198). */
+int flk_blake3_proof_bound(const flk_blake3_setup *s, size_t *bytes);
+
 /* One proof: generate_witness_with_ab_packed_and_lincheck of the device
  * compressions (padding slots are the all-zero compression), pcs::commit,
  * prover::prove_ligerito (CP/prover.rs:373-472: bind_statement, zerocheck,

@@ -34,7 +34,7 @@ static void *range_main(void *p) {
 }
 
 /* Run fn over [0, n) in n_threads contiguous ranges; the calling thread takes the first. */
-static int parallel_for(size_t n, size_t n_threads, range_fn fn, void *ctx) {
+int orc_parallel_for(size_t n, size_t n_threads, range_fn fn, void *ctx) {
     if (n_threads < 1) n_threads = 1;
     if (n_threads > n) n_threads = n ? n : 1;
     if (n_threads > 256) n_threads = 256;
@@ -117,15 +117,15 @@ int orc_commit_mt(const orc_f128 *msg, size_t n_msg, size_t num_ntts, size_t log
     memset(codeword_out + n_msg, 0, (n_cw - n_msg) * sizeof(orc_f128));
     for (size_t layer = 0; layer < log_d; layer++) {
         struct layer_ctx c = {codeword_out, tw, (size_t)1 << layer, (size_t)1 << (log_d - layer - 1), num_ntts};
-        parallel_for(n_pos / 2, n_threads, layer_range, &c);
+        orc_parallel_for(n_pos / 2, n_threads, layer_range, &c);
     }
     free(tw);
     struct leaf_ctx lc = {(const uint8_t *)codeword_out, num_ntts * 16, tree_out};
-    parallel_for(n_pos, n_threads, leaf_range, &lc);
+    orc_parallel_for(n_pos, n_threads, leaf_range, &lc);
     size_t read_start = 0, read_len = n_pos;
     while (read_len > 1) {
         struct level_ctx vc = {tree_out, read_start, read_len};
-        parallel_for(read_len / 2, n_threads, level_range, &vc);
+        orc_parallel_for(read_len / 2, n_threads, level_range, &vc);
         read_start += read_len;
         read_len >>= 1;
     }

@@ -82,6 +82,8 @@ int orc_merkle_verify(const uint8_t root[32], const uint8_t leaf_hash[32], size_
 /* orc_commit on n_threads POSIX threads (same results; commit_mt.c). */
 int orc_commit_mt(const orc_f128 *msg, size_t n_msg, size_t num_ntts, size_t log_inv_rate,
                   orc_f128 *codeword_out, uint8_t *tree_out, uint8_t root_out[32], size_t n_threads);
+/* Runs fn(ctx, lo, hi) over [0, n) in n_threads contiguous ranges (commit_mt.c). */
+int orc_parallel_for(size_t n, size_t n_threads, void (*fn)(void *ctx, size_t lo, size_t hi), void *ctx);
 int orc_commit(const orc_f128 *msg, size_t n_msg, size_t num_ntts, size_t log_inv_rate,
                orc_f128 *codeword_out, uint8_t *tree_out, uint8_t root_out[32]);
 
@@ -123,6 +125,9 @@ int orc_gf8_forward(size_t k, uint8_t beta, uint8_t *v);
 int orc_gf8_inverse(size_t k, uint8_t beta, uint8_t *v);
 int orc_zc_round1(const uint8_t *a, const uint8_t *b, const uint8_t *c, size_t m, size_t k_skip,
                   const orc_f128 *r, orc_f128 *p_ab, orc_f128 *p_c);
+/* orc_zc_round1 for k_skip = 6 on n_threads threads, regrouped sums (zerocheck_round1_mt.c). */
+int orc_zc_round1_mt(const uint8_t *a, const uint8_t *b, const uint8_t *c, size_t m, size_t k_skip,
+                     const orc_f128 *r, orc_f128 *p_ab, orc_f128 *p_c, size_t n_threads);
 
 
 /* ---- BLAKE3 R1CS witness generation (witness.c; CP/r1cs_hashes/blake3.rs) */

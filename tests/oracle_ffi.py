@@ -93,6 +93,7 @@ class Oracle:
             "orc_gf8_forward": (ctypes.c_int, [SZ, ctypes.c_uint8, P]),
             "orc_gf8_inverse": (ctypes.c_int, [SZ, ctypes.c_uint8, P]),
             "orc_zc_round1": (ctypes.c_int, [P, P, P, SZ, SZ, P, P, P]),
+            "orc_zc_round1_mt": (ctypes.c_int, [P, P, P, SZ, SZ, P, P, P, SZ]),
             "orc_witness_layout": (None, [P]),
             "orc_blake3_block_witness": (None, [P, P, P, P]),
             "orc_blake3_witness": (ctypes.c_int, [P, SZ, SZ, P, P, P, P]),
@@ -514,6 +515,16 @@ class Oracle:
         pc = np.zeros(1 << k_skip, dtype=F128_DT)
         args = [np.ascontiguousarray(x, dtype=np.uint8) for x in (a, b, c)]
         assert self.L.orc_zc_round1(_p(args[0]), _p(args[1]), _p(args[2]), m, k_skip, _p(rr), _p(pab), _p(pc)) == 0
+        return pab, pc
+
+    def zc_round1_mt(self, a, b, c, m, k_skip, r, threads):
+        """orc_zc_round1 on `threads` threads (k_skip = 6; same outputs)."""
+        rr = ints_to_f128(r)
+        pab = np.zeros(1 << k_skip, dtype=F128_DT)
+        pc = np.zeros(1 << k_skip, dtype=F128_DT)
+        args = [np.ascontiguousarray(x, dtype=np.uint8) for x in (a, b, c)]
+        assert self.L.orc_zc_round1_mt(_p(args[0]), _p(args[1]), _p(args[2]), m, k_skip, _p(rr), _p(pab), _p(pc),
+                                       threads) == 0
         return pab, pc
 
 

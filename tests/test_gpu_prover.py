@@ -5,6 +5,8 @@ the restated verifier (tests/flock_verifier.py) and the oracle.
   configs 0 and 2   generate_compressions(12, seed) (m = 26) and (18, seed) (m = 32):
                     the proof is accepted, including the commitment root recomputed
                     on the CPU from the regenerated witness, and stays under 500,000 B
+  oracle prover     the proof equals the oracle CPU prover's bytes (config 0, a padded
+                    m = 22 batch)
   determinism       the same inputs give the same bytes (smallest-nonce grinds), on
                     another stream too; the golden fixture is reproduced exactly
   stages            round 1's messages equal the oracle's round1_naive at the
@@ -52,7 +54,21 @@ def test_proof_verifies(orc, log2, seed):
     proof, stages = s.prove(blocks)
     assert s.m == 14 + log2
     assert len(proof) < fv.MAX_PROOF_BYTES
+    assert len(proof) <= s._cap  # flk_blake3_proof_bound
     fv.verify_proof(orc, proof, comps.view(COMPRESSION_DT).reshape(-1), log2)
+
+
+@pytest.mark.parametrize("log2,seed,n", [(8, 0xA11CE, 200), (12, 0x00C0FFEEBEEFD15C, None)],
+                         ids=["m22_padded200", "config0_m26"])
+def test_proof_equals_oracle_prover(orc, log2, seed, n):
+    """The device proof equals, byte for byte, the proof of the oracle's CPU prover
+    (tests/oracle_prover.py) on the same compressions."""
+    import oracle_prover as op
+    print(f"case oracle-prover log2={log2} seed={seed:#x} n={n}", flush=True)
+    comps, blocks = _blocks(log2, seed, n)
+    proof, _ = setup_for(len(comps)).prove(blocks)
+    want, _ = op.prove(orc, comps.view(COMPRESSION_DT).reshape(-1), log2, threads=8)
+    assert proof == want
 
 
 def test_padded_batch_verifies(orc):
@@ -160,6 +176,28 @@ def test_lincheck_prove_matches_oracle(orc):
     zbits = np.frombuffer(z.cpu().numpy().tobytes(), np.uint8)
     mlv = orc.zc_fold_at_z(zbits, m, 6, orc.zc_lagrange_weights(6, lc["r_inner_skip"]))
     assert f128_to_ints(orc.partial_eval_lsb(mlv, lc["r_inner_rest"] + xab["x_outer"]))[0] == lc["w"]
+
+
+def test_proof_bound_and_small_buffer(orc):
+    """flk_blake3_proof_bound bounds the proof of each profile at m = 22 (the slim
+    profile commits at rate 1/4); the slim and secure proofs are accepted by the
+    restated verifier and equal the oracle prover's; a buffer one byte short of the
+    proof is refused."""
+    import oracle_prover as op
+    comps, blocks = _blocks(8, 3)
+    s = setup_for(256)
+    proof, _ = s.prove(blocks)
+    with pytest.raises(pv.nat.FlockGpuError):
+        s.prove(blocks, cap=len(proof) - 1)
+    for prof in ("slim", "secure"):
+        print(f"case profile={prof}", flush=True)
+        t = pv.Blake3Setup(256, profile=prof)
+        p2, _ = t.prove(blocks)
+        assert len(p2) <= t._cap
+        fv.verify_r1cs(orc, fv.parse_bundle(p2))
+        want, _ = op.prove(orc, comps.view(COMPRESSION_DT).reshape(-1), 8, threads=8,
+                           profile=pv.PROFILES[prof])
+        assert p2 == want
 
 
 def test_setup_rejects_unsupported_sizes():

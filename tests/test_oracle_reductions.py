@@ -222,3 +222,18 @@ def test_zc_round1_closed_form(orc):
             want_c[i] ^= pf.mul(e, phi[cl])
     assert f128_to_ints(pab) == want_ab
     assert f128_to_ints(pc) == want_c
+
+
+@pytest.mark.parametrize("m,threads", [(6, 1), (9, 3), (16, 8)])
+def test_zc_round1_mt_equals_naive(orc, m, threads):
+    """The CPU baseline's threaded, bucketed round 1 equals round1_naive bit for bit,
+    on random bits and on all-ones / all-zeros columns."""
+    rng = random.Random(m * 7 + threads)
+    nb = (1 << m) // 8
+    a, b, c = (np.frombuffer(rng.randbytes(nb), np.uint8).copy() for _ in range(3))
+    a[: nb // 4] = 0xFF
+    b[nb // 8: nb // 2] = 0
+    r = [rng.getrandbits(128) for _ in range(m)]
+    want = orc.zc_round1(a, b, c, m, 6, r)
+    got = orc.zc_round1_mt(a, b, c, m, 6, r, threads)
+    assert all(f128_to_ints(g) == f128_to_ints(w) for g, w in zip(got, want))

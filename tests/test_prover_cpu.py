@@ -2,7 +2,7 @@
 product library against the restated one, the Ligerito ladder data against the
 reference's TOML files, and the restated verifier on a device proof fixture
 (tests/golden/proof_m22_seed123c.bin, written by tests/golden/make_proof_fixture.py):
-accepted as is, rejected after any single-field mutation (the reference's
+reproduced byte for byte by the oracle's CPU prover, accepted as is, rejected after any single-field mutation (the reference's
 tamper tests, CF/zerocheck.rs:2096-2170, CF/lincheck.rs:7195-7336,
 CF/pcs/ring_switch.rs:5199-5220)."""
 import copy
@@ -68,6 +68,15 @@ def test_ladders_match_reference_toml():
             assert cfg["ood_samples"] == [l["ood_samples"] for l in levels]
             assert cfg["log_inv_rates"] == [l["log_inv_rate"] for l in levels]
             assert cfg["grinding_bits"] == [l["grinding_bits"] for l in levels]
+
+
+def test_oracle_prover_reproduces_fixture(orc, fixture_bytes, comps):
+    """The oracle's CPU prover (tests/oracle_prover.py), which shares no code with the
+    device library, writes the device fixture byte for byte; threaded round 1 too."""
+    import oracle_prover as op
+    for threads in (1, 4):
+        proof, _ = op.prove(orc, comps, LOG2, threads=threads)
+        assert proof == fixture_bytes
 
 
 def test_fixture_round_trips(fixture_bytes):
