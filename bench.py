@@ -640,7 +640,7 @@ def proof_leg(args, torch, nat, hp):
         "proof_bytes": max(len(p) for p in proofs),
         "setup_s": setup_s,
         "m": setup.m,
-        "device_bytes_per_proof": setup.device_bytes,
+        "device_bytes_per_proof": setup.arena_stats()["peak"],
     }, proofs, comps, inputs
 
 

@@ -126,6 +126,7 @@ SIGNATURES = {
     "flk_blake3_setup_free": (None, [_P]),
     "flk_blake3_setup_info": (_I, [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ), ctypes.POINTER(_SZ), _P]),
     "flk_blake3_proof_bound": (_I, [_P, ctypes.POINTER(_SZ)]),
+    "flk_blake3_setup_arena_stats": (_I, [_P] + [ctypes.POINTER(_SZ)] * 4),
     "flk_blake3_prove": (_I, [_P, _P, _SZ, _P, _SZ, _P, _SZ, ctypes.POINTER(_SZ), _P, _P]),
 }
 

@@ -30,4 +30,27 @@ void zc_fixed_challenges(flk_f128 out[7]);
 
 inline flk_f128 fadd(flk_f128 a, flk_f128 b) { return flk_f128{a.lo ^ b.lo, a.hi ^ b.hi}; }
 
+// Device arenas (runtime.hip): one device block carved by a best-fit free list.
+// Inside an ArenaScope the calling thread's stream-ordered scratch (the library's
+// temporaries and flk_alloc_async) comes from the arena, with no HIP call per
+// allocation; a block freed there is handed out again at once, which is safe
+// because everything in the scope is ordered on one stream.  Outside a scope,
+// or when the arena is full, the stream-ordered pool serves the allocation.
+class Arena;
+class ArenaScope {
+ public:
+  explicit ArenaScope(Arena* a);
+  ~ArenaScope();
+  ArenaScope(const ArenaScope&) = delete;
+  ArenaScope& operator=(const ArenaScope&) = delete;
+
+ private:
+  Arena* prev_;
+};
+// A new arena of `bytes` (nullptr and the error recorded when the device
+// allocation fails), its release (no block may be in use), its statistics.
+Arena* arena_new(size_t bytes, int* status);
+void arena_delete(Arena* a);
+void arena_stats(const Arena* a, size_t* capacity, size_t* in_use, size_t* peak, size_t* fallbacks);
+
 }  // namespace flk

@@ -228,7 +228,7 @@ extern "C" int flk_induce_sumcheck_poly(size_t log_msg_cols, const flk_f128* row
   const size_t n_S = (size_t)nq * (L ? L : 1), n_lo = (size_t)nq << s_lo, n_hi = (size_t)nq << s_hi;
   const size_t total = n_consts + n_aq + n_coef + n_S + n_lo + n_hi;
   uint4* scratch = nullptr;
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&scratch), total * sizeof(uint4), st);
+  hipError_t e = flk::stream_alloc(reinterpret_cast<void**>(&scratch), total * sizeof(uint4), st);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return set_error(FLK_E_NOMEM, "induce: scratch allocation of %zu bytes failed", total * sizeof(uint4));
@@ -269,7 +269,7 @@ extern "C" int flk_induce_sumcheck_poly(size_t log_msg_cols, const flk_f128* row
   hipLaunchKernelGGL(dot_kernel, dim3((unsigned)g), dim3(IND_WG), 0, st, reinterpret_cast<const uint4*>(rows), d_coef,
                      (uint64_t)n_coef, sum);
   FLK_LAUNCH_CHECK("dot_kernel");
-  e = hipFreeAsync(scratch, st);
+  e = flk::stream_free(scratch, st);
   if (e != hipSuccess) return device_failure(e, "induce: free scratch");
   return FLK_OK;
 }

@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "../../include/flock_gpu.h"
+#include "host_api.hpp"
 
 namespace flk {
 
@@ -35,6 +36,16 @@ inline int log2_exact(size_t n) {
     hipError_t e_ = hipGetLastError();                       \
     if (e_ != hipSuccess) return ::flk::device_failure(e_, where); \
   } while (0)
+
+// Stream-ordered device scratch.  Inside an ArenaScope the calling thread's
+// allocations come from that arena (one device block carved by a best-fit free
+// list, set up once and reused: no HIP call per allocation); elsewhere they come
+// from the stream-ordered pool (hipMallocAsync / hipFreeAsync).  Arena blocks are
+// handed out again as soon as they are freed, which is safe because everything
+// in an arena scope is ordered on one stream.  stream_free recognises arena
+// blocks of any live arena, in or out of the scope.
+hipError_t stream_alloc(void** p, size_t bytes, hipStream_t st);
+hipError_t stream_free(void* p, hipStream_t st);
 
 // Work-group partial sums of the reduction kernels.  A kernel launched on a grid
 // of G work-groups stores the XOR over its work-group of output k at

@@ -29,7 +29,7 @@ int flk_ligerito_ood(flk_fs* fs, const flk_f128* f, flk_f128* b, size_t log_n, s
   hipStream_t st = static_cast<hipStream_t>(stream);
   const size_t n = size_t(1) << log_n;
   uint4* ws = nullptr;  // eq(z) then the three reduction outputs
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&ws), (n + 4) * sizeof(uint4), st);
+  hipError_t e = flk::stream_alloc(reinterpret_cast<void**>(&ws), (n + 4) * sizeof(uint4), st);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return set_error(FLK_E_NOMEM, "ligerito_ood: workspace allocation failed");
@@ -60,7 +60,7 @@ int flk_ligerito_ood(flk_fs* fs, const flk_f128* f, flk_f128* b, size_t log_n, s
     t_r->lo ^= p.lo;
     t_r->hi ^= p.hi;
   }
-  e = hipFreeAsync(ws, st);
+  e = flk::stream_free(ws, st);
   if (e == hipSuccess && rc == FLK_OK) e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
     (void)hipGetLastError();

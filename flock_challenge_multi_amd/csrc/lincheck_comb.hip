@@ -196,7 +196,7 @@ int flk_lincheck_comb(const flk_r1cs* r, flk_f128 z_skip, const flk_f128* x_inne
   }
   for (unsigned j = 0; j < N_REST; ++j) args.xr[j] = u4(x_inner_rest[j]);
   uint4* tabs = nullptr;
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&tabs), 2 * K * sizeof(uint4), st);
+  hipError_t e = flk::stream_alloc(reinterpret_cast<void**>(&tabs), 2 * K * sizeof(uint4), st);
   if (e != hipSuccess) return flk::set_error(FLK_E_NOMEM, "lincheck_comb: scratch allocation failed");
   hipLaunchKernelGGL(lc_quirky_eq_kernel, dim3(K / 256), dim3(256), 0, st, args, tabs, tabs + K);
   FLK_LAUNCH_CHECK("lc_quirky_eq_kernel");
@@ -204,7 +204,7 @@ int flk_lincheck_comb(const flk_r1cs* r, flk_f128 z_skip, const flk_f128* x_inne
                      r->csc.a_rows, r->csc.b_ptr, r->csc.b_rows, tabs, tabs + K, (unsigned)flk::r1cs::CONST_POS,
                      u4(beta), reinterpret_cast<uint4*>(comb_out));
   FLK_LAUNCH_CHECK("lc_comb_gather_kernel");
-  if ((e = hipFreeAsync(tabs, st)) != hipSuccess) return flk::device_failure(e, "lincheck_comb: hipFreeAsync");
+  if ((e = flk::stream_free(tabs, st)) != hipSuccess) return flk::device_failure(e, "lincheck_comb: hipFreeAsync");
   return FLK_OK;
 }
 

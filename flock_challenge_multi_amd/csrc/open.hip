@@ -40,7 +40,7 @@ int gather(const uint4* src, const std::vector<uint64_t>& idx, uint32_t per, uin
            const char* what) {
   if (idx.empty()) return FLK_OK;
   uint64_t* d_idx = nullptr;
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&d_idx), idx.size() * sizeof(uint64_t), st);
+  hipError_t e = flk::stream_alloc(reinterpret_cast<void**>(&d_idx), idx.size() * sizeof(uint64_t), st);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return set_error(FLK_E_NOMEM, "%s: index scratch allocation failed", what);
@@ -52,7 +52,7 @@ int gather(const uint4* src, const std::vector<uint64_t>& idx, uint32_t per, uin
   hipLaunchKernelGGL(gather_kernel, dim3((unsigned)std::min<uint64_t>(wgs, 4096)), dim3(GATHER_WG), 0, st, src, d_idx,
                      (uint64_t)idx.size(), per, dst);
   FLK_LAUNCH_CHECK("gather_kernel");
-  e = hipFreeAsync(d_idx, st);
+  e = flk::stream_free(d_idx, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return device_failure(e, what);
   return FLK_OK;

@@ -20,6 +20,7 @@
 #include <array>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -293,6 +294,11 @@ struct flk_blake3_setup {
   const Ladder* ladder;
   flk_r1cs* r1cs;
   uint8_t digest[32];
+  // Device arenas for the proves of this setup, one per concurrent prove: created
+  // on demand, reused by later proves (no device allocation per prove).
+  size_t arena_bytes = 0;
+  std::mutex arena_mu;
+  std::vector<flk::Arena*> arenas, idle;
 };
 
 namespace {
@@ -556,18 +562,27 @@ int flk_blake3_setup_new(size_t n_blocks, int profile, flk_blake3_setup** out) {
                           profile);
   flk_r1cs* r = nullptr;
   if (int rc = flk_blake3_r1cs_new(nbl, &r)) return rc;
-  flk_blake3_setup* s = new (std::nothrow) flk_blake3_setup{nbl, m, profile, l, r, {}};
+  flk_blake3_setup* s = new (std::nothrow) flk_blake3_setup();
+  if (s) {
+    s->n_blocks_log = nbl;
+    s->m = m;
+    s->profile = profile;
+    s->ladder = l;
+    s->r1cs = r;
+  }
   if (!s) {
     flk_r1cs_free(r);
     return flk::set_error(FLK_E_NOMEM, "blake3_setup_new: out of host memory");
   }
   flk::r1cs::statement_digest(m, s->digest);
+  s->arena_bytes = 2 * prove_footprint(m, l->levels[0].log_inv_rate);
   *out = s;
   return FLK_OK;
 }
 
 void flk_blake3_setup_free(flk_blake3_setup* s) {
   if (!s) return;
+  for (flk::Arena* a : s->arenas) flk::arena_delete(a);
   flk_r1cs_free(s->r1cs);
   delete s;
 }
@@ -613,8 +628,56 @@ int flk_blake3_prove(const flk_blake3_setup* s, const flk_compression* blocks, s
   if (n_blocks > (size_t(1) << s->n_blocks_log))
     return flk::set_error(FLK_E_SHAPE, "blake3_prove: %zu compressions for %zu slots", n_blocks,
                           size_t(1) << s->n_blocks_log);
+  flk_blake3_setup* ms = const_cast<flk_blake3_setup*>(s);  // the arena list is the only mutable state
+  flk::Arena* arena = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(ms->arena_mu);
+    if (!ms->idle.empty()) {
+      arena = ms->idle.back();
+      ms->idle.pop_back();
+    }
+  }
+  if (!arena) {
+    int st = FLK_OK;
+    arena = flk::arena_new(s->arena_bytes, &st);  // on failure the prove uses the stream-ordered pool
+    if (arena) {
+      std::lock_guard<std::mutex> lk(ms->arena_mu);
+      ms->arenas.push_back(arena);
+    }
+  }
   ProveCall c{s, blocks, n_blocks, domain, domain_len, proof_out, cap, proof_len, stage_seconds, stream};
-  return guarded("blake3_prove", prove_impl, &c);
+  int rc;
+  {
+    flk::ArenaScope scope(arena);
+    rc = guarded("blake3_prove", prove_impl, &c);
+  }
+  if (arena) {
+    // Nothing still queued may touch the arena when another prove takes it.
+    const int sr = flk_synchronize(stream);
+    if (rc == FLK_OK) rc = sr;
+    std::lock_guard<std::mutex> lk(ms->arena_mu);
+    ms->idle.push_back(arena);
+  }
+  return rc;
+}
+
+int flk_blake3_setup_arena_stats(const flk_blake3_setup* s, size_t* capacity, size_t* peak, size_t* n_arenas,
+                                 size_t* fallbacks) {
+  if (!s) return flk::set_error(FLK_E_SHAPE, "blake3_setup_arena_stats: null pointer");
+  flk_blake3_setup* ms = const_cast<flk_blake3_setup*>(s);
+  std::lock_guard<std::mutex> lk(ms->arena_mu);
+  size_t pk = 0, fb = 0;
+  for (flk::Arena* a : ms->arenas) {
+    size_t p = 0, f = 0;
+    flk::arena_stats(a, nullptr, nullptr, &p, &f);
+    pk = std::max(pk, p);
+    fb += f;
+  }
+  if (capacity) *capacity = s->arena_bytes;
+  if (peak) *peak = pk;
+  if (n_arenas) *n_arenas = ms->arenas.size();
+  if (fallbacks) *fallbacks = fb;
+  return FLK_OK;
 }
 
 }  // extern "C"

@@ -128,8 +128,6 @@ __global__ void __launch_bounds__(RC_WG) rs_combine_split_kernel(const uint4* __
   }
 }
 
-inline flk_f128 fadd(flk_f128 a, flk_f128 b) { return flk_f128{a.lo ^ b.lo, a.hi ^ b.hi}; }
-
 // build_eq (CF/zerocheck/univariate_skip.rs:31-73) on the host: LSB-first table of 2^d entries.
 std::vector<flk_f128> host_eq(const flk_f128* r, size_t d) {
   std::vector<flk_f128> t(size_t(1) << d, flk_f128{0, 0});
@@ -190,7 +188,7 @@ int prove_batched(flk_fs* fs, const flk_f128* z, size_t log_len, const flk_f128*
   // its gamma-scaled eq(r'').
   uint4* ws = nullptr;
   const size_t ws_elems = n_claims * (per_claim + 256);
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&ws), ws_elems * sizeof(uint4), st);
+  hipError_t e = flk::stream_alloc(reinterpret_cast<void**>(&ws), ws_elems * sizeof(uint4), st);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return set_error(FLK_E_NOMEM, "rs_prove_batched: %zu-byte workspace allocation failed", ws_elems * sizeof(uint4));
@@ -201,7 +199,7 @@ int prove_batched(flk_fs* fs, const flk_f128* z, size_t log_len, const flk_f128*
   std::vector<flk_f128> eqg(n_claims * 128);
   int rc = FLK_OK;
   auto finish = [&](int status) {
-    hipError_t fe = hipFreeAsync(ws, st);
+    hipError_t fe = flk::stream_free(ws, st);
     if (status == FLK_OK && fe != hipSuccess) return device_failure(fe, "rs_prove_batched");
     if (fe != hipSuccess) (void)hipGetLastError();
     return status;

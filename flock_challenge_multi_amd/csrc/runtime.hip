@@ -10,8 +10,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <mutex>
+#include <set>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -268,6 +271,149 @@ __global__ void __launch_bounds__(PSUM_WG) partials_sum_kernel(const uint4* __re
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Device arena: one hipMalloc'd block, best-fit free list with coalescing.
+// Host bookkeeping only; blocks are 256-byte aligned.
+// ---------------------------------------------------------------------------
+class Arena {
+ public:
+  static constexpr size_t ALIGN = 256;
+  bool init(size_t cap) {
+    cap = (cap + ALIGN - 1) & ~(ALIGN - 1);
+    if (hipMalloc(reinterpret_cast<void**>(&base_), cap) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    cap_ = cap;
+    add_free(0, cap);
+    return true;
+  }
+  ~Arena() {
+    if (base_) (void)hipFree(base_);
+  }
+  bool contains(const void* p) const {
+    const char* c = static_cast<const char*>(p);
+    return base_ && c >= base_ && c < base_ + cap_;
+  }
+  void* alloc(size_t bytes) {
+    bytes = bytes ? (bytes + ALIGN - 1) & ~(ALIGN - 1) : ALIGN;
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = by_size_.lower_bound({bytes, 0});
+    if (it == by_size_.end()) {
+      ++fallbacks_;
+      return nullptr;
+    }
+    const size_t off = it->second, len = it->first;
+    by_size_.erase(it);
+    by_off_.erase(off);
+    if (len > bytes) add_free(off + bytes, len - bytes);
+    used_[off] = bytes;
+    in_use_ += bytes;
+    peak_ = std::max(peak_, in_use_);
+    return base_ + off;
+  }
+  bool release(void* p) {
+    if (!contains(p)) return false;
+    std::lock_guard<std::mutex> lk(mu_);
+    size_t off = static_cast<size_t>(static_cast<char*>(p) - base_);
+    auto u = used_.find(off);
+    if (u == used_.end()) return false;
+    size_t len = u->second;
+    used_.erase(u);
+    in_use_ -= len;
+    auto next = by_off_.find(off + len);
+    if (next != by_off_.end()) {
+      by_size_.erase({next->second, next->first});
+      len += next->second;
+      by_off_.erase(next);
+    }
+    auto prev = by_off_.lower_bound(off);
+    if (prev != by_off_.begin()) {
+      --prev;
+      if (prev->first + prev->second == off) {
+        by_size_.erase({prev->second, prev->first});
+        off = prev->first;
+        len += prev->second;
+        by_off_.erase(prev);
+      }
+    }
+    add_free(off, len);
+    return true;
+  }
+  void stats(size_t* capacity, size_t* in_use, size_t* peak, size_t* fallbacks) const {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (capacity) *capacity = cap_;
+    if (in_use) *in_use = in_use_;
+    if (peak) *peak = peak_;
+    if (fallbacks) *fallbacks = fallbacks_;
+  }
+
+ private:
+  void add_free(size_t off, size_t len) {
+    by_off_[off] = len;
+    by_size_.insert({len, off});
+  }
+  char* base_ = nullptr;
+  size_t cap_ = 0, in_use_ = 0, peak_ = 0, fallbacks_ = 0;
+  std::map<size_t, size_t> by_off_;              // free: offset -> length
+  std::set<std::pair<size_t, size_t>> by_size_;  // free: (length, offset)
+  std::unordered_map<size_t, size_t> used_;      // in use: offset -> length
+  mutable std::mutex mu_;
+};
+
+static thread_local Arena* t_arena = nullptr;
+static std::mutex g_arenas_mu;
+static std::vector<Arena*> g_arenas;  // live arenas (stream_free looks blocks up here)
+
+ArenaScope::ArenaScope(Arena* a) : prev_(t_arena) { t_arena = a; }
+ArenaScope::~ArenaScope() { t_arena = prev_; }
+
+Arena* arena_new(size_t bytes, int* status) {
+  *status = FLK_OK;
+  Arena* a = new (std::nothrow) Arena();
+  if (!a || !a->init(bytes)) {
+    delete a;
+    *status = set_error(FLK_E_NOMEM, "arena: hipMalloc of %zu bytes failed", bytes);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_arenas_mu);
+  g_arenas.push_back(a);
+  return a;
+}
+
+void arena_delete(Arena* a) {
+  if (!a) return;
+  {
+    std::lock_guard<std::mutex> lk(g_arenas_mu);
+    g_arenas.erase(std::remove(g_arenas.begin(), g_arenas.end(), a), g_arenas.end());
+  }
+  delete a;
+}
+
+void arena_stats(const Arena* a, size_t* capacity, size_t* in_use, size_t* peak, size_t* fallbacks) {
+  a->stats(capacity, in_use, peak, fallbacks);
+}
+
+static void keep_pool_memory();
+
+hipError_t stream_alloc(void** p, size_t bytes, hipStream_t st) {
+  if (t_arena && (*p = t_arena->alloc(bytes)) != nullptr) return hipSuccess;
+  keep_pool_memory();
+  return hipMallocAsync(p, bytes, st);
+}
+
+hipError_t stream_free(void* p, hipStream_t st) {
+  if (!p) return hipSuccess;
+  if (t_arena && t_arena->release(p)) return hipSuccess;
+  {
+    std::lock_guard<std::mutex> lk(g_arenas_mu);
+    for (Arena* a : g_arenas)
+      if (a->contains(p)) return a->release(p) ? hipSuccess : hipErrorInvalidValue;
+  }
+  return hipFreeAsync(p, st);
+}
+
 // The default stream-ordered pool of each device keeps what the reduction
 // scratch frees (release threshold raised once per device), so repeated calls
 // reuse it instead of returning it to the HIP runtime at every synchronisation.
@@ -296,12 +442,11 @@ int partials_alloc(uint4** parts, unsigned grid, int n_out, uint4* out, hipStrea
     *parts = out;
     return FLK_OK;
   }
-  keep_pool_memory();
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(parts), (size_t)grid * (size_t)n_out * sizeof(uint4), st);
+  hipError_t e = stream_alloc(reinterpret_cast<void**>(parts), (size_t)grid * (size_t)n_out * sizeof(uint4), st);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     *parts = nullptr;
-    return set_error(FLK_E_NOMEM, "reduction scratch: hipMallocAsync of %zu bytes failed",
+    return set_error(FLK_E_NOMEM, "reduction scratch: allocation of %zu bytes failed",
                      (size_t)grid * (size_t)n_out * sizeof(uint4));
   }
   return FLK_OK;
@@ -318,9 +463,9 @@ int partials_sum_free(uint4* parts, unsigned grid, int n_out, uint4* out, hipStr
   if (parts == out) return FLK_OK;  // one work-group wrote out directly
   hipLaunchKernelGGL(partials_sum_kernel, dim3((unsigned)n_out), dim3(PSUM_WG), 0, st, parts, (uint32_t)grid, out);
   hipError_t e = hipGetLastError();
-  const hipError_t ef = hipFreeAsync(parts, st);
+  const hipError_t ef = stream_free(parts, st);
   if (e != hipSuccess) return device_failure(e, "partials_sum_kernel");
-  if (ef != hipSuccess) return device_failure(ef, "reduction scratch: hipFreeAsync");
+  if (ef != hipSuccess) return device_failure(ef, "reduction scratch: free");
   return FLK_OK;
 }
 
@@ -378,8 +523,7 @@ int flk_alloc_async(size_t bytes, void* stream, void** out) {
   int rc = flk::check_enabled();
   if (rc) return rc;
   if (bytes == 0) return FLK_OK;
-  flk::keep_pool_memory();
-  hipError_t e = hipMallocAsync(out, bytes, static_cast<hipStream_t>(stream));
+  hipError_t e = flk::stream_alloc(out, bytes, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) {
     (void)hipGetLastError();
     *out = nullptr;
@@ -390,7 +534,7 @@ int flk_alloc_async(size_t bytes, void* stream, void** out) {
 
 int flk_free_async(void* p, void* stream) {
   if (!p) return FLK_OK;
-  hipError_t e = hipFreeAsync(p, static_cast<hipStream_t>(stream));
+  hipError_t e = flk::stream_free(p, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return flk::device_failure(e, "flk_free_async");
   return FLK_OK;
 }

@@ -422,7 +422,7 @@ int flk_partial_eval_lsb(const flk_f128* evals, size_t n, const flk_f128* rs, si
   // regions (n/16 and n/256 elements) and the last pass writes `out`.
   uint4* tmp = nullptr;
   if (n_passes > 1) {
-    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&tmp), ((n >> 4) + (n >> 8) + 1) * sizeof(uint4), st);
+    hipError_t e = flk::stream_alloc(reinterpret_cast<void**>(&tmp), ((n >> 4) + (n >> 8) + 1) * sizeof(uint4), st);
     if (e != hipSuccess) {
       (void)hipGetLastError();
       return flk::set_error(FLK_E_NOMEM, "partial_eval: scratch allocation failed");
@@ -449,7 +449,7 @@ int flk_partial_eval_lsb(const flk_f128* evals, size_t n, const flk_f128* rs, si
     done += t;
   }
   if (tmp) {
-    hipError_t e = hipFreeAsync(tmp, st);
+    hipError_t e = flk::stream_free(tmp, st);
     if (e != hipSuccess) return flk::device_failure(e, "partial_eval free");
   }
   return FLK_OK;

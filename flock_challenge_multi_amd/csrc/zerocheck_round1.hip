@@ -408,7 +408,7 @@ extern "C" int flk_zc_round1(const uint8_t* a_bits, const uint8_t* b_bits, const
   if (rc) return rc;
   const size_t n_outer = size_t(1) << (m - 13);
   uint4* eq = nullptr;
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&eq), n_outer * sizeof(uint4), st);
+  hipError_t e = flk::stream_alloc(reinterpret_cast<void**>(&eq), n_outer * sizeof(uint4), st);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return set_error(FLK_E_NOMEM, "zc_round1: eq_outer allocation failed");
@@ -422,7 +422,7 @@ extern "C" int flk_zc_round1(const uint8_t* a_bits, const uint8_t* b_bits, const
   const unsigned grid = (unsigned)(n_iter < 1024 ? n_iter : 1024);
   uint4* parts = nullptr;
   if ((rc = partials_alloc(&parts, grid, 128, out, st))) {
-    (void)hipFreeAsync(eq, st);
+    (void)flk::stream_free(eq, st);
     return rc;
   }
   hipLaunchKernelGGL(zc_round1_kernel, dim3(grid), dim3(R1_WG), 0, st, reinterpret_cast<const uint64_t*>(a_bits),
@@ -436,7 +436,7 @@ extern "C" int flk_zc_round1(const uint8_t* a_bits, const uint8_t* b_bits, const
   hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(128), 0, st, out, 128u,
                      make_uint4((uint32_t)sc.lo, (uint32_t)(sc.lo >> 32), (uint32_t)sc.hi, (uint32_t)(sc.hi >> 32)));
   FLK_LAUNCH_CHECK("scale_kernel");
-  e = hipFreeAsync(eq, st);
+  e = flk::stream_free(eq, st);
   if (e != hipSuccess) return device_failure(e, "zc_round1: free eq_outer");
   return FLK_OK;
 }

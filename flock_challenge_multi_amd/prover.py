@@ -138,6 +138,12 @@ class Blake3Setup:
         nat.call("flk_blake3_proof_bound", h, ctypes.byref(bound))
         self._cap = bound.value
 
+    def arena_stats(self):
+        """{capacity, peak, n_arenas, fallbacks} of the setup's device arenas (bytes)."""
+        v = [ctypes.c_size_t() for _ in range(4)]
+        nat.call("flk_blake3_setup_arena_stats", self._h, *[ctypes.byref(x) for x in v])
+        return dict(zip(("capacity", "peak", "n_arenas", "fallbacks"), (x.value for x in v)))
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

@@ -519,6 +519,17 @@ int flk_blake3_setup_info(const flk_blake3_setup *s, size_t *m, size_t *n_blocks
  * (CP/proof_io.rs:160-198). */
 int flk_blake3_proof_bound(const flk_blake3_setup *s, size_t *bytes);
 
+/* Device memory of setup s's proves.  Each concurrent prove of s takes a device
+ * arena of `capacity` bytes (created at its first use, kept for later proves:
+ * no device allocation inside a prove); `peak` is the largest number of bytes
+ * any of them has held, `n_arenas` how many exist, `fallbacks` the allocations
+ * that did not fit and went to the stream-ordered pool.  Host-only.  The
+ * arenas are the device counterpart of the prover's scratch pool, which
+ * recycles its large transient buffers across phases and proves
+ * (CF/scratch.rs:1-20). */
+int flk_blake3_setup_arena_stats(const flk_blake3_setup *s, size_t *capacity, size_t *peak, size_t *n_arenas,
+                                 size_t *fallbacks);
+
 /* One proof: generate_witness_with_ab_packed_and_lincheck of the device
  * compressions (padding slots are the all-zero compression), pcs::commit,
  * prover::prove_ligerito (CP/prover.rs:373-472: bind_statement, zerocheck,

@@ -93,6 +93,35 @@ def test_proof_is_deterministic_and_stream_independent():
     assert s.prove(other)[0] != p1
 
 
+def test_arena_reuse_and_concurrent_proves():
+    """Proves of one setup draw their device memory from the setup's arenas (no
+    pool fallback, peak within capacity); two host threads proving on two streams
+    at once take two arenas and write the same bytes as a lone prove."""
+    import threading
+    _, blocks = _blocks(8, 41)
+    s = pv.Blake3Setup(256)
+    want, _ = s.prove(blocks)
+    st = s.arena_stats()
+    assert st["n_arenas"] == 1 and st["fallbacks"] == 0 and 0 < st["peak"] <= st["capacity"]
+    assert s.prove(blocks)[0] == want and s.arena_stats()["n_arenas"] == 1  # the arena is reused
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = [None, None]
+
+    def run(i):
+        for _ in range(2):
+            outs[i] = s.prove(blocks, stream=streams[i])[0]
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert outs == [want, want]
+    st = s.arena_stats()
+    assert 1 <= st["n_arenas"] <= 2 and st["fallbacks"] == 0
+
+
 def test_reproduces_golden_fixture():
     _, blocks = _blocks(8, 0x123C)
     proof, _ = setup_for(256).prove(blocks)
