@@ -50,7 +50,11 @@ def parse():
     ap.add_argument("--commit-warmup", type=int, default=30)
     ap.add_argument("--no-verify", action="store_true", help="skip the untimed verification of the proofs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-log2", type=int, default=15, help="compressions of the CPU baseline's proof = 2^cpu_log2")
+    ap.add_argument("--cpu-log2", type=int, default=16, help="compressions of the CPU baseline's proof = 2^cpu_log2")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched (48 proofs) and served legs")
+    ap.add_argument("--batch-jobs", type=int, default=48)
+    ap.add_argument("--served-requests", type=int, default=400, help="requests per load level of the served leg")
+    ap.add_argument("--workers", type=str, default="1,2,3,4,6,8", help="worker counts the batch autotuner tries")
     ap.add_argument("--no-commit", action="store_true", help="skip the standalone config-1 commit leg")
     ap.add_argument("--no-m32", action="store_true")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel table at m = 32 sizes")
@@ -679,6 +683,150 @@ def proof_kernels(args, torch, nat, hp, blocks):
             "rs_encode_l0": {"ms": enc_ms, "bytes": enc_bytes, "gbs": enc_bytes / (enc_ms / 1e3) / 1e9}}
 
 
+# ---------------------------------------------------------------------------
+# Batched (48 uneven proofs) and served (open-loop Poisson arrivals) proving
+# ---------------------------------------------------------------------------
+def _splitmix64(state):
+    state = (state + 0x9E3779B97F4A7C15) & (2**64 - 1)
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2**64 - 1)
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & (2**64 - 1)
+    return state, z ^ (z >> 31)
+
+
+def batch_sizes(n_jobs, seed):
+    """n_i = floor(2^U_i), U_i ~ Uniform[10, 18] from a seeded SplitMix64 (53-bit uniforms)."""
+    st, out = seed, []
+    for _ in range(n_jobs):
+        st, z = _splitmix64(st)
+        u = (z >> 11) * 2.0**-53
+        out.append(int(2.0 ** (10 + 8 * u)))
+    return out
+
+
+def _job_input(torch, np, n, seed):
+    """The first n of generate_compressions(ceil_log2(n), seed), resident on the device."""
+    from flock_challenge_multi_amd.inputs import generate_compressions
+    log2 = max(3, (n - 1).bit_length())
+    c = np.ascontiguousarray(generate_compressions(log2, seed)[:n])
+    return torch.from_numpy(c.view(np.int32)).cuda()
+
+
+def _run_batch(pv, pool, inputs):
+    t0 = time.perf_counter()
+    tickets = [pool.submit(b) for b in inputs]
+    res = [pool.wait(t) for t in tickets]
+    return time.perf_counter() - t0, res
+
+
+def batch_leg(args, torch, pv):
+    """48 proofs of log-uniform 2^10..2^18 compressions submitted at once to the prover
+    pool; the autotuner tries each worker count and keeps the fastest.  Every proof is
+    compared with the same input proved alone (flk_blake3_prove), which also gives the
+    one-at-a-time throughput."""
+    import numpy as np
+    sizes = batch_sizes(args.batch_jobs, 0xBA7C4ED)
+    inputs = [_job_input(torch, np, n, 0xB000 + i) for i, n in enumerate(sizes)]
+    torch.cuda.synchronize()
+    total = sum(sizes)
+    setups = {}
+
+    def setup_for(n):
+        k = max(3, (n - 1).bit_length())
+        if k not in setups:
+            setups[k] = pv.Blake3Setup(n)
+        return setups[k]
+
+    # one at a time (the reference harness's form), also the byte-identity reference
+    for b in inputs[:4]:
+        setup_for(b.shape[0]).prove(b)  # warm-up of the setups and arenas
+    alone, t_alone = [], []
+    for b in inputs:
+        s = setup_for(b.shape[0])
+        t1 = time.perf_counter()
+        alone.append(s.prove(b)[0])
+        t_alone.append(time.perf_counter() - t1)
+    seq_s = sum(t_alone)
+    tune = {}
+    best = None
+    cap32 = setup_for(max(sizes)).arena_stats()["capacity"]
+    for w in [int(x) for x in args.workers.split(",")]:
+        pool = pv.ProverPool(workers=w, hbm_budget=w * cap32 + (1 << 30))
+        for n in sorted({max(3, (n - 1).bit_length()) for n in sizes}):
+            pool.prepare(1 << n)
+        _run_batch(pv, pool, inputs[:8])  # warm-up: arenas views, streams, caches
+        wall, res = _run_batch(pv, pool, inputs)
+        st = pool.stats()
+        same = sum(r[0] == a for r, a in zip(res, alone))
+        tune[w] = {"wall_s": wall, "value": total / wall, "identical": same, "peak_running": st["peak_running"],
+                   "peak_reserved_bytes": st["peak_reserved"]}
+        if best is None or wall < tune[best]["wall_s"]:
+            best = w
+            best_res = res
+        pool.close()
+    lat = [r[1][2] * 1e3 for r in best_res]
+    return {
+        "metric": "batched verified BLAKE3 compressions/s (48 proofs, 2^10..2^18 each, 1 GPU)",
+        "value": tune[best]["value"], "unit": "compressions/s", "jobs": len(sizes), "compressions": total,
+        "workers": best, "wall_s": tune[best]["wall_s"], "p50_ms": pctl(lat, 0.5), "p99_ms": pctl(lat, 0.99),
+        "one_at_a_time_value": total / seq_s, "identical_to_alone": tune[best]["identical"],
+        "hbm_budget": "workers x the m = 32 arena + 1 GiB", "autotune": tune, "sizes_seed": "0xBA7C4ED", "inputs": "generate_compressions(ceil_log2(n_i), 0xB000 + i)[:n_i]",
+    }, best, alone, inputs, sizes
+
+
+def served_leg(args, torch, pv, workers, capacity_rps, alone, inputs):
+    """Open-loop Poisson arrivals at 25/50/75/95% of the batch leg's capacity (jobs/s,
+    same size mix): each request is one of the batch leg's inputs (cycled in a seeded
+    random order), submitted at its arrival time; latency = arrival to proof ready.
+    Every response is compared with the input's proof made alone."""
+    import random
+    import threading
+    out = {"capacity_jobs_per_s": capacity_rps, "workers": workers, "requests_per_level": args.served_requests}
+    cap32 = pv.Blake3Setup(max(b.shape[0] for b in inputs)).arena_stats()["capacity"]
+    pool = pv.ProverPool(workers=workers, hbm_budget=workers * cap32 + (1 << 30))
+    for b in inputs:
+        pool.prepare(b.shape[0])
+    _run_batch(pv, pool, inputs[:8])
+    for load in (0.25, 0.5, 0.75, 0.95):
+        rng = random.Random(int(load * 100))
+        lam = load * capacity_rps
+        arrivals, t = [], 0.0
+        for _ in range(args.served_requests):
+            t += rng.expovariate(lam)
+            arrivals.append(t)
+        picks = [rng.randrange(len(inputs)) for _ in arrivals]
+        tickets = [None] * len(arrivals)
+        lag = [0.0] * len(arrivals)
+        t0 = time.perf_counter()
+
+        def submitter():
+            for k, (a, j) in enumerate(zip(arrivals, picks)):
+                d = t0 + a - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                lag[k] = time.perf_counter() - t0 - a
+                tickets[k] = pool.submit(inputs[j])
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, ok = [], 0
+        for k in range(len(arrivals)):
+            while tickets[k] is None:
+                time.sleep(0.0002)
+            proof, times = pool.wait(tickets[k])
+            lat.append((times[2] + lag[k]) * 1e3)
+            ok += proof == alone[picks[k]]
+        th.join()
+        wall = time.perf_counter() - t0
+        out[f"load_{int(load * 100)}"] = {
+            "offered_jobs_per_s": lam, "achieved_jobs_per_s": len(arrivals) / wall,
+            "p50_ms": pctl(lat, 0.5), "p99_ms": pctl(lat, 0.99), "mean_ms": sum(lat) / len(lat),
+            "identical_to_alone": ok, "requests": len(arrivals), "max_submit_lag_ms": max(lag) * 1e3,
+        }
+    pool.close()
+    return out
+
+
 def calib_copy(torch, mib):
     """One device-to-device copy of `mib` MiB: a plain kernel of known traffic whose
     FETCH_SIZE / WRITE_SIZE readings scale the counters of the proof's kernels."""
@@ -747,6 +895,14 @@ def main():
             return s.prove(torch.from_numpy(np.ascontiguousarray(c).view(np.int32)).cuda())[0]
 
         cpu = proof_cpu_baseline(args, device_prove)
+    batch = served = None
+    if not args.no_batch:
+        from flock_challenge_multi_amd import prover as pv
+        batch, workers, alone, b_inputs, b_sizes = batch_leg(args, torch, pv)
+        if args.served_requests > 0:
+            served = served_leg(args, torch, pv, workers, len(b_sizes) / batch["wall_s"], alone, b_inputs)
+        del alone, b_inputs
+        torch.cuda.empty_cache()
     commit = None if args.no_commit else commit_leg(args, torch, nat, hp)
     dom = kern["zc_round1"] if kern else None
     roofline = None if dom is None else {
@@ -785,6 +941,8 @@ def main():
         "roofline": roofline,
         "proof_kernels": kern,
         "cpu_baseline": cpu,
+        "batched_config3": batch,
+        "served_config4": served,
         "commit_config1": commit,
     }
     print(json.dumps(line), flush=True)

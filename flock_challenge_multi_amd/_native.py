@@ -127,6 +127,12 @@ SIGNATURES = {
     "flk_blake3_setup_info": (_I, [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ), ctypes.POINTER(_SZ), _P]),
     "flk_blake3_proof_bound": (_I, [_P, ctypes.POINTER(_SZ)]),
     "flk_blake3_setup_arena_stats": (_I, [_P] + [ctypes.POINTER(_SZ)] * 4),
+    "flk_prover_pool_new": (_I, [ctypes.c_int, _SZ, _SZ, ctypes.POINTER(_P)]),
+    "flk_prover_pool_free": (None, [_P]),
+    "flk_prover_pool_prepare": (_I, [_P, _SZ]),
+    "flk_prover_pool_submit": (_I, [_P, _P, _SZ, _P, _SZ, ctypes.POINTER(ctypes.c_uint64)]),
+    "flk_prover_pool_wait": (_I, [_P, ctypes.c_uint64, _P, _SZ, ctypes.POINTER(_SZ), _P]),
+    "flk_prover_pool_stats": (_I, [_P] + [ctypes.POINTER(_SZ)] * 4),
     "flk_blake3_prove": (_I, [_P, _P, _SZ, _P, _SZ, _P, _SZ, ctypes.POINTER(_SZ), _P, _P]),
 }
 

@@ -50,7 +50,19 @@ class ArenaScope {
 // A new arena of `bytes` (nullptr and the error recorded when the device
 // allocation fails), its release (no block may be in use), its statistics.
 Arena* arena_new(size_t bytes, int* status);
+// An arena over `bytes` carved from `parent`, given back to the parent by
+// arena_delete (nullptr and FLK_E_NOMEM when the parent has no room).
+Arena* arena_view_new(Arena* parent, size_t bytes, int* status);
 void arena_delete(Arena* a);
 void arena_stats(const Arena* a, size_t* capacity, size_t* in_use, size_t* peak, size_t* fallbacks);
+
+// The BLAKE3 prover of a setup (prover.cpp): the device bytes one prove draws
+// from its arena, the setup's m, and a prove whose scratch comes from `arena`
+// (the stream-ordered pool when null); same arguments as flk_blake3_prove.
+size_t blake3_arena_bytes(const flk_blake3_setup* s);
+size_t blake3_setup_m(const flk_blake3_setup* s);
+int blake3_prove_in(const flk_blake3_setup* s, Arena* arena, const flk_compression* blocks, size_t n_blocks,
+                    const uint8_t* domain, size_t domain_len, uint8_t* proof_out, size_t cap, size_t* proof_len,
+                    double* stage_seconds, void* stream);
 
 }  // namespace flk

@@ -289,8 +289,22 @@ class Arena {
     add_free(0, cap);
     return true;
   }
+  // A view over `cap` bytes carved from `parent` (returned to it on deletion).
+  bool init_view(Arena* parent, size_t cap) {
+    cap = (cap + ALIGN - 1) & ~(ALIGN - 1);
+    void* p = parent->alloc(cap);
+    if (!p) return false;
+    parent_ = parent;
+    base_ = static_cast<char*>(p);
+    cap_ = cap;
+    add_free(0, cap);
+    return true;
+  }
   ~Arena() {
-    if (base_) (void)hipFree(base_);
+    if (parent_)
+      parent_->release(base_);
+    else if (base_)
+      (void)hipFree(base_);
   }
   bool contains(const void* p) const {
     const char* c = static_cast<const char*>(p);
@@ -354,6 +368,7 @@ class Arena {
     by_off_[off] = len;
     by_size_.insert({len, off});
   }
+  Arena* parent_ = nullptr;
   char* base_ = nullptr;
   size_t cap_ = 0, in_use_ = 0, peak_ = 0, fallbacks_ = 0;
   std::map<size_t, size_t> by_off_;              // free: offset -> length
@@ -375,6 +390,19 @@ Arena* arena_new(size_t bytes, int* status) {
   if (!a || !a->init(bytes)) {
     delete a;
     *status = set_error(FLK_E_NOMEM, "arena: hipMalloc of %zu bytes failed", bytes);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_arenas_mu);
+  g_arenas.push_back(a);
+  return a;
+}
+
+Arena* arena_view_new(Arena* parent, size_t bytes, int* status) {
+  *status = FLK_OK;
+  Arena* a = new (std::nothrow) Arena();
+  if (!a || !a->init_view(parent, bytes)) {
+    delete a;
+    *status = set_error(FLK_E_NOMEM, "arena view: %zu bytes do not fit in the parent arena", bytes);
     return nullptr;
   }
   std::lock_guard<std::mutex> lk(g_arenas_mu);
@@ -407,9 +435,16 @@ hipError_t stream_free(void* p, hipStream_t st) {
   if (!p) return hipSuccess;
   if (t_arena && t_arena->release(p)) return hipSuccess;
   {
+    // Newest first: a view is created after the arena it is carved from, so a
+    // block of a view is released there, never as the view's block of the parent.
     std::lock_guard<std::mutex> lk(g_arenas_mu);
-    for (Arena* a : g_arenas)
-      if (a->contains(p)) return a->release(p) ? hipSuccess : hipErrorInvalidValue;
+    bool inside = false;
+    for (auto it = g_arenas.rbegin(); it != g_arenas.rend(); ++it)
+      if ((*it)->contains(p)) {
+        inside = true;
+        if ((*it)->release(p)) return hipSuccess;
+      }
+    if (inside) return hipErrorInvalidValue;
   }
   return hipFreeAsync(p, st);
 }

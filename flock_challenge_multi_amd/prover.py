@@ -169,3 +169,60 @@ class Blake3Setup:
         nat.call("flk_blake3_prove", self._h, hp._ptr(blocks), blocks.shape[0], dbuf, len(d),
                  buf.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(plen), stages, st)
         return buf[:plen.value].tobytes(), list(stages)
+
+
+class ProverPool:
+    """Batched and served proving (flk_prover_pool_*): `workers` host threads, each on
+    its own stream, proving batches of any supported size through one device memory
+    budget (bytes; None = 60% of the free device memory), admitted in submission
+    order.  submit() returns a ticket at once; wait(ticket) returns the proof bytes
+    and (seconds queued, seconds proving, seconds submit to done)."""
+
+    def __init__(self, profile="fast", workers=2, hbm_budget=None):
+        h = ctypes.c_void_p()
+        nat.call("flk_prover_pool_new", PROFILES[profile], workers, hbm_budget or 0, ctypes.byref(h))
+        self._h = h
+        self._held = {}  # ticket -> the input tensor, kept alive until the job is waited for
+        self.workers = workers
+
+    def prepare(self, n_blocks):
+        nat.call("flk_prover_pool_prepare", self._h, n_blocks)
+
+    def submit(self, blocks, domain=DOMAIN):
+        if not (isinstance(blocks, torch.Tensor) and blocks.is_cuda and blocks.dim() == 2
+                and blocks.shape[1] == 28 and blocks.element_size() == 4 and blocks.is_contiguous()):
+            raise ValueError("blocks must be a contiguous CUDA 32-bit tensor of shape (n, 28)")
+        d = bytes(domain)
+        dbuf = (ctypes.c_uint8 * max(len(d), 1)).from_buffer_copy(d or b"\0")
+        t = ctypes.c_uint64()
+        nat.call("flk_prover_pool_submit", self._h, hp._ptr(blocks), blocks.shape[0], dbuf, len(d), ctypes.byref(t))
+        self._held[t.value] = blocks
+        return t.value
+
+    def wait(self, ticket):
+        n = ctypes.c_size_t()
+        times = (ctypes.c_double * 3)()
+        buf = np.empty(1 << 20, np.uint8)
+        rc = nat.load().flk_prover_pool_wait(self._h, ticket, buf.ctypes.data_as(ctypes.c_void_p), buf.size,
+                                             ctypes.byref(n), times)
+        if rc == nat.FLK_E_SHAPE and n.value > buf.size:  # a proof larger than 1 MiB: fetch it again
+            buf = np.empty(n.value, np.uint8)
+            rc = nat.load().flk_prover_pool_wait(self._h, ticket, buf.ctypes.data_as(ctypes.c_void_p), buf.size,
+                                                 ctypes.byref(n), times)
+        self._held.pop(ticket, None)
+        nat.check(rc, "flk_prover_pool_wait")
+        return buf[:n.value].tobytes(), tuple(times)
+
+    def stats(self):
+        v = [ctypes.c_size_t() for _ in range(4)]
+        nat.call("flk_prover_pool_stats", self._h, *[ctypes.byref(x) for x in v])
+        return dict(zip(("budget", "peak_reserved", "peak_running", "completed"), (x.value for x in v)))
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            nat.load().flk_prover_pool_free(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -530,6 +530,35 @@ int flk_blake3_proof_bound(const flk_blake3_setup *s, size_t *bytes);
 int flk_blake3_setup_arena_stats(const flk_blake3_setup *s, size_t *capacity, size_t *peak, size_t *n_arenas,
                                  size_t *fallbacks);
 
+/* ---- Batched and served proving (pool.hip) --------------------------------
+ * A pool of `workers` host threads (1..16), each with its own HIP stream,
+ * proving BLAKE3 R1CS batches of any supported size (m = 22..35) for one
+ * Ligerito profile.  Device memory is one arena of `hbm_budget` bytes (0 = 60%
+ * of the free device memory): a job starts when its setup's footprint can be
+ * carved from it, in submission order.  Setups are made per batch size class on
+ * first use (or by flk_prover_pool_prepare).  A job's proof bytes equal
+ * flk_blake3_prove's on the same input.  The reference proves one batch per
+ * call (CP/prover.rs:373-472) and its harness runs one proof per worker process
+ * (benchmark-tools/harness/src/main.rs:360-407); this is the device-side batch
+ * and serving form of that loop. */
+typedef struct flk_prover_pool flk_prover_pool;
+int flk_prover_pool_new(int profile, size_t workers, size_t hbm_budget, flk_prover_pool **out);
+/* Running jobs finish; queued jobs are dropped (their waits fail). */
+void flk_prover_pool_free(flk_prover_pool *p);
+/* Make the setup for batches of n_blocks compressions now (statement digest, R1CS). */
+int flk_prover_pool_prepare(flk_prover_pool *p, size_t n_blocks);
+/* Queue a proof of n_blocks device compressions (ready on the device, valid until
+ * the job is done) under the transcript domain; returns at once with a ticket. */
+int flk_prover_pool_submit(flk_prover_pool *p, const flk_compression *blocks, size_t n_blocks, const uint8_t *domain,
+                           size_t domain_len, uint64_t *ticket);
+/* Wait for a ticket and copy its proof out.  *proof_len is always set; with a
+ * buffer too small (or NULL) the call fails with FLK_E_SHAPE and keeps the job.
+ * times (optional, 3 doubles): seconds queued, proving, submit to done. */
+int flk_prover_pool_wait(flk_prover_pool *p, uint64_t ticket, uint8_t *proof_out, size_t cap, size_t *proof_len,
+                         double *times);
+int flk_prover_pool_stats(flk_prover_pool *p, size_t *budget, size_t *peak_reserved, size_t *peak_running,
+                          size_t *completed);
+
 /* One proof: generate_witness_with_ab_packed_and_lincheck of the device
  * compressions (padding slots are the all-zero compression), pcs::commit,
  * prover::prove_ligerito (CP/prover.rs:373-472: bind_statement, zerocheck,
