@@ -202,11 +202,7 @@ extern "C" int flk_induce_sumcheck_poly(size_t log_msg_cols, const flk_f128* row
 
   hipLaunchKernelGGL(zero4_kernel, dim3(1), dim3(64), 0, st, sum);
   FLK_LAUNCH_CHECK("zero4_kernel");
-  if (nq == 0) {
-    hipError_t e = hipMemsetAsync(basis, 0, (size_t(1) << L) * 16, st);
-    if (e != hipSuccess) return device_failure(e, "induce: clear basis");
-    return FLK_OK;
-  }
+  if (nq == 0) return zero_n(basis, uint64_t(1) << L, st);  // a kernel: safe inside a captured graph
 
   // Host constants: alpha_q, eq(v), sks_vks and their inverses, and alpha_q * eq_k.
   std::vector<flk_f128> aq = host_eq(alpha, n_alpha);

@@ -53,17 +53,7 @@ __device__ __forceinline__ void wg_reduce_store(uint4 v, uint4* red, uint4* part
   }
 }
 
-__global__ void zero_n_kernel(uint4* p, uint64_t n) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = make_uint4(0u, 0u, 0u, 0u);
-}
 
-static int zero_n(uint4* p, uint64_t n, hipStream_t st) {
-  if (!n) return FLK_OK;
-  hipLaunchKernelGGL(zero_n_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
-  FLK_LAUNCH_CHECK("zero_n_kernel");
-  return FLK_OK;
-}
 
 // ---------------------------------------------------------------------------
 // Lincheck partial fold.  out[i] = sum over stripes s and bits r of byte

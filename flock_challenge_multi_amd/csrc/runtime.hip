@@ -44,10 +44,31 @@ static bool env_disabled() {
   return v && v[0] && strcmp(v, "0") != 0;
 }
 
-int check_enabled() {
+// Failure injection for tests of the latch (flk_inject_device_failures): after
+// g_inject_skip more entry checks, the next g_inject_fail ones fail as a device
+// runtime failure would.
+static std::atomic<bool> g_inject_armed{false};
+static std::mutex g_inject_mu;
+static unsigned g_inject_skip = 0, g_inject_fail = 0;
+
+static int enabled_state() {
   if (g_latched.load()) return set_error(FLK_E_DISABLED, "device path latched off after a runtime failure");
   if (!g_enabled.load()) return set_error(FLK_E_DISABLED, "device path disabled by flk_set_enabled(0)");
   if (env_disabled()) return set_error(FLK_E_DISABLED, "device path disabled by FLOCK_NO_GPU");
+  return FLK_OK;
+}
+
+int check_enabled() {
+  if (int rc = enabled_state()) return rc;
+  if (g_inject_armed.load(std::memory_order_relaxed)) {
+    std::lock_guard<std::mutex> lk(g_inject_mu);
+    if (g_inject_skip) {
+      --g_inject_skip;
+    } else if (g_inject_fail) {
+      if (--g_inject_fail == 0) g_inject_armed.store(false);
+      return device_failure(hipErrorUnknown, "injected device failure");
+    }
+  }
   return FLK_OK;
 }
 
@@ -471,6 +492,19 @@ static void keep_pool_memory() {
   (void)hipGetLastError();
 }
 
+__global__ void zero_n_kernel(uint4* __restrict__ p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+int zero_n(uint4* p, uint64_t n, hipStream_t st) {
+  if (!n) return FLK_OK;
+  const uint64_t wgs = (n + 255) / 256;
+  hipLaunchKernelGGL(zero_n_kernel, dim3((unsigned)(wgs < 65535 ? wgs : 65535)), dim3(256), 0, st, p, n);
+  FLK_LAUNCH_CHECK("zero_n_kernel");
+  return FLK_OK;
+}
+
 int partials_alloc(uint4** parts, unsigned grid, int n_out, uint4* out, hipStream_t st) {
   *parts = nullptr;
   if (grid == 1) {  // parts[k * 1 + 0] is out[k]: no scratch, no second launch
@@ -518,9 +552,16 @@ int flk_device_available(void) {
   return n > 0;
 }
 
-int flk_is_disabled(void) { return flk::check_enabled() != FLK_OK; }
+int flk_is_disabled(void) { return flk::enabled_state() != FLK_OK; }
 
 void flk_undisable(void) { flk::g_latched.store(0); }
+
+void flk_inject_device_failures(unsigned skip, unsigned n) {
+  std::lock_guard<std::mutex> lk(flk::g_inject_mu);
+  flk::g_inject_skip = n ? skip : 0;
+  flk::g_inject_fail = n;
+  flk::g_inject_armed.store(n != 0);
+}
 
 void flk_set_enabled(int enabled) { flk::g_enabled.store(enabled ? 1 : 0); }
 

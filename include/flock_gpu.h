@@ -68,6 +68,12 @@ int flk_device_available(void);
 int flk_is_disabled(void);
 /* Clear the failure latch (CF/gpu.rs undisable). */
 void flk_undisable(void);
+/* Test hook: after `skip` more entry-point calls, the next n fail as a device
+ * runtime failure would (FLK_E_DEVICE, the latch set) without touching the
+ * device, so callers can exercise their fallback (CF/gpu.rs:455-516); a skip
+ * lands the failure inside a composite call such as flk_blake3_prove.  n = 0
+ * cancels. */
+void flk_inject_device_failures(unsigned skip, unsigned n);
 /* Enable/disable the library (CF/gpu.rs merkle::set_enabled). */
 void flk_set_enabled(int enabled);
 /* Message for the last non-OK status returned on this thread (host string). */

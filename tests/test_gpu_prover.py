@@ -229,6 +229,30 @@ def test_proof_bound_and_small_buffer(orc):
         assert p2 == want
 
 
+@pytest.mark.parametrize("skip", [0, 3, 40, 120])
+def test_device_failure_inside_prove_latches(skip):
+    """A device failure at any point of a prove (injected after `skip` entry-point
+    calls, from the host) makes the prove return FLK_E_DEVICE and latches the
+    library; after flk_undisable the same setup proves the same bytes again."""
+    lib = pv.nat.load()
+    _, blocks = _blocks(8, 51)
+    s = setup_for(256)
+    want, _ = s.prove(blocks)
+    lib.flk_inject_device_failures(skip, 1)
+    try:
+        with pytest.raises(pv.nat.FlockGpuError) as ei:
+            s.prove(blocks)
+        assert "status 2" in str(ei.value)  # FLK_E_DEVICE
+        assert lib.flk_is_disabled() == 1
+        with pytest.raises(pv.nat.FlockGpuError):
+            s.prove(blocks)  # FLK_E_DISABLED while latched
+    finally:
+        lib.flk_inject_device_failures(0, 0)
+        lib.flk_undisable()
+    assert s.prove(blocks)[0] == want
+    assert s.arena_stats()["fallbacks"] == 0
+
+
 def test_setup_rejects_unsupported_sizes():
     with pytest.raises(pv.nat.FlockGpuError):
         pv.Blake3Setup(1 << 6)  # m = 20: no Ligerito ladder below m = 22

@@ -70,3 +70,36 @@ def test_env_kill_switch(monkeypatch):
     assert lib.flk_is_disabled() == 1
     monkeypatch.setenv("FLOCK_NO_GPU", "0")
     assert lib.flk_is_disabled() == 0
+
+
+def test_injected_device_failure_latches():
+    """A device runtime failure latches the library off for every later call until
+    flk_undisable() (the permanent fallback latch of CF/gpu.rs:455-516), forced from
+    the host with flk_inject_device_failures: no device is touched."""
+    lib = nat.load()
+    assert lib.flk_is_disabled() == 0
+    lib.flk_inject_device_failures(0, 1)
+    try:
+        assert lib.flk_is_disabled() == 0  # asking does not consume the injection
+        assert lib.flk_sc_round_msg(None, None, 8, None, None) == nat.FLK_E_DEVICE
+        assert b"injected" in lib.flk_last_error()
+        assert lib.flk_is_disabled() == 1
+        # latched: every entry point refuses, shape errors included, until undisable
+        assert lib.flk_sc_round_msg(None, None, 8, None, None) == nat.FLK_E_DISABLED
+        assert lib.flk_zc_round1(None, None, None, 20, None, None, None, None) == nat.FLK_E_DISABLED
+        assert b"latched" in lib.flk_last_error()
+    finally:
+        lib.flk_inject_device_failures(0, 0)
+        lib.flk_undisable()
+    assert lib.flk_is_disabled() == 0
+    assert lib.flk_sc_round_msg(None, None, 8, None, None) == nat.FLK_E_SHAPE  # back to ordinary refusals
+    lib.flk_inject_device_failures(1, 2)  # one call passes, then two fail (the second after undisable)
+    try:
+        assert lib.flk_sc_round_msg(None, None, 8, None, None) == nat.FLK_E_SHAPE
+        assert lib.flk_sc_round_msg(None, None, 8, None, None) == nat.FLK_E_DEVICE
+        lib.flk_undisable()
+        assert lib.flk_sc_round_msg(None, None, 8, None, None) == nat.FLK_E_DEVICE
+    finally:
+        lib.flk_inject_device_failures(0, 0)
+        lib.flk_undisable()
+    assert lib.flk_is_disabled() == 0
