@@ -111,37 +111,89 @@ void zerocheck_prove(Fs& ch, const uint8_t* a_bits, const uint8_t* b_bits, const
 
   // Fold at z on S, then the multilinear rounds binding the low variable first
   // (fold_at_z_naive, round_pair_naive with r_next = [1, r[k_skip + i + 1 ..]],
-  // fold_in_place_pair; multilinear.rs:575-640, 3194-3213).
+  // fold_in_place_pair; multilinear.rs:575-640, 3194-3213).  While the round's eq
+  // has >= 8 coordinates the device runs each round as one fused pass: the bind by
+  // the previous challenge, the new tables written, and the message against the eq
+  // split into its first 8 coordinates and the rest (flk_zc_fold_round).  The last
+  // rounds (tables of <= 512 entries) run on the host.
   chk(flk::lagrange_weights_nodes(K_SKIP, z, 0, ws.data()), "lagrange_weights(S)");
   DeviceVec<F128> wd(ws, st);
   size_t n = size_t(1) << n_mlv;
   DeviceVec<F128> am(n, st), bm(n, st), msg(2, st);
   chk(flk_zc_fold_at_z(a_bits, m, K_SKIP, wd.data(), am.data(), st), "zc_fold_at_z(a)");
   chk(flk_zc_fold_at_z(b_bits, m, K_SKIP, wd.data(), bm.data(), st), "zc_fold_at_z(b)");
-  std::vector<F128> rhos;
-  auto fold = [&](F128 rho) {
-    DeviceVec<F128> an(n / 2, st), bn(n / 2, st);
-    chk(flk_partial_eval_lsb(am.data(), n, &rho, 1, an.data(), st), "zc fold(a)");
-    chk(flk_partial_eval_lsb(bm.data(), n, &rho, 1, bn.data(), st), "zc fold(b)");
-    am = std::move(an);
-    bm = std::move(bn);
-    n /= 2;
+  const size_t d0 = n_mlv - 1;
+  DeviceVec<F128> eq_lo(256, st), eq_hi(d0 >= 8 ? size_t(1) << (d0 - 8) : 1, st);
+  std::vector<F128> rhos, ha, hb;  // ha, hb: the host tail's tables
+  bool on_device = true;
+  auto host_fold = [&](std::vector<F128>& t, F128 rho) {
+    std::vector<F128> o(t.size() / 2);
+    for (size_t j = 0; j < o.size(); ++j)
+      o[j] = flk::fadd(t[2 * j], flk::host_mul(rho, flk::fadd(t[2 * j], t[2 * j + 1])));
+    t.swap(o);
   };
   for (size_t i = 0; i < n_mlv; ++i) {
-    if (i) fold(rhos.back());
     const size_t d = n_mlv - i - 1;  // eq over r[k_skip + i + 1 .. m)
-    DeviceVec<F128> eq(size_t(1) << d, st);
-    chk(flk_build_eq(r.data() + K_SKIP + i + 1, d, eq.data(), st), "build_eq");
-    chk(flk_zc_round_pair(am.data(), bm.data(), n, eq.data(), msg.data(), st), "zc_round_pair");
-    const std::vector<F128> g = msg.to_host();
-    pf.rounds.push_back({g[0], g[1]});
+    const F128* pt = r.data() + K_SKIP + i + 1;
+    std::array<F128, 2> g;
+    if (on_device && d >= 8) {
+      chk(flk_build_eq(pt, 8, eq_lo.data(), st), "build_eq(lo)");
+      chk(flk_build_eq(pt + 8, d - 8, eq_hi.data(), st), "build_eq(hi)");
+      if (i == 0) {
+        chk(flk_zc_fold_round(am.data(), bm.data(), n, nullptr, eq_lo.data(), eq_hi.data(), d - 8, nullptr, nullptr,
+                              msg.data(), st),
+            "zc_fold_round");
+      } else {
+        DeviceVec<F128> an(n / 2, st), bn(n / 2, st);
+        chk(flk_zc_fold_round(am.data(), bm.data(), n, &rhos.back(), eq_lo.data(), eq_hi.data(), d - 8, an.data(),
+                              bn.data(), msg.data(), st),
+            "zc_fold_round");
+        am = std::move(an);
+        bm = std::move(bn);
+        n /= 2;
+      }
+      const std::vector<F128> h = msg.to_host();
+      g = {h[0], h[1]};
+    } else {
+      if (on_device) {  // the tables move to the host once they are small
+        ha = am.to_host();
+        hb = bm.to_host();
+        am = DeviceVec<F128>();
+        bm = DeviceVec<F128>();
+        on_device = false;
+      }
+      if (i) {
+        host_fold(ha, rhos.back());
+        host_fold(hb, rhos.back());
+      }
+      std::vector<F128> eq(size_t(1) << d);
+      eq[0] = F128{1, 0};
+      for (size_t j = 0; j < d; ++j)
+        for (size_t k = 0; k < (size_t(1) << j); ++k) {
+          eq[k + (size_t(1) << j)] = flk::host_mul(eq[k], pt[j]);
+          eq[k] = flk::host_mul(eq[k], F128{pt[j].lo ^ 1, pt[j].hi});
+        }
+      F128 g1{0, 0}, gi{0, 0};
+      for (size_t x = 0; x < eq.size(); ++x) {
+        const F128 a0 = ha[2 * x], a1 = ha[2 * x + 1], b0 = hb[2 * x], b1 = hb[2 * x + 1];
+        g1 = flk::fadd(g1, flk::host_mul(eq[x], flk::host_mul(a1, b1)));
+        gi = flk::fadd(gi, flk::host_mul(eq[x], flk::host_mul(flk::fadd(a0, a1), flk::fadd(b0, b1))));
+      }
+      g = {g1, gi};
+    }
+    pf.rounds.push_back(g);
     ch.f128(g[0]);
     ch.f128(g[1]);
     rhos.push_back(ch.sample());
   }
-  fold(rhos.back());
-  pf.final_a = am.to_host()[0];
-  pf.final_b = bm.to_host()[0];
+  if (on_device) {  // only when no round ran on the host (m - 6 rounds all with >= 8 eq coordinates: never)
+    ha = am.to_host();
+    hb = bm.to_host();
+  }
+  host_fold(ha, rhos.back());
+  host_fold(hb, rhos.back());
+  pf.final_a = ha[0];
+  pf.final_b = hb[0];
   ch.f128(pf.final_a);
   ch.f128(pf.final_b);
   cl.z = z;

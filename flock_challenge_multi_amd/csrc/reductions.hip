@@ -8,6 +8,8 @@
 //   flk_zc_fold_at_z        fold_at_z                     CF/zerocheck/multilinear.rs:575-597
 //   flk_zc_round_pair       round_pair (wire form)        CF/zerocheck/multilinear.rs:616-640,
 //                                                         CF/zerocheck.rs:1748-1781
+//   flk_zc_fold_round       fold_in_place_pair + round_pair against a split eq
+//                                                         CF/zerocheck/multilinear.rs:616-640,3194-3213
 //
 // These are XOR reductions of F2-linear maps over bit-packed witness data.  The
 // sums of known values selected by bits ("sum eq[i] over the set bits") use
@@ -433,6 +435,72 @@ __global__ void __launch_bounds__(RD_WG) zc_round_pair_kernel(const uint4* __res
   wg_reduce_store(f_reduce(ginf), red[1], out, 1);
 }
 
+// ---------------------------------------------------------------------------
+// Zerocheck multilinear round with the bind of the previous round fused in and
+// the eq table split: T' = fold(T, rho) (when FOLD; T' = T otherwise) is written
+// and the wire message over T' is summed, G(1) = sum_x eq[x] a1 b1 and
+// G(inf) = sum_x eq[x] (a0+a1)(b0+b1) on the pairs (2x, 2x+1) of T', with
+// eq[x] = e_lo[x mod 256] * e_hi[x >> 8].  A work-group iteration takes the 256
+// x of one chunk x >> 8, so e_hi[chunk] is one value for the work-group (its
+// nibble-position tables are built once per chunk) and e_lo[lane] one value per
+// lane, applied once to the lane's sums at the end.  No eq table is written.
+// ---------------------------------------------------------------------------
+constexpr int ZF_WG = 256;
+
+template <bool FOLD>
+__global__ void __launch_bounds__(ZF_WG) zc_fold_round_kernel(const uint4* __restrict__ a, const uint4* __restrict__ b,
+                                                             uint64_t n_chunks, uint4 rho,
+                                                             const uint4* __restrict__ e_lo,
+                                                             const uint4* __restrict__ e_hi, uint4* __restrict__ a_out,
+                                                             uint4* __restrict__ b_out, uint4* __restrict__ out) {
+  __shared__ uint4 tabs[16 * ZF_WG];  // per-lane tables of the data x data products
+  __shared__ uint4 rtab[128], rnib[32];
+  __shared__ uint4 htab[128], hnib[32];
+  __shared__ uint4 red[2][ZF_WG / 64];
+  const uint32_t tid = threadIdx.x;
+  if (FOLD) build_pos_tables(rtab, rnib, 1, [&](uint32_t) { return rho; });
+  U256 g1, gi;
+  u256_zero(g1);
+  u256_zero(gi);
+  uint4* my = tabs + tid;
+  for (uint64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const uint4 eh = e_hi[c];
+    build_pos_tables(htab, hnib, 1, [&](uint32_t) { return eh; });  // two barriers: the last chunk's lookups are done
+    const uint64_t x = c * ZF_WG + tid;
+    uint4 a0, a1, b0, b1;
+    if (FOLD) {
+      const uint4* ap = a + 4 * x;
+      const uint4* bp = b + 4 * x;
+      const uint4 u0 = ap[0], u1 = ap[1], u2 = ap[2], u3 = ap[3];
+      const uint4 v0 = bp[0], v1 = bp[1], v2 = bp[2], v3 = bp[3];
+      a0 = combjr_mul_add(f_add(u0, u1), u0, rtab);  // T'[j] = T[2j] + rho (T[2j] + T[2j+1])
+      a1 = combjr_mul_add(f_add(u2, u3), u2, rtab);
+      b0 = combjr_mul_add(f_add(v0, v1), v0, rtab);
+      b1 = combjr_mul_add(f_add(v2, v3), v2, rtab);
+      a_out[2 * x] = a0;
+      a_out[2 * x + 1] = a1;
+      b_out[2 * x] = b0;
+      b_out[2 * x + 1] = b1;
+    } else {
+      a0 = a[2 * x];
+      a1 = a[2 * x + 1];
+      b0 = b[2 * x];
+      b1 = b[2 * x + 1];
+    }
+    const uint4 ea1 = combjr_mul_add(a1, f_zero(), htab);
+    const uint4 eas = combjr_mul_add(f_add(a0, a1), f_zero(), htab);
+    table_store_private(tabs, ZF_WG, tid, ea1);
+    comb_mul_acc_rt(b1, my, ZF_WG, g1);
+    table_store_private(tabs, ZF_WG, tid, eas);
+    comb_mul_acc_rt(f_add(b0, b1), my, ZF_WG, gi);
+  }
+  table_store_private(tabs, ZF_WG, tid, e_lo[tid]);  // the lane's own table: no barrier needed
+  const uint4 r1 = comb_mul_rt(f_reduce(g1), my, ZF_WG);
+  const uint4 ri = comb_mul_rt(f_reduce(gi), my, ZF_WG);
+  wg_reduce_store(r1, red[0], out, 0);
+  wg_reduce_store(ri, red[1], out, 1);
+}
+
 static unsigned grid_for(uint64_t items, int wg, unsigned cap) {
   uint64_t g = (items + wg - 1) / wg;
   if (g > cap) g = cap;
@@ -442,6 +510,7 @@ static unsigned grid_for(uint64_t items, int wg, unsigned cap) {
 static inline uint4 u4(flk_f128 a) {
   return make_uint4((uint32_t)a.lo, (uint32_t)(a.lo >> 32), (uint32_t)a.hi, (uint32_t)(a.hi >> 32));
 }
+static inline uint4 f_zero_host() { return make_uint4(0u, 0u, 0u, 0u); }
 
 int rs_fold_1b_rows_split(const flk_f128* z, const flk_f128* eq_lo, size_t lo_log, const flk_f128* eq_hi, size_t hi_log,
                           flk_f128* out128, hipStream_t st) {
@@ -565,6 +634,38 @@ int flk_zc_fold_at_z(const uint8_t* bits, size_t m, size_t k_skip, const flk_f12
                      reinterpret_cast<uint4*>(out));
   FLK_LAUNCH_CHECK("zc_fold_at_z_kernel");
   return FLK_OK;
+}
+
+int flk_zc_fold_round(const flk_f128* a, const flk_f128* b, size_t n, const flk_f128* rho, const flk_f128* eq_lo,
+                      const flk_f128* eq_hi, size_t hi_log, flk_f128* a_out, flk_f128* b_out, flk_f128* out2,
+                      void* stream) {
+  int rc = check_enabled();
+  if (rc) return rc;
+  if (!a || !b || !eq_lo || !eq_hi || !out2 || (rho && (!a_out || !b_out)))
+    return set_error(FLK_E_SHAPE, "zc_fold_round: null pointer");
+  if (hi_log > 32) return set_error(FLK_E_SHAPE, "zc_fold_round: hi_log=%zu", hi_log);
+  const uint64_t n_pairs = uint64_t(ZF_WG) << hi_log;  // pairs of the (folded) table
+  if (n != (rho ? 4 * n_pairs : 2 * n_pairs))
+    return set_error(FLK_E_SHAPE, "zc_fold_round: n=%zu, expected %llu for hi_log=%zu%s", n,
+                     (unsigned long long)(rho ? 4 * n_pairs : 2 * n_pairs), hi_log, rho ? " with a fold" : "");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  uint4* o = reinterpret_cast<uint4*>(out2);
+  const uint64_t n_chunks = uint64_t(1) << hi_log;
+  const unsigned grid = (unsigned)std::min<uint64_t>(n_chunks, 2048);
+  uint4* parts = nullptr;
+  if ((rc = partials_alloc(&parts, grid, 2, o, st))) return rc;
+  const uint4 r = rho ? u4(*rho) : f_zero_host();
+  if (rho)
+    hipLaunchKernelGGL(zc_fold_round_kernel<true>, dim3(grid), dim3(ZF_WG), 0, st, reinterpret_cast<const uint4*>(a),
+                       reinterpret_cast<const uint4*>(b), n_chunks, r, reinterpret_cast<const uint4*>(eq_lo),
+                       reinterpret_cast<const uint4*>(eq_hi), reinterpret_cast<uint4*>(a_out),
+                       reinterpret_cast<uint4*>(b_out), parts);
+  else
+    hipLaunchKernelGGL(zc_fold_round_kernel<false>, dim3(grid), dim3(ZF_WG), 0, st, reinterpret_cast<const uint4*>(a),
+                       reinterpret_cast<const uint4*>(b), n_chunks, r, reinterpret_cast<const uint4*>(eq_lo),
+                       reinterpret_cast<const uint4*>(eq_hi), nullptr, nullptr, parts);
+  FLK_LAUNCH_CHECK("zc_fold_round_kernel");
+  return partials_sum_free(parts, grid, 2, o, st);
 }
 
 int flk_zc_round_pair(const flk_f128* a, const flk_f128* b, size_t n, const flk_f128* eq_rest, flk_f128* out2,

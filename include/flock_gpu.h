@@ -342,6 +342,16 @@ int flk_zc_fold_at_z(const uint8_t *bits, size_t m, size_t k_skip, const flk_f12
  *                          sum_x eq_rest[x] (a[2x]+a[2x+1]) (b[2x]+b[2x+1])}, eq_rest n/2. */
 int flk_zc_round_pair(const flk_f128 *a, const flk_f128 *b, size_t n, const flk_f128 *eq_rest,
                       flk_f128 *out2, void *stream);
+/* The next round with the previous bind fused in and the eq table split
+ * (fold_in_place_pair + round_pair, multilinear.rs:616-640,3194-3213): with
+ * rho, T' = fold(T, rho) (T'[j] = T[2j] + rho (T[2j] + T[2j+1]), n inputs, n/2
+ * outputs into a_out, b_out); without (rho = NULL) T' = T (n entries).  out2 is
+ * the wire message over T' as flk_zc_round_pair computes it with
+ * eq_rest[x] = eq_lo[x mod 256] * eq_hi[x >> 8]: eq_lo has 256 entries (the first
+ * eight coordinates), eq_hi 2^hi_log, and T' has 2 * 256 * 2^hi_log entries. */
+int flk_zc_fold_round(const flk_f128 *a, const flk_f128 *b, size_t n, const flk_f128 *rho, const flk_f128 *eq_lo,
+                      const flk_f128 *eq_hi, size_t hi_log, flk_f128 *a_out, flk_f128 *b_out, flk_f128 *out2,
+                      void *stream);
 
 /* Round 1 of the zerocheck (univariate skip over k_skip = 6): round1_naive
  * (univariate_skip.rs:90-152) for the protocol's challenge layout

@@ -115,6 +115,54 @@ def test_zc_round_pair(orc, dev, logn):
     assert same(from_dev(got), orc.zc_round_pair(a, b, eq_rest))
 
 
+def _fold_round(ta, tb, rho, eq_lo, eq_hi, hi_log):
+    import ctypes
+    from flock_challenge_multi_amd import _native as nat
+    n = ta.shape[0]
+    out = hp.f128_empty(2)
+    ao = bo = None
+    r = None
+    if rho is not None:
+        r = nat.F128.from_int(rho)
+        ao, bo = hp.f128_empty(n // 2), hp.f128_empty(n // 2)
+    p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+    nat.call("flk_zc_fold_round", p(ta), p(tb), n, ctypes.byref(r) if r is not None else None, p(eq_lo), p(eq_hi),
+             hi_log, p(ao), p(bo), p(out), hp._stream())
+    return out, ao, bo
+
+
+@pytest.mark.parametrize("hi_log,fold", [(0, False), (0, True), (3, True), (5, False), (12, True)])
+def test_zc_fold_round_matches_oracle(orc, dev, hi_log, fold):
+    """The fused bind + round with a split eq equals partial_eval_lsb then round_pair
+    against the full eq table (eq[x] = eq_lo[x mod 256] eq_hi[x >> 8])."""
+    _log(f"zc_fold_round hi_log={hi_log} fold={fold}")
+    rng = random.Random(hi_log * 2 + fold)
+    n_pairs = 256 << hi_log
+    n = 4 * n_pairs if fold else 2 * n_pairs
+    a, b = orc.fill(90 + hi_log, n), orc.fill(91 + hi_log, n)
+    pt = [rng.getrandbits(128) for _ in range(8 + hi_log)]
+    rho = rng.getrandbits(128) if fold else None
+    out, ao, bo = _fold_round(to_dev(a), to_dev(b), rho, hp.build_eq_table(pt[:8]), hp.build_eq_table(pt[8:]),
+                              hi_log)
+    torch.cuda.synchronize()
+    if fold:
+        a, b = orc.partial_eval_lsb(a, [rho]), orc.partial_eval_lsb(b, [rho])
+        assert same(from_dev(ao), a) and same(from_dev(bo), b)
+    assert same(from_dev(out), orc.zc_round_pair(a, b, orc.build_eq(pt)))
+
+
+def test_zc_fold_round_rejects_shapes(dev):
+    import ctypes
+    from flock_challenge_multi_amd import _native as nat
+    t = hp.f128_empty(1024)
+    p = ctypes.c_void_p(t.data_ptr())
+    lib = nat.load()
+    assert lib.flk_zc_fold_round(p, p, 1000, None, p, p, 0, None, None, p, None) == nat.FLK_E_SHAPE
+    assert lib.flk_zc_fold_round(p, p, 1024, ctypes.byref(nat.F128.from_int(3)), p, p, 0, None, None, p,
+                                 None) == nat.FLK_E_SHAPE
+    assert lib.flk_is_disabled() == 0
+
+
 def test_zerocheck_multilinear_rounds_chain(orc, dev):
     """fold at z, then round message + LSB bind per round with the eq table of the
     remaining challenges, as the prover chains them (CF/zerocheck.rs:1748-1781)."""
