@@ -45,6 +45,9 @@ inline int log2_exact(size_t n) {
 // in an arena scope is ordered on one stream.  stream_free recognises arena
 // blocks of any live arena, in or out of the scope.
 hipError_t stream_alloc(void** p, size_t bytes, hipStream_t st);
+// Copy bytes from the device to host memory and wait for st (a pinned bounce
+// buffer per thread for reads up to 1 MiB).
+hipError_t read_to_host(void* dst, const void* src, size_t bytes, hipStream_t st);
 // p[0 .. n) = 0 by a kernel on st (graph-capturable, unlike hipMemsetAsync).
 int zero_n(uint4* p, uint64_t n, hipStream_t st);
 hipError_t stream_free(void* p, hipStream_t st);

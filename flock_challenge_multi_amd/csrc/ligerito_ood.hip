@@ -41,8 +41,7 @@ int flk_ligerito_ood(flk_fs* fs, const flk_f128* f, flk_f128* b, size_t log_n, s
     if ((rc = flk_fs_sample_f128_vec(fs, log_n, z))) break;
     if ((rc = flk_build_eq(z, log_n, eq, st))) break;
     if ((rc = flk_sc_round_msg_and_eval(f, eq, n, red, st))) break;
-    e = hipMemcpyAsync(out3, red, sizeof out3, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    e = read_to_host(out3, red, sizeof out3, st);
     if (e != hipSuccess) {
       rc = device_failure(e, "ligerito_ood: readback");
       break;

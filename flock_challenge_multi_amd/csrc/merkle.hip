@@ -224,8 +224,7 @@ int flk_commit(const flk_f128* poly, size_t log_msg_cols, size_t log_num_interle
   rc = flk::merkle_tree_device(reinterpret_cast<const uint8_t*>(codeword_out), n_cw * 16, n_pos, hash_kind, tree_out, st);
   if (rc) return rc;
   if (root_out) {
-    hipError_t e = hipMemcpyAsync(root_out, tree_out + 32 * (2 * n_pos - 2), 32, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipError_t e = flk::read_to_host(root_out, tree_out + 32 * (2 * n_pos - 2), 32, st);
     if (e != hipSuccess) return flk::device_failure(e, "commit: root copy");
   }
   return FLK_OK;

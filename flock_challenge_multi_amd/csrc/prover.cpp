@@ -243,8 +243,7 @@ void lincheck_prove(Fs& ch, const flk_r1cs* r1cs, const uint8_t* z_stripe, size_
     n /= 2;
   }
   pf.z_partial.resize(n);
-  chk(flk_copy_to_host(pf.z_partial.data(), zv.data(), n * sizeof(F128), st), "copy z_partial");
-  chk(flk_synchronize(st), "synchronize");
+  chk(flk_read_to_host(pf.z_partial.data(), zv.data(), n * sizeof(F128), st), "read z_partial");
   ch.slice(pf.z_partial);
   cl.r_inner_skip = ch.sample();
   std::vector<F128> lam(size_t(1) << K_SKIP);

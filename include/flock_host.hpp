@@ -83,14 +83,13 @@ class DeviceVec {
   T back() const {
     if (!n_) throw std::out_of_range("DeviceVec::back: empty");
     T h;
-    check(flk_copy_to_host(&h, p_ + (n_ - 1), sizeof(T), stream_), "flk_copy_to_host");
-    check(flk_synchronize(stream_), "flk_synchronize");
+    check(flk_read_to_host(&h, p_ + (n_ - 1), sizeof(T), stream_), "flk_read_to_host");
     return h;
   }
   std::vector<T> to_host() const {
     std::vector<T> h(n_);
-    if (n_) check(flk_copy_to_host(h.data(), p_, n_ * sizeof(T), stream_), "flk_copy_to_host");
-    check(flk_synchronize(stream_), "flk_synchronize");
+    if (n_) check(flk_read_to_host(h.data(), p_, n_ * sizeof(T), stream_), "flk_read_to_host");
+    else check(flk_synchronize(stream_), "flk_synchronize");
     return h;
   }
 
@@ -377,8 +376,7 @@ inline std::vector<Hash> merkle_multi_proof(const DeviceVec<Hash>& tree, size_t 
         "merkle_multi_proof");
   std::vector<Hash> out(k);
   if (k) {
-    check(flk_copy_to_host(out.data(), tmp.data(), k * sizeof(Hash), stream), "flk_copy_to_host");
-    check(flk_synchronize(stream), "flk_synchronize");
+    check(flk_read_to_host(out.data(), tmp.data(), k * sizeof(Hash), stream), "flk_read_to_host");
   }
   return out;
 }
