@@ -128,7 +128,8 @@ SIGNATURES = {
     "flk_blake3_proof_bound": (_I, [_P, ctypes.POINTER(_SZ)]),
     "flk_inject_device_failures": (None, [ctypes.c_uint, ctypes.c_uint]),
     "flk_read_to_host": (_I, [_P, _P, _SZ, _P]),
-    "flk_zc_fold_round": (_I, [_P, _P, _SZ, _P, _P, _P, _SZ, _P, _P, _P, _P]),
+    "flk_zc_fold_round": (_I, [_P, _P, _SZ, _P, _P, _P, _SZ, _P, _P, _P, _P, _P]),
+    "flk_zc_fold_at_z_round": (_I, [_P, _P, _SZ, _P, _P, _P, _P, _P, _P, _P, _P]),
     "flk_blake3_setup_arena_stats": (_I, [_P] + [ctypes.POINTER(_SZ)] * 4),
     "flk_prover_pool_new": (_I, [ctypes.c_int, _SZ, _SZ, ctypes.POINTER(_P)]),
     "flk_prover_pool_free": (None, [_P]),

@@ -120,10 +120,17 @@ void zerocheck_prove(Fs& ch, const uint8_t* a_bits, const uint8_t* b_bits, const
   DeviceVec<F128> wd(ws, st);
   size_t n = size_t(1) << n_mlv;
   DeviceVec<F128> am(n, st), bm(n, st), msg(2, st);
-  chk(flk_zc_fold_at_z(a_bits, m, K_SKIP, wd.data(), am.data(), st), "zc_fold_at_z(a)");
-  chk(flk_zc_fold_at_z(b_bits, m, K_SKIP, wd.data(), bm.data(), st), "zc_fold_at_z(b)");
   const size_t d0 = n_mlv - 1;
-  DeviceVec<F128> eq_lo(256, st), eq_hi(d0 >= 8 ? size_t(1) << (d0 - 8) : 1, st);
+  // eq_lo: the round's first 8 eq coordinates; eq_hi (two buffers, alternating): the
+  // rest, handed from round to round by summing out the lowest coordinate.
+  DeviceVec<F128> eq_lo(256, st), eq_hi[2] = {DeviceVec<F128>(d0 >= 8 ? size_t(1) << (d0 - 8) : 1, st),
+                                               DeviceVec<F128>(d0 >= 9 ? size_t(1) << (d0 - 9) : 1, st)};
+  if (d0 >= 8) {
+    chk(flk_build_eq(r.data() + K_SKIP + 1 + 8, d0 - 8, eq_hi[0].data(), st), "build_eq(hi)");
+  } else {
+    chk(flk_zc_fold_at_z(a_bits, m, K_SKIP, wd.data(), am.data(), st), "zc_fold_at_z(a)");
+    chk(flk_zc_fold_at_z(b_bits, m, K_SKIP, wd.data(), bm.data(), st), "zc_fold_at_z(b)");
+  }
   std::vector<F128> rhos, ha, hb;  // ha, hb: the host tail's tables
   bool on_device = true;
   auto host_fold = [&](std::vector<F128>& t, F128 rho) {
@@ -137,16 +144,17 @@ void zerocheck_prove(Fs& ch, const uint8_t* a_bits, const uint8_t* b_bits, const
     const F128* pt = r.data() + K_SKIP + i + 1;
     std::array<F128, 2> g;
     if (on_device && d >= 8) {
+      DeviceVec<F128>& hi = eq_hi[i & 1];
+      DeviceVec<F128>& hi_next = eq_hi[(i + 1) & 1];
       chk(flk_build_eq(pt, 8, eq_lo.data(), st), "build_eq(lo)");
-      chk(flk_build_eq(pt + 8, d - 8, eq_hi.data(), st), "build_eq(hi)");
-      if (i == 0) {
-        chk(flk_zc_fold_round(am.data(), bm.data(), n, nullptr, eq_lo.data(), eq_hi.data(), d - 8, nullptr, nullptr,
-                              msg.data(), st),
-            "zc_fold_round");
+      if (i == 0) {  // the fold at z and the first message in one pass over the bits
+        chk(flk_zc_fold_at_z_round(a_bits, b_bits, m, wd.data(), eq_lo.data(), hi.data(), am.data(), bm.data(),
+                                   hi_next.data(), msg.data(), st),
+            "zc_fold_at_z_round");
       } else {
         DeviceVec<F128> an(n / 2, st), bn(n / 2, st);
-        chk(flk_zc_fold_round(am.data(), bm.data(), n, &rhos.back(), eq_lo.data(), eq_hi.data(), d - 8, an.data(),
-                              bn.data(), msg.data(), st),
+        chk(flk_zc_fold_round(am.data(), bm.data(), n, &rhos.back(), eq_lo.data(), hi.data(), d - 8, an.data(),
+                              bn.data(), hi_next.data(), msg.data(), st),
             "zc_fold_round");
         am = std::move(an);
         bm = std::move(bn);

@@ -446,29 +446,48 @@ __global__ void __launch_bounds__(RD_WG) zc_round_pair_kernel(const uint4* __res
 // lane, applied once to the lane's sums at the end.  No eq table is written.
 // ---------------------------------------------------------------------------
 constexpr int ZF_WG = 256;
+enum ZfMode { ZF_PLAIN = 0, ZF_FOLD = 1, ZF_BITS = 2 };
 
-template <bool FOLD>
+// MODE: ZF_PLAIN reads T; ZF_FOLD reads T and writes fold(T, rho); ZF_BITS reads
+// the 64-bit rows of the a and b bit vectors and writes T = fold_at_z(bits, w)
+// (the weights' nibble tables, w_nib, built per work-group).  eq_hi_next (when
+// not null) receives eq_hi with its lowest coordinate summed out, the next
+// round's eq_hi: eq_hi_next[y] = eq_hi[2y] + eq_hi[2y + 1].
+template <int MODE>
 __global__ void __launch_bounds__(ZF_WG) zc_fold_round_kernel(const uint4* __restrict__ a, const uint4* __restrict__ b,
                                                              uint64_t n_chunks, uint4 rho,
+                                                             const uint4* __restrict__ w,
                                                              const uint4* __restrict__ e_lo,
                                                              const uint4* __restrict__ e_hi, uint4* __restrict__ a_out,
-                                                             uint4* __restrict__ b_out, uint4* __restrict__ out) {
+                                                             uint4* __restrict__ b_out, uint4* __restrict__ e_hi_next,
+                                                             uint4* __restrict__ out) {
   __shared__ uint4 tabs[16 * ZF_WG];  // per-lane tables of the data x data products
   __shared__ uint4 rtab[128], rnib[32];
   __shared__ uint4 htab[128], hnib[32];
+  __shared__ uint4 wnib[MODE == ZF_BITS ? 16 * 16 : 1];  // 16 nibble positions x 16 subset sums of w
   __shared__ uint4 red[2][ZF_WG / 64];
   const uint32_t tid = threadIdx.x;
-  if (FOLD) build_pos_tables(rtab, rnib, 1, [&](uint32_t) { return rho; });
+  if (MODE == ZF_FOLD) build_pos_tables(rtab, rnib, 1, [&](uint32_t) { return rho; });
+  if (MODE == ZF_BITS) {
+    const uint32_t np = tid >> 4, k = tid & 15u;  // 256 lanes = 16 positions x 16 values
+    uint4 v = f_zero();
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if ((k >> j) & 1u) v = f_add(v, w[4 * np + j]);
+    wnib[tid] = v;
+    __syncthreads();
+  }
   U256 g1, gi;
   u256_zero(g1);
   u256_zero(gi);
   uint4* my = tabs + tid;
   for (uint64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
     const uint4 eh = e_hi[c];
+    if (e_hi_next && tid == 0 && !(c & 1)) e_hi_next[c >> 1] = f_add(eh, e_hi[c + 1]);
     build_pos_tables(htab, hnib, 1, [&](uint32_t) { return eh; });  // two barriers: the last chunk's lookups are done
     const uint64_t x = c * ZF_WG + tid;
     uint4 a0, a1, b0, b1;
-    if (FOLD) {
+    if (MODE == ZF_FOLD) {
       const uint4* ap = a + 4 * x;
       const uint4* bp = b + 4 * x;
       const uint4 u0 = ap[0], u1 = ap[1], u2 = ap[2], u3 = ap[3];
@@ -477,15 +496,24 @@ __global__ void __launch_bounds__(ZF_WG) zc_fold_round_kernel(const uint4* __res
       a1 = combjr_mul_add(f_add(u2, u3), u2, rtab);
       b0 = combjr_mul_add(f_add(v0, v1), v0, rtab);
       b1 = combjr_mul_add(f_add(v2, v3), v2, rtab);
-      a_out[2 * x] = a0;
-      a_out[2 * x + 1] = a1;
-      b_out[2 * x] = b0;
-      b_out[2 * x + 1] = b1;
+    } else if (MODE == ZF_BITS) {
+      // rows 2x and 2x + 1 of each bit vector: 16 bytes = one uint4
+      const uint4 ra = a[x], rb = b[x];
+      a0 = f_add(nibtab8_sum(ra.x, wnib), nibtab8_sum(ra.y, wnib + 128));
+      a1 = f_add(nibtab8_sum(ra.z, wnib), nibtab8_sum(ra.w, wnib + 128));
+      b0 = f_add(nibtab8_sum(rb.x, wnib), nibtab8_sum(rb.y, wnib + 128));
+      b1 = f_add(nibtab8_sum(rb.z, wnib), nibtab8_sum(rb.w, wnib + 128));
     } else {
       a0 = a[2 * x];
       a1 = a[2 * x + 1];
       b0 = b[2 * x];
       b1 = b[2 * x + 1];
+    }
+    if (MODE != ZF_PLAIN) {
+      a_out[2 * x] = a0;
+      a_out[2 * x + 1] = a1;
+      b_out[2 * x] = b0;
+      b_out[2 * x + 1] = b1;
     }
     const uint4 ea1 = combjr_mul_add(a1, f_zero(), htab);
     const uint4 eas = combjr_mul_add(f_add(a0, a1), f_zero(), htab);
@@ -636,9 +664,40 @@ int flk_zc_fold_at_z(const uint8_t* bits, size_t m, size_t k_skip, const flk_f12
   return FLK_OK;
 }
 
+static int zc_fold_round_launch(int mode, const void* a, const void* b, uint64_t hi_log, const flk_f128* rho,
+                                const flk_f128* w, const flk_f128* eq_lo, const flk_f128* eq_hi, flk_f128* a_out,
+                                flk_f128* b_out, flk_f128* eq_hi_next, flk_f128* out2, hipStream_t st) {
+  uint4* o = reinterpret_cast<uint4*>(out2);
+  const uint64_t n_chunks = uint64_t(1) << hi_log;
+  const unsigned grid = (unsigned)std::min<uint64_t>(n_chunks, 2048);
+  uint4* parts = nullptr;
+  int rc = partials_alloc(&parts, grid, 2, o, st);
+  if (rc) return rc;
+  const uint4 r = rho ? u4(*rho) : f_zero_host();
+  auto* ka = reinterpret_cast<const uint4*>(a);
+  auto* kb = reinterpret_cast<const uint4*>(b);
+  auto* kw = reinterpret_cast<const uint4*>(w);
+  auto* kl = reinterpret_cast<const uint4*>(eq_lo);
+  auto* kh = reinterpret_cast<const uint4*>(eq_hi);
+  auto* ao = reinterpret_cast<uint4*>(a_out);
+  auto* bo = reinterpret_cast<uint4*>(b_out);
+  auto* hn = hi_log ? reinterpret_cast<uint4*>(eq_hi_next) : nullptr;
+  if (mode == ZF_FOLD)
+    hipLaunchKernelGGL(zc_fold_round_kernel<ZF_FOLD>, dim3(grid), dim3(ZF_WG), 0, st, ka, kb, n_chunks, r, kw, kl, kh,
+                       ao, bo, hn, parts);
+  else if (mode == ZF_BITS)
+    hipLaunchKernelGGL(zc_fold_round_kernel<ZF_BITS>, dim3(grid), dim3(ZF_WG), 0, st, ka, kb, n_chunks, r, kw, kl, kh,
+                       ao, bo, hn, parts);
+  else
+    hipLaunchKernelGGL(zc_fold_round_kernel<ZF_PLAIN>, dim3(grid), dim3(ZF_WG), 0, st, ka, kb, n_chunks, r, kw, kl,
+                       kh, ao, bo, hn, parts);
+  FLK_LAUNCH_CHECK("zc_fold_round_kernel");
+  return partials_sum_free(parts, grid, 2, o, st);
+}
+
 int flk_zc_fold_round(const flk_f128* a, const flk_f128* b, size_t n, const flk_f128* rho, const flk_f128* eq_lo,
-                      const flk_f128* eq_hi, size_t hi_log, flk_f128* a_out, flk_f128* b_out, flk_f128* out2,
-                      void* stream) {
+                      const flk_f128* eq_hi, size_t hi_log, flk_f128* a_out, flk_f128* b_out, flk_f128* eq_hi_next,
+                      flk_f128* out2, void* stream) {
   int rc = check_enabled();
   if (rc) return rc;
   if (!a || !b || !eq_lo || !eq_hi || !out2 || (rho && (!a_out || !b_out)))
@@ -648,24 +707,23 @@ int flk_zc_fold_round(const flk_f128* a, const flk_f128* b, size_t n, const flk_
   if (n != (rho ? 4 * n_pairs : 2 * n_pairs))
     return set_error(FLK_E_SHAPE, "zc_fold_round: n=%zu, expected %llu for hi_log=%zu%s", n,
                      (unsigned long long)(rho ? 4 * n_pairs : 2 * n_pairs), hi_log, rho ? " with a fold" : "");
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  uint4* o = reinterpret_cast<uint4*>(out2);
-  const uint64_t n_chunks = uint64_t(1) << hi_log;
-  const unsigned grid = (unsigned)std::min<uint64_t>(n_chunks, 2048);
-  uint4* parts = nullptr;
-  if ((rc = partials_alloc(&parts, grid, 2, o, st))) return rc;
-  const uint4 r = rho ? u4(*rho) : f_zero_host();
-  if (rho)
-    hipLaunchKernelGGL(zc_fold_round_kernel<true>, dim3(grid), dim3(ZF_WG), 0, st, reinterpret_cast<const uint4*>(a),
-                       reinterpret_cast<const uint4*>(b), n_chunks, r, reinterpret_cast<const uint4*>(eq_lo),
-                       reinterpret_cast<const uint4*>(eq_hi), reinterpret_cast<uint4*>(a_out),
-                       reinterpret_cast<uint4*>(b_out), parts);
-  else
-    hipLaunchKernelGGL(zc_fold_round_kernel<false>, dim3(grid), dim3(ZF_WG), 0, st, reinterpret_cast<const uint4*>(a),
-                       reinterpret_cast<const uint4*>(b), n_chunks, r, reinterpret_cast<const uint4*>(eq_lo),
-                       reinterpret_cast<const uint4*>(eq_hi), nullptr, nullptr, parts);
-  FLK_LAUNCH_CHECK("zc_fold_round_kernel");
-  return partials_sum_free(parts, grid, 2, o, st);
+  return zc_fold_round_launch(rho ? ZF_FOLD : ZF_PLAIN, a, b, hi_log, rho, nullptr, eq_lo, eq_hi, a_out, b_out,
+                              eq_hi_next, out2, static_cast<hipStream_t>(stream));
+}
+
+int flk_zc_fold_at_z_round(const uint8_t* a_bits, const uint8_t* b_bits, size_t m, const flk_f128* weights,
+                           const flk_f128* eq_lo, const flk_f128* eq_hi, flk_f128* a_out, flk_f128* b_out,
+                           flk_f128* eq_hi_next, flk_f128* out2, void* stream) {
+  int rc = check_enabled();
+  if (rc) return rc;
+  if (!a_bits || !b_bits || !weights || !eq_lo || !eq_hi || !a_out || !b_out || !out2)
+    return set_error(FLK_E_SHAPE, "zc_fold_at_z_round: null pointer");
+  if (m < 6 + 1 + 8 || m > 40) return set_error(FLK_E_SHAPE, "zc_fold_at_z_round: m=%zu outside 15..40", m);
+  if (((uintptr_t)a_bits | (uintptr_t)b_bits) % 16)
+    return set_error(FLK_E_SHAPE, "zc_fold_at_z_round: bit vectors must be 16-byte aligned");
+  // 2^(m-6) rows of 64 bits; the pairs of rows are 2^(m-7) = 256 * 2^hi_log
+  return zc_fold_round_launch(ZF_BITS, a_bits, b_bits, m - 7 - 8, nullptr, weights, eq_lo, eq_hi, a_out, b_out,
+                              eq_hi_next, out2, static_cast<hipStream_t>(stream));
 }
 
 int flk_zc_round_pair(const flk_f128* a, const flk_f128* b, size_t n, const flk_f128* eq_rest, flk_f128* out2,

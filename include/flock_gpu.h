@@ -351,10 +351,20 @@ int flk_zc_round_pair(const flk_f128 *a, const flk_f128 *b, size_t n, const flk_
  * outputs into a_out, b_out); without (rho = NULL) T' = T (n entries).  out2 is
  * the wire message over T' as flk_zc_round_pair computes it with
  * eq_rest[x] = eq_lo[x mod 256] * eq_hi[x >> 8]: eq_lo has 256 entries (the first
- * eight coordinates), eq_hi 2^hi_log, and T' has 2 * 256 * 2^hi_log entries. */
+ * eight coordinates), eq_hi 2^hi_log, and T' has 2 * 256 * 2^hi_log entries.
+ * When hi_log >= 1 and eq_hi_next is not NULL it receives the next round's eq_hi
+ * (eq_hi with its lowest coordinate summed out: 2^(hi_log-1) entries). */
 int flk_zc_fold_round(const flk_f128 *a, const flk_f128 *b, size_t n, const flk_f128 *rho, const flk_f128 *eq_lo,
-                      const flk_f128 *eq_hi, size_t hi_log, flk_f128 *a_out, flk_f128 *b_out, flk_f128 *out2,
-                      void *stream);
+                      const flk_f128 *eq_hi, size_t hi_log, flk_f128 *a_out, flk_f128 *b_out, flk_f128 *eq_hi_next,
+                      flk_f128 *out2, void *stream);
+/* The first multilinear round from the bits (fold_at_z_naive then round_pair,
+ * multilinear.rs:575-640) for k_skip = 6: a_out, b_out = the fold at z of the
+ * 2^(m-6) rows of 64 bits with the 64 device weights (as flk_zc_fold_at_z), and
+ * out2 the message over them as flk_zc_fold_round without rho (hi_log = m - 15;
+ * eq_hi_next as there).  m in 15..40; bit vectors 16-byte aligned. */
+int flk_zc_fold_at_z_round(const uint8_t *a_bits, const uint8_t *b_bits, size_t m, const flk_f128 *weights,
+                           const flk_f128 *eq_lo, const flk_f128 *eq_hi, flk_f128 *a_out, flk_f128 *b_out,
+                           flk_f128 *eq_hi_next, flk_f128 *out2, void *stream);
 
 /* Round 1 of the zerocheck (univariate skip over k_skip = 6): round1_naive
  * (univariate_skip.rs:90-152) for the protocol's challenge layout

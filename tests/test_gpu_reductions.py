@@ -115,7 +115,7 @@ def test_zc_round_pair(orc, dev, logn):
     assert same(from_dev(got), orc.zc_round_pair(a, b, eq_rest))
 
 
-def _fold_round(ta, tb, rho, eq_lo, eq_hi, hi_log):
+def _fold_round(ta, tb, rho, eq_lo, eq_hi, hi_log, with_next=False):
     import ctypes
     from flock_challenge_multi_amd import _native as nat
     n = ta.shape[0]
@@ -125,16 +125,18 @@ def _fold_round(ta, tb, rho, eq_lo, eq_hi, hi_log):
     if rho is not None:
         r = nat.F128.from_int(rho)
         ao, bo = hp.f128_empty(n // 2), hp.f128_empty(n // 2)
+    nxt = hp.f128_empty(max(1, (1 << hi_log) // 2)) if with_next else None
     p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
     nat.call("flk_zc_fold_round", p(ta), p(tb), n, ctypes.byref(r) if r is not None else None, p(eq_lo), p(eq_hi),
-             hi_log, p(ao), p(bo), p(out), hp._stream())
-    return out, ao, bo
+             hi_log, p(ao), p(bo), p(nxt), p(out), hp._stream())
+    return out, ao, bo, nxt
 
 
 @pytest.mark.parametrize("hi_log,fold", [(0, False), (0, True), (3, True), (5, False), (12, True)])
 def test_zc_fold_round_matches_oracle(orc, dev, hi_log, fold):
     """The fused bind + round with a split eq equals partial_eval_lsb then round_pair
-    against the full eq table (eq[x] = eq_lo[x mod 256] eq_hi[x >> 8])."""
+    against the full eq table (eq[x] = eq_lo[x mod 256] eq_hi[x >> 8]); the next
+    round's eq_hi is eq_hi with its lowest coordinate summed out."""
     _log(f"zc_fold_round hi_log={hi_log} fold={fold}")
     rng = random.Random(hi_log * 2 + fold)
     n_pairs = 256 << hi_log
@@ -142,13 +144,42 @@ def test_zc_fold_round_matches_oracle(orc, dev, hi_log, fold):
     a, b = orc.fill(90 + hi_log, n), orc.fill(91 + hi_log, n)
     pt = [rng.getrandbits(128) for _ in range(8 + hi_log)]
     rho = rng.getrandbits(128) if fold else None
-    out, ao, bo = _fold_round(to_dev(a), to_dev(b), rho, hp.build_eq_table(pt[:8]), hp.build_eq_table(pt[8:]),
-                              hi_log)
+    out, ao, bo, nxt = _fold_round(to_dev(a), to_dev(b), rho, hp.build_eq_table(pt[:8]), hp.build_eq_table(pt[8:]),
+                                   hi_log, with_next=hi_log > 0)
     torch.cuda.synchronize()
     if fold:
         a, b = orc.partial_eval_lsb(a, [rho]), orc.partial_eval_lsb(b, [rho])
         assert same(from_dev(ao), a) and same(from_dev(bo), b)
     assert same(from_dev(out), orc.zc_round_pair(a, b, orc.build_eq(pt)))
+    if hi_log > 0:
+        assert same(from_dev(nxt), orc.build_eq(pt[9:]))
+
+
+@pytest.mark.parametrize("m", [15, 16, 22])
+def test_zc_fold_at_z_round_matches_oracle(orc, dev, m):
+    """The first round straight from the bits: the tables equal the oracle's fold at
+    z and the message its round pair against the full eq table."""
+    import ctypes
+    from flock_challenge_multi_amd import _native as nat
+    _log(f"zc_fold_at_z_round m={m}")
+    rng = random.Random(m)
+    abits, bbits = _rand_bytes(m + 7, (1 << m) // 8), _rand_bytes(m + 8, (1 << m) // 8)
+    w = orc.zc_lagrange_weights(6, rng.getrandbits(128))
+    hi_log = m - 15
+    pt = [rng.getrandbits(128) for _ in range(8 + hi_log)]
+    ta, tb = torch.from_numpy(abits).cuda(), torch.from_numpy(bbits).cuda()
+    ao, bo, out = hp.f128_empty(1 << (m - 6)), hp.f128_empty(1 << (m - 6)), hp.f128_empty(2)
+    nxt = hp.f128_empty(max(1, (1 << hi_log) // 2))
+    lo, hi, wd = hp.build_eq_table(pt[:8]), hp.build_eq_table(pt[8:]), to_dev(w)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    nat.call("flk_zc_fold_at_z_round", p(ta), p(tb), m, p(wd), p(lo), p(hi), p(ao), p(bo),
+             p(nxt) if hi_log else None, p(out), hp._stream())
+    torch.cuda.synchronize()
+    a, b = orc.zc_fold_at_z(abits, m, 6, w), orc.zc_fold_at_z(bbits, m, 6, w)
+    assert same(from_dev(ao), a) and same(from_dev(bo), b)
+    assert same(from_dev(out), orc.zc_round_pair(a, b, orc.build_eq(pt)))
+    if hi_log:
+        assert same(from_dev(nxt), orc.build_eq(pt[9:]))
 
 
 def test_zc_fold_round_rejects_shapes(dev):
@@ -157,9 +188,10 @@ def test_zc_fold_round_rejects_shapes(dev):
     t = hp.f128_empty(1024)
     p = ctypes.c_void_p(t.data_ptr())
     lib = nat.load()
-    assert lib.flk_zc_fold_round(p, p, 1000, None, p, p, 0, None, None, p, None) == nat.FLK_E_SHAPE
-    assert lib.flk_zc_fold_round(p, p, 1024, ctypes.byref(nat.F128.from_int(3)), p, p, 0, None, None, p,
+    assert lib.flk_zc_fold_round(p, p, 1000, None, p, p, 0, None, None, None, p, None) == nat.FLK_E_SHAPE
+    assert lib.flk_zc_fold_round(p, p, 1024, ctypes.byref(nat.F128.from_int(3)), p, p, 0, None, None, None, p,
                                  None) == nat.FLK_E_SHAPE
+    assert lib.flk_zc_fold_at_z_round(p, p, 14, p, p, p, p, p, None, p, None) == nat.FLK_E_SHAPE
     assert lib.flk_is_disabled() == 0
 
 
