@@ -5,8 +5,8 @@ the restated verifier (tests/flock_verifier.py) and the oracle.
   configs 0 and 2   generate_compressions(12, seed) (m = 26) and (18, seed) (m = 32):
                     the proof is accepted, including the commitment root recomputed
                     on the CPU from the regenerated witness, and stays under 500,000 B
-  oracle prover     the proof equals the oracle CPU prover's bytes (config 0, a padded
-                    m = 22 batch)
+  oracle prover     the proof equals the oracle CPU prover's bytes (a padded m = 22
+                    batch, config 0 at m = 26, config 2 at m = 32: the bench's first seed)
   determinism       the same inputs give the same bytes (smallest-nonce grinds), on
                     another stream too; the golden fixture is reproduced exactly
   stages            round 1's messages equal the oracle's round1_naive at the
@@ -58,8 +58,8 @@ def test_proof_verifies(orc, log2, seed):
     fv.verify_proof(orc, proof, comps.view(COMPRESSION_DT).reshape(-1), log2)
 
 
-@pytest.mark.parametrize("log2,seed,n", [(8, 0xA11CE, 200), (12, 0x00C0FFEEBEEFD15C, None)],
-                         ids=["m22_padded200", "config0_m26"])
+@pytest.mark.parametrize("log2,seed,n", [(8, 0xA11CE, 200), (12, 0x00C0FFEEBEEFD15C, None), (18, 0x5EED0000, None)],
+                         ids=["m22_padded200", "config0_m26", "config2_m32"])
 def test_proof_equals_oracle_prover(orc, log2, seed, n):
     """The device proof equals, byte for byte, the proof of the oracle's CPU prover
     (tests/oracle_prover.py) on the same compressions."""
@@ -67,7 +67,8 @@ def test_proof_equals_oracle_prover(orc, log2, seed, n):
     print(f"case oracle-prover log2={log2} seed={seed:#x} n={n}", flush=True)
     comps, blocks = _blocks(log2, seed, n)
     proof, _ = setup_for(len(comps)).prove(blocks)
-    want, _ = op.prove(orc, comps.view(COMPRESSION_DT).reshape(-1), log2, threads=8)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    want, _ = op.prove(orc, comps.view(COMPRESSION_DT).reshape(-1), log2, threads=threads)
     assert proof == want
 
 
