@@ -19,18 +19,20 @@
 // like P_AB, with the row's bits c_x[s] as 0/1 bytes in place of LDE values, and
 // a 64-thread epilogue applies M.
 //
-// A work-group processes three 128-row blocks (three x_outer) side by side; per block:
+// A work-group processes four 128-row blocks (four x_outer) side by side; per block:
 //   phase 1, lane = row (K = lane & 7, J = lane >> 3): the 64-point LDE of the
 //     row's a and b bits (a fixed GF(2)-linear map 64 bits -> 64 bytes, applied
-//     with 16 nibble tables of 64 bytes), v_ab = a_l * b_l in GF(2^8) by
-//     split-nibble product tables (two lookups), and c's bits as bytes; both
-//     stored to LDS;
+//     with 16 nibble tables of 64 bytes), v_ab = a_l * b_l in GF(2^8) from the
+//     full 64 KiB product table (one lookup of a * 256 + b, the index formed by
+//     one byte permute; FLOCK_R1_FULLTAB=0 selects split-nibble tables, two
+//     lookups), and c's bits as bytes; both stored to LDS;
 //   phase 2a, lane = (byte word w, which, J-group): Horner over K for 4 J values
 //     and 4 l at once (packed bytes), phi8 by two nibble lookups, shifted by J;
 //   phase 2b, lane = (l, which): the sum over the four J-groups, then one
 //     product by eq_outer with nibble-pair tables (comb4p_mul_add).
-// Every data-indexed LDS table has 16 entries of 16 bytes or 256 single bytes,
-// or is laid out so (bank conflicts cost more than the extra lookups).
+// Every data-indexed 16-byte LDS table has 16 entries (conflict-free); the byte
+// product table takes one lookup per product where the split tables take two
+// and 2.3x the address arithmetic, which outweighs its bank conflicts.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
@@ -47,6 +49,7 @@ struct Round1Tables {
   uint32_t lde[16 * 16 * 16];  // [nibble pos p][value v][word w]: bytes l = 4w..4w+3
   uint8_t mul_lo[256 * 16];    // [a][n] = a * n        in GF(2^8)
   uint8_t mul_hi[256 * 16];    // [a][n] = a * (n << 4) in GF(2^8)
+  uint8_t mul64[256 * 256];    // [a][b] = a * b in GF(2^8)
   uint8_t M[64 * 64];          // [s][l]: the LDE of the unit vector e_s at point l
   uint4 phi_nib[32];           // phi8(n), then phi8(n << 4), n < 16
   uint4 phi_pow[8];            // phi8(2^i)
@@ -134,6 +137,8 @@ static void build_round1_tables(Round1Tables& T) {
       T.mul_lo[a * 16 + n] = g8mul((uint8_t)a, (uint8_t)n);
       T.mul_hi[a * 16 + n] = g8mul((uint8_t)a, (uint8_t)(n << 4));
     }
+  for (int a = 0; a < 256; ++a)
+    for (int b = 0; b < 256; ++b) T.mul64[a * 256 + b] = g8mul((uint8_t)a, (uint8_t)b);
   memcpy(T.M, M, sizeof(T.M));
   auto as_u4 = [](flk_f128 f) {
     return make_uint4((uint32_t)f.lo, (uint32_t)(f.lo >> 32), (uint32_t)f.hi, (uint32_t)(f.hi >> 32));
@@ -189,9 +194,12 @@ static const Round1Tables* round1_tables_device(int* status) {
   return d;
 }
 
-constexpr int R1_ROWS = 128;             // rows of one block (one x_outer)
-constexpr int R1_SUB = 3;                // blocks a work-group processes per iteration
-constexpr int R1_WG = R1_ROWS * R1_SUB;  // 6 waves; ~75 KiB of LDS -> 2 work-groups per CU
+constexpr int R1_ROWS = 128;  // rows of one block (one x_outer)
+// Blocks a work-group processes per iteration: with the split-nibble product
+// tables (8 KiB) three (6 waves, ~75 KiB of LDS, 2 work-groups per CU); with
+// the full 64 KiB product table four (8 waves, ~146 KiB, 1 work-group per CU).
+template <bool FULL>
+constexpr int r1_sub() { return FULL ? 4 : 3; }
 
 __device__ __forceinline__ uint32_t xtime4(uint32_t w) {
   return ((w & 0x7F7F7F7Fu) << 1) ^ (((w >> 7) & 0x01010101u) * 0x1Bu);
@@ -200,25 +208,38 @@ __device__ __forceinline__ uint32_t xtime4(uint32_t w) {
 // Work-group = R1_SUB blocks side by side (lanes sub * 128 + t), sharing the
 // tables; each block follows the phases described at the top of the file.
 // out: work-group partials, out[k * gridDim.x + blockIdx.x] for the 128 outputs k.
-__global__ void __launch_bounds__(R1_WG) zc_round1_kernel(const uint64_t* __restrict__ A, const uint64_t* __restrict__ B,
-                                                         const uint64_t* __restrict__ C, uint64_t n_blocks,
-                                                         const uint4* __restrict__ eq_outer,
-                                                         const Round1Tables* __restrict__ T, uint4* __restrict__ out) {
+template <bool FULL>
+__global__ void __launch_bounds__(r1_sub<FULL>() * R1_ROWS) zc_round1_kernel(
+    const uint64_t* __restrict__ A, const uint64_t* __restrict__ B, const uint64_t* __restrict__ C, uint64_t n_blocks,
+    const uint4* __restrict__ eq_outer, const Round1Tables* __restrict__ T, uint4* __restrict__ out) {
+  constexpr int R1_SUB = r1_sub<FULL>(), R1_WG = R1_SUB * R1_ROWS;
   // LDE contributions, chunk-major: lde16[(p * 4 + q) * 16 + v] is bytes 16q..16q+15
   // of the 64-byte image of nibble value v at nibble position p.  For a fixed
   // (p, q) the 16 values sit in 16 distinct LDS bank groups, so the data-indexed
   // lookups of a wave never conflict (16 KiB).
-  __shared__ uint4 lde16[16 * 4 * 16];
-  __shared__ uint8_t mul_lo[256 * 16];  // a * n, a * (n << 4) in GF(2^8): two independent lookups
-  __shared__ uint8_t mul_hi[256 * 16];
-  __shared__ uint4 phin[32];  // phi8(n), phi8(n << 4): phi8(v) = phin[v & 15] ^ phin[16 + (v >> 4)]
-  // Per block: [row][ab words 0..15 | c words 16..31] with pitch RP; after phase
-  // 2a has read it, the same memory holds the partial sums ypart[(q*4 + j)*YP + t].
+  // One LDS block with the byte product tables first: their lookups then fit the
+  // 16-bit offset field of ds_read_u8 (no address add per lookup).
+  // Per block rowdata: [row][ab words 0..15 | c words 16..31] with pitch RP; after
+  // phase 2a has read it, the same memory holds the partial sums ypart[(q*4 + j)*YP + t].
   constexpr uint32_t RP = 33, YP = R1_ROWS + 16;  // pitches keep every access free of bank conflicts
   static_assert(20 * YP <= R1_ROWS * RP, "ypart must fit in rowdata");
   static_assert((R1_SUB - 1) * R1_ROWS * 4 <= R1_ROWS * RP, "the sub-block sums must fit in rowdata[0]");
-  __shared__ uint32_t rowdata[R1_SUB][R1_ROWS * RP];
-  __shared__ uint4 etab[R1_SUB][32];  // eq_outer * n and eq_outer * x^4 * n (comb4p_mul_add)
+  struct alignas(16) Lds {
+    // FULL: mul[a * 256 + b] = a * b (one lookup per product); else
+    // mul[a * 16 + n] = a * n and mul[4096 + a * 16 + n] = a * (n << 4)
+    uint8_t mul[FULL ? 65536 : 8192];
+    uint4 lde16[16 * 4 * 16];
+    uint4 phin[32];  // phi8(n), phi8(n << 4): phi8(v) = phin[v & 15] ^ phin[16 + (v >> 4)]
+    uint4 etab[R1_SUB][32];  // eq_outer * n and eq_outer * x^4 * n (comb4p_mul_add)
+    uint32_t rowdata[R1_SUB][R1_ROWS * RP];
+  };
+  __shared__ Lds L;
+  uint4* const lde16 = L.lde16;
+  uint8_t* const mul_lo = L.mul;
+  uint8_t* const mul_hi = L.mul + 4096;
+  uint4* const phin = L.phin;
+  auto& rowdata = L.rowdata;
+  auto& etab = L.etab;
   const uint32_t tid = threadIdx.x, sub = tid / R1_ROWS, t = tid % R1_ROWS;
   uint32_t* const rows = rowdata[sub];
   uint32_t* const ypart = rows;
@@ -228,11 +249,16 @@ __global__ void __launch_bounds__(R1_WG) zc_round1_kernel(const uint64_t* __rest
       const uint32_t p = i >> 6, q = (i >> 4) & 3u, v = i & 15u;
       lde16[i] = src[(p * 16 + v) * 4 + q];
     }
-    const uint32_t* mlo = reinterpret_cast<const uint32_t*>(T->mul_lo);
-    const uint32_t* mhi = reinterpret_cast<const uint32_t*>(T->mul_hi);
-    for (uint32_t i = tid; i < 1024; i += R1_WG) {
-      reinterpret_cast<uint32_t*>(mul_lo)[i] = mlo[i];
-      reinterpret_cast<uint32_t*>(mul_hi)[i] = mhi[i];
+    if (FULL) {
+      const uint4* m64 = reinterpret_cast<const uint4*>(T->mul64);
+      for (uint32_t i = tid; i < 4096; i += R1_WG) reinterpret_cast<uint4*>(L.mul)[i] = m64[i];
+    } else {
+      const uint32_t* mlo = reinterpret_cast<const uint32_t*>(T->mul_lo);
+      const uint32_t* mhi = reinterpret_cast<const uint32_t*>(T->mul_hi);
+      for (uint32_t i = tid; i < 1024; i += R1_WG) {
+        reinterpret_cast<uint32_t*>(mul_lo)[i] = mlo[i];
+        reinterpret_cast<uint32_t*>(mul_hi)[i] = mhi[i];
+      }
     }
     if (tid < 32) phin[tid] = T->phi_nib[tid];
   }
@@ -276,8 +302,14 @@ __global__ void __launch_bounds__(R1_WG) zc_round1_kernel(const uint64_t* __rest
       uint32_t v = 0;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const uint32_t x16 = ((ah[q] >> (8 * j)) & 255u) << 4, y = (bh[q] >> (8 * j)) & 255u;
-        v |= (uint32_t)(mul_lo[x16 | (y & 15u)] ^ mul_hi[x16 | (y >> 4)]) << (8 * j);
+        if (FULL) {
+          // index (byte j of ah) * 256 + (byte j of bh) in one byte permute
+          const uint32_t idx = __builtin_amdgcn_perm(ah[q], bh[q], 0x0c0c0000u | ((4u + j) << 8) | (uint32_t)j);
+          v |= (uint32_t)L.mul[idx] << (8 * j);
+        } else {
+          const uint32_t x16 = ((ah[q] >> (8 * j)) & 255u) << 4, y = (bh[q] >> (8 * j)) & 255u;
+          v |= (uint32_t)(mul_lo[x16 | (y & 15u)] ^ mul_hi[x16 | (y >> 4)]) << (8 * j);
+        }
       }
       rd[q] = v;
       // bits 4q..4q+3 of c as the bytes 0/1 of one word (spread by a multiply:
@@ -416,18 +448,31 @@ extern "C" int flk_zc_round1(const uint8_t* a_bits, const uint8_t* b_bits, const
   rc = flk_build_eq(r_outer, m - 13, reinterpret_cast<flk_f128*>(eq), stream);
   if (rc) return rc;
   uint4* out = reinterpret_cast<uint4*>(out_ab);
-  // Two work-groups per CU (LDS): 512 fill the chip; 1024 leave room for
-  // uneven block counts per work-group.
-  const uint64_t n_iter = (n_outer + R1_SUB - 1) / R1_SUB;
-  const unsigned grid = (unsigned)(n_iter < 1024 ? n_iter : 1024);
+  // The full 64 KiB product table (one work-group of 4 blocks per CU; 4.72 ms at
+  // m = 32 against 6.03 ms) unless FLOCK_R1_FULLTAB=0 selects the split-nibble
+  // tables (two work-groups of 3 blocks per CU).  Grids: 512 work-groups
+  // (full) or 1024 (split) leave room for uneven block counts per work-group.
+  static const bool full = [] {
+    const char* v = getenv("FLOCK_R1_FULLTAB");
+    return !(v && v[0] == '0');
+  }();
+  const int sub = full ? r1_sub<true>() : r1_sub<false>();
+  const uint64_t n_iter = (n_outer + sub - 1) / sub;
+  const uint64_t cap = full ? 512 : 1024;
+  const unsigned grid = (unsigned)(n_iter < cap ? n_iter : cap);
   uint4* parts = nullptr;
   if ((rc = partials_alloc(&parts, grid, 128, out, st))) {
     (void)flk::stream_free(eq, st);
     return rc;
   }
-  hipLaunchKernelGGL(zc_round1_kernel, dim3(grid), dim3(R1_WG), 0, st, reinterpret_cast<const uint64_t*>(a_bits),
-                     reinterpret_cast<const uint64_t*>(b_bits), reinterpret_cast<const uint64_t*>(c_bits),
-                     (uint64_t)n_outer, eq, T, parts);
+  if (full)
+    hipLaunchKernelGGL(zc_round1_kernel<true>, dim3(grid), dim3(sub * R1_ROWS), 0, st,
+                       reinterpret_cast<const uint64_t*>(a_bits), reinterpret_cast<const uint64_t*>(b_bits),
+                       reinterpret_cast<const uint64_t*>(c_bits), (uint64_t)n_outer, eq, T, parts);
+  else
+    hipLaunchKernelGGL(zc_round1_kernel<false>, dim3(grid), dim3(sub * R1_ROWS), 0, st,
+                       reinterpret_cast<const uint64_t*>(a_bits), reinterpret_cast<const uint64_t*>(b_bits),
+                       reinterpret_cast<const uint64_t*>(c_bits), (uint64_t)n_outer, eq, T, parts);
   FLK_LAUNCH_CHECK("zc_round1_kernel");
   if ((rc = partials_sum_free(parts, grid, 128, out, st))) return rc;
   hipLaunchKernelGGL(zc_round1_lift_c_kernel, dim3(1), dim3(64), 0, st, out + 64, T);

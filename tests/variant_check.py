@@ -10,7 +10,8 @@ tests/test_gpu_variants.py with one setting each:
                                pair, lincheck round message and ring-switch
                                fold_1b_rows at 2^13 and 2^16;
   variant_check.py witness     the BLAKE3 witness (z, a, b, stripe) of 100 and
-                               1000 compressions in 2^7 and 2^10 slots.
+                               1000 compressions in 2^7 and 2^10 slots;
+  variant_check.py round1      zerocheck round 1 at m = 13, 16 and 20.
 
 Uses the C ABI through ctypes only (flk_alloc / flk_copy_*), no torch.  Exit
 status 0 means every check matched the oracle; the lines printed name each check.
@@ -177,11 +178,44 @@ def check_witness(lib, orc):
     return ok
 
 
+def check_round1(lib, orc):
+    """flk_zc_round1 at m = 13, 16, 20 (the protocol's fixed inner challenges) against
+    round1_naive, plus the all-ones rows of an honest-looking witness (a AND b = c)."""
+    import random
+    import flock_verifier as fv
+    ok = True
+    for m in (13, 16, 20):
+        rng = random.Random(m)
+        nb = (1 << m) // 8
+        a, b, c = (np.frombuffer(rng.randbytes(nb), np.uint8).copy() for _ in range(3))
+        if m == 16:
+            a[: nb // 4] = 0xFF
+            c = a & b
+        r_outer = [rng.getrandbits(128) for _ in range(m - 13)]
+        r = [rng.getrandbits(128) for _ in range(6)] + fv.fixed_challenges(orc) + r_outer
+        want_ab, want_c = orc.zc_round1(a, b, c, m, 6, r)
+        da, db, dc, dout = Dev(lib, nb), Dev(lib, nb), Dev(lib, nb), Dev(lib, 128 * 16)
+        da.put(a)
+        db.put(b)
+        dc.put(c)
+        ro = (nat.F128 * max(len(r_outer), 1))(*[nat.F128.from_int(v) for v in r_outer])
+        nat.check(lib.flk_zc_round1(da.p, db.p, dc.p, m, ro, dout.p, ctypes.c_void_p(dout.p.value + 64 * 16), None),
+                  "zc_round1")
+        got = dout.get(F128_DT, 128)
+        good = same(got[:64], want_ab) and same(got[64:], want_c)
+        print(f"zc_round1 m={m}: {'ok' if good else 'MISMATCH'}", flush=True)
+        ok = ok and good
+        for d in (da, db, dc, dout):
+            d.free()
+    return ok
+
+
 def main():
     lib = nat.load()
     orc = Oracle()
     mode = sys.argv[1] if len(sys.argv) > 1 else "ntt"
-    check = {"ntt": check_ntt, "reductions": check_reductions, "witness": check_witness}[mode]
+    check = {"ntt": check_ntt, "reductions": check_reductions, "witness": check_witness,
+             "round1": check_round1}[mode]
     ok = check(lib, orc)
     return 0 if ok else 1
 
