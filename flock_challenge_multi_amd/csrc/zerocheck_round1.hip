@@ -15,21 +15,23 @@
 //
 // P_C needs no per-row LDE: the LDE is GF(2^8)-linear and c is a bit, so
 //     P_C[l] = sum_s phi8(M[s][l]) * Q[s],   Q[s] = sum_x eq(x) c_x[s],
-// with M the 64 x 64 LDE matrix (S -> Lambda).  The kernel accumulates Q exactly
-// like P_AB, with the row's bits c_x[s] as 0/1 bytes in place of LDE values, and
-// a 64-thread epilogue applies M.
+// with M the 64 x 64 LDE matrix (S -> Lambda).  Q is a column sum of the bit
+// matrix c: the ring-switch fold_1b_rows kernel computes it over row pairs (c
+// read as 2^(m-7) elements of 128 bits, weighted by eq over r[7..m)), and a
+// 64-thread epilogue binds r[6] and applies M.
 //
-// A work-group processes four 128-row blocks (four x_outer) side by side; per block:
-//   phase 1, lane = row (K = lane & 7, J = lane >> 3): the 64-point LDE of the
-//     row's a and b bits (a fixed GF(2)-linear map 64 bits -> 64 bytes, applied
+// In the P_AB kernel one wave processes one 128-row block (one x_outer) at a
+// time; per block:
+//   phase 1, lane t = rows t and t + 64 (row r = (K, J) = (r & 7, r >> 3)): the
+//     64-point LDE of the row's a and b bits (a fixed GF(2)-linear map 64 bits -> 64 bytes, applied
 //     with 16 nibble tables of 64 bytes), v_ab = a_l * b_l in GF(2^8) from the
 //     full 64 KiB product table (one lookup of a * 256 + b, the index formed by
 //     one byte permute; FLOCK_R1_FULLTAB=0 selects split-nibble tables, two
-//     lookups), and c's bits as bytes; both stored to LDS;
-//   phase 2a, lane = (byte word w, which, J-group): Horner over K for 4 J values
-//     and 4 l at once (packed bytes), phi8 by two nibble lookups, shifted by J;
-//   phase 2b, lane = (l, which): the sum over the four J-groups, then one
-//     product by eq_outer with nibble-pair tables (comb4p_mul_add).
+//     lookups), stored to LDS;
+//   phase 2a, lane = (byte word w, J-group): Horner over K for 4 J values and
+//     4 l at once (packed bytes), phi8 by two nibble lookups, shifted by J;
+//   phase 2b, lane = l: the sum over the four J-groups, then one product by
+//     eq_outer with nibble-pair tables (comb4p_mul_add).
 // Every data-indexed 16-byte LDS table has 16 entries (conflict-free); the byte
 // product table takes one lookup per product where the split tables take two
 // and 2.3x the address arithmetic, which outweighs its bank conflicts.
@@ -151,6 +153,10 @@ static void build_round1_tables(Round1Tables& T) {
 }
 
 // C_s / D with C_s = phi8(0x1C), D = prod_{i<4} (1 + x^(2^i)).
+static inline uint4 u4_of(flk_f128 a) {
+  return make_uint4((uint32_t)a.lo, (uint32_t)(a.lo >> 32), (uint32_t)a.hi, (uint32_t)(a.hi >> 32));
+}
+
 static flk_f128 round1_scale() {
   flk_f128 d{1, 0};
   for (int i = 0; i < 4; ++i) d = host_mul(d, flk_f128{1ull ^ (1ull << (1u << i)), 0});
@@ -195,54 +201,64 @@ static const Round1Tables* round1_tables_device(int* status) {
 }
 
 constexpr int R1_ROWS = 128;  // rows of one block (one x_outer)
-// Blocks a work-group processes per iteration: with the split-nibble product
-// tables (8 KiB) three (6 waves, ~75 KiB of LDS, 2 work-groups per CU); with
-// the full 64 KiB product table four (8 waves, ~146 KiB, 1 work-group per CU).
+// Waves per work-group: each wave processes whole blocks on its own (no
+// work-group barrier inside the block loop), so the waves of a CU drift apart
+// and one wave's table lookups overlap another's arithmetic.  With the full
+// 64 KiB product table one work-group of 8 waves fills the LDS (154 KiB); with
+// the split-nibble tables (8 KiB) two work-groups of 6 waves (78 KiB each).
 template <bool FULL>
-constexpr int r1_sub() { return FULL ? 4 : 3; }
+constexpr int r1_waves() { return FULL ? 8 : 6; }
 
 __device__ __forceinline__ uint32_t xtime4(uint32_t w) {
   return ((w & 0x7F7F7F7Fu) << 1) ^ (((w >> 7) & 0x01010101u) * 0x1Bu);
 }
 
-// Work-group = R1_SUB blocks side by side (lanes sub * 128 + t), sharing the
-// tables; each block follows the phases described at the top of the file.
-// out: work-group partials, out[k * gridDim.x + blockIdx.x] for the 128 outputs k.
+// Orders the LDS accesses of one wave: the compiler moves no memory access
+// across it and the wave's outstanding LDS operations have completed, so data
+// one lane stored is visible to every lane of the wave afterwards.
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// Per-wave block scratch: row r's 16 product words at r * 17 + 16 * (r >> 5)
+// (the skew keeps the phase-2a column reads of the four J-groups in distinct
+// banks), later the phase-2a partial sums ypart[(q * 4 + j) * 80 + lane].
+constexpr uint32_t R1_RP = 17, R1_YP = 80;
+constexpr uint32_t R1_ROWWORDS = R1_ROWS * R1_RP + 48;
+__device__ __forceinline__ uint32_t r1_row(uint32_t r) { return r * R1_RP + 16u * (r >> 5); }
+
+// Each wave takes blocks blockIdx.x * NW + wave, then strides by gridDim.x * NW;
+// each block follows the phases described at the top of the file.
+// out: work-group partials, out[l * gridDim.x + blockIdx.x] for the 64 outputs l.
 template <bool FULL>
-__global__ void __launch_bounds__(r1_sub<FULL>() * R1_ROWS) zc_round1_kernel(
-    const uint64_t* __restrict__ A, const uint64_t* __restrict__ B, const uint64_t* __restrict__ C, uint64_t n_blocks,
+__global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
+    const uint64_t* __restrict__ A, const uint64_t* __restrict__ B, uint64_t n_blocks,
     const uint4* __restrict__ eq_outer, const Round1Tables* __restrict__ T, uint4* __restrict__ out) {
-  constexpr int R1_SUB = r1_sub<FULL>(), R1_WG = R1_SUB * R1_ROWS;
+  constexpr int NW = r1_waves<FULL>(), R1_WG = NW * 64;
+  static_assert(20 * R1_YP <= R1_ROWWORDS, "ypart must fit in the block scratch");
+  static_assert(64 * 4 <= R1_ROWWORDS, "a wave's sums must fit in its block scratch");
   // LDE contributions, chunk-major: lde16[(p * 4 + q) * 16 + v] is bytes 16q..16q+15
   // of the 64-byte image of nibble value v at nibble position p.  For a fixed
   // (p, q) the 16 values sit in 16 distinct LDS bank groups, so the data-indexed
   // lookups of a wave never conflict (16 KiB).
   // One LDS block with the byte product tables first: their lookups then fit the
   // 16-bit offset field of ds_read_u8 (no address add per lookup).
-  // Per block rowdata: [row][ab words 0..15 | c words 16..31] with pitch RP; after
-  // phase 2a has read it, the same memory holds the partial sums ypart[(q*4 + j)*YP + t].
-  constexpr uint32_t RP = 33, YP = R1_ROWS + 16;  // pitches keep every access free of bank conflicts
-  static_assert(20 * YP <= R1_ROWS * RP, "ypart must fit in rowdata");
-  static_assert((R1_SUB - 1) * R1_ROWS * 4 <= R1_ROWS * RP, "the sub-block sums must fit in rowdata[0]");
   struct alignas(16) Lds {
     // FULL: mul[a * 256 + b] = a * b (one lookup per product); else
     // mul[a * 16 + n] = a * n and mul[4096 + a * 16 + n] = a * (n << 4)
     uint8_t mul[FULL ? 65536 : 8192];
     uint4 lde16[16 * 4 * 16];
     uint4 phin[32];  // phi8(n), phi8(n << 4): phi8(v) = phin[v & 15] ^ phin[16 + (v >> 4)]
-    uint4 etab[R1_SUB][32];  // eq_outer * n and eq_outer * x^4 * n (comb4p_mul_add)
-    uint32_t rowdata[R1_SUB][R1_ROWS * RP];
+    uint4 etab[NW][32];  // eq_outer * n and eq_outer * x^4 * n (comb4p_mul_add)
+    uint32_t rowdata[NW][R1_ROWWORDS];
   };
   __shared__ Lds L;
   uint4* const lde16 = L.lde16;
   uint8_t* const mul_lo = L.mul;
   uint8_t* const mul_hi = L.mul + 4096;
   uint4* const phin = L.phin;
-  auto& rowdata = L.rowdata;
-  auto& etab = L.etab;
-  const uint32_t tid = threadIdx.x, sub = tid / R1_ROWS, t = tid % R1_ROWS;
-  uint32_t* const rows = rowdata[sub];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, t = tid & 63u;
+  uint32_t* const rows = L.rowdata[wave];
   uint32_t* const ypart = rows;
+  uint4* const etab = L.etab[wave];
   {
     const uint4* src = reinterpret_cast<const uint4*>(T->lde);  // [p][v][q]
     for (uint32_t i = tid; i < 16 * 4 * 16; i += R1_WG) {
@@ -262,79 +278,78 @@ __global__ void __launch_bounds__(r1_sub<FULL>() * R1_ROWS) zc_round1_kernel(
     }
     if (tid < 32) phin[tid] = T->phi_nib[tid];
   }
-  // phase-2a role: 16 words x 2 (ab, c) x 4 J-groups
-  const uint32_t w = t & 15u, which = (t >> 4) & 1u, jg = t >> 5;
+  // phase-2a role: 16 words x 4 J-groups of four J
+  const uint32_t w = t & 15u, jg = t >> 4;
   uint4 acc = f_zero();
-  __syncthreads();
+  __syncthreads();  // the tables are read-only from here; the waves run apart
 
-  for (uint64_t base = (uint64_t)blockIdx.x * R1_SUB; base < n_blocks; base += (uint64_t)gridDim.x * R1_SUB) {
-    const uint64_t blk = base + sub;
-    const bool live = blk < n_blocks;  // a dead block reads zeros and adds nothing
-    // ---- phase 1: one row per lane, row (K, J) = (t & 7, t >> 3) ----
-    const uint64_t row = blk * R1_ROWS + t;
-    const uint64_t a = live ? A[row] : 0ull, b = live ? B[row] : 0ull, c = live ? C[row] : 0ull;
-    uint32_t ah[16], bh[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) ah[q] = bh[q] = 0u;
-    // Rolled over nibble-position pairs: at most 16 table loads in flight.
+  for (uint64_t blk = (uint64_t)blockIdx.x * NW + wave; blk < n_blocks; blk += (uint64_t)gridDim.x * NW) {
+    // ---- phase 1: rows t and t + 64, row r = (K, J) = (r & 7, r >> 3) ----
 #pragma unroll 1
-    for (uint32_t p = 0; p < 16; p += 2) {
-      const uint32_t na0 = (uint32_t)(a >> (4 * p)) & 15u, na1 = (uint32_t)(a >> (4 * p + 4)) & 15u;
-      const uint32_t nb0 = (uint32_t)(b >> (4 * p)) & 15u, nb1 = (uint32_t)(b >> (4 * p + 4)) & 15u;
+    for (uint32_t half = 0; half < 2; ++half) {
+      const uint32_t r = t + 64u * half;
+      const uint64_t row = blk * R1_ROWS + r;
+      const uint64_t a = A[row], b = B[row];
+      uint32_t ah[16], bh[16];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const uint4* c0 = lde16 + (p * 4 + q) * 16;
-        const uint4* c1 = c0 + 64;
-        const uint4 x0 = c0[na0], x1 = c1[na1], y0 = c0[nb0], y1 = c1[nb1];
-        ah[4 * q] = xor3(ah[4 * q], x0.x, x1.x);
-        ah[4 * q + 1] = xor3(ah[4 * q + 1], x0.y, x1.y);
-        ah[4 * q + 2] = xor3(ah[4 * q + 2], x0.z, x1.z);
-        ah[4 * q + 3] = xor3(ah[4 * q + 3], x0.w, x1.w);
-        bh[4 * q] = xor3(bh[4 * q], y0.x, y1.x);
-        bh[4 * q + 1] = xor3(bh[4 * q + 1], y0.y, y1.y);
-        bh[4 * q + 2] = xor3(bh[4 * q + 2], y0.z, y1.z);
-        bh[4 * q + 3] = xor3(bh[4 * q + 3], y0.w, y1.w);
-      }
-    }
-    uint32_t* rd = rows + t * RP;
+      for (int q = 0; q < 16; ++q) ah[q] = bh[q] = 0u;
+      // Rolled over nibble-position pairs: at most 16 table loads in flight.
+#pragma unroll 1
+      for (uint32_t p = 0; p < 16; p += 2) {
+        const uint32_t na0 = (uint32_t)(a >> (4 * p)) & 15u, na1 = (uint32_t)(a >> (4 * p + 4)) & 15u;
+        const uint32_t nb0 = (uint32_t)(b >> (4 * p)) & 15u, nb1 = (uint32_t)(b >> (4 * p + 4)) & 15u;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      uint32_t v = 0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (FULL) {
-          // index (byte j of ah) * 256 + (byte j of bh) in one byte permute
-          const uint32_t idx = __builtin_amdgcn_perm(ah[q], bh[q], 0x0c0c0000u | ((4u + j) << 8) | (uint32_t)j);
-          v |= (uint32_t)L.mul[idx] << (8 * j);
-        } else {
-          const uint32_t x16 = ((ah[q] >> (8 * j)) & 255u) << 4, y = (bh[q] >> (8 * j)) & 255u;
-          v |= (uint32_t)(mul_lo[x16 | (y & 15u)] ^ mul_hi[x16 | (y >> 4)]) << (8 * j);
+        for (int q = 0; q < 4; ++q) {
+          const uint4* c0 = lde16 + (p * 4 + q) * 16;
+          const uint4* c1 = c0 + 64;
+          const uint4 x0 = c0[na0], x1 = c1[na1], y0 = c0[nb0], y1 = c1[nb1];
+          ah[4 * q] = xor3(ah[4 * q], x0.x, x1.x);
+          ah[4 * q + 1] = xor3(ah[4 * q + 1], x0.y, x1.y);
+          ah[4 * q + 2] = xor3(ah[4 * q + 2], x0.z, x1.z);
+          ah[4 * q + 3] = xor3(ah[4 * q + 3], x0.w, x1.w);
+          bh[4 * q] = xor3(bh[4 * q], y0.x, y1.x);
+          bh[4 * q + 1] = xor3(bh[4 * q + 1], y0.y, y1.y);
+          bh[4 * q + 2] = xor3(bh[4 * q + 2], y0.z, y1.z);
+          bh[4 * q + 3] = xor3(bh[4 * q + 3], y0.w, y1.w);
         }
       }
-      rd[q] = v;
-      // bits 4q..4q+3 of c as the bytes 0/1 of one word (spread by a multiply:
-      // the four shifted copies of the nibble do not overlap)
-      rd[16 + q] = (((uint32_t)(c >> (4 * q)) & 15u) * 0x00204081u) & 0x01010101u;
+      uint32_t* rd = rows + r1_row(r);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (FULL) {
+            // index (byte j of ah) * 256 + (byte j of bh) in one byte permute
+            const uint32_t idx = __builtin_amdgcn_perm(ah[q], bh[q], 0x0c0c0000u | ((4u + j) << 8) | (uint32_t)j);
+            v |= (uint32_t)L.mul[idx] << (8 * j);
+          } else {
+            const uint32_t x16 = ((ah[q] >> (8 * j)) & 255u) << 4, y = (bh[q] >> (8 * j)) & 255u;
+            v |= (uint32_t)(mul_lo[x16 | (y & 15u)] ^ mul_hi[x16 | (y >> 4)]) << (8 * j);
+          }
+        }
+        rd[q] = v;
+      }
     }
     // eq_outer[blk] nibble tables for comb4p_mul_add
     if (t < 32) {
-      uint4 e = live ? eq_outer[blk] : f_zero();
+      uint4 e = eq_outer[blk];
       if (t >= 16) e = f_mul_x4(e);
-      etab[sub][t] = comb_entry(e, t & 15u);
+      etab[t] = comb_entry(e, t & 15u);
     }
-    __syncthreads();
-    // ---- phase 2a: lane (w, which, jg): Horner over K in alpha = 0x02 (xtime on
-    // packed bytes: 4 l per word) for the four J of its group ----
+    wave_lds_sync();
+    // ---- phase 2a: lane (w, jg): Horner over K in alpha = 0x02 (xtime on packed
+    // bytes: 4 l per word) for the four J of its group ----
     uint32_t wj[4];
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
-      const uint32_t* col = rows + (8 * (4 * jg + jj)) * RP + which * 16 + w;
+      const uint32_t* col = rows + r1_row(32u * jg + 8u * jj) + w;  // rows 32jg + 8jj + K share r >> 5
       uint32_t h = 0;
 #pragma unroll
-      for (int K = 7; K >= 0; --K) h = xtime4(h) ^ col[RP * K];
+      for (int K = 7; K >= 0; --K) h = xtime4(h) ^ col[R1_RP * K];
       wj[jj] = h;
     }
-    __syncthreads();  // every lane has read rowdata; ypart reuses it
+    wave_lds_sync();  // every lane has read the rows; ypart reuses them
     uint32_t Y[4][5];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
@@ -362,46 +377,50 @@ __global__ void __launch_bounds__(r1_sub<FULL>() * R1_ROWS) zc_round1_kernel(
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int q = 0; q < 5; ++q) ypart[(q * 4 + j) * YP + t] = Y[j][q];
-    __syncthreads();
-    // ---- phase 2b: lane (l, which2): finish the J sum, multiply by eq_outer ----
+      for (int q = 0; q < 5; ++q) ypart[(q * 4 + j) * R1_YP + t] = Y[j][q];
+    wave_lds_sync();
+    // ---- phase 2b: lane l: finish the J sum over the four J-groups, multiply by
+    // eq_outer ----
     {
-      const uint32_t l = t & 63u, wh = t >> 6, ww = l >> 2, j = l & 3u;
-      const uint32_t src = ww + 16 * wh;  // writer lane of J-group g is src + 32 g
+      const uint32_t l = t, ww = l >> 2, j = l & 3u;
       uint32_t y[8];
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
-        const uint32_t* yp = ypart + (q * 4 + j) * YP + src;
-        y[q] = xor3(yp[0], yp[32], yp[64]) ^ yp[96];
+        const uint32_t* yp = ypart + (q * 4 + j) * R1_YP + ww;  // writer lane of J-group g: ww + 16 g
+        y[q] = xor3(yp[0], yp[16], yp[32]) ^ yp[48];
       }
       y[5] = y[6] = y[7] = 0u;
-      acc = comb4p_mul_add(f_reduce_add(y, f_zero()), acc, etab[sub]);
+      acc = comb4p_mul_add(f_reduce_add(y, f_zero()), acc, etab);
     }
-    __syncthreads();
+    wave_lds_sync();  // the next block overwrites rows, ypart and etab
   }
-  // Sum the sub-blocks' accumulators for output t = which * 64 + l, then store
-  // the work-group's partial at out[t * gridDim.x + blockIdx.x].
-  uint4* const red = reinterpret_cast<uint4*>(rowdata[0]);
-  if (sub > 0) red[(sub - 1) * R1_ROWS + t] = acc;
+  // Sum the waves' accumulators for output l = t, then store the work-group's
+  // partial at out[l * gridDim.x + blockIdx.x].
   __syncthreads();
-  if (sub == 0) {
+  reinterpret_cast<uint4*>(rows)[t] = acc;
+  __syncthreads();
+  if (wave == 0) {
 #pragma unroll
-    for (int k = 1; k < R1_SUB; ++k) acc = f_add(acc, red[(k - 1) * R1_ROWS + t]);
+    for (int k = 1; k < NW; ++k) acc = f_add(acc, reinterpret_cast<const uint4*>(L.rowdata[k])[t]);
     out[(size_t)t * gridDim.x + blockIdx.x] = acc;
   }
 }
 
-// P_C[l] = sum_s phi8(M[s][l]) * Q[s] for the 64 accumulated Q values in out_c,
-// in place: R[s][i] = phi8(2^i) * Q[s], and lane l XORs the R[s][i] selected by
-// the bits of M[s][l].
-__global__ void __launch_bounds__(64) zc_round1_lift_c_kernel(uint4* __restrict__ out_c,
-                                                             const Round1Tables* __restrict__ T) {
-  __shared__ uint4 tabs[8 * 16];
+// P_C[l] = sum_s phi8(M[s][l]) * Q[s], written to out_c[l].  S holds the
+// column sums over row pairs (S[s]: even rows, S[64 + s]: odd rows, each
+// weighted by eq over r[7..m)), so Q[s] = S[s] + r6 * (S[s] + S[64 + s]).
+// R[s][i] = phi8(2^i) * Q[s], and lane l XORs the R[s][i] selected by the bits
+// of M[s][l].
+__global__ void __launch_bounds__(64) zc_round1_lift_c_kernel(const uint4* __restrict__ S, uint4 r6,
+                                                             const Round1Tables* __restrict__ T,
+                                                             uint4* __restrict__ out_c) {
+  __shared__ uint4 tabs[9 * 16];
   __shared__ uint4 R[64 * 8];
   const uint32_t t = threadIdx.x;
-  for (uint32_t e = t; e < 128; e += 64) tabs[e] = comb_entry(T->phi_pow[e >> 4], e & 15u);
-  const uint4 q = out_c[t];
+  for (uint32_t e = t; e < 144; e += 64) tabs[e] = comb_entry(e < 128 ? T->phi_pow[e >> 4] : r6, e & 15u);
   __syncthreads();
+  const uint4 se = S[t], so = S[64 + t];
+  const uint4 q = f_add(se, comb_mul<1>(f_add(se, so), tabs + 128));
   for (int i = 0; i < 8; ++i) R[t * 8 + i] = comb_mul<1>(q, tabs + i * 16);
   __syncthreads();
   uint4 acc = f_zero();
@@ -432,56 +451,80 @@ extern "C" int flk_zc_round1(const uint8_t* a_bits, const uint8_t* b_bits, const
   if (!a_bits || !b_bits || !c_bits || !out_ab || !out_c || (m > 13 && !r_outer))
     return set_error(FLK_E_SHAPE, "zc_round1: null pointer");
   if (m < 13 || m > 40) return set_error(FLK_E_SHAPE, "zc_round1: m=%zu out of range (13..40)", m);
-  if (((uintptr_t)a_bits | (uintptr_t)b_bits | (uintptr_t)c_bits) % 8)
-    return set_error(FLK_E_SHAPE, "zc_round1: bit vectors must be 8-byte aligned");
+  if (((uintptr_t)a_bits | (uintptr_t)b_bits) % 8 || (uintptr_t)c_bits % 16)
+    return set_error(FLK_E_SHAPE, "zc_round1: a_bits and b_bits must be 8-byte aligned, c_bits 16-byte aligned");
   if (out_c != out_ab + 64) return set_error(FLK_E_SHAPE, "zc_round1: out_c must follow out_ab (one 128-entry buffer)");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const Round1Tables* T = round1_tables_device(&rc);
   if (rc) return rc;
+  // eq over r[7..m) = (fixed r[7..13), r_outer) for the column sums of c, split
+  // as eq_lo (2^lo_log) x eq_hi when there are at least 2^8 row pairs.
+  flk_f128 fixed[7];
+  zc_fixed_challenges(fixed);
+  std::vector<flk_f128> r7(fixed + 1, fixed + 7);
+  r7.insert(r7.end(), r_outer, r_outer + (m - 13));
+  const size_t n_pairs_log = m - 7;
+  const bool split = n_pairs_log >= 8;
+  const size_t lo_log = split ? (n_pairs_log < 12 ? n_pairs_log : 12) : n_pairs_log;
+  const size_t hi_log = n_pairs_log - lo_log;
   const size_t n_outer = size_t(1) << (m - 13);
-  uint4* eq = nullptr;
-  hipError_t e = flk::stream_alloc(reinterpret_cast<void**>(&eq), n_outer * sizeof(uint4), st);
+  const size_t n_lo = size_t(1) << lo_log, n_hi = split ? size_t(1) << hi_log : 0;
+  uint4* eq = nullptr;  // [eq_outer | eq_lo | eq_hi | S (128)]
+  hipError_t e = flk::stream_alloc(reinterpret_cast<void**>(&eq), (n_outer + n_lo + n_hi + 128) * sizeof(uint4), st);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    return set_error(FLK_E_NOMEM, "zc_round1: eq_outer allocation failed");
+    return set_error(FLK_E_NOMEM, "zc_round1: scratch allocation failed");
   }
-  rc = flk_build_eq(r_outer, m - 13, reinterpret_cast<flk_f128*>(eq), stream);
-  if (rc) return rc;
+  uint4* const eq_lo = eq + n_outer;
+  uint4* const eq_hi = eq_lo + n_lo;
+  uint4* const S = eq_hi + n_hi;
+  auto fail = [&](int status) {
+    (void)flk::stream_free(eq, st);
+    return status;
+  };
+  if ((rc = flk_build_eq(r_outer, m - 13, reinterpret_cast<flk_f128*>(eq), stream))) return fail(rc);
+  if ((rc = flk_build_eq(r7.data(), lo_log, reinterpret_cast<flk_f128*>(eq_lo), stream))) return fail(rc);
+  const flk_f128* c128 = reinterpret_cast<const flk_f128*>(c_bits);
+  if (split) {
+    if ((rc = flk_build_eq(r7.data() + lo_log, hi_log, reinterpret_cast<flk_f128*>(eq_hi), stream))) return fail(rc);
+    rc = rs_fold_1b_rows_split(c128, reinterpret_cast<const flk_f128*>(eq_lo), lo_log,
+                               reinterpret_cast<const flk_f128*>(eq_hi), hi_log, reinterpret_cast<flk_f128*>(S), st);
+  } else {
+    rc = flk_rs_fold_1b_rows(c128, reinterpret_cast<const flk_f128*>(eq_lo), n_lo, reinterpret_cast<flk_f128*>(S),
+                             stream);
+  }
+  if (rc) return fail(rc);
   uint4* out = reinterpret_cast<uint4*>(out_ab);
-  // The full 64 KiB product table (one work-group of 4 blocks per CU; 4.72 ms at
-  // m = 32 against 6.03 ms) unless FLOCK_R1_FULLTAB=0 selects the split-nibble
-  // tables (two work-groups of 3 blocks per CU).  Grids: 512 work-groups
-  // (full) or 1024 (split) leave room for uneven block counts per work-group.
+  hipLaunchKernelGGL(zc_round1_lift_c_kernel, dim3(1), dim3(64), 0, st, S, u4_of(fixed[0]), T, out + 64);
+  FLK_LAUNCH_CHECK("zc_round1_lift_c_kernel");
+  // The full 64 KiB product table (one work-group of 8 waves per CU) unless
+  // FLOCK_R1_FULLTAB=0 selects the split-nibble tables (two work-groups of 6
+  // waves per CU).  Grids of 512 (full) or 1024 (split) work-groups: several
+  // blocks per wave, so the waves' uneven block counts even out.
   static const bool full = [] {
     const char* v = getenv("FLOCK_R1_FULLTAB");
     return !(v && v[0] == '0');
   }();
-  const int sub = full ? r1_sub<true>() : r1_sub<false>();
-  const uint64_t n_iter = (n_outer + sub - 1) / sub;
+  const int nw = full ? r1_waves<true>() : r1_waves<false>();
+  const uint64_t n_iter = (n_outer + nw - 1) / nw;
   const uint64_t cap = full ? 512 : 1024;
   const unsigned grid = (unsigned)(n_iter < cap ? n_iter : cap);
   uint4* parts = nullptr;
-  if ((rc = partials_alloc(&parts, grid, 128, out, st))) {
-    (void)flk::stream_free(eq, st);
-    return rc;
-  }
+  if ((rc = partials_alloc(&parts, grid, 64, out, st))) return fail(rc);
   if (full)
-    hipLaunchKernelGGL(zc_round1_kernel<true>, dim3(grid), dim3(sub * R1_ROWS), 0, st,
+    hipLaunchKernelGGL(zc_round1_kernel<true>, dim3(grid), dim3(nw * 64), 0, st,
                        reinterpret_cast<const uint64_t*>(a_bits), reinterpret_cast<const uint64_t*>(b_bits),
-                       reinterpret_cast<const uint64_t*>(c_bits), (uint64_t)n_outer, eq, T, parts);
+                       (uint64_t)n_outer, eq, T, parts);
   else
-    hipLaunchKernelGGL(zc_round1_kernel<false>, dim3(grid), dim3(sub * R1_ROWS), 0, st,
+    hipLaunchKernelGGL(zc_round1_kernel<false>, dim3(grid), dim3(nw * 64), 0, st,
                        reinterpret_cast<const uint64_t*>(a_bits), reinterpret_cast<const uint64_t*>(b_bits),
-                       reinterpret_cast<const uint64_t*>(c_bits), (uint64_t)n_outer, eq, T, parts);
+                       (uint64_t)n_outer, eq, T, parts);
   FLK_LAUNCH_CHECK("zc_round1_kernel");
-  if ((rc = partials_sum_free(parts, grid, 128, out, st))) return rc;
-  hipLaunchKernelGGL(zc_round1_lift_c_kernel, dim3(1), dim3(64), 0, st, out + 64, T);
-  FLK_LAUNCH_CHECK("zc_round1_lift_c_kernel");
+  if ((rc = partials_sum_free(parts, grid, 64, out, st))) return fail(rc);
   const flk_f128 sc = round1_scale();
-  hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(128), 0, st, out, 128u,
-                     make_uint4((uint32_t)sc.lo, (uint32_t)(sc.lo >> 32), (uint32_t)sc.hi, (uint32_t)(sc.hi >> 32)));
+  hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(64), 0, st, out, 64u, u4_of(sc));
   FLK_LAUNCH_CHECK("scale_kernel");
   e = flk::stream_free(eq, st);
-  if (e != hipSuccess) return device_failure(e, "zc_round1: free eq_outer");
+  if (e != hipSuccess) return device_failure(e, "zc_round1: free scratch");
   return FLK_OK;
 }

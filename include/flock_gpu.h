@@ -371,7 +371,7 @@ int flk_zc_fold_at_z_round(const uint8_t *a_bits, const uint8_t *b_bits, size_t 
  * (zerocheck.rs:1680-1692): r[6..9) = phi8(0xF7, 0x53, 0xB5), r[9..13) =
  * x^(2^i) / (1 + x^(2^i)) (univariate_skip_optimized.rs:136-232), r[13..m) =
  * r_outer (HOST array of m - 13 elements).  a_bits, b_bits, c_bits hold 2^m bits
- * each (8-byte aligned).  Writes P_AB on Lambda to out_ab[0..64) and P_C to
+ * each (a_bits, b_bits 8-byte aligned, c_bits 16-byte aligned).  Writes P_AB on Lambda to out_ab[0..64) and P_C to
  * out_c[0..64); out_c must equal out_ab + 64.  13 <= m <= 40. */
 int flk_zc_round1(const uint8_t *a_bits, const uint8_t *b_bits, const uint8_t *c_bits, size_t m,
                   const flk_f128 *r_outer, flk_f128 *out_ab, flk_f128 *out_c, void *stream);
@@ -510,7 +510,8 @@ int flk_lincheck_comb(const flk_r1cs *r, flk_f128 z_skip, const flk_f128 *x_inne
 
 /* zerocheck::prove_packed_padded (CF/zerocheck.rs:384-535; transcript order of
  * verify, :1642-1790) for a(y) b(y) + c(y) = 0 over 2^m points, a_bits, b_bits,
- * c_bits device bit vectors (LSB-first, 8-byte aligned), 13 <= m <= 40.
+ * c_bits device bit vectors (LSB-first; a_bits, b_bits 8-byte aligned, c_bits
+ * 16-byte aligned), 13 <= m <= 40.
  * Outputs: round1_ab[64], round1_c[64] (naive convention), rounds[2 (m - 6)]
  * = (G(1), G(inf)) per multilinear round, finals = {final_a, final_b,
  * final_c}; the claim: *z_out, mlv_out[m - 6] (the rho challenges), r_rest_out[m - 6]. */

@@ -231,7 +231,7 @@ def _fixed_inner_challenges():
     return small + medium
 
 
-@pytest.mark.parametrize("m", [13, 14, 16, 20])
+@pytest.mark.parametrize("m", [13, 14, 16, 20, 22])
 def test_zc_round1_matches_oracle(orc, dev, m):
     """Round 1 for the protocol challenge layout: r[0..6) unused by round 1,
     r[6..13) fixed, r[13..m) random; against round1_naive on the oracle."""
