@@ -121,10 +121,16 @@ void zerocheck_prove(Fs& ch, const uint8_t* a_bits, const uint8_t* b_bits, const
   size_t n = size_t(1) << n_mlv;
   DeviceVec<F128> am(n, st), bm(n, st), msg(2, st);
   const size_t d0 = n_mlv - 1;
-  // eq_lo: the round's first 8 eq coordinates; eq_hi (two buffers, alternating): the
-  // rest, handed from round to round by summing out the lowest coordinate.
-  DeviceVec<F128> eq_lo(256, st), eq_hi[2] = {DeviceVec<F128>(d0 >= 8 ? size_t(1) << (d0 - 8) : 1, st),
-                                               DeviceVec<F128>(d0 >= 9 ? size_t(1) << (d0 - 9) : 1, st)};
+  // eq_lo: each device round's first 8 eq coordinates (all known before round 0,
+  // so the tables are built up front, off the rounds' critical path); eq_hi (two
+  // buffers, alternating): the rest, handed from round to round by summing out
+  // the lowest coordinate.
+  const size_t n_dev = d0 >= 8 ? d0 - 7 : 0;  // rounds i with n_mlv - i - 1 >= 8
+  DeviceVec<F128> eq_lo(256 * (n_dev ? n_dev : 1), st),
+      eq_hi[2] = {DeviceVec<F128>(d0 >= 8 ? size_t(1) << (d0 - 8) : 1, st),
+                  DeviceVec<F128>(d0 >= 9 ? size_t(1) << (d0 - 9) : 1, st)};
+  for (size_t i = 0; i < n_dev; ++i)
+    chk(flk_build_eq(r.data() + K_SKIP + i + 1, 8, eq_lo.data() + 256 * i, st), "build_eq(lo)");
   if (d0 >= 8) {
     chk(flk_build_eq(r.data() + K_SKIP + 1 + 8, d0 - 8, eq_hi[0].data(), st), "build_eq(hi)");
   } else {
@@ -146,14 +152,14 @@ void zerocheck_prove(Fs& ch, const uint8_t* a_bits, const uint8_t* b_bits, const
     if (on_device && d >= 8) {
       DeviceVec<F128>& hi = eq_hi[i & 1];
       DeviceVec<F128>& hi_next = eq_hi[(i + 1) & 1];
-      chk(flk_build_eq(pt, 8, eq_lo.data(), st), "build_eq(lo)");
+      const F128* lo = eq_lo.data() + 256 * i;  // eq over pt[0 .. 8)
       if (i == 0) {  // the fold at z and the first message in one pass over the bits
-        chk(flk_zc_fold_at_z_round(a_bits, b_bits, m, wd.data(), eq_lo.data(), hi.data(), am.data(), bm.data(),
+        chk(flk_zc_fold_at_z_round(a_bits, b_bits, m, wd.data(), lo, hi.data(), am.data(), bm.data(),
                                    hi_next.data(), msg.data(), st),
             "zc_fold_at_z_round");
       } else {
         DeviceVec<F128> an(n / 2, st), bn(n / 2, st);
-        chk(flk_zc_fold_round(am.data(), bm.data(), n, &rhos.back(), eq_lo.data(), hi.data(), d - 8, an.data(),
+        chk(flk_zc_fold_round(am.data(), bm.data(), n, &rhos.back(), lo, hi.data(), d - 8, an.data(),
                               bn.data(), hi_next.data(), msg.data(), st),
             "zc_fold_round");
         am = std::move(an);

@@ -481,33 +481,50 @@ __global__ void __launch_bounds__(ZF_WG) zc_fold_round_kernel(const uint4* __res
   u256_zero(g1);
   u256_zero(gi);
   uint4* my = tabs + tid;
+  // The chunk's inputs are loaded one iteration ahead (u: a, v: b; FOLD 4 each,
+  // PLAIN 2 each, BITS 1 each), so their latency overlaps the previous chunk's
+  // products and this chunk's table build.
+  constexpr int NL = MODE == ZF_FOLD ? 4 : (MODE == ZF_PLAIN ? 2 : 1);
+  uint4 u[NL], v[NL];
+  auto load = [&](uint64_t c) {
+    const uint64_t x = c * ZF_WG + tid;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      u[i] = a[NL * x + i];
+      v[i] = b[NL * x + i];
+    }
+  };
+  if (blockIdx.x < n_chunks) load(blockIdx.x);
   for (uint64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
     const uint4 eh = e_hi[c];
     if (e_hi_next && tid == 0 && !(c & 1)) e_hi_next[c >> 1] = f_add(eh, e_hi[c + 1]);
+    uint4 cu[NL], cv[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      cu[i] = u[i];
+      cv[i] = v[i];
+    }
+    if (c + gridDim.x < n_chunks) load(c + gridDim.x);
     build_pos_tables(htab, hnib, 1, [&](uint32_t) { return eh; });  // two barriers: the last chunk's lookups are done
     const uint64_t x = c * ZF_WG + tid;
     uint4 a0, a1, b0, b1;
     if (MODE == ZF_FOLD) {
-      const uint4* ap = a + 4 * x;
-      const uint4* bp = b + 4 * x;
-      const uint4 u0 = ap[0], u1 = ap[1], u2 = ap[2], u3 = ap[3];
-      const uint4 v0 = bp[0], v1 = bp[1], v2 = bp[2], v3 = bp[3];
-      a0 = combjr_mul_add(f_add(u0, u1), u0, rtab);  // T'[j] = T[2j] + rho (T[2j] + T[2j+1])
-      a1 = combjr_mul_add(f_add(u2, u3), u2, rtab);
-      b0 = combjr_mul_add(f_add(v0, v1), v0, rtab);
-      b1 = combjr_mul_add(f_add(v2, v3), v2, rtab);
+      a0 = combjr_mul_add(f_add(cu[0], cu[1]), cu[0], rtab);  // T'[j] = T[2j] + rho (T[2j] + T[2j+1])
+      a1 = combjr_mul_add(f_add(cu[2], cu[3]), cu[2], rtab);
+      b0 = combjr_mul_add(f_add(cv[0], cv[1]), cv[0], rtab);
+      b1 = combjr_mul_add(f_add(cv[2], cv[3]), cv[2], rtab);
     } else if (MODE == ZF_BITS) {
       // rows 2x and 2x + 1 of each bit vector: 16 bytes = one uint4
-      const uint4 ra = a[x], rb = b[x];
+      const uint4 ra = cu[0], rb = cv[0];
       a0 = f_add(nibtab8_sum(ra.x, wnib), nibtab8_sum(ra.y, wnib + 128));
       a1 = f_add(nibtab8_sum(ra.z, wnib), nibtab8_sum(ra.w, wnib + 128));
       b0 = f_add(nibtab8_sum(rb.x, wnib), nibtab8_sum(rb.y, wnib + 128));
       b1 = f_add(nibtab8_sum(rb.z, wnib), nibtab8_sum(rb.w, wnib + 128));
     } else {
-      a0 = a[2 * x];
-      a1 = a[2 * x + 1];
-      b0 = b[2 * x];
-      b1 = b[2 * x + 1];
+      a0 = cu[0];
+      a1 = cu[1];
+      b0 = cv[0];
+      b1 = cv[1];
     }
     if (MODE != ZF_PLAIN) {
       a_out[2 * x] = a0;

@@ -148,23 +148,21 @@ struct EqFactors {
   uint4 e[16];  // e[k] = prod over the pass's t coordinates, selected by the bits of k
 };
 
-// First 2^s entries (s <= 8) by one work-group: thread i multiplies its s factors
-// with a private comb table of its running product.
-__global__ void __launch_bounds__(256) eq_seed_kernel(uint4* __restrict__ out, EqFactors one_plus, EqFactors rr,
+// First 2^s entries (s <= 8) by one work-group: entry i = lo[i & 15] * hi[i >> 4]
+// (hi only when s > 4), where lo and hi hold the products over the first
+// min(s, 4) coordinates and over coordinates 4 .. s-1 for every bit pattern:
+// one product per entry, on the lane's private comb table.
+__global__ void __launch_bounds__(256) eq_seed_kernel(uint4* __restrict__ out, EqFactors lo, EqFactors hi,
                                                       uint32_t s) {
   __shared__ uint4 tabs[16 * 256];
   const uint32_t i = threadIdx.x;
-  uint4 acc = make_uint4(1u, 0u, 0u, 0u);
-  for (uint32_t j = 0; j < s; ++j) {
-    const uint4 fac = ((i >> j) & 1u) ? rr.e[j] : one_plus.e[j];
-    if (j == 0) {
-      acc = fac;
-    } else {
-      table_store_private(tabs, blockDim.x, i, acc);
-      acc = comb_mul_rt(fac, tabs + i, blockDim.x);
-    }
+  if (i >= (1u << s)) return;  // no barrier below: each lane reads only its own table
+  uint4 v = lo.e[i & 15u];
+  if (s > 4) {
+    table_store_private(tabs, blockDim.x, i, hi.e[i >> 4]);
+    v = comb_mul_rt(v, tabs + i, blockDim.x);
   }
-  if (i < (1u << s)) out[i] = acc;
+  out[i] = v;
 }
 
 // out[i + k*len] = out[i] * E[k] for k < 2^t, i < len (in place; reads [0,len)).
@@ -367,32 +365,29 @@ int flk_build_eq(const flk_f128* point, size_t d, flk_f128* out, void* stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   uint4* o = reinterpret_cast<uint4*>(out);
   const size_t s = d < 8 ? d : 8;
-  flk::EqFactors onep{}, rr{};
-  for (size_t j = 0; j < s; ++j) {
-    rr.e[j] = flk::to_u4(point[j]);
-    onep.e[j] = flk::to_u4(flk_f128{point[j].lo ^ 1ull, point[j].hi});
-  }
-  if (d == 0) {
-    // single entry 1
-    hipLaunchKernelGGL(flk::eq_seed_kernel, dim3(1), dim3(64), 0, st, o, onep, rr, 0u);
-    FLK_LAUNCH_CHECK("eq_seed_kernel");
-    return FLK_OK;
-  }
-  hipLaunchKernelGGL(flk::eq_seed_kernel, dim3(1), dim3(256), 0, st, o, onep, rr, (uint32_t)s);
-  FLK_LAUNCH_CHECK("eq_seed_kernel");
-  size_t done = s;
-  while (done < d) {
-    const size_t t = (d - done) < 4 ? (d - done) : 4;
+  // Products over point[first .. first + t) for every bit pattern k < 2^t.
+  auto combos = [&](size_t first, size_t t) {
     flk::EqFactors E{};
     for (size_t k = 0; k < (size_t(1) << t); ++k) {
       flk_f128 acc{1, 0};
       for (size_t bit = 0; bit < t; ++bit) {
-        flk_f128 r = point[done + bit];
-        flk_f128 fac = ((k >> bit) & 1) ? r : flk_f128{r.lo ^ 1ull, r.hi};
-        acc = flk::host_mul(acc, fac);
+        const flk_f128 r = point[first + bit];
+        acc = flk::host_mul(acc, ((k >> bit) & 1) ? r : flk_f128{r.lo ^ 1ull, r.hi});
       }
       E.e[k] = flk::to_u4(acc);
     }
+    return E;
+  };
+  const size_t t_lo = s < 4 ? s : 4;
+  const flk::EqFactors lo = combos(0, t_lo);
+  const flk::EqFactors hi = s > 4 ? combos(4, s - 4) : flk::EqFactors{};
+  hipLaunchKernelGGL(flk::eq_seed_kernel, dim3(1), dim3(256), 0, st, o, lo, hi, (uint32_t)s);
+  FLK_LAUNCH_CHECK("eq_seed_kernel");
+  if (d == 0) return FLK_OK;
+  size_t done = s;
+  while (done < d) {
+    const size_t t = (d - done) < 4 ? (d - done) : 4;
+    const flk::EqFactors E = combos(done, t);
     const uint64_t len = uint64_t(1) << done;
     // Each work-group builds 2^t tables of 2 KiB, so it strides over several
     // inputs: at most 1024 work-groups.
