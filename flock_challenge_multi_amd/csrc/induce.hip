@@ -10,9 +10,11 @@
 //      s_0 = x, s_i = s_{i-1}^2 + s_{i-1}(v_{i-1}) s_{i-1}, scaled by 1/s_i(v_i);
 //   2. per query, the low and high halves of the tensor expansion
 //      lo_q[j] (2^s_lo entries) and hi_q[j] = alpha_q * (2^s_hi entries);
-//   3. basis[(jh << s_lo) | jl] = sum_q lo_q[jl] * hi_q[jh]: a work-group owns one jh,
-//      hi_q[jh] is its shared multiplier, lanes XOR-accumulate unreduced products;
-//   4. the enforced sum as a dot product of the opened rows with alpha_q eq_k.
+//   3. basis[(jh << s_lo) | jl] = sum_q lo_q[jl] * hi_q[jh]: a work-group owns one jh
+//      and 256 jl, hi_q[jh] is its shared multiplier, lanes XOR-accumulate
+//      unreduced products;
+//   4. the enforced sum as a dot product of the opened rows with alpha_q eq_k
+//      (both factors applied on the device).
 // Every sum is an XOR, so this order equals the reference's per-thread partials.
 #include <hip/hip_runtime.h>
 
@@ -69,7 +71,10 @@ __global__ void __launch_bounds__(IND_WG) induce_expand_kernel(const uint4* __re
   }
 }
 
-// 3. basis[(jh << s_lo) | jl] = sum_q lo[q][jl] * hi[q][jh]; one work-group per jh.
+// 3. basis[(jh << s_lo) | jl] = sum_q lo[q][jl] * hi[q][jh]; work-group (jh, y)
+//    takes jl in [y * IND_WG * ITEMS, (y + 1) * IND_WG * ITEMS), so a large
+//    basis runs as 2^s_hi x 2^s_lo / (IND_WG * ITEMS) work-groups (each builds the
+//    nibble-position tables of its hi[.][jh]).
 template <int ITEMS>
 __global__ void __launch_bounds__(IND_WG) induce_basis_kernel(const uint4* __restrict__ lo, const uint4* __restrict__ hi,
                                                              uint32_t nq, uint32_t s_lo, uint32_t s_hi,
@@ -79,6 +84,7 @@ __global__ void __launch_bounds__(IND_WG) induce_basis_kernel(const uint4* __res
   __shared__ uint4 nib[QC * 32];
   const uint32_t jh = blockIdx.x;
   const uint32_t n_lo = 1u << s_lo;
+  const uint32_t jl0 = blockIdx.y * (IND_WG * ITEMS);
   uint32_t acc[ITEMS][7];
 #pragma unroll
   for (int t = 0; t < ITEMS; ++t)
@@ -91,14 +97,14 @@ __global__ void __launch_bounds__(IND_WG) induce_basis_kernel(const uint4* __res
     for (uint32_t c = 0; c < nc; ++c) {
 #pragma unroll
       for (int t = 0; t < ITEMS; ++t) {
-        const uint32_t jl = threadIdx.x + t * IND_WG;
+        const uint32_t jl = jl0 + threadIdx.x + t * IND_WG;
         if (jl < n_lo) combjr_acc(lo[((size_t)(q0 + c) << s_lo) + jl], tab + 128 * c, acc[t]);
       }
     }
   }
 #pragma unroll
   for (int t = 0; t < ITEMS; ++t) {
-    const uint32_t jl = threadIdx.x + t * IND_WG;
+    const uint32_t jl = jl0 + threadIdx.x + t * IND_WG;
     if (jl < n_lo) {
       const uint32_t r[8] = {acc[t][0], acc[t][1], acc[t][2], acc[t][3], acc[t][4], acc[t][5], acc[t][6], 0u};
       basis[((size_t)jh << s_lo) + jl] = f_reduce_add(r, f_zero());
@@ -106,16 +112,22 @@ __global__ void __launch_bounds__(IND_WG) induce_basis_kernel(const uint4* __res
   }
 }
 
-// 4. sum_i a[i] * c[i] XOR-accumulated into *out (out zeroed by the caller).
-__global__ void __launch_bounds__(IND_WG) dot_kernel(const uint4* __restrict__ a, const uint4* __restrict__ c,
-                                                    uint64_t n, uint4* __restrict__ out) {
+// 4. sum over rows[q * 2^log_row + k] * eqv[k] * aq[q], XOR-accumulated into *out
+//    (out zeroed by the caller): two products per element on the lane's private
+//    comb tables.
+__global__ void __launch_bounds__(IND_WG) dot_kernel(const uint4* __restrict__ rows, const uint4* __restrict__ eqv,
+                                                    const uint4* __restrict__ aq, uint32_t log_row, uint64_t n,
+                                                    uint4* __restrict__ out) {
   __shared__ uint4 tabs[16 * IND_WG];
   __shared__ uint4 red[IND_WG / 64];
   U256 acc;
   u256_zero(acc);
+  const uint64_t kmask = (uint64_t(1) << log_row) - 1;
   for (uint64_t i = (uint64_t)blockIdx.x * IND_WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * IND_WG) {
-    table_store_private(tabs, IND_WG, threadIdx.x, c[i]);
-    comb_mul_acc_rt(a[i], tabs + threadIdx.x, IND_WG, acc);
+    table_store_private(tabs, IND_WG, threadIdx.x, eqv[i & kmask]);
+    const uint4 p = comb_mul_rt(rows[i], tabs + threadIdx.x, IND_WG);
+    table_store_private(tabs, IND_WG, threadIdx.x, aq[i >> log_row]);
+    comb_mul_acc_rt(p, tabs + threadIdx.x, IND_WG, acc);
   }
   uint4 v = f_reduce(acc);
 #pragma unroll
@@ -204,25 +216,36 @@ extern "C" int flk_induce_sumcheck_poly(size_t log_msg_cols, const flk_f128* row
   FLK_LAUNCH_CHECK("zero4_kernel");
   if (nq == 0) return zero_n(basis, uint64_t(1) << L, st);  // a kernel: safe inside a captured graph
 
-  // Host constants: alpha_q, eq(v), sks_vks and their inverses, and alpha_q * eq_k.
+  // Host constants: alpha_q, eq(v), sks_vks and their inverses (one inversion
+  // for all of them: prefix products, invert the last, walk back).
   std::vector<flk_f128> aq = host_eq(alpha, n_alpha);
   aq.resize(nq);
   std::vector<flk_f128> eqv = host_eq(v_challenges, n_v);
   std::vector<flk_f128> sks = host_sks_vks(L);
   std::vector<flk_f128> consts(2 * (L + 1));
-  for (uint32_t i = 0; i <= L; ++i) {
-    consts[i] = sks[i];
-    consts[L + 1 + i] = (sks[i].lo | sks[i].hi) ? host_inv(sks[i]) : sks[i];
+  {
+    std::vector<flk_f128> pre(L + 2, flk_f128{1, 0});
+    for (uint32_t i = 0; i <= L; ++i) {
+      const bool nz = sks[i].lo | sks[i].hi;
+      pre[i + 1] = nz ? host_mul(pre[i], sks[i]) : pre[i];
+    }
+    flk_f128 inv = host_inv(pre[L + 1]);  // inverse of the product of the nonzero sks
+    for (uint32_t i = L + 1; i-- > 0;) {
+      consts[i] = sks[i];
+      if (sks[i].lo | sks[i].hi) {
+        consts[L + 1 + i] = host_mul(inv, pre[i]);
+        inv = host_mul(inv, sks[i]);
+      } else {
+        consts[L + 1 + i] = sks[i];
+      }
+    }
   }
-  std::vector<flk_f128> coef(nq * row_len);
-  for (uint32_t q = 0; q < nq; ++q)
-    for (size_t k = 0; k < row_len; ++k) coef[q * row_len + k] = host_mul(aq[q], eqv[k]);
 
   const uint32_t s_lo = L < 10 ? L : 10;
   const uint32_t s_hi = L - s_lo;
-  const size_t n_consts = consts.size(), n_aq = nq, n_coef = coef.size();
+  const size_t n_consts = consts.size(), n_aq = nq, n_eqv = row_len;
   const size_t n_S = (size_t)nq * (L ? L : 1), n_lo = (size_t)nq << s_lo, n_hi = (size_t)nq << s_hi;
-  const size_t total = n_consts + n_aq + n_coef + n_S + n_lo + n_hi;
+  const size_t total = n_consts + n_aq + n_eqv + n_S + n_lo + n_hi;
   uint4* scratch = nullptr;
   hipError_t e = flk::stream_alloc(reinterpret_cast<void**>(&scratch), total * sizeof(uint4), st);
   if (e != hipSuccess) {
@@ -231,15 +254,15 @@ extern "C" int flk_induce_sumcheck_poly(size_t log_msg_cols, const flk_f128* row
   }
   uint4* d_consts = scratch;
   uint4* d_aq = d_consts + n_consts;
-  uint4* d_coef = d_aq + n_aq;
-  uint4* d_S = d_coef + n_coef;
+  uint4* d_eqv = d_aq + n_aq;
+  uint4* d_S = d_eqv + n_eqv;
   uint4* d_lo = d_S + n_S;
   uint4* d_hi = d_lo + n_lo;
   std::vector<flk_f128> up;
-  up.reserve(n_consts + n_aq + n_coef);
+  up.reserve(n_consts + n_aq + n_eqv);
   up.insert(up.end(), consts.begin(), consts.end());
   up.insert(up.end(), aq.begin(), aq.end());
-  up.insert(up.end(), coef.begin(), coef.end());
+  up.insert(up.end(), eqv.begin(), eqv.end());
   e = hipMemcpyAsync(scratch, up.data(), up.size() * sizeof(flk_f128), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);  // `up` is pageable host memory
   if (e != hipSuccess) return device_failure(e, "induce: upload constants");
@@ -254,16 +277,18 @@ extern "C" int flk_induce_sumcheck_poly(size_t log_msg_cols, const flk_f128* row
   hipLaunchKernelGGL(induce_expand_kernel, dim3(nq), dim3(IND_WG), 0, st, d_S, L, s_lo, s_hi,
                      static_cast<const uint4*>(d_aq), d_hi);
   FLK_LAUNCH_CHECK("induce_expand_kernel hi");
+  // One basis entry per thread, 2^s_lo / IND_WG work-groups per jh: at the
+  // largest level (2^19 entries) 2048 work-groups, eight per CU.
   const uint32_t n_lo_cols = 1u << s_lo;
-  if (n_lo_cols <= (uint32_t)IND_WG)
-    hipLaunchKernelGGL(induce_basis_kernel<1>, dim3(1u << s_hi), dim3(IND_WG), 0, st, d_lo, d_hi, nq, s_lo, s_hi, basis);
-  else
-    hipLaunchKernelGGL(induce_basis_kernel<4>, dim3(1u << s_hi), dim3(IND_WG), 0, st, d_lo, d_hi, nq, s_lo, s_hi, basis);
+  const uint32_t ny = n_lo_cols > (uint32_t)IND_WG ? n_lo_cols / IND_WG : 1u;
+  hipLaunchKernelGGL(induce_basis_kernel<1>, dim3(1u << s_hi, ny), dim3(IND_WG), 0, st, d_lo, d_hi, nq, s_lo, s_hi,
+                     basis);
   FLK_LAUNCH_CHECK("induce_basis_kernel");
-  uint64_t g = (n_coef + IND_WG - 1) / IND_WG;
+  const uint64_t n_rows = (uint64_t)nq * row_len;
+  uint64_t g = (n_rows + IND_WG - 1) / IND_WG;
   if (g > 1024) g = 1024;
-  hipLaunchKernelGGL(dot_kernel, dim3((unsigned)g), dim3(IND_WG), 0, st, reinterpret_cast<const uint4*>(rows), d_coef,
-                     (uint64_t)n_coef, sum);
+  hipLaunchKernelGGL(dot_kernel, dim3((unsigned)g), dim3(IND_WG), 0, st, reinterpret_cast<const uint4*>(rows), d_eqv,
+                     d_aq, (uint32_t)n_v, n_rows, sum);
   FLK_LAUNCH_CHECK("dot_kernel");
   e = flk::stream_free(scratch, st);
   if (e != hipSuccess) return device_failure(e, "induce: free scratch");
