@@ -20,41 +20,52 @@ constexpr size_t CHUNK_LEN = 1024, BLOCK_LEN = 64;
 
 inline uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
 
-inline void g(uint32_t* s, int a, int b, int c, int d, uint32_t x, uint32_t y) {
-  s[a] = s[a] + s[b] + x;
-  s[d] = rotr(s[d] ^ s[a], 16);
-  s[c] = s[c] + s[d];
-  s[b] = rotr(s[b] ^ s[c], 12);
-  s[a] = s[a] + s[b] + y;
-  s[d] = rotr(s[d] ^ s[a], 8);
-  s[c] = s[c] + s[d];
-  s[b] = rotr(s[b] ^ s[c], 7);
-}
+// Message word order of round r: PERM applied r times to the identity.
+struct Schedule {
+  uint8_t w[7][16];
+  constexpr Schedule() : w() {
+    for (int i = 0; i < 16; ++i) w[0][i] = (uint8_t)i;
+    for (int r = 1; r < 7; ++r)
+      for (int i = 0; i < 16; ++i) w[r][i] = w[r - 1][PERM[i]];
+  }
+};
+constexpr Schedule SCHEDULE{};
 
-// The full 16-word output of the compression function.
-inline void compress(const uint32_t cv[8], const uint32_t m_in[16], uint64_t counter, uint32_t block_len,
+// The full 16-word output of the compression function.  The state lives in
+// sixteen scalars (not an indexed array) so that it stays in registers, and
+// each round reads the message through the precomputed schedule instead of
+// permuting a copy.
+inline void compress(const uint32_t cv[8], const uint32_t m[16], uint64_t counter, uint32_t block_len,
                      uint32_t flags, uint32_t out[16]) {
-  uint32_t s[16] = {cv[0], cv[1], cv[2], cv[3], cv[4], cv[5], cv[6], cv[7],
-                    IV[0], IV[1], IV[2], IV[3], (uint32_t)counter, (uint32_t)(counter >> 32), block_len, flags};
-  uint32_t m[16];
-  std::memcpy(m, m_in, sizeof m);
+  uint32_t s0 = cv[0], s1 = cv[1], s2 = cv[2], s3 = cv[3], s4 = cv[4], s5 = cv[5], s6 = cv[6], s7 = cv[7];
+  uint32_t s8 = IV[0], s9 = IV[1], s10 = IV[2], s11 = IV[3];
+  uint32_t s12 = (uint32_t)counter, s13 = (uint32_t)(counter >> 32), s14 = block_len, s15 = flags;
+  auto g = [](uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d, uint32_t x, uint32_t y) {
+    a = a + b + x;
+    d = rotr(d ^ a, 16);
+    c = c + d;
+    b = rotr(b ^ c, 12);
+    a = a + b + y;
+    d = rotr(d ^ a, 8);
+    c = c + d;
+    b = rotr(b ^ c, 7);
+  };
+#pragma GCC unroll 7
   for (int r = 0; r < 7; ++r) {
-    g(s, 0, 4, 8, 12, m[0], m[1]);
-    g(s, 1, 5, 9, 13, m[2], m[3]);
-    g(s, 2, 6, 10, 14, m[4], m[5]);
-    g(s, 3, 7, 11, 15, m[6], m[7]);
-    g(s, 0, 5, 10, 15, m[8], m[9]);
-    g(s, 1, 6, 11, 12, m[10], m[11]);
-    g(s, 2, 7, 8, 13, m[12], m[13]);
-    g(s, 3, 4, 9, 14, m[14], m[15]);
-    uint32_t p[16];
-    for (int i = 0; i < 16; ++i) p[i] = m[PERM[i]];
-    std::memcpy(m, p, sizeof m);
+    const uint8_t* z = SCHEDULE.w[r];
+    g(s0, s4, s8, s12, m[z[0]], m[z[1]]);
+    g(s1, s5, s9, s13, m[z[2]], m[z[3]]);
+    g(s2, s6, s10, s14, m[z[4]], m[z[5]]);
+    g(s3, s7, s11, s15, m[z[6]], m[z[7]]);
+    g(s0, s5, s10, s15, m[z[8]], m[z[9]]);
+    g(s1, s6, s11, s12, m[z[10]], m[z[11]]);
+    g(s2, s7, s8, s13, m[z[12]], m[z[13]]);
+    g(s3, s4, s9, s14, m[z[14]], m[z[15]]);
   }
-  for (int i = 0; i < 8; ++i) {
-    out[i] = s[i] ^ s[i + 8];
-    out[i + 8] = s[i + 8] ^ cv[i];
-  }
+  out[0] = s0 ^ s8, out[1] = s1 ^ s9, out[2] = s2 ^ s10, out[3] = s3 ^ s11;
+  out[4] = s4 ^ s12, out[5] = s5 ^ s13, out[6] = s6 ^ s14, out[7] = s7 ^ s15;
+  out[8] = s8 ^ cv[0], out[9] = s9 ^ cv[1], out[10] = s10 ^ cv[2], out[11] = s11 ^ cv[3];
+  out[12] = s12 ^ cv[4], out[13] = s13 ^ cv[5], out[14] = s14 ^ cv[6], out[15] = s15 ^ cv[7];
 }
 
 inline void words_le(const uint8_t* b, uint32_t w[16]) {
