@@ -22,14 +22,15 @@
 //
 // In the P_AB kernel one wave processes one 128-row block (one x_outer) at a
 // time; per block:
-//   phase 1, lane t = rows t and t + 64 (row r = (K, J) = (r & 7, r >> 3)): the
-//     64-point LDE of the row's a and b bits (a fixed GF(2)-linear map 64 bits -> 64 bytes, applied
+//   phase 1, per 64-row half, lane t = row 64 * half + t (row r = (K, J) =
+//     (r & 7, r >> 3)): the 64-point LDE of the row's a and b bits (a fixed GF(2)-linear map 64 bits -> 64 bytes, applied
 //     with 16 nibble tables of 64 bytes), v_ab = a_l * b_l in GF(2^8) from the
 //     full 64 KiB product table (one lookup of a * 256 + b, the index formed by
 //     one byte permute; FLOCK_R1_FULLTAB=0 selects split-nibble tables, two
 //     lookups), stored to LDS;
-//   phase 2a, lane = (byte word w, J-group): Horner over K for 4 J values and
-//     4 l at once (packed bytes), phi8 by two nibble lookups, shifted by J;
+//   phase 2a, lane = (byte word w, J-group): Horner over K for the half's 2 J
+//     values of its group and 4 l at once (packed bytes); after both halves,
+//     phi8 by two nibble lookups, shifted by J, for its 4 J;
 //   phase 2b, lane = l: the sum over the four J-groups, then one product by
 //     eq_outer with nibble-pair tables (comb4p_mul_add).
 // Every data-indexed 16-byte LDS table has 16 entries (conflict-free); the byte
@@ -203,11 +204,12 @@ static const Round1Tables* round1_tables_device(int* status) {
 constexpr int R1_ROWS = 128;  // rows of one block (one x_outer)
 // Waves per work-group: each wave processes whole blocks on its own (no
 // work-group barrier inside the block loop), so the waves of a CU drift apart
-// and one wave's table lookups overlap another's arithmetic.  With the full
-// 64 KiB product table one work-group of 8 waves fills the LDS (154 KiB); with
-// the split-nibble tables (8 KiB) two work-groups of 6 waves (78 KiB each).
+// and one wave's table lookups overlap another's arithmetic.  A wave's block
+// scratch holds one 64-row half of a block at a time (5.5 KiB), so with the
+// full 64 KiB product table one work-group of 12 waves fits the LDS (146 KiB);
+// with the split-nibble tables (8 KiB) two work-groups of 8 waves (69 KiB each).
 template <bool FULL>
-constexpr int r1_waves() { return FULL ? 8 : 6; }
+constexpr int r1_waves() { return FULL ? 12 : 8; }
 
 __device__ __forceinline__ uint32_t xtime4(uint32_t w) {
   return ((w & 0x7F7F7F7Fu) << 1) ^ (((w >> 7) & 0x01010101u) * 0x1Bu);
@@ -218,12 +220,13 @@ __device__ __forceinline__ uint32_t xtime4(uint32_t w) {
 // one lane stored is visible to every lane of the wave afterwards.
 __device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
-// Per-wave block scratch: row r's 16 product words at r * 17 + 16 * (r >> 5)
-// (the skew keeps the phase-2a column reads of the four J-groups in distinct
-// banks), later the phase-2a partial sums ypart[(q * 4 + j) * 80 + lane].
-constexpr uint32_t R1_RP = 17, R1_YP = 80;
-constexpr uint32_t R1_ROWWORDS = R1_ROWS * R1_RP + 48;
-__device__ __forceinline__ uint32_t r1_row(uint32_t r) { return r * R1_RP + 16u * (r >> 5); }
+// Per-wave block scratch: row r (0..63) of the current half at r * 17 (16
+// product words; 17 * 16 = 16 mod 64, so the phase-2a column reads of the four
+// J-groups fall in distinct banks), later the phase-2a partial sums at
+// ypart[(q * 4 + j) * 64 + (lane ^ 16 j)] (the XOR keeps the phase-2b gathers of
+// the four j in distinct banks).
+constexpr uint32_t R1_RP = 17, R1_YP = 64;
+constexpr uint32_t R1_SCRATCH = 20 * R1_YP;  // words; >= 64 * R1_RP
 
 // Each wave takes blocks blockIdx.x * NW + wave, then strides by gridDim.x * NW;
 // each block follows the phases described at the top of the file.
@@ -233,8 +236,8 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
     const uint64_t* __restrict__ A, const uint64_t* __restrict__ B, uint64_t n_blocks,
     const uint4* __restrict__ eq_outer, const Round1Tables* __restrict__ T, uint4* __restrict__ out) {
   constexpr int NW = r1_waves<FULL>(), R1_WG = NW * 64;
-  static_assert(20 * R1_YP <= R1_ROWWORDS, "ypart must fit in the block scratch");
-  static_assert(64 * 4 <= R1_ROWWORDS, "a wave's sums must fit in its block scratch");
+  static_assert(64 * R1_RP <= R1_SCRATCH, "a half block's rows must fit in the scratch");
+  static_assert(64 * 4 <= R1_SCRATCH, "a wave's sums must fit in its scratch");
   // LDE contributions, chunk-major: lde16[(p * 4 + q) * 16 + v] is bytes 16q..16q+15
   // of the 64-byte image of nibble value v at nibble position p.  For a fixed
   // (p, q) the 16 values sit in 16 distinct LDS bank groups, so the data-indexed
@@ -248,7 +251,7 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
     uint4 lde16[16 * 4 * 16];
     uint4 phin[32];  // phi8(n), phi8(n << 4): phi8(v) = phin[v & 15] ^ phin[16 + (v >> 4)]
     uint4 etab[NW][32];  // eq_outer * n and eq_outer * x^4 * n (comb4p_mul_add)
-    uint32_t rowdata[NW][R1_ROWWORDS];
+    uint32_t scratch[NW][R1_SCRATCH];
   };
   __shared__ Lds L;
   uint4* const lde16 = L.lde16;
@@ -256,7 +259,7 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
   uint8_t* const mul_hi = L.mul + 4096;
   uint4* const phin = L.phin;
   const uint32_t tid = threadIdx.x, wave = tid >> 6, t = tid & 63u;
-  uint32_t* const rows = L.rowdata[wave];
+  uint32_t* const rows = L.scratch[wave];
   uint32_t* const ypart = rows;
   uint4* const etab = L.etab[wave];
   {
@@ -278,17 +281,25 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
     }
     if (tid < 32) phin[tid] = T->phi_nib[tid];
   }
-  // phase-2a role: 16 words x 4 J-groups of four J
+  // phase-2a role: 16 words x 4 J-groups; the group jg takes J = 2jg, 2jg + 1 of
+  // each half (J = 8 * half + 2 * jg + jj)
   const uint32_t w = t & 15u, jg = t >> 4;
   uint4 acc = f_zero();
   __syncthreads();  // the tables are read-only from here; the waves run apart
 
   for (uint64_t blk = (uint64_t)blockIdx.x * NW + wave; blk < n_blocks; blk += (uint64_t)gridDim.x * NW) {
-    // ---- phase 1: rows t and t + 64, row r = (K, J) = (r & 7, r >> 3) ----
+    // eq_outer[blk] nibble tables for comb4p_mul_add
+    if (t < 32) {
+      uint4 e = eq_outer[blk];
+      if (t >= 16) e = f_mul_x4(e);
+      etab[t] = comb_entry(e, t & 15u);
+    }
+    uint32_t wj[4];
 #pragma unroll 1
     for (uint32_t half = 0; half < 2; ++half) {
-      const uint32_t r = t + 64u * half;
-      const uint64_t row = blk * R1_ROWS + r;
+      // ---- phase 1: row t of the half, block row r = 64 * half + t = (K, J) =
+      // (r & 7, r >> 3) ----
+      const uint64_t row = blk * R1_ROWS + 64u * half + t;
       const uint64_t a = A[row], b = B[row];
       uint32_t ah[16], bh[16];
 #pragma unroll
@@ -313,7 +324,7 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
           bh[4 * q + 3] = xor3(bh[4 * q + 3], y0.w, y1.w);
         }
       }
-      uint32_t* rd = rows + r1_row(r);
+      uint32_t* rd = rows + t * R1_RP;
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         uint32_t v = 0;
@@ -330,37 +341,38 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
         }
         rd[q] = v;
       }
-    }
-    // eq_outer[blk] nibble tables for comb4p_mul_add
-    if (t < 32) {
-      uint4 e = eq_outer[blk];
-      if (t >= 16) e = f_mul_x4(e);
-      etab[t] = comb_entry(e, t & 15u);
-    }
-    wave_lds_sync();
-    // ---- phase 2a: lane (w, jg): Horner over K in alpha = 0x02 (xtime on packed
-    // bytes: 4 l per word) for the four J of its group ----
-    uint32_t wj[4];
+      wave_lds_sync();
+      // ---- phase 2a (this half): lane (w, jg): Horner over K in alpha = 0x02
+      // (xtime on packed bytes: 4 l per word) for J = 8 * half + 2 * jg + jj ----
+      uint32_t h[2];
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-      const uint32_t* col = rows + r1_row(32u * jg + 8u * jj) + w;  // rows 32jg + 8jj + K share r >> 5
-      uint32_t h = 0;
+      for (int jj = 0; jj < 2; ++jj) {
+        const uint32_t* col = rows + (8u * (2u * jg + jj)) * R1_RP + w;
+        h[jj] = 0;
 #pragma unroll
-      for (int K = 7; K >= 0; --K) h = xtime4(h) ^ col[R1_RP * K];
-      wj[jj] = h;
+        for (int K = 7; K >= 0; --K) h[jj] = xtime4(h[jj]) ^ col[R1_RP * K];
+      }
+      // constant indices on both sides of the (wave-uniform) branch keep wj in registers
+      if (half == 0) {
+        wj[0] = h[0];
+        wj[1] = h[1];
+      } else {
+        wj[2] = h[0];
+        wj[3] = h[1];
+      }
+      wave_lds_sync();  // every lane has read the half's rows; the next writes reuse them
     }
-    wave_lds_sync();  // every lane has read the rows; ypart reuses them
     uint32_t Y[4][5];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int q = 0; q < 5; ++q) Y[j][q] = 0u;
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-      const uint32_t J = 4 * jg + jj;  // shift phi8(w) by J < 16 bits
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t J = 8 * (i >> 1) + 2 * jg + (i & 1);  // shift phi8(w) by J < 16 bits
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const uint32_t byte = (wj[jj] >> (8 * j)) & 255u;
+        const uint32_t byte = (wj[i] >> (8 * j)) & 255u;
         const uint4 f0 = phin[byte & 15u], f1 = phin[16 + (byte >> 4)];
         const uint4 f = make_uint4(f0.x ^ f1.x, f0.y ^ f1.y, f0.z ^ f1.z, f0.w ^ f1.w);
         if (J == 0) {
@@ -377,7 +389,7 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int q = 0; q < 5; ++q) ypart[(q * 4 + j) * R1_YP + t] = Y[j][q];
+      for (int q = 0; q < 5; ++q) ypart[(q * 4 + j) * R1_YP + (t ^ (16u * j))] = Y[j][q];
     wave_lds_sync();
     // ---- phase 2b: lane l: finish the J sum over the four J-groups, multiply by
     // eq_outer ----
@@ -386,13 +398,15 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
       uint32_t y[8];
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
-        const uint32_t* yp = ypart + (q * 4 + j) * R1_YP + ww;  // writer lane of J-group g: ww + 16 g
-        y[q] = xor3(yp[0], yp[16], yp[32]) ^ yp[48];
+        // writer lane of J-group g: ww + 16 g, stored at (ww + 16 g) ^ 16 j
+        const uint32_t* yp = ypart + (q * 4 + j) * R1_YP;
+        y[q] = xor3(yp[ww ^ (16u * j)], yp[(ww + 16u) ^ (16u * j)], yp[(ww + 32u) ^ (16u * j)]) ^
+               yp[(ww + 48u) ^ (16u * j)];
       }
       y[5] = y[6] = y[7] = 0u;
       acc = comb4p_mul_add(f_reduce_add(y, f_zero()), acc, etab);
     }
-    wave_lds_sync();  // the next block overwrites rows, ypart and etab
+    wave_lds_sync();  // the next block overwrites the scratch and etab
   }
   // Sum the waves' accumulators for output l = t, then store the work-group's
   // partial at out[l * gridDim.x + blockIdx.x].
@@ -401,7 +415,7 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
   __syncthreads();
   if (wave == 0) {
 #pragma unroll
-    for (int k = 1; k < NW; ++k) acc = f_add(acc, reinterpret_cast<const uint4*>(L.rowdata[k])[t]);
+    for (int k = 1; k < NW; ++k) acc = f_add(acc, reinterpret_cast<const uint4*>(L.scratch[k])[t]);
     out[(size_t)t * gridDim.x + blockIdx.x] = acc;
   }
 }
@@ -409,28 +423,37 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
 // P_C[l] = sum_s phi8(M[s][l]) * Q[s], written to out_c[l].  S holds the
 // column sums over row pairs (S[s]: even rows, S[64 + s]: odd rows, each
 // weighted by eq over r[7..m)), so Q[s] = S[s] + r6 * (S[s] + S[64 + s]).
-// R[s][i] = phi8(2^i) * Q[s], and lane l XORs the R[s][i] selected by the bits
-// of M[s][l].
-__global__ void __launch_bounds__(64) zc_round1_lift_c_kernel(const uint4* __restrict__ S, uint4 r6,
-                                                             const Round1Tables* __restrict__ T,
-                                                             uint4* __restrict__ out_c) {
+// R[s][i] = phi8(2^i) * Q[s] (two products per thread), then thread (l, part)
+// XORs the R[s][i] selected by the bits of M[s][l] for its 16 s, and the four
+// parts of each l are summed.
+__global__ void __launch_bounds__(256) zc_round1_lift_c_kernel(const uint4* __restrict__ S, uint4 r6,
+                                                              const Round1Tables* __restrict__ T,
+                                                              uint4* __restrict__ out_c) {
   __shared__ uint4 tabs[9 * 16];
+  __shared__ uint4 Q[64];
   __shared__ uint4 R[64 * 8];
+  __shared__ uint4 part_sum[3][64];
   const uint32_t t = threadIdx.x;
-  for (uint32_t e = t; e < 144; e += 64) tabs[e] = comb_entry(e < 128 ? T->phi_pow[e >> 4] : r6, e & 15u);
+  if (t < 144) tabs[t] = comb_entry(t < 128 ? T->phi_pow[t >> 4] : r6, t & 15u);
   __syncthreads();
-  const uint4 se = S[t], so = S[64 + t];
-  const uint4 q = f_add(se, comb_mul<1>(f_add(se, so), tabs + 128));
-  for (int i = 0; i < 8; ++i) R[t * 8 + i] = comb_mul<1>(q, tabs + i * 16);
+  if (t < 64) {
+    const uint4 se = S[t], so = S[64 + t];
+    Q[t] = f_add(se, comb_mul<1>(f_add(se, so), tabs + 128));
+  }
   __syncthreads();
+  for (uint32_t e = t; e < 64 * 8; e += 256) R[e] = comb_mul<1>(Q[e >> 3], tabs + (e & 7u) * 16);
+  __syncthreads();
+  const uint32_t l = t & 63u, part = t >> 6;
   uint4 acc = f_zero();
-  for (int s = 0; s < 64; ++s) {
-    const uint32_t m = T->M[s * 64 + t];
+  for (uint32_t s = 16 * part; s < 16 * part + 16; ++s) {
+    const uint32_t m = T->M[s * 64 + l];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
       if ((m >> i) & 1u) acc = f_add(acc, R[s * 8 + i]);
   }
-  out_c[t] = acc;
+  if (part) part_sum[part - 1][l] = acc;
+  __syncthreads();
+  if (part == 0) out_c[l] = f_add(f_add(acc, part_sum[0][l]), f_add(part_sum[1][l], part_sum[2][l]));
 }
 
 __global__ void scale_kernel(uint4* __restrict__ v, uint32_t n, uint4 c) {
@@ -495,10 +518,10 @@ extern "C" int flk_zc_round1(const uint8_t* a_bits, const uint8_t* b_bits, const
   }
   if (rc) return fail(rc);
   uint4* out = reinterpret_cast<uint4*>(out_ab);
-  hipLaunchKernelGGL(zc_round1_lift_c_kernel, dim3(1), dim3(64), 0, st, S, u4_of(fixed[0]), T, out + 64);
+  hipLaunchKernelGGL(zc_round1_lift_c_kernel, dim3(1), dim3(256), 0, st, S, u4_of(fixed[0]), T, out + 64);
   FLK_LAUNCH_CHECK("zc_round1_lift_c_kernel");
-  // The full 64 KiB product table (one work-group of 8 waves per CU) unless
-  // FLOCK_R1_FULLTAB=0 selects the split-nibble tables (two work-groups of 6
+  // The full 64 KiB product table (one work-group of 12 waves per CU) unless
+  // FLOCK_R1_FULLTAB=0 selects the split-nibble tables (two work-groups of 8
   // waves per CU).  Grids of 512 (full) or 1024 (split) work-groups: several
   // blocks per wave, so the waves' uneven block counts even out.
   static const bool full = [] {
