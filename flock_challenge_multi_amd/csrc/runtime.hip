@@ -6,6 +6,7 @@
 #include <immintrin.h>
 
 #include <atomic>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -497,14 +498,48 @@ __global__ void zero_n_kernel(uint4* __restrict__ p, uint64_t n) {
     p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+// Small device -> host reads (<= 64 KiB, the round messages and challenges of
+// the provers) go through the calling thread's mailbox: coherent, mapped pinned
+// memory that a one-work-group kernel fills with system-scope stores before it
+// sets a sequence word; the host spins on that word instead of synchronizing
+// the stream (7 us against 14 us per read-back on the MI355X box).  When the
+// word has not appeared after MAILBOX_SPIN (a long kernel ahead on the stream,
+// or a failed one) the host synchronizes the stream, which also reports a
+// device failure.  Seeing the word means every earlier operation on the stream
+// has completed, as after a synchronization.
+constexpr uint32_t MAILBOX_BYTES = 64u << 10;
+constexpr double MAILBOX_SPIN_S = 2e-3;
+
+__global__ void __launch_bounds__(256) publish_kernel(const uint8_t* __restrict__ src, uint8_t* __restrict__ box,
+                                                      uint32_t bytes, uint32_t* flag, uint32_t seq) {
+  if (((uintptr_t)src | bytes) % 16 == 0) {
+    for (uint32_t i = threadIdx.x; i < bytes / 16; i += blockDim.x)
+      reinterpret_cast<uint4*>(box)[i] = reinterpret_cast<const uint4*>(src)[i];
+  } else {
+    for (uint32_t i = threadIdx.x; i < bytes; i += blockDim.x) box[i] = src[i];
+  }
+  __threadfence_system();  // every thread's stores reach host memory before the flag
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+struct Mailbox {
+  uint8_t* host = nullptr;  // MAILBOX_BYTES of data, then the sequence word
+  uint8_t* dev = nullptr;
+  uint32_t seq = 0;
+  int device = -1;
+  bool failed = false;
+};
+
 // Synchronous device -> host read through a per-thread pinned bounce buffer
 // (1 MiB, allocated on first use and kept): a pageable destination makes the
 // runtime stage the copy itself, 23 us per small read-back on the MI355X box
 // against 14 us through pinned memory.
-hipError_t read_to_host(void* dst, const void* src, size_t bytes, hipStream_t st) {
+static hipError_t read_bounce(void* dst, const void* src, size_t bytes, hipStream_t st) {
   constexpr size_t BOUNCE = size_t(1) << 20;
   static thread_local void* bounce = nullptr;
-  if (bytes == 0) return hipSuccess;
   if (bytes <= BOUNCE && !bounce) {
     if (hipHostMalloc(&bounce, BOUNCE, hipHostMallocDefault) != hipSuccess) {
       (void)hipGetLastError();
@@ -521,6 +556,57 @@ hipError_t read_to_host(void* dst, const void* src, size_t bytes, hipStream_t st
     if (e == hipSuccess) e = hipStreamSynchronize(st);
   }
   return e;
+}
+
+hipError_t read_to_host(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return hipSuccess;
+  static thread_local Mailbox mb;
+  int dev = 0;
+  if (bytes > MAILBOX_BYTES || hipGetDevice(&dev) != hipSuccess) return read_bounce(dst, src, bytes, st);
+  if (mb.device != dev && !mb.failed) {
+    // One mailbox per thread, for its first device (a thread that moves to
+    // another device falls back to the bounce buffer).
+    if (mb.device == -1) {
+      void* h = nullptr;
+      void* d = nullptr;
+      if (hipHostMalloc(&h, MAILBOX_BYTES + 256, hipHostMallocCoherent | hipHostMallocMapped) == hipSuccess &&
+          hipHostGetDevicePointer(&d, h, 0) == hipSuccess) {
+        memset(h, 0, MAILBOX_BYTES + 256);
+        mb.host = static_cast<uint8_t*>(h);
+        mb.dev = static_cast<uint8_t*>(d);
+        mb.device = dev;
+      } else {
+        (void)hipGetLastError();
+        if (h) (void)hipHostFree(h);
+        mb.failed = true;
+      }
+    }
+  }
+  if (mb.failed || mb.device != dev) return read_bounce(dst, src, bytes, st);
+  const uint32_t seq = ++mb.seq;
+  uint32_t* flag_dev = reinterpret_cast<uint32_t*>(mb.dev + MAILBOX_BYTES);
+  const uint32_t* flag_host = reinterpret_cast<const uint32_t*>(mb.host + MAILBOX_BYTES);
+  hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(256), 0, st, static_cast<const uint8_t*>(src), mb.dev,
+                     (uint32_t)bytes, flag_dev, seq);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const auto t0 = std::chrono::steady_clock::now();
+  bool seen = false;
+  for (;;) {
+    if (__atomic_load_n(flag_host, __ATOMIC_ACQUIRE) == seq) {
+      seen = true;
+      break;
+    }
+    if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > MAILBOX_SPIN_S) break;
+    _mm_pause();
+  }
+  if (!seen) {
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    if (__atomic_load_n(flag_host, __ATOMIC_ACQUIRE) != seq) return hipErrorUnknown;
+  }
+  memcpy(dst, mb.host, bytes);
+  return hipSuccess;
 }
 
 int zero_n(uint4* p, uint64_t n, hipStream_t st) {
