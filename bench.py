@@ -14,10 +14,10 @@ second, whole job.  p50/p99 are the per-proof latencies of the same loop, and
 checked (untimed) by the restated verifier in tests/, with the commitment root
 recomputed on the CPU from the regenerated witness.
 
-`roofline` is that of zerocheck round 1 (flk_zc_round1, the proof's longest
-kernel), timed alone with HIP events on its launch stream; `proof_kernels` adds
+`roofline` is that of zerocheck round 1 (flk_zc_round1: the proof's longest
+kernel and the column sums of c), timed alone with HIP events on its launch stream; `proof_kernels` adds
 the L0 RS encode.  `cpu_baseline` is the oracle's CPU prover (tests/oracle_prover.py)
-on one proof of 2^15 compressions, whose bytes must equal the device proof of the
+on one proof of 2^16 compressions, whose bytes must equal the device proof of the
 same input.  `commit_config1` is the standalone L0 commit of 2^15 compressions
 (m = 29) with its NTT roofline, witness and proof-of-work figures.
 """
@@ -32,10 +32,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
-# HBM bytes per zerocheck round-1 launch at m = 32 from two rocprofv3 --pmc passes
-# (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB device copy: read = 2.00 x FETCH_SIZE,
-# write = 1.00 x WRITE_SIZE (profiles/proof_m32.md)
-ZC_ROUND1_TRAFFIC_M32 = 1_643_786_880
+# HBM bytes per flk_zc_round1 call at m = 32 (the c column-sum fold and the a.b row
+# kernel, with their eq tables) from two rocprofv3 --pmc passes (FETCH_SIZE,
+# WRITE_SIZE) scaled by a 1 GiB device copy: read = 2.00 x FETCH_SIZE, write =
+# 1.00 x WRITE_SIZE (profiles/proof_m32.md)
+ZC_ROUND1_TRAFFIC_M32 = 1_655_464_456
 
 
 def parse():
@@ -908,8 +909,8 @@ def main():
     roofline = None if dom is None else {
         "bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": ZC_ROUND1_TRAFFIC_M32 if head["m"] == 32 else None,
-        "kernel": f"zerocheck round 1 (zc_round1_kernel, one launch per proof): algorithmic bytes "
-                  f"3 * 2^m / 8 = {dom['bytes']} per launch, {dom['ms']:.3f} ms per launch",
+        "kernel": f"zerocheck round 1 (flk_zc_round1, one call per proof: the c column-sum fold and the a.b "
+                  f"row kernel): algorithmic bytes 3 * 2^m / 8 = {dom['bytes']} per call, {dom['ms']:.3f} ms per call",
     }
     line = {
         "metric": "verified BLAKE3 compressions/s (2^18 batch, 1 GPU) + p50/p99 proof latency",

@@ -130,6 +130,7 @@ SIGNATURES = {
     "flk_read_to_host": (_I, [_P, _P, _SZ, _P]),
     "flk_zc_fold_round": (_I, [_P, _P, _SZ, _P, _P, _P, _SZ, _P, _P, _P, _P, _P]),
     "flk_zc_fold_at_z_round": (_I, [_P, _P, _SZ, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "flk_zc_fold_at_z_bind_round": (_I, [_P, _P, _SZ, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "flk_blake3_setup_arena_stats": (_I, [_P] + [ctypes.POINTER(_SZ)] * 4),
     "flk_prover_pool_new": (_I, [ctypes.c_int, _SZ, _SZ, ctypes.POINTER(_P)]),
     "flk_prover_pool_free": (None, [_P]),

@@ -114,18 +114,28 @@ void zerocheck_prove(Fs& ch, const uint8_t* a_bits, const uint8_t* b_bits, const
   // fold_in_place_pair; multilinear.rs:575-640, 3194-3213).  While the round's eq
   // has >= 8 coordinates the device runs each round as one fused pass: the bind by
   // the previous challenge, the new tables written, and the message against the eq
-  // split into its first 8 coordinates and the rest (flk_zc_fold_round).  The last
-  // rounds (tables of <= 512 entries) run on the host.
+  // split into its first 8 coordinates and the rest (flk_zc_fold_round).  Rounds 0
+  // and 1 read the a and b bits themselves (flk_zc_fold_at_z_round without tables,
+  // flk_zc_fold_at_z_bind_round), so the 2^(m-6)-entry fold at z is never stored.
+  // The last rounds (tables of <= 512 entries) run on the host.
   chk(flk::lagrange_weights_nodes(K_SKIP, z, 0, ws.data()), "lagrange_weights(S)");
   DeviceVec<F128> wd(ws, st);
   size_t n = size_t(1) << n_mlv;
-  DeviceVec<F128> am(n, st), bm(n, st), msg(2, st);
+  DeviceVec<F128> am, bm, msg(2, st);
   const size_t d0 = n_mlv - 1;
   // eq_lo: each device round's first 8 eq coordinates (all known before round 0,
   // so the tables are built up front, off the rounds' critical path); eq_hi (two
   // buffers, alternating): the rest, handed from round to round by summing out
   // the lowest coordinate.
   const size_t n_dev = d0 >= 8 ? d0 - 7 : 0;  // rounds i with n_mlv - i - 1 >= 8
+  // With two such rounds, rounds 0 and 1 both read the bits and the fold at z is
+  // never written (round 0 sends the message alone, round 1 writes the table
+  // bound by rho_0); otherwise the fold at z is written in full.
+  const bool from_bits = n_dev >= 2;
+  if (!from_bits) {
+    am = DeviceVec<F128>(n, st);
+    bm = DeviceVec<F128>(n, st);
+  }
   DeviceVec<F128> eq_lo(256 * (n_dev ? n_dev : 1), st),
       eq_hi[2] = {DeviceVec<F128>(d0 >= 8 ? size_t(1) << (d0 - 8) : 1, st),
                   DeviceVec<F128>(d0 >= 9 ? size_t(1) << (d0 - 9) : 1, st)};
@@ -154,9 +164,16 @@ void zerocheck_prove(Fs& ch, const uint8_t* a_bits, const uint8_t* b_bits, const
       DeviceVec<F128>& hi_next = eq_hi[(i + 1) & 1];
       const F128* lo = eq_lo.data() + 256 * i;  // eq over pt[0 .. 8)
       if (i == 0) {  // the fold at z and the first message in one pass over the bits
-        chk(flk_zc_fold_at_z_round(a_bits, b_bits, m, wd.data(), lo, hi.data(), am.data(), bm.data(),
-                                   hi_next.data(), msg.data(), st),
+        chk(flk_zc_fold_at_z_round(a_bits, b_bits, m, wd.data(), lo, hi.data(), from_bits ? nullptr : am.data(),
+                                   from_bits ? nullptr : bm.data(), hi_next.data(), msg.data(), st),
             "zc_fold_at_z_round");
+      } else if (i == 1 && from_bits) {  // the fold at z bound by rho_0, and the second message
+        am = DeviceVec<F128>(n / 2, st);
+        bm = DeviceVec<F128>(n / 2, st);
+        chk(flk_zc_fold_at_z_bind_round(a_bits, b_bits, m, wd.data(), &rhos.back(), lo, hi.data(), am.data(),
+                                        bm.data(), hi_next.data(), msg.data(), st),
+            "zc_fold_at_z_bind_round");
+        n /= 2;
       } else {
         DeviceVec<F128> an(n / 2, st), bn(n / 2, st);
         chk(flk_zc_fold_round(am.data(), bm.data(), n, &rhos.back(), lo, hi.data(), d - 8, an.data(),

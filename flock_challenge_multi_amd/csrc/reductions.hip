@@ -10,6 +10,9 @@
 //                                                         CF/zerocheck.rs:1748-1781
 //   flk_zc_fold_round       fold_in_place_pair + round_pair against a split eq
 //                                                         CF/zerocheck/multilinear.rs:616-640,3194-3213
+//   flk_zc_fold_at_z_round  fold_at_z + round_pair from the bits (with or without the table)
+//   flk_zc_fold_at_z_bind_round  fold_at_z + fold_in_place_pair + round_pair from the bits
+//                                                         CF/zerocheck/multilinear.rs:575-683,744-790
 //
 // These are XOR reductions of F2-linear maps over bit-packed witness data.  The
 // sums of known values selected by bits ("sum eq[i] over the set bits") use
@@ -546,6 +549,153 @@ __global__ void __launch_bounds__(ZF_WG) zc_fold_round_kernel(const uint4* __res
   wg_reduce_store(ri, red[1], out, 1);
 }
 
+// ---------------------------------------------------------------------------
+// The first two multilinear rounds straight from the 64-bit rows of the a and b
+// bit vectors, with no table of the fold at z written (the reference's
+// uni_skip_fold_and_round_pair over its UniSkipFoldTable,
+// CF/zerocheck/multilinear.rs:659-683,744-790).  fold_at_z is GF(2)-linear in a
+// row's bits: T[x] = sum_s w_s a(s, x) = N(row x), 16 lookups in the nibble
+// tables of the weights (nib[16 p + k] = sum of w_{4p+j} over the bits j of k).
+//
+// ZB_MSG (round 0): the message over T against eq[x] = eq_lo[x mod 256] *
+//   eq_hi[x >> 8].  eq_hi[c] is one value per chunk c of 256 pairs; it goes into
+//   the weights of the b side instead of into each pair: chunk c reads its b rows
+//   through the nibble tables of eq_hi[c] * w_s (ew[64 c + s],
+//   zc_scaled_weights_kernel), so G(1) = sum_x eq_lo * a[2x+1] * (eq_hi b)[2x+1]
+//   takes no eq product per pair, and eq_lo is applied per lane at the end.
+// ZB_FOLD (round 1): T' = fold(T, rho) from the rows, T'[y] = T[2y] + rho (T[2y] +
+//   T[2y+1]) = N(row 2y) + N_rho(row 2y ^ row 2y+1), with N_rho the nibble tables
+//   of rho * w_s; T' is written and its message summed as zc_fold_round_kernel
+//   sums it (eq_hi through the chunk's nibble-position tables).
+// ---------------------------------------------------------------------------
+enum ZbMode { ZB_MSG = 0, ZB_FOLD = 1 };
+
+// Entry e (< 256) of the nibble subset-sum tables of the 64 values v[s]:
+// XOR of v[4p + j] over the bits j of k, p = e >> 4, k = e & 15.
+__device__ __forceinline__ uint4 nib_entry(const uint4* v, uint32_t e) {
+  const uint32_t p = e >> 4, k = e & 15u;
+  uint4 r = f_zero();
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if ((k >> j) & 1u) r = f_add(r, v[4 * p + j]);
+  return r;
+}
+
+// The sum selected by one 64-bit row (low word lo: nibbles 0..7, hi: 8..15).
+__device__ __forceinline__ uint4 row_sum(uint32_t lo, uint32_t hi, const uint4* nib) {
+  return f_add(nibtab8_sum(lo, nib), nibtab8_sum(hi, nib + 128));
+}
+
+// ew[64 c + s] = e_hi[c] * w[s] for c < n_hi, s < 64.  Work-group (x, y) builds the
+// nibble-position tables of w[16 y .. 16 y + 16); wave v multiplies by weight
+// 16 y + 4 k + v (k = 0..3), one e_hi value per lane, so the 64 lanes of a lookup
+// share one table (conflict-free).
+__global__ void __launch_bounds__(256) zc_scaled_weights_kernel(const uint4* __restrict__ w,
+                                                                const uint4* __restrict__ e_hi, uint64_t n_hi,
+                                                                uint4* __restrict__ ew) {
+  __shared__ uint4 tab[16 * 128], nib[16 * 32];
+  const uint32_t s0 = blockIdx.y * 16u, wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  build_pos_tables(tab, nib, 16, [&](uint32_t c) { return w[s0 + c]; });
+  for (uint64_t c = (uint64_t)blockIdx.x * 64 + lane; c < n_hi; c += (uint64_t)gridDim.x * 64) {
+    const uint4 e = e_hi[c];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+      const uint32_t s = 4 * k + wave;
+      ew[c * 64 + s0 + s] = combjr_mul_add(e, f_zero(), tab + s * 128);
+    }
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(ZF_WG) zc_bits_round_kernel(const uint4* __restrict__ a, const uint4* __restrict__ b,
+                                                             uint64_t n_chunks, uint4 rho,
+                                                             const uint4* __restrict__ w,
+                                                             const uint4* __restrict__ ew,
+                                                             const uint4* __restrict__ e_lo,
+                                                             const uint4* __restrict__ e_hi, uint4* __restrict__ a_out,
+                                                             uint4* __restrict__ b_out, uint4* __restrict__ e_hi_next,
+                                                             uint4* __restrict__ out) {
+  __shared__ uint4 tabs[16 * ZF_WG];  // per-lane tables of the data x data products
+  __shared__ uint4 wnib[256];         // N: nibble tables of w
+  __shared__ uint4 xnib[256];         // ZB_MSG: of eq_hi[c] * w (per chunk); ZB_FOLD: of rho * w (N_rho)
+  __shared__ uint4 htab[MODE == ZB_FOLD ? 128 : 1], hnib[MODE == ZB_FOLD ? 32 : 1];
+  __shared__ uint4 red[2][ZF_WG / 64];
+  const uint32_t tid = threadIdx.x;
+  wnib[tid] = nib_entry(w, tid);
+  if (MODE == ZB_FOLD) {
+    // rho * w_s through rho's nibble-position tables (htab is rebuilt per chunk
+    // afterwards), staged in the per-lane table area before its first use
+    build_pos_tables(htab, hnib, 1, [&](uint32_t) { return rho; });
+    if (tid < 64) tabs[tid] = combjr_mul_add(w[tid], f_zero(), htab);
+    __syncthreads();
+    xnib[tid] = nib_entry(tabs, tid);
+  }
+  __syncthreads();
+  U256 g1, gi;
+  u256_zero(g1);
+  u256_zero(gi);
+  uint4* my = tabs + tid;
+  // A pair's rows are loaded one chunk ahead: ZB_MSG rows 2x, 2x + 1 (one uint4 per
+  // vector), ZB_FOLD rows 4y .. 4y + 3 (two).
+  constexpr int NL = MODE == ZB_FOLD ? 2 : 1;
+  uint4 u[NL], v[NL];
+  auto load = [&](uint64_t c) {
+    const uint64_t x = c * ZF_WG + tid;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      u[i] = a[NL * x + i];
+      v[i] = b[NL * x + i];
+    }
+  };
+  if (blockIdx.x < n_chunks) load(blockIdx.x);
+  for (uint64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const uint4 eh = e_hi[c];
+    if (e_hi_next && tid == 0 && !(c & 1)) e_hi_next[c >> 1] = f_add(eh, e_hi[c + 1]);
+    uint4 cu[NL], cv[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      cu[i] = u[i];
+      cv[i] = v[i];
+    }
+    if (c + gridDim.x < n_chunks) load(c + gridDim.x);
+    const uint64_t x = c * ZF_WG + tid;
+    uint4 a1, as, b1, bs;
+    if (MODE == ZB_MSG) {
+      const uint4 xe = nib_entry(ew + c * 64, tid);  // this lane's entry of the chunk's b tables
+      __syncthreads();  // the previous chunk's lookups in xnib are done
+      xnib[tid] = xe;
+      __syncthreads();
+      const uint4 ra = cu[0], rb = cv[0];  // rows 2x (x, y words) and 2x + 1 (z, w)
+      a1 = row_sum(ra.z, ra.w, wnib);
+      as = row_sum(ra.x ^ ra.z, ra.y ^ ra.w, wnib);
+      b1 = row_sum(rb.z, rb.w, xnib);
+      bs = row_sum(rb.x ^ rb.z, rb.y ^ rb.w, xnib);
+    } else {
+      build_pos_tables(htab, hnib, 1, [&](uint32_t) { return eh; });  // two barriers: the last chunk's lookups are done
+      const uint4 a0 = f_add(row_sum(cu[0].x, cu[0].y, wnib), row_sum(cu[0].x ^ cu[0].z, cu[0].y ^ cu[0].w, xnib));
+      const uint4 ta1 = f_add(row_sum(cu[1].x, cu[1].y, wnib), row_sum(cu[1].x ^ cu[1].z, cu[1].y ^ cu[1].w, xnib));
+      const uint4 b0 = f_add(row_sum(cv[0].x, cv[0].y, wnib), row_sum(cv[0].x ^ cv[0].z, cv[0].y ^ cv[0].w, xnib));
+      b1 = f_add(row_sum(cv[1].x, cv[1].y, wnib), row_sum(cv[1].x ^ cv[1].z, cv[1].y ^ cv[1].w, xnib));
+      a_out[2 * x] = a0;
+      a_out[2 * x + 1] = ta1;
+      b_out[2 * x] = b0;
+      b_out[2 * x + 1] = b1;
+      bs = f_add(b0, b1);
+      a1 = combjr_mul_add(ta1, f_zero(), htab);
+      as = combjr_mul_add(f_add(a0, ta1), f_zero(), htab);
+    }
+    table_store_private(tabs, ZF_WG, tid, a1);
+    comb_mul_acc_rt(b1, my, ZF_WG, g1);
+    table_store_private(tabs, ZF_WG, tid, as);
+    comb_mul_acc_rt(bs, my, ZF_WG, gi);
+  }
+  table_store_private(tabs, ZF_WG, tid, e_lo[tid]);  // the lane's own table: no barrier needed
+  const uint4 r1 = comb_mul_rt(f_reduce(g1), my, ZF_WG);
+  const uint4 ri = comb_mul_rt(f_reduce(gi), my, ZF_WG);
+  wg_reduce_store(r1, red[0], out, 0);
+  wg_reduce_store(ri, red[1], out, 1);
+}
+
 static unsigned grid_for(uint64_t items, int wg, unsigned cap) {
   uint64_t g = (items + wg - 1) / wg;
   if (g > cap) g = cap;
@@ -712,6 +862,65 @@ static int zc_fold_round_launch(int mode, const void* a, const void* b, uint64_t
   return partials_sum_free(parts, grid, 2, o, st);
 }
 
+// The two bits rounds (zc_bits_round_kernel): ZB_MSG takes the eq_hi-scaled
+// weights from a scratch table written first by zc_scaled_weights_kernel.
+static int zc_bits_round_launch(int mode, const uint8_t* a_bits, const uint8_t* b_bits, uint64_t hi_log,
+                                const flk_f128* rho, const flk_f128* w, const flk_f128* eq_lo, const flk_f128* eq_hi,
+                                flk_f128* a_out, flk_f128* b_out, flk_f128* eq_hi_next, flk_f128* out2,
+                                hipStream_t st) {
+  uint4* o = reinterpret_cast<uint4*>(out2);
+  const uint64_t n_chunks = uint64_t(1) << hi_log;
+  const unsigned grid = (unsigned)std::min<uint64_t>(n_chunks, 2048);
+  auto* kw = reinterpret_cast<const uint4*>(w);
+  auto* kh = reinterpret_cast<const uint4*>(eq_hi);
+  uint4* ew = nullptr;
+  if (mode == ZB_MSG) {
+    hipError_t e = stream_alloc(reinterpret_cast<void**>(&ew), n_chunks * 64 * sizeof(uint4), st);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(FLK_E_NOMEM, "zc_fold_at_z_round: scaled-weight scratch of %llu bytes",
+                       (unsigned long long)(n_chunks * 64 * sizeof(uint4)));
+    }
+    const unsigned gx = (unsigned)std::min<uint64_t>((n_chunks + 63) / 64, 1024);
+    hipLaunchKernelGGL(zc_scaled_weights_kernel, dim3(gx, 4), dim3(256), 0, st, kw, kh, n_chunks, ew);
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+      (void)stream_free(ew, st);
+      return device_failure(le, "zc_scaled_weights_kernel");
+    }
+  }
+  uint4* parts = nullptr;
+  int rc = partials_alloc(&parts, grid, 2, o, st);
+  if (rc) {
+    if (ew) (void)stream_free(ew, st);
+    return rc;
+  }
+  const uint4 r = rho ? u4(*rho) : f_zero_host();
+  auto* ka = reinterpret_cast<const uint4*>(a_bits);
+  auto* kb = reinterpret_cast<const uint4*>(b_bits);
+  auto* kl = reinterpret_cast<const uint4*>(eq_lo);
+  auto* ao = reinterpret_cast<uint4*>(a_out);
+  auto* bo = reinterpret_cast<uint4*>(b_out);
+  auto* hn = hi_log ? reinterpret_cast<uint4*>(eq_hi_next) : nullptr;
+  if (mode == ZB_MSG)
+    hipLaunchKernelGGL(zc_bits_round_kernel<ZB_MSG>, dim3(grid), dim3(ZF_WG), 0, st, ka, kb, n_chunks, r, kw, ew, kl,
+                       kh, ao, bo, hn, parts);
+  else
+    hipLaunchKernelGGL(zc_bits_round_kernel<ZB_FOLD>, dim3(grid), dim3(ZF_WG), 0, st, ka, kb, n_chunks, r, kw, ew, kl,
+                       kh, ao, bo, hn, parts);
+  hipError_t le = hipGetLastError();
+  if (le != hipSuccess) {
+    if (ew) (void)stream_free(ew, st);
+    return device_failure(le, "zc_bits_round_kernel");
+  }
+  rc = partials_sum_free(parts, grid, 2, o, st);
+  if (ew) {
+    hipError_t fe = stream_free(ew, st);
+    if (!rc && fe != hipSuccess) rc = device_failure(fe, "zc_fold_at_z_round: scratch free");
+  }
+  return rc;
+}
+
 int flk_zc_fold_round(const flk_f128* a, const flk_f128* b, size_t n, const flk_f128* rho, const flk_f128* eq_lo,
                       const flk_f128* eq_hi, size_t hi_log, flk_f128* a_out, flk_f128* b_out, flk_f128* eq_hi_next,
                       flk_f128* out2, void* stream) {
@@ -733,13 +942,31 @@ int flk_zc_fold_at_z_round(const uint8_t* a_bits, const uint8_t* b_bits, size_t 
                            flk_f128* eq_hi_next, flk_f128* out2, void* stream) {
   int rc = check_enabled();
   if (rc) return rc;
-  if (!a_bits || !b_bits || !weights || !eq_lo || !eq_hi || !a_out || !b_out || !out2)
+  if (!a_bits || !b_bits || !weights || !eq_lo || !eq_hi || !out2 || (!a_out != !b_out))
     return set_error(FLK_E_SHAPE, "zc_fold_at_z_round: null pointer");
   if (m < 6 + 1 + 8 || m > 40) return set_error(FLK_E_SHAPE, "zc_fold_at_z_round: m=%zu outside 15..40", m);
   if (((uintptr_t)a_bits | (uintptr_t)b_bits) % 16)
     return set_error(FLK_E_SHAPE, "zc_fold_at_z_round: bit vectors must be 16-byte aligned");
   // 2^(m-6) rows of 64 bits; the pairs of rows are 2^(m-7) = 256 * 2^hi_log
+  if (!a_out)  // the message alone: nothing written but out2 (and eq_hi_next)
+    return zc_bits_round_launch(ZB_MSG, a_bits, b_bits, m - 7 - 8, nullptr, weights, eq_lo, eq_hi, nullptr, nullptr,
+                                eq_hi_next, out2, static_cast<hipStream_t>(stream));
   return zc_fold_round_launch(ZF_BITS, a_bits, b_bits, m - 7 - 8, nullptr, weights, eq_lo, eq_hi, a_out, b_out,
+                              eq_hi_next, out2, static_cast<hipStream_t>(stream));
+}
+
+int flk_zc_fold_at_z_bind_round(const uint8_t* a_bits, const uint8_t* b_bits, size_t m, const flk_f128* weights,
+                                const flk_f128* rho, const flk_f128* eq_lo, const flk_f128* eq_hi, flk_f128* a_out,
+                                flk_f128* b_out, flk_f128* eq_hi_next, flk_f128* out2, void* stream) {
+  int rc = check_enabled();
+  if (rc) return rc;
+  if (!a_bits || !b_bits || !weights || !rho || !eq_lo || !eq_hi || !a_out || !b_out || !out2)
+    return set_error(FLK_E_SHAPE, "zc_fold_at_z_bind_round: null pointer");
+  if (m < 6 + 2 + 8 || m > 40) return set_error(FLK_E_SHAPE, "zc_fold_at_z_bind_round: m=%zu outside 16..40", m);
+  if (((uintptr_t)a_bits | (uintptr_t)b_bits) % 16)
+    return set_error(FLK_E_SHAPE, "zc_fold_at_z_bind_round: bit vectors must be 16-byte aligned");
+  // T' has 2^(m-7) entries: 2^(m-8) = 256 * 2^hi_log pairs, each from four rows
+  return zc_bits_round_launch(ZB_FOLD, a_bits, b_bits, m - 8 - 8, rho, weights, eq_lo, eq_hi, a_out, b_out,
                               eq_hi_next, out2, static_cast<hipStream_t>(stream));
 }
 

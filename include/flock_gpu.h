@@ -361,10 +361,21 @@ int flk_zc_fold_round(const flk_f128 *a, const flk_f128 *b, size_t n, const flk_
  * multilinear.rs:575-640) for k_skip = 6: a_out, b_out = the fold at z of the
  * 2^(m-6) rows of 64 bits with the 64 device weights (as flk_zc_fold_at_z), and
  * out2 the message over them as flk_zc_fold_round without rho (hi_log = m - 15;
- * eq_hi_next as there).  m in 15..40; bit vectors 16-byte aligned. */
+ * eq_hi_next as there).  With a_out = b_out = NULL the tables are not written:
+ * only out2 (and eq_hi_next), as uni_skip_fold_and_round_pair (multilinear.rs:659-683)
+ * computes the round over its UniSkipFoldTable (:744-790).  m in 15..40; bit
+ * vectors 16-byte aligned. */
 int flk_zc_fold_at_z_round(const uint8_t *a_bits, const uint8_t *b_bits, size_t m, const flk_f128 *weights,
                            const flk_f128 *eq_lo, const flk_f128 *eq_hi, flk_f128 *a_out, flk_f128 *b_out,
                            flk_f128 *eq_hi_next, flk_f128 *out2, void *stream);
+/* The second multilinear round from the bits: T' = fold(T, rho) with T the fold
+ * at z of the rows (fold_at_z_naive then fold_in_place_pair, multilinear.rs:575-597,
+ * 3194-3213), 2^(m-7) entries into a_out, b_out, and out2 the message over T' as
+ * flk_zc_fold_round computes it (hi_log = m - 16; eq_hi_next as there).  rho is a
+ * HOST pointer.  m in 16..40; bit vectors 16-byte aligned. */
+int flk_zc_fold_at_z_bind_round(const uint8_t *a_bits, const uint8_t *b_bits, size_t m, const flk_f128 *weights,
+                                const flk_f128 *rho, const flk_f128 *eq_lo, const flk_f128 *eq_hi, flk_f128 *a_out,
+                                flk_f128 *b_out, flk_f128 *eq_hi_next, flk_f128 *out2, void *stream);
 
 /* Round 1 of the zerocheck (univariate skip over k_skip = 6): round1_naive
  * (univariate_skip.rs:90-152) for the protocol's challenge layout

@@ -182,6 +182,63 @@ def test_zc_fold_at_z_round_matches_oracle(orc, dev, m):
         assert same(from_dev(nxt), orc.build_eq(pt[9:]))
 
 
+@pytest.mark.parametrize("m", [15, 16, 22])
+def test_zc_fold_at_z_round_message_only(orc, dev, m):
+    """Round 0 without the tables (a_out = b_out = NULL): the eq_hi factor rides in
+    the b side's scaled weights, and the message still equals the oracle's round
+    pair over the fold at z against the full eq table."""
+    import ctypes
+    from flock_challenge_multi_amd import _native as nat
+    _log(f"zc_fold_at_z_round message-only m={m}")
+    rng = random.Random(100 + m)
+    abits, bbits = _rand_bytes(m + 17, (1 << m) // 8), _rand_bytes(m + 18, (1 << m) // 8)
+    w = orc.zc_lagrange_weights(6, rng.getrandbits(128))
+    hi_log = m - 15
+    pt = [rng.getrandbits(128) for _ in range(8 + hi_log)]
+    ta, tb = torch.from_numpy(abits).cuda(), torch.from_numpy(bbits).cuda()
+    out = hp.f128_empty(2)
+    nxt = hp.f128_empty(max(1, (1 << hi_log) // 2))
+    lo, hi, wd = hp.build_eq_table(pt[:8]), hp.build_eq_table(pt[8:]), to_dev(w)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    nat.call("flk_zc_fold_at_z_round", p(ta), p(tb), m, p(wd), p(lo), p(hi), None, None,
+             p(nxt) if hi_log else None, p(out), hp._stream())
+    torch.cuda.synchronize()
+    a, b = orc.zc_fold_at_z(abits, m, 6, w), orc.zc_fold_at_z(bbits, m, 6, w)
+    assert same(from_dev(out), orc.zc_round_pair(a, b, orc.build_eq(pt)))
+    if hi_log:
+        assert same(from_dev(nxt), orc.build_eq(pt[9:]))
+
+
+@pytest.mark.parametrize("m", [16, 17, 23])
+def test_zc_fold_at_z_bind_round_matches_oracle(orc, dev, m):
+    """Round 1 from the bits: the written tables equal the oracle's fold at z bound
+    by rho (partial_eval_lsb), and the message its round pair over them."""
+    import ctypes
+    from flock_challenge_multi_amd import _native as nat
+    _log(f"zc_fold_at_z_bind_round m={m}")
+    rng = random.Random(200 + m)
+    abits, bbits = _rand_bytes(m + 27, (1 << m) // 8), _rand_bytes(m + 28, (1 << m) // 8)
+    w = orc.zc_lagrange_weights(6, rng.getrandbits(128))
+    rho = rng.getrandbits(128)
+    hi_log = m - 16
+    pt = [rng.getrandbits(128) for _ in range(8 + hi_log)]
+    ta, tb = torch.from_numpy(abits).cuda(), torch.from_numpy(bbits).cuda()
+    ao, bo, out = hp.f128_empty(1 << (m - 7)), hp.f128_empty(1 << (m - 7)), hp.f128_empty(2)
+    nxt = hp.f128_empty(max(1, (1 << hi_log) // 2))
+    lo, hi, wd = hp.build_eq_table(pt[:8]), hp.build_eq_table(pt[8:]), to_dev(w)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    r = nat.F128.from_int(rho)
+    nat.call("flk_zc_fold_at_z_bind_round", p(ta), p(tb), m, p(wd), ctypes.byref(r), p(lo), p(hi), p(ao), p(bo),
+             p(nxt) if hi_log else None, p(out), hp._stream())
+    torch.cuda.synchronize()
+    a = orc.partial_eval_lsb(orc.zc_fold_at_z(abits, m, 6, w), [rho])
+    b = orc.partial_eval_lsb(orc.zc_fold_at_z(bbits, m, 6, w), [rho])
+    assert same(from_dev(ao), a) and same(from_dev(bo), b)
+    assert same(from_dev(out), orc.zc_round_pair(a, b, orc.build_eq(pt)))
+    if hi_log:
+        assert same(from_dev(nxt), orc.build_eq(pt[9:]))
+
+
 def test_zc_fold_round_rejects_shapes(dev):
     import ctypes
     from flock_challenge_multi_amd import _native as nat
@@ -192,6 +249,10 @@ def test_zc_fold_round_rejects_shapes(dev):
     assert lib.flk_zc_fold_round(p, p, 1024, ctypes.byref(nat.F128.from_int(3)), p, p, 0, None, None, None, p,
                                  None) == nat.FLK_E_SHAPE
     assert lib.flk_zc_fold_at_z_round(p, p, 14, p, p, p, p, p, None, p, None) == nat.FLK_E_SHAPE
+    assert lib.flk_zc_fold_at_z_round(p, p, 16, p, p, p, p, None, None, p, None) == nat.FLK_E_SHAPE
+    r = nat.F128.from_int(5)
+    assert lib.flk_zc_fold_at_z_bind_round(p, p, 15, p, ctypes.byref(r), p, p, p, p, None, p, None) == nat.FLK_E_SHAPE
+    assert lib.flk_zc_fold_at_z_bind_round(p, p, 16, p, None, p, p, p, p, None, p, None) == nat.FLK_E_SHAPE
     assert lib.flk_is_disabled() == 0
 
 
