@@ -142,16 +142,19 @@ __device__ __forceinline__ void comb_mul_acc(uint4 v, const uint4* __restrict__ 
 
 // Same with a run-time stride (per-thread tables laid out entry-major).
 // Each Horner step folds the four looked-up entries into the 7-word window
-// with three-input XORs.
+// with three-input XORs.  UNROLL = 1 keeps only four table loads in flight per
+// lane, which bounds register pressure when several products are inlined
+// together; UNROLL = 8 lets the loads of later steps issue ahead of this step's
+// XORs (for kernels whose per-lane tables already hold them at two waves per
+// SIMD, where the registers are there).
+template <int UNROLL = 1>
 __device__ __forceinline__ void comb_mul_acc_rt(uint4 v, const uint4* __restrict__ tab, uint32_t stride,
                                                 U256& acc) {
   const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
   uint32_t a[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) a[i] = 0u;
-  // Not unrolled over j: keeps only four table loads in flight per lane, which
-  // bounds register pressure when several products are inlined together.
-#pragma unroll 1
+#pragma unroll UNROLL
   for (int j = 7; j >= 0; --j) {
     // entry nib of the lane's table: one bit-field extract and one shift-add
     const uint4 e0 = tab[__builtin_amdgcn_ubfe(vw[0], 4 * j, 4) * stride];

@@ -187,9 +187,9 @@ __global__ void __launch_bounds__(RD_WG) lc_round_msg_msb_kernel(const uint4* __
   for (uint64_t i = (uint64_t)blockIdx.x * RD_WG + threadIdx.x; i < half; i += (uint64_t)gridDim.x * RD_WG) {
     const uint4 lo = comb[i], hi = comb[i + half], zl = z[i], zh = z[i + half];
     table_store_private(tabs, RD_WG, threadIdx.x, hi);
-    comb_mul_acc_rt(zh, my, RD_WG, e1);
+    comb_mul_acc_rt<2>(zh, my, RD_WG, e1);
     table_store_private(tabs, RD_WG, threadIdx.x, f_add(hi, lo));
-    comb_mul_acc_rt(f_add(zh, zl), my, RD_WG, einf);
+    comb_mul_acc_rt<2>(f_add(zh, zl), my, RD_WG, einf);
   }
   wg_reduce_store(f_reduce(e1), red[0], out, 0);
   wg_reduce_store(f_reduce(einf), red[1], out, 1);
@@ -538,9 +538,9 @@ __global__ void __launch_bounds__(ZF_WG) zc_fold_round_kernel(const uint4* __res
     const uint4 ea1 = combjr_mul_add(a1, f_zero(), htab);
     const uint4 eas = combjr_mul_add(f_add(a0, a1), f_zero(), htab);
     table_store_private(tabs, ZF_WG, tid, ea1);
-    comb_mul_acc_rt(b1, my, ZF_WG, g1);
+    comb_mul_acc_rt<8>(b1, my, ZF_WG, g1);
     table_store_private(tabs, ZF_WG, tid, eas);
-    comb_mul_acc_rt(f_add(b0, b1), my, ZF_WG, gi);
+    comb_mul_acc_rt<8>(f_add(b0, b1), my, ZF_WG, gi);
   }
   table_store_private(tabs, ZF_WG, tid, e_lo[tid]);  // the lane's own table: no barrier needed
   const uint4 r1 = comb_mul_rt(f_reduce(g1), my, ZF_WG);
@@ -685,9 +685,9 @@ __global__ void __launch_bounds__(ZF_WG) zc_bits_round_kernel(const uint4* __res
       as = combjr_mul_add(f_add(a0, ta1), f_zero(), htab);
     }
     table_store_private(tabs, ZF_WG, tid, a1);
-    comb_mul_acc_rt(b1, my, ZF_WG, g1);
+    comb_mul_acc_rt<8>(b1, my, ZF_WG, g1);
     table_store_private(tabs, ZF_WG, tid, as);
-    comb_mul_acc_rt(bs, my, ZF_WG, gi);
+    comb_mul_acc_rt<8>(bs, my, ZF_WG, gi);
   }
   table_store_private(tabs, ZF_WG, tid, e_lo[tid]);  // the lane's own table: no barrier needed
   const uint4 r1 = comb_mul_rt(f_reduce(g1), my, ZF_WG);

@@ -62,12 +62,12 @@ __global__ void __launch_bounds__(SC_WG) round_msg_kernel(const uint4* __restric
   for (uint64_t j = (uint64_t)blockIdx.x * SC_WG + tid; j < n_pairs; j += (uint64_t)gridDim.x * SC_WG) {
     const uint4 f0 = f[2 * j], f1 = f[2 * j + 1], b0 = b[2 * j], b1 = b[2 * j + 1];
     table_store_private(tabs, SC_WG, tid, f0);
-    comb_mul_acc_rt(b0, my, SC_WG, a0);
+    comb_mul_acc_rt<2>(b0, my, SC_WG, a0);
     table_store_private(tabs, SC_WG, tid, f_add(f0, f1));
-    comb_mul_acc_rt(f_add(b0, b1), my, SC_WG, a2);
+    comb_mul_acc_rt<2>(f_add(b0, b1), my, SC_WG, a2);
     if (EVAL) {
       table_store_private(tabs, SC_WG, tid, f1);
-      comb_mul_acc_rt(b1, my, SC_WG, a1);
+      comb_mul_acc_rt<2>(b1, my, SC_WG, a1);
     }
   }
   const uint4 u0 = f_reduce(a0), u2 = f_reduce(a2);
@@ -121,9 +121,9 @@ __global__ void __launch_bounds__(SC_WG) fold_and_msg_kernel(const uint4* __rest
     b_out[2 * q] = nb0;
     b_out[2 * q + 1] = nb1;
     table_store_private(tabs, SC_WG, tid, nf0);
-    comb_mul_acc_rt(nb0, my, SC_WG, a0);
+    comb_mul_acc_rt<8>(nb0, my, SC_WG, a0);
     table_store_private(tabs, SC_WG, tid, f_add(nf0, nf1));
-    comb_mul_acc_rt(f_add(nb0, nb1), my, SC_WG, a2);
+    comb_mul_acc_rt<8>(f_add(nb0, nb1), my, SC_WG, a2);
   }
   wg_xor_reduce_store(f_reduce(a0), red[0], out, 0);
   wg_xor_reduce_store(f_reduce(a2), red[1], out, 1);
