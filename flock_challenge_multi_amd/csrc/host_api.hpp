@@ -30,6 +30,20 @@ void zc_fixed_challenges(flk_f128 out[7]);
 
 inline flk_f128 fadd(flk_f128 a, flk_f128 b) { return flk_f128{a.lo ^ b.lo, a.hi ^ b.hi}; }
 
+// flk_zc_round1 that also keeps the column sums of c in s_out (device, 128
+// entries; NULL: not kept): S = fold_1b_rows(c as 128-bit elements, eq over
+// r[7..m)), which is the ring-switch s_hat_v of a claim whose suffix point is
+// r[7..m) over the same bits.
+int zc_round1_keep_s(const uint8_t* a_bits, const uint8_t* b_bits, const uint8_t* c_bits, size_t m,
+                     const flk_f128* r_outer, flk_f128* out_ab, flk_f128* out_c, flk_f128* s_out, void* stream);
+// flk_pcs_combined_basis_and_target with some claims' s_hat_v already on the
+// device: known[k] (128 entries) is taken as claim k's s_hat_v instead of
+// folding the witness again (NULL entries, or known = NULL, fold as usual).
+int pcs_combined_basis_and_target_known(flk_fs* fs, const flk_f128* packed_witness, size_t log_len,
+                                        const flk_f128* x_outers, size_t n_claims, const flk_f128* const* known,
+                                        flk_f128* s_hat_v_out, flk_f128* claims_out, flk_f128* gammas_out,
+                                        flk_f128* b_combined, flk_f128* target_out, void* stream);
+
 // Device arenas (runtime.hip): one device block carved by a best-fit free list.
 // Inside an ArenaScope the calling thread's stream-ordered scratch (the library's
 // temporaries and flk_alloc_async) comes from the arena, with no HIP call per

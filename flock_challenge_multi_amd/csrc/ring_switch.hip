@@ -175,7 +175,7 @@ int validate(const char* what, flk_fs* fs, const flk_f128* z, size_t log_len, co
 
 int prove_batched(flk_fs* fs, const flk_f128* z, size_t log_len, const flk_f128* x_outers, size_t n_claims,
                   flk_f128* s_hat_v_out, flk_f128* claims_out, flk_f128* gammas_out, flk_f128* b_combined,
-                  hipStream_t st) {
+                  hipStream_t st, const flk_f128* const* known = nullptr) {
   const size_t n = size_t(1) << log_len, d = log_len + 1;  // x_outer has log_len + 1 coordinates
   // From 2^20 elements on (>= 256 blocks of 2^12, one work-group each), the
   // suffix tensor stays split: per claim eq_lo (2^12 elements) and eq_hi
@@ -205,21 +205,27 @@ int prove_batched(flk_fs* fs, const flk_f128* z, size_t log_len, const flk_f128*
     return status;
   };
 
-  // 1. Suffix tensors and s_hat_v (independent of the transcript).
+  // 1. Suffix tensors and s_hat_v (independent of the transcript); a known
+  // s_hat_v is copied instead of folding the witness again.
   const size_t B = size_t(1) << CS_LO_LOG;
   uint4* his = sfx + n_claims * B;
   for (size_t k = 0; k < n_claims && rc == FLK_OK; ++k) {
     flk_f128* out128 = reinterpret_cast<flk_f128*>(shv + 128 * k);
+    const flk_f128* kn = known ? known[k] : nullptr;
     if (split) {
       flk_f128* lo = reinterpret_cast<flk_f128*>(sfx + k * B);
       flk_f128* hi = reinterpret_cast<flk_f128*>(his + k * n_hi);
       rc = flk_build_eq(x_outers + k * d + 1, CS_LO_LOG, lo, st);
       if (rc == FLK_OK) rc = flk_build_eq(x_outers + k * d + 1 + CS_LO_LOG, hi_log, hi, st);
-      if (rc == FLK_OK) rc = rs_fold_1b_rows_split(z, lo, CS_LO_LOG, hi, hi_log, out128, st);
+      if (rc == FLK_OK && !kn) rc = rs_fold_1b_rows_split(z, lo, CS_LO_LOG, hi, hi_log, out128, st);
     } else {
       flk_f128* sk = reinterpret_cast<flk_f128*>(sfx + k * n);
       rc = flk_build_eq(x_outers + k * d + 1, log_len, sk, st);
-      if (rc == FLK_OK) rc = flk_rs_fold_1b_rows(z, sk, n, out128, st);
+      if (rc == FLK_OK && !kn) rc = flk_rs_fold_1b_rows(z, sk, n, out128, st);
+    }
+    if (rc == FLK_OK && kn) {
+      e = hipMemcpyAsync(out128, kn, 128 * sizeof(flk_f128), hipMemcpyDeviceToDevice, st);
+      if (e != hipSuccess) rc = device_failure(e, "rs_prove_batched: known s_hat_v copy");
     }
   }
   if (rc != FLK_OK) return finish(rc);
@@ -290,6 +296,18 @@ int flk_pcs_combined_basis_and_target(flk_fs* fs, const flk_f128* packed_witness
                                       const flk_f128* x_outers, size_t n_claims, flk_f128* s_hat_v_out,
                                       flk_f128* claims_out, flk_f128* gammas_out, flk_f128* b_combined,
                                       flk_f128* target_out, void* stream) {
+  return flk::pcs_combined_basis_and_target_known(fs, packed_witness, log_len, x_outers, n_claims, nullptr,
+                                                  s_hat_v_out, claims_out, gammas_out, b_combined, target_out,
+                                                  stream);
+}
+
+}  // extern "C"
+
+int flk::pcs_combined_basis_and_target_known(flk_fs* fs, const flk_f128* packed_witness, size_t log_len,
+                                             const flk_f128* x_outers, size_t n_claims,
+                                             const flk_f128* const* known, flk_f128* s_hat_v_out,
+                                             flk_f128* claims_out, flk_f128* gammas_out, flk_f128* b_combined,
+                                             flk_f128* target_out, void* stream) {
   int rc = check_enabled();
   if (rc) return rc;
   if ((rc = validate("pcs_combined_basis_and_target", fs, packed_witness, log_len, x_outers, n_claims, s_hat_v_out,
@@ -299,12 +317,10 @@ int flk_pcs_combined_basis_and_target(flk_fs* fs, const flk_f128* packed_witness
   rc = flk_fs_observe_label(fs, PCS_LABEL, sizeof(PCS_LABEL) - 1);
   if (rc == FLK_OK)
     rc = prove_batched(fs, packed_witness, log_len, x_outers, n_claims, s_hat_v_out, claims_out, gammas_out,
-                       b_combined, static_cast<hipStream_t>(stream));
+                       b_combined, static_cast<hipStream_t>(stream), known);
   if (rc != FLK_OK) return rc;
   flk_f128 t{0, 0};
   for (size_t k = 0; k < n_claims; ++k) t = fadd(t, host_mul(gammas_out[k], claims_out[k]));
   *target_out = t;
   return FLK_OK;
 }
-
-}  // extern "C"

@@ -466,8 +466,21 @@ __global__ void scale_kernel(uint4* __restrict__ v, uint32_t n, uint4 c) {
 }  // namespace flk
 
 
+namespace flk {
+int zc_round1_keep_s(const uint8_t* a_bits, const uint8_t* b_bits, const uint8_t* c_bits, size_t m,
+                     const flk_f128* r_outer, flk_f128* out_ab, flk_f128* out_c, flk_f128* s_out, void* stream);
+}
+
 extern "C" int flk_zc_round1(const uint8_t* a_bits, const uint8_t* b_bits, const uint8_t* c_bits, size_t m,
                              const flk_f128* r_outer, flk_f128* out_ab, flk_f128* out_c, void* stream) {
+  return flk::zc_round1_keep_s(a_bits, b_bits, c_bits, m, r_outer, out_ab, out_c, nullptr, stream);
+}
+
+// flk_zc_round1, and with s_out (device, 128 entries) the column sums S of c
+// kept there: S = fold_1b_rows(c as 128-bit elements, eq over r[7..m)).
+int flk::zc_round1_keep_s(const uint8_t* a_bits, const uint8_t* b_bits, const uint8_t* c_bits, size_t m,
+                          const flk_f128* r_outer, flk_f128* out_ab, flk_f128* out_c, flk_f128* s_out,
+                          void* stream) {
   using namespace flk;
   int rc = check_enabled();
   if (rc) return rc;
@@ -500,7 +513,7 @@ extern "C" int flk_zc_round1(const uint8_t* a_bits, const uint8_t* b_bits, const
   }
   uint4* const eq_lo = eq + n_outer;
   uint4* const eq_hi = eq_lo + n_lo;
-  uint4* const S = eq_hi + n_hi;
+  uint4* const S = s_out ? reinterpret_cast<uint4*>(s_out) : eq_hi + n_hi;
   auto fail = [&](int status) {
     (void)flk::stream_free(eq, st);
     return status;
