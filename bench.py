@@ -609,7 +609,11 @@ def proof_leg(args, torch, nat, hp):
     from flock_challenge_multi_amd import prover as pv
     from flock_challenge_multi_amd.inputs import generate_compressions
 
+    from flock_challenge_multi_amd import autotune
     n = 1 << args.log2
+    # the L0 encode's pass schedule, autotuned before the timed region (fast
+    # profile: rate 1/2, log_d = m - 12)
+    ntt_tune = autotune.tune_encode(args.log2 + 2, 1)
     t0 = time.perf_counter()
     setup = pv.Blake3Setup(n)
     setup_s = time.perf_counter() - t0
@@ -646,6 +650,8 @@ def proof_leg(args, torch, nat, hp):
         "setup_s": setup_s,
         "m": setup.m,
         "device_bytes_per_proof": setup.arena_stats()["peak"],
+        "ntt_schedule": {"best": ntt_tune["best"], "best_ms": ntt_tune["best_ms"], "default": ntt_tune["default"],
+                         "default_ms": ntt_tune["default_ms"]},
     }, proofs, comps, inputs
 
 
@@ -726,10 +732,14 @@ def batch_leg(args, torch, pv):
     compared with the same input proved alone (flk_blake3_prove), which also gives the
     one-at-a-time throughput."""
     import numpy as np
+    from flock_challenge_multi_amd import autotune
     sizes = batch_sizes(args.batch_jobs, 0xBA7C4ED)
     inputs = [_job_input(torch, np, n, 0xB000 + i) for i, n in enumerate(sizes)]
     torch.cuda.synchronize()
     total = sum(sizes)
+    # the autotuner's first half: the RS-encode pass schedule of every size class's
+    # L0 codeword (fast profile, rate 1/2), installed before any proof of the leg
+    ntt_tune = autotune.tune_batch([14 + max(3, (n - 1).bit_length()) for n in sizes], rate=1)
     setups = {}
 
     def setup_for(n):
@@ -771,7 +781,10 @@ def batch_leg(args, torch, pv):
         "value": tune[best]["value"], "unit": "compressions/s", "jobs": len(sizes), "compressions": total,
         "workers": best, "wall_s": tune[best]["wall_s"], "p50_ms": pctl(lat, 0.5), "p99_ms": pctl(lat, 0.99),
         "one_at_a_time_value": total / seq_s, "identical_to_alone": tune[best]["identical"],
-        "hbm_budget": "workers x the m = 32 arena + 1 GiB", "autotune": tune, "sizes_seed": "0xBA7C4ED", "inputs": "generate_compressions(ceil_log2(n_i), 0xB000 + i)[:n_i]",
+        "hbm_budget": "workers x the m = 32 arena + 1 GiB", "autotune": tune,
+        "ntt_schedules": {f"m{t['log_d'] + 12}": {"best": t["best"], "best_ms": t["best_ms"], "default": t["default"],
+                                                  "default_ms": t["default_ms"]} for t in ntt_tune},
+        "sizes_seed": "0xBA7C4ED", "inputs": "generate_compressions(ceil_log2(n_i), 0xB000 + i)[:n_i]",
     }, best, alone, inputs, sizes
 
 
@@ -938,6 +951,7 @@ def main():
         "score_median": head["score_median"],
         "stage_ms_p50": head["stage_ms_p50"],
         "device_bytes_per_proof": head["device_bytes_per_proof"],
+        "ntt_schedule_l0": head["ntt_schedule"],
         "setup_s": head["setup_s"],
         "roofline": roofline,
         "proof_kernels": kern,

@@ -60,6 +60,8 @@ SIGNATURES = {
     "flk_copy_to_host": (_I, [_P, _P, _SZ, _P]),
     "flk_synchronize": (_I, [_P]),
     "flk_ntt_twiddles": (_I, [_SZ, _P]),
+    "flk_ntt_set_schedule": (_I, [_SZ, _SZ, _P, _SZ]),
+    "flk_ntt_get_schedule": (_I, [_SZ, _SZ, _P, _SZ, _P]),
     "flk_ntt_forward_interleaved": (_I, [_SZ, _P, _SZ, _SZ, _SZ, _P]),
     "flk_ntt_rs_encode_interleaved": (_I, [_SZ, _P, _SZ, _P, _SZ, _SZ, _P]),
     "flk_ntt_inverse_interleaved": (_I, [_SZ, _P, _SZ, _SZ, _P]),

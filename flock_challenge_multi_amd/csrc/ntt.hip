@@ -29,6 +29,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
 
 #include "gf128.hpp"
 #include "internal.hpp"
@@ -464,7 +468,13 @@ static int launch_pass(const uint4* src, uint4* dst, const uint4* tw, uint32_t l
 // Measured on MI355X (2^17 x 64 encode): 2,2,3,3,3,3 0.409 ms, 1,3,3,3,3,3
 // 0.433 ms, 4,3,3,3,3 0.412 ms (its 4-layer pass spills at 4 waves/SIMD).
 // FLOCK_NTT_SCHEDULE="4,3,3,3,3" overrides it (must sum to the layer count).
-static int ntt_schedule(uint32_t n, int kmax, int* ks) {
+// Schedules installed through flk_ntt_set_schedule (the autotuner's choices),
+// keyed by (layers, log2 lanes); consulted after FLOCK_NTT_SCHEDULE and before
+// the default rule.
+static std::mutex g_sched_mu;
+static std::map<std::pair<uint32_t, uint32_t>, std::vector<int>> g_sched;
+
+static int ntt_schedule(uint32_t n, uint32_t log_lanes, int kmax, int* ks) {
   int cnt = 0;
   const char* env = getenv("FLOCK_NTT_SCHEDULE");
   if (env && env[0]) {
@@ -479,6 +489,14 @@ static int ntt_schedule(uint32_t n, int kmax, int* ks) {
     }
     if (sum == n) return cnt;
     cnt = 0;  // does not match this transform: fall back to the default
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_sched_mu);
+    auto it = g_sched.find({n, log_lanes});
+    if (it != g_sched.end()) {
+      for (int k : it->second) ks[cnt++] = k;
+      return cnt;
+    }
   }
   if (n == 0) return 0;
   const uint32_t deep = n < (uint32_t)kmax ? n : (uint32_t)kmax;
@@ -514,7 +532,7 @@ static int run_layers(const uint4* src, uint4* dst, const uint4* tw, uint32_t lo
   const int kgroup = (int)log_group + 1;
   int kmax = ntt_radix_cap();
   int ks[64];
-  const int n_pass = ntt_schedule(log_d - start, kmax, ks);
+  const int n_pass = ntt_schedule(log_d - start, log_lanes, kmax, ks);
   if (n_pass < 0) return set_error(FLK_E_SHAPE, "FLOCK_NTT_SCHEDULE is malformed");
   // Launch plan in forward order: (first layer, layers), splitting passes the
   // per-group kernel cannot hold.
@@ -630,6 +648,40 @@ int flk_ntt_inverse_interleaved(size_t dim, flk_f128* data, size_t n_total, size
   uint4* d = reinterpret_cast<uint4*>(data);
   return flk::run_layers(d, d, tw, (uint32_t)log_d, 0, (uint32_t)flk::log2_exact(num_ntts), ~0ull,
                          static_cast<hipStream_t>(stream), true);
+}
+
+int flk_ntt_set_schedule(size_t n_layers, size_t log_lanes, const int* ks, size_t n) {
+  if (n_layers == 0 || n_layers > 40 || log_lanes > 16)
+    return flk::set_error(FLK_E_SHAPE, "ntt_set_schedule: n_layers=%zu log_lanes=%zu", n_layers, log_lanes);
+  std::lock_guard<std::mutex> lk(flk::g_sched_mu);
+  const std::pair<uint32_t, uint32_t> key((uint32_t)n_layers, (uint32_t)log_lanes);
+  if (n == 0) {
+    flk::g_sched.erase(key);
+    return FLK_OK;
+  }
+  if (!ks || n > 40) return flk::set_error(FLK_E_SHAPE, "ntt_set_schedule: null or too long schedule");
+  size_t sum = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (ks[i] < 1 || ks[i] > flk::NTT_KMAX)
+      return flk::set_error(FLK_E_SHAPE, "ntt_set_schedule: pass %zu has %d layers (1..%d)", i, ks[i], flk::NTT_KMAX);
+    sum += (size_t)ks[i];
+  }
+  if (sum != n_layers) return flk::set_error(FLK_E_SHAPE, "ntt_set_schedule: passes sum to %zu, not %zu", sum, n_layers);
+  flk::g_sched[key] = std::vector<int>(ks, ks + n);
+  return FLK_OK;
+}
+
+int flk_ntt_get_schedule(size_t n_layers, size_t log_lanes, int* ks, size_t cap, size_t* n) {
+  if (!ks || !n) return flk::set_error(FLK_E_SHAPE, "ntt_get_schedule: null pointer");
+  if (n_layers > 40 || log_lanes > 16)
+    return flk::set_error(FLK_E_SHAPE, "ntt_get_schedule: n_layers=%zu log_lanes=%zu", n_layers, log_lanes);
+  int tmp[64];
+  const int cnt = flk::ntt_schedule((uint32_t)n_layers, (uint32_t)log_lanes, flk::ntt_radix_cap(), tmp);
+  if (cnt < 0) return flk::set_error(FLK_E_SHAPE, "FLOCK_NTT_SCHEDULE is malformed");
+  if ((size_t)cnt > cap) return flk::set_error(FLK_E_SHAPE, "ntt_get_schedule: %d passes, capacity %zu", cnt, cap);
+  for (int i = 0; i < cnt; ++i) ks[i] = tmp[i];
+  *n = (size_t)cnt;
+  return FLK_OK;
 }
 
 }  // extern "C"

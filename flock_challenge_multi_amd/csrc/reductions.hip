@@ -219,6 +219,10 @@ constexpr int F1_WG = 256;
 // quads' table stores (16 groups per wave) spread over the LDS banks instead of
 // all hitting the same ones; a lookup still reads one 16-entry table.
 constexpr int F1_TS = 17;
+// Words per staged element in LDS: 4 plus one of padding, so the transpose's
+// reads (lane (g, w) reads word w of the group's four elements) fall in 64
+// distinct banks.
+constexpr int F1_ZS = 5;
 
 // Lane K of the caller's aligned lane quad, broadcast to the quad (DPP).
 template <int K>
@@ -231,13 +235,18 @@ __device__ __forceinline__ uint4 quad_bcast4(uint4 v) {
   return make_uint4(quad_bcast<K>(v.x), quad_bcast<K>(v.y), quad_bcast<K>(v.z), quad_bcast<K>(v.w));
 }
 
-// Bits b of the byte x moved to bit 4b (b < 8).
-__device__ __forceinline__ uint32_t spread_byte4(uint32_t x) {
-  x &= 0xFFu;
-  x = (x | (x << 12)) & 0x000F000Fu;
-  x = (x | (x << 6)) & 0x03030303u;
-  x = (x | (x << 3)) & 0x11111111u;
-  return x;
+// A 4 x 8 bit transpose inside one word: byte k's bit b (bit 8k + b) moves to
+// bit 4b + k.  Four delta swaps exchange the index bits (4,2), (3,1), (2,0) and
+// (1,0), which turns (k1 k0 b2 b1 b0) into (b2 b1 b0 k1 k0).
+__device__ __forceinline__ uint32_t delta_swap(uint32_t x, uint32_t d, uint32_t m) {
+  const uint32_t t = __builtin_amdgcn_bitop3_b32(x >> d, x, m, 0x28);  // (x >> d ^ x) & m
+  return xor3(x, t, t << d);
+}
+__device__ __forceinline__ uint32_t transpose_4x8(uint32_t x) {
+  x = delta_swap(x, 12, 0x0000F0F0u);
+  x = delta_swap(x, 6, 0x00CC00CCu);
+  x = delta_swap(x, 3, 0x0A0A0A0Au);
+  return delta_swap(x, 1, 0x22222222u);
 }
 
 // One tile of 256 elements (z value zv and eq value ev per lane) folded into
@@ -246,7 +255,13 @@ __device__ __forceinline__ uint32_t spread_byte4(uint32_t x) {
 __device__ __forceinline__ void f1_tile(uint4 zv, uint4 ev, uint32_t t, uint32_t* zt, uint32_t* nibw, uint4* tabs,
                                         uint4& acc) {
   const uint32_t r = t & 127u, halfsel = t >> 7;
-  reinterpret_cast<uint4*>(zt)[t] = zv;  // one 16-byte store per lane (conflict-free)
+  {
+    uint32_t* zr = zt + F1_ZS * t;  // stride 5 words: conflict-free stores and transpose reads
+    zr[0] = zv.x;
+    zr[1] = zv.y;
+    zr[2] = zv.z;
+    zr[3] = zv.w;
+  }
   // Table of group g = t/4 holds sums over subsets of its 4 eq values; this lane
   // writes the 4 entries whose bits 2,3 equal q = t%4.  The group is the
   // lane's aligned quad, so its eq values arrive by DPP quad broadcasts
@@ -262,17 +277,23 @@ __device__ __forceinline__ void f1_tile(uint4 zv, uint4 ev, uint32_t t, uint32_t
   tg[2] = f_add(base, e1);
   tg[3] = f_add(f_add(base, e0), e1);
   __syncthreads();
-  // Transpose: item (g, qw) packs the nibbles of r = 8qw .. 8qw+7, i.e. byte
-  // qw & 3 of word qw >> 2 of the group's four elements, bit b of element k
-  // at bit 4b + k.
-#pragma unroll
-  for (int it = 0; it < 4; ++it) {
-    const uint32_t item = (uint32_t)it * F1_WG + t, g = item >> 4, qw = item & 15u;
-    const uint32_t w = qw >> 2, sh = 8u * (qw & 3u);
-    uint32_t v = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v |= spread_byte4(zt[(4 * g + k) * 4 + w] >> sh) << k;
-    nibw[item] = v;
+  // Transpose: word qw = 4w + j of group g packs the nibbles of r = 8qw .. 8qw+7,
+  // i.e. byte j of word w of the group's four elements, bit b of element k at
+  // bit 4b + k.  Lane t = 4g + w gathers byte j of its four words into one word
+  // (byte k = element k) with byte permutes, transposes it in registers, and
+  // stores the four words of (g, w) with one 16-byte store.
+  {
+    const uint32_t g = t >> 2, w = t & 3u;
+    const uint32_t* zg = zt + F1_ZS * 4 * g + w;
+    const uint32_t e0 = zg[0], e1 = zg[F1_ZS], e2 = zg[2 * F1_ZS], e3 = zg[3 * F1_ZS];
+    const uint32_t lo01 = __builtin_amdgcn_perm(e1, e0, 0x05010400u), hi01 = __builtin_amdgcn_perm(e1, e0, 0x07030602u);
+    const uint32_t lo23 = __builtin_amdgcn_perm(e3, e2, 0x05010400u), hi23 = __builtin_amdgcn_perm(e3, e2, 0x07030602u);
+    uint4 v;
+    v.x = transpose_4x8(__builtin_amdgcn_perm(lo23, lo01, 0x05040100u));
+    v.y = transpose_4x8(__builtin_amdgcn_perm(lo23, lo01, 0x07060302u));
+    v.z = transpose_4x8(__builtin_amdgcn_perm(hi23, hi01, 0x05040100u));
+    v.w = transpose_4x8(__builtin_amdgcn_perm(hi23, hi01, 0x07060302u));
+    reinterpret_cast<uint4*>(nibw)[t] = v;
   }
   __syncthreads();
   const uint32_t qr = r >> 3, shr = 4u * (r & 7u);
@@ -289,8 +310,8 @@ __device__ __forceinline__ void f1_tile(uint4 zv, uint4 ev, uint32_t t, uint32_t
 __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_kernel(const uint4* __restrict__ zp,
                                                                const uint4* __restrict__ eq, uint64_t n,
                                                                uint4* __restrict__ out) {
-  __shared__ __align__(16) uint32_t zt[F1_WG * 4];
-  __shared__ uint32_t nibw[(F1_WG / 4) * 16];  // [group][r / 8]: nibbles of r = 8q .. 8q+7
+  __shared__ __align__(16) uint32_t zt[F1_WG * F1_ZS];
+  __shared__ __align__(16) uint32_t nibw[(F1_WG / 4) * 16];  // [group][r / 8]: nibbles of r = 8q .. 8q+7
   __shared__ uint4 tabs[(F1_WG / 4) * F1_TS];
   __shared__ uint4 red[128];
   const uint32_t t = threadIdx.x;
@@ -323,8 +344,8 @@ __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_split_kernel(const uint
                                                                      uint32_t lo_log,
                                                                      const uint4* __restrict__ eq_hi,
                                                                      uint64_t n_hi, uint4* __restrict__ out) {
-  __shared__ __align__(16) uint32_t zt[F1_WG * 4];
-  __shared__ uint32_t nibw[(F1_WG / 4) * 16];
+  __shared__ __align__(16) uint32_t zt[F1_WG * F1_ZS];
+  __shared__ __align__(16) uint32_t nibw[(F1_WG / 4) * 16];
   __shared__ uint4 tabs[(F1_WG / 4) * F1_TS];
   __shared__ uint4 red[128];
   __shared__ uint4 etab[16];

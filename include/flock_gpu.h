@@ -135,6 +135,19 @@ int flk_ntt_forward_interleaved(size_t dim, flk_f128 *data, size_t n_total, size
  * deepest first (v += u; u += v * twiddle).  num_ntts = 1 is inverse_transform. */
 int flk_ntt_inverse_interleaved(size_t dim, flk_f128 *data, size_t n_total, size_t num_ntts, void *stream);
 
+/* The pass schedule of the transforms above for n_layers layers over 2^log_lanes
+ * interleaved lanes (n_layers = log_d - start_layer; the RS encode runs
+ * log_d - log_inv_rate): ks[0..n) layers per pass in forward order, each 1..4,
+ * summing to n_layers; n = 0 removes the entry.  Process-wide; shapes without an
+ * entry use the library's default (3-layer passes, the remainder as leading
+ * 2-layer passes).  The schedule changes only the speed, never a value.  It has
+ * no counterpart in the reference, whose CPU schedule is fixed by its top/deep
+ * split (:4865-4900); the autotuner of the batched prover
+ * (flock_challenge_multi_amd/autotune.py) installs its choices here. */
+int flk_ntt_set_schedule(size_t n_layers, size_t log_lanes, const int *ks, size_t n);
+/* The schedule in use for (n_layers, log_lanes) into ks[0..*n), *n <= cap. */
+int flk_ntt_get_schedule(size_t n_layers, size_t log_lanes, int *ks, size_t cap, size_t *n);
+
 /* AdditiveNttF128::rs_encode_interleaved (:2785-2830): codeword = forward
  * transform of [msg, 0, ...]; n_cw / n_msg is the inverse rate (a power of
  * two >= 2).  msg and codeword must not overlap. */

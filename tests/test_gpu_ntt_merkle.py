@@ -197,3 +197,37 @@ def test_commit_from_host_buffers_via_c_abi(orc, dev):
     finally:
         for p in ptrs:
             nat.call("flk_free", p)
+
+
+@pytest.mark.parametrize("log_d,r,num_ntts", [(12, 1, 64), (11, 2, 8), (9, 1, 64)])
+def test_every_autotuner_schedule_gives_the_same_codeword(orc, dev, log_d, r, num_ntts):
+    """Each candidate pass schedule of the autotuner (flk_ntt_set_schedule) groups
+    the same layers differently: the codeword equals the oracle's for all of them,
+    and the inverse transform under a custom schedule undoes the forward one."""
+    from flock_challenge_multi_amd import autotune
+    lanes_log = num_ntts.bit_length() - 1
+    n_layers = log_d - r
+    n_msg = (1 << (log_d - r)) * num_ntts
+    msg = orc.fill(7000 + log_d, n_msg)
+    want = orc.rs_encode(log_d, msg, n_msg << r, num_ntts)
+    ntt = hp.AdditiveNttF128.standard(log_d)
+    try:
+        for sched in autotune.candidates(n_layers):
+            _log(f"schedule {sched} log_d={log_d} r={r} lanes={num_ntts}")
+            autotune.install(n_layers, sched, lanes_log)
+            assert autotune.current(n_layers, lanes_log) == sched
+            cw = hp.f128_empty(n_msg << r)
+            cw.fill_(-1)
+            ntt.rs_encode_interleaved(to_dev(msg), cw, num_ntts)
+            torch.cuda.synchronize()
+            assert same(from_dev(cw), want), sched
+        sched = autotune.candidates(log_d)[0]
+        autotune.install(log_d, sched, lanes_log)
+        data = to_dev(want)
+        ntt.inverse_transform_interleaved(data, num_ntts)
+        ntt.forward_transform_interleaved(data, num_ntts)
+        torch.cuda.synchronize()
+        assert same(from_dev(data), want)
+    finally:
+        autotune.install(n_layers, (), lanes_log)
+        autotune.install(log_d, (), lanes_log)

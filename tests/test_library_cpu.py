@@ -103,3 +103,33 @@ def test_injected_device_failure_latches():
         lib.flk_inject_device_failures(0, 0)
         lib.flk_undisable()
     assert lib.flk_is_disabled() == 0
+
+
+def test_ntt_schedule_registry_and_candidates():
+    """The autotuner's candidate schedules are valid groupings of the layers, and
+    flk_ntt_set_schedule / flk_ntt_get_schedule install, report and remove one
+    (no device work: the registry is host state); malformed schedules are refused
+    without latching."""
+    import ctypes
+
+    from flock_challenge_multi_amd import _native as nat
+    from flock_challenge_multi_amd import autotune
+    lib = nat.load()
+    for n in range(1, 24):
+        cands = autotune.candidates(n)
+        assert cands, n
+        for c in cands:
+            assert sum(c) == n and all(1 <= k <= 3 for k in c), (n, c)
+    default = autotune.current(19)
+    assert default == (2, 2, 3, 3, 3, 3, 3)
+    autotune.install(19, (3, 3, 3, 3, 3, 2, 2))
+    assert autotune.current(19) == (3, 3, 3, 3, 3, 2, 2)
+    assert autotune.current(19, 3) == default  # another lane count keeps the default
+    autotune.install(19, ())
+    assert autotune.current(19) == default
+    bad = (ctypes.c_int * 3)(3, 3, 3)
+    assert lib.flk_ntt_set_schedule(10, 6, bad, 3) == nat.FLK_E_SHAPE  # sums to 9
+    five = (ctypes.c_int * 2)(5, 5)
+    assert lib.flk_ntt_set_schedule(10, 6, five, 2) == nat.FLK_E_SHAPE  # 5-layer passes
+    assert lib.flk_ntt_set_schedule(0, 6, bad, 0) == nat.FLK_E_SHAPE
+    assert lib.flk_is_disabled() == 0
