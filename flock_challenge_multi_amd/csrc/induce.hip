@@ -263,8 +263,7 @@ extern "C" int flk_induce_sumcheck_poly(size_t log_msg_cols, const flk_f128* row
   up.insert(up.end(), consts.begin(), consts.end());
   up.insert(up.end(), aq.begin(), aq.end());
   up.insert(up.end(), eqv.begin(), eqv.end());
-  e = hipMemcpyAsync(scratch, up.data(), up.size() * sizeof(flk_f128), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);  // `up` is pageable host memory
+  e = upload_staged(scratch, up.data(), up.size() * sizeof(flk_f128), st);
   if (e != hipSuccess) return device_failure(e, "induce: upload constants");
 
   if (L > 0) {

@@ -48,6 +48,10 @@ hipError_t stream_alloc(void** p, size_t bytes, hipStream_t st);
 // Copy bytes from the device to host memory and wait for st (a pinned bounce
 // buffer per thread for reads up to 1 MiB).
 hipError_t read_to_host(void* dst, const void* src, size_t bytes, hipStream_t st);
+// Host -> device copy enqueued on st through the calling thread's pinned upload
+// ring: src may be reused as soon as this returns, and nothing waits for the
+// copy (uploads above 8 MiB are copied and waited for).
+hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t st);
 // p[0 .. n) = 0 by a kernel on st (graph-capturable, unlike hipMemsetAsync).
 int zero_n(uint4* p, uint64_t n, hipStream_t st);
 hipError_t stream_free(void* p, hipStream_t st);

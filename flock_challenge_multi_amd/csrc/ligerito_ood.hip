@@ -60,7 +60,6 @@ int flk_ligerito_ood(flk_fs* fs, const flk_f128* f, flk_f128* b, size_t log_n, s
     t_r->hi ^= p.hi;
   }
   e = flk::stream_free(ws, st);
-  if (e == hipSuccess && rc == FLK_OK) e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     if (rc == FLK_OK) rc = device_failure(e, "ligerito_ood");

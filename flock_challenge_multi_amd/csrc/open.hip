@@ -45,7 +45,7 @@ int gather(const uint4* src, const std::vector<uint64_t>& idx, uint32_t per, uin
     (void)hipGetLastError();
     return set_error(FLK_E_NOMEM, "%s: index scratch allocation failed", what);
   }
-  e = hipMemcpyAsync(d_idx, idx.data(), idx.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st);
+  e = upload_staged(d_idx, idx.data(), idx.size() * sizeof(uint64_t), st);
   if (e != hipSuccess) return device_failure(e, what);
   const uint64_t total = idx.size() * per;
   const uint64_t wgs = (total + GATHER_WG - 1) / GATHER_WG;
@@ -53,7 +53,6 @@ int gather(const uint4* src, const std::vector<uint64_t>& idx, uint32_t per, uin
                      (uint64_t)idx.size(), per, dst);
   FLK_LAUNCH_CHECK("gather_kernel");
   e = flk::stream_free(d_idx, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return device_failure(e, what);
   return FLK_OK;
 }

@@ -108,8 +108,7 @@ int scan_range(const PowDigest& dg, uint64_t start, uint64_t len, uint32_t bits,
     hipLaunchKernelGGL(pow_scan_kernel, dim3(grid), dim3(POW_WG), 0, st, dg.lo, dg.hi, start, end, bits, best);
     FLK_LAUNCH_CHECK("pow_scan_kernel");
   }
-  hipError_t e = hipMemcpyAsync(host, best, sizeof *host, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipError_t e = read_to_host(host, best, sizeof *host, st);  // the mailbox: no stream synchronisation
   if (e != hipSuccess) return device_failure(e, "pow_scan readback");
   *out = *host;
   return FLK_OK;

@@ -250,7 +250,7 @@ int prove_batched(flk_fs* fs, const flk_f128* z, size_t log_len, const flk_f128*
     for (unsigned b = 0; b < 128; ++b) eqg[128 * k + b] = host_mul(gammas_out[k], eq_r[k][b]);
 
   // 4. b = sum_k fold_b128_elems(suffix_k, gamma_k eq(r''_k)), RC_MAXC claims per pass.
-  e = hipMemcpyAsync(eqg_dev, eqg.data(), eqg.size() * sizeof(flk_f128), hipMemcpyHostToDevice, st);
+  e = upload_staged(eqg_dev, eqg.data(), eqg.size() * sizeof(flk_f128), st);
   if (e != hipSuccess) return finish(device_failure(e, "rs_prove_batched: eq upload"));
   const unsigned grid = split ? (unsigned)std::min<uint64_t>(n_hi, 2048)
                              : (unsigned)std::min<uint64_t>((n + RC_WG - 1) / RC_WG, 1024);
@@ -267,9 +267,6 @@ int prove_batched(flk_fs* fs, const flk_f128* z, size_t log_len, const flk_f128*
       FLK_LAUNCH_CHECK("rs_combine_kernel");
     }
   }
-  // the host eq table must outlive its upload
-  e = hipStreamSynchronize(st);
-  if (e != hipSuccess) return finish(device_failure(e, "rs_prove_batched"));
   return finish(FLK_OK);
 }
 

@@ -525,6 +525,74 @@ __global__ void __launch_bounds__(256) publish_kernel(const uint8_t* __restrict_
   }
 }
 
+// Host -> device uploads staged through a per-thread pinned ring (8 MiB,
+// allocated on first use): the bytes are copied into the ring and the DMA is
+// enqueued on the stream, so the caller's (pageable) buffer may be reused at
+// once and no synchronisation is needed.  A region of the ring is reused only
+// after the stream that read it has been seen to pass its copies: every
+// read-back through read_to_host on that stream waits for the whole stream and
+// recycles the ring; a full ring, or an upload on another stream, synchronises
+// the stream the pending copies belong to first.  Larger uploads are copied
+// and waited for.
+struct UploadRing {
+  uint8_t* base = nullptr;
+  size_t off = 0;
+  hipStream_t pending = nullptr;  // the stream of the copies in [0, off)
+  bool failed = false;
+};
+constexpr size_t UPLOAD_RING_BYTES = size_t(8) << 20;
+static thread_local UploadRing t_ring;
+
+static hipError_t ring_drain() {
+  UploadRing& r = t_ring;
+  if (r.off == 0) return hipSuccess;
+  hipError_t e = hipStreamSynchronize(r.pending);
+  r.off = 0;
+  r.pending = nullptr;
+  return e;
+}
+
+hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return hipSuccess;
+  UploadRing& r = t_ring;
+  if (!r.base && !r.failed) {
+    void* h = nullptr;
+    if (hipHostMalloc(&h, UPLOAD_RING_BYTES, hipHostMallocDefault) == hipSuccess) {
+      r.base = static_cast<uint8_t*>(h);
+    } else {
+      (void)hipGetLastError();
+      r.failed = true;
+    }
+  }
+  const size_t need = (bytes + 255) & ~size_t(255);
+  if (!r.base || need > UPLOAD_RING_BYTES) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
+  }
+  if (r.off && (r.pending != st || r.off + need > UPLOAD_RING_BYTES)) {
+    hipError_t e = ring_drain();
+    if (e != hipSuccess) return e;
+  }
+  uint8_t* slot = r.base + r.off;
+  memcpy(slot, src, bytes);
+  hipError_t e = hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return e;
+  r.off += need;
+  r.pending = st;
+  return hipSuccess;
+}
+
+// The stream st has completed everything enqueued before this point: the ring's
+// copies on it are done.
+static void ring_passed(hipStream_t st) {
+  UploadRing& r = t_ring;
+  if (r.off && r.pending == st) {
+    r.off = 0;
+    r.pending = nullptr;
+  }
+}
+
 struct Mailbox {
   uint8_t* host = nullptr;  // MAILBOX_BYTES of data, then the sequence word
   uint8_t* dev = nullptr;
@@ -555,6 +623,7 @@ static hipError_t read_bounce(void* dst, const void* src, size_t bytes, hipStrea
     e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
   }
+  if (e == hipSuccess) ring_passed(st);
   return e;
 }
 
@@ -606,6 +675,7 @@ hipError_t read_to_host(void* dst, const void* src, size_t bytes, hipStream_t st
     if (__atomic_load_n(flag_host, __ATOMIC_ACQUIRE) != seq) return hipErrorUnknown;
   }
   memcpy(dst, mb.host, bytes);
+  ring_passed(st);
   return hipSuccess;
 }
 
@@ -732,7 +802,7 @@ int flk_copy_to_device(void* dst, const void* src, size_t bytes, void* stream) {
   if (rc) return rc;
   if (!bytes) return FLK_OK;
   if (!dst || !src) return flk::set_error(FLK_E_SHAPE, "flk_copy_to_device: null pointer");
-  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream));
+  hipError_t e = flk::upload_staged(dst, src, bytes, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return flk::device_failure(e, "flk_copy_to_device");
   return FLK_OK;
 }
@@ -770,6 +840,7 @@ int flk_copy_device_to_device(void* dst, const void* src, size_t bytes, void* st
 int flk_synchronize(void* stream) {
   hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return flk::device_failure(e, "flk_synchronize");
+  flk::ring_passed(static_cast<hipStream_t>(stream));
   return FLK_OK;
 }
 

@@ -17,8 +17,11 @@
  * released with hipFreeAsync on the same stream); zerocheck round 1 also
  * builds its eq table there, and the opening gathers their index lists.  The
  * proof-of-work calls keep one 8-byte device word and one pinned host word per
- * device.  A failed allocation returns FLK_E_NOMEM.  The PoW, opening and
- * transcript-grind calls return host results and synchronise `stream`.
+ * device.  A failed allocation returns FLK_E_NOMEM.  Calls that return host
+ * results (PoW, transcript grinds, read-backs) wait for `stream`; calls that take
+ * small HOST arrays (index lists, challenges, constants) stage them in a
+ * per-thread pinned upload ring, so the arrays may be reused on return and the
+ * call does not wait.
  *
  * Element layout: flk_f128 is 16 bytes, `lo` (x^0..x^63) then `hi`
  * (x^64..x^127), little-endian — identical to the reference F128
@@ -103,7 +106,10 @@ int flk_free(void *p);
  * commits); no reference counterpart. */
 int flk_alloc_async(size_t bytes, void *stream, void **out);
 int flk_free_async(void *p, void *stream);
-/* Enqueue a host -> device copy on `stream` (src is HOST memory). */
+/* Enqueue a host -> device copy on `stream` (src is HOST memory).  Up to 8 MiB the
+ * bytes are staged in the calling thread's pinned upload ring, so src may be
+ * reused as soon as the call returns and nothing waits; larger copies wait for
+ * the copy. */
 int flk_copy_to_device(void *dst, const void *src, size_t bytes, void *stream);
 /* Enqueue a device -> host copy on `stream` (dst is HOST memory). */
 int flk_copy_to_host(void *dst, const void *src, size_t bytes, void *stream);
@@ -313,7 +319,7 @@ struct flk_fs;
  * (CF/pcs.rs:865-870 and its b_combined sum).  1 <= n_claims <= 64.  The suffix
  * tensors live in stream-ordered scratch for the call (from log_len 20 on only
  * their split factors, 2^12 + 2^(log_len - 12) elements per claim); the call
- * synchronises `stream`.  PaddingSpec only lets the reference
+ * waits for `stream` once, to read s_hat_v.  PaddingSpec only lets the reference
  * skip all-zero chunks; the values equal the dense ones computed here. */
 int flk_rs_prove_batched(struct flk_fs *fs, const flk_f128 *packed_witness, size_t log_len,
                          const flk_f128 *x_outers, size_t n_claims, flk_f128 *s_hat_v_out,
@@ -327,7 +333,8 @@ int flk_rs_prove_batched(struct flk_fs *fs, const flk_f128 *packed_witness, size
  * observe_f128(u_2); beta = sample_f128(); b += beta eq(z); *t_r += beta y (glue,
  * :9213-9256).  ood_values_out[s] = y and msgs_out[2s..2s+2) = (u_0, u_2) are HOST
  * arrays (the proof's ood_values and sumcheck messages).  eq(z) lives in
- * stream-ordered scratch; the call synchronises `stream`. */
+ * stream-ordered scratch; the call waits for `stream` once per sample, to read
+ * (u_0, u_2, y); the last glue stays enqueued. */
 int flk_ligerito_ood(struct flk_fs *fs, const flk_f128 *f, flk_f128 *b, size_t log_n, size_t n_samples,
                      flk_f128 *t_r, flk_f128 *ood_values_out, flk_f128 *msgs_out, void *stream);
 
@@ -490,13 +497,15 @@ int flk_fs_verify_pow(flk_fs *t, uint64_t nonce, uint32_t bits, int *ok_out);
 
 /* The opened rows: out[i * row_len + j] = codeword[positions[i] * row_len + j]
  * (SoA codeword of n_rows rows of row_len elements, e.g. row_len = num_ntts).
- * positions is a HOST array of n entries < n_rows; synchronises `stream`. */
+ * positions is a HOST array of n entries < n_rows (staged; the call does not
+ * wait). */
 int flk_gather_rows(const flk_f128 *codeword, size_t n_rows, size_t row_len, const uint64_t *positions, size_t n,
                     flk_f128 *out, void *stream);
 /* merkle_multi_proof (CF/merkle.rs:1020-1060) of a device flat tree (2 num_leaves
  * - 1 hashes): the sibling hashes in emission order into proof_out (device,
  * room for max_hashes), their count in *n_hashes_out (host).  positions is a
- * HOST array (any order, repeats allowed); synchronises `stream`. */
+ * HOST array (any order, repeats allowed); the sibling indices are staged and
+ * the gather enqueued without waiting. */
 int flk_merkle_multi_proof(const uint8_t *tree, size_t num_leaves, const uint64_t *positions, size_t n,
                            uint8_t *proof_out, size_t max_hashes, size_t *n_hashes_out, void *stream);
 

@@ -37,9 +37,10 @@ inline bool operator==(F128 a, F128 b) { return a.lo == b.lo && a.hi == b.hi; }
 // stream: the allocation, uploads, read-backs and the free are all ordered on
 // that stream, so work the caller enqueues on the same stream needs no further
 // synchronisation.  Work on another stream that touches the buffer must be
-// ordered against this one by the caller.  Uploads and read-backs go through
-// pageable host memory and synchronise the stream before returning, so the
-// host vector may be released as soon as they return.
+// ordered against this one by the caller.  Uploads are staged by
+// flk_copy_to_device (the library's pinned upload ring), so the host vector may
+// be released as soon as they return and nothing waits; read-backs wait for the
+// stream.
 template <class T>
 class DeviceVec {
  public:
@@ -77,7 +78,6 @@ class DeviceVec {
     if (host.size() != n_) throw std::invalid_argument("DeviceVec::upload: length mismatch");
     if (!n_) return;
     check(flk_copy_to_device(p_, host.data(), n_ * sizeof(T), stream_), "flk_copy_to_device");
-    check(flk_synchronize(stream_), "flk_synchronize");
   }
   // The last element (e.g. a Merkle tree's root) without copying the rest.
   T back() const {
@@ -610,20 +610,50 @@ inline LigeritoProof recursive_prover_with_basis(const ProverConfig& cfg, Device
     ch.observe_f128(m.u_0);
     ch.observe_f128(m.u_2);
   };
-  auto open = [&](const DeviceVec<F128>& cw, const DeviceVec<Hash>& tree, size_t row_len,
-                  const std::vector<uint64_t>& q) {
-    return OpenedLevel{pcs::gather_rows(cw, row_len, q, stream).to_host(),
-                       merkle::merkle_multi_proof(tree, cw.len() / row_len, q, stream)};
+  // One opening of a level and, for every level but the last, the basis it
+  // induces: the rows are gathered on the device and used there by the
+  // induction, the multi-proof hashes gathered beside them, and rows, hashes
+  // and the induced sum come back in one read-back (the index lists and
+  // constants go up through the staged uploads, which do not wait).
+  struct Opening {
+    OpenedLevel level;
+    DeviceVec<F128> basis;
+    F128 h{0, 0};
   };
-  auto induce = [&](size_t log_cols, const OpenedLevel& op, const std::vector<F128>& v,
-                    const std::vector<uint64_t>& q, const std::vector<F128>& alpha) {
-    DeviceVec<F128> rows(op.rows, stream), basis(size_t(1) << log_cols, stream), sum(1, stream);
-    DeviceVec<uint64_t> qd(q, stream);
-    check(flk_induce_sumcheck_poly(log_cols, rows.data(), q.size(), v.data(), v.size(), qd.data(), alpha.data(),
-                                   alpha.size(), basis.data(), sum.data(), stream),
-          "induce_sumcheck_poly");
-    F128 h = sum.to_host()[0];
-    return std::make_pair(std::move(basis), h);
+  auto open = [&](const DeviceVec<F128>& cw, const DeviceVec<Hash>& tree, size_t row_len,
+                  const std::vector<uint64_t>& q, size_t log_cols, const std::vector<F128>* v,
+                  const std::vector<F128>* alpha) {
+    const size_t n_rows = cw.len() / row_len, nr = q.size() * row_len;
+    size_t depth = 0;
+    while ((size_t(1) << depth) < n_rows) ++depth;
+    const size_t cap = q.size() * (depth ? depth : 1);
+    // [rows (nr F128) | induced sum (1 F128) | hashes (cap x 32 B)] in one device buffer
+    const size_t n128 = nr + 1 + 2 * cap;
+    DeviceVec<F128> buf(n128, stream);
+    F128* rows = buf.data();
+    F128* sum = rows + nr;
+    uint8_t* hashes = reinterpret_cast<uint8_t*>(sum + 1);
+    check(flk_gather_rows(cw.data(), n_rows, row_len, q.data(), q.size(), rows, stream), "gather_rows");
+    size_t k = 0;
+    check(flk_merkle_multi_proof(reinterpret_cast<const uint8_t*>(tree.data()), n_rows, q.data(), q.size(), hashes,
+                                 cap, &k, stream),
+          "merkle_multi_proof");
+    Opening out;
+    if (v) {
+      DeviceVec<uint64_t> qd(q, stream);
+      out.basis = DeviceVec<F128>(size_t(1) << log_cols, stream);
+      check(flk_induce_sumcheck_poly(log_cols, rows, q.size(), v->data(), v->size(), qd.data(), alpha->data(),
+                                     alpha->size(), out.basis.data(), sum, stream),
+            "induce_sumcheck_poly");
+    }
+    const size_t n_read = nr + 1 + 2 * k;
+    std::vector<F128> host(n_read);
+    check(flk_read_to_host(host.data(), buf.data(), n_read * sizeof(F128), stream), "flk_read_to_host");
+    out.level.rows.assign(host.begin(), host.begin() + nr);
+    out.h = host[nr];
+    out.level.merkle_proof.resize(k);
+    if (k) memcpy(out.level.merkle_proof.data(), host.data() + nr + 1, k * sizeof(Hash));
+    return out;
   };
 
   ch.observe_label("flock-ligerito-basis-v0");
@@ -663,10 +693,10 @@ inline LigeritoProof recursive_prover_with_basis(const ProverConfig& cfg, Device
   proof.grinding_nonces.push_back(ch.grind_pow(cfg.grinding_bits[0], stream));
   auto q0 = ch.sample_distinct_queries(block0, cfg.queries[0]);
   auto a0 = ch.sample_f128_vec(ceil_log2(cfg.queries[0]));
-  proof.initial_proof = open(l0_codeword, l0_tree, size_t(1) << k0, q0);
   {
-    auto [basis, h] = induce(n_cur, proof.initial_proof, r_lane, q0, a0);
-    introduce(std::move(basis), h);
+    Opening o = open(l0_codeword, l0_tree, size_t(1) << k0, q0, n_cur, &r_lane, &a0);
+    proof.initial_proof = std::move(o.level);
+    introduce(std::move(o.basis), o.h);
   }
   for (size_t i = 0; i < r; ++i) {
     const std::vector<F128> level_rs = fold_rounds(cfg.recursive_ks[i], i + 1);
@@ -676,7 +706,7 @@ inline LigeritoProof recursive_prover_with_basis(const ProverConfig& cfg, Device
       for (const F128& v : proof.yr) ch.observe_f128(v);
       proof.grinding_nonces.push_back(ch.grind_pow(cfg.grinding_bits[i + 1], stream));
       auto ql = ch.sample_distinct_queries(prev_block, cfg.queries[i + 1]);
-      proof.final_proof = open(prev.codeword, prev.merkle_tree, size_t(1) << prev_k, ql);
+      proof.final_proof = open(prev.codeword, prev.merkle_tree, size_t(1) << prev_k, ql, 0, nullptr, nullptr).level;
       proof.sumcheck_transcript = sp.transcript();
       return proof;
     }
@@ -689,10 +719,10 @@ inline LigeritoProof recursive_prover_with_basis(const ProverConfig& cfg, Device
     proof.grinding_nonces.push_back(ch.grind_pow(cfg.grinding_bits[i + 1], stream));
     auto qi = ch.sample_distinct_queries(prev_block, cfg.queries[i + 1]);
     auto ai = ch.sample_f128_vec(ceil_log2(cfg.queries[i + 1]));
-    proof.recursive_proofs.push_back(open(prev.codeword, prev.merkle_tree, size_t(1) << prev_k, qi));
     {
-      auto [basis, h] = induce(n_cur, proof.recursive_proofs.back(), level_rs, qi, ai);
-      introduce(std::move(basis), h);
+      Opening o = open(prev.codeword, prev.merkle_tree, size_t(1) << prev_k, qi, n_cur, &level_rs, &ai);
+      proof.recursive_proofs.push_back(std::move(o.level));
+      introduce(std::move(o.basis), o.h);
     }
     prev = std::move(nxt);
     prev_k = k_next;
