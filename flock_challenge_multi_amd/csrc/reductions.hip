@@ -178,18 +178,19 @@ static unsigned rd_grid_cap() {
 __global__ void __launch_bounds__(RD_WG) lc_round_msg_msb_kernel(const uint4* __restrict__ comb,
                                                                 const uint4* __restrict__ z, uint64_t half,
                                                                 uint4* __restrict__ out) {
-  __shared__ uint4 tabs[16 * RD_WG];
+  __shared__ uint4 tabs[8 * RD_WG];
   __shared__ uint4 red[2][RD_WG / 64];
   U256 e1, einf;
   u256_zero(e1);
   u256_zero(einf);
   uint4* my = tabs + threadIdx.x;
+  table3_init(tabs, threadIdx.x);
   for (uint64_t i = (uint64_t)blockIdx.x * RD_WG + threadIdx.x; i < half; i += (uint64_t)gridDim.x * RD_WG) {
     const uint4 lo = comb[i], hi = comb[i + half], zl = z[i], zh = z[i + half];
-    table_store_private(tabs, RD_WG, threadIdx.x, hi);
-    comb_mul_acc_rt<2>(zh, my, RD_WG, e1);
-    table_store_private(tabs, RD_WG, threadIdx.x, f_add(hi, lo));
-    comb_mul_acc_rt<2>(f_add(zh, zl), my, RD_WG, einf);
+    table3_store_private(tabs, RD_WG, threadIdx.x, hi);
+    comb3_mul_acc_rt(zh, my, RD_WG, e1);
+    table3_store_private(tabs, RD_WG, threadIdx.x, f_add(hi, lo));
+    comb3_mul_acc_rt(f_add(zh, zl), my, RD_WG, einf);
   }
   wg_reduce_store(f_reduce(e1), red[0], out, 0);
   wg_reduce_store(f_reduce(einf), red[1], out, 1);
@@ -439,21 +440,22 @@ __global__ void __launch_bounds__(FZ_WG) zc_fold_at_z_kernel(const uint8_t* __re
 __global__ void __launch_bounds__(RD_WG) zc_round_pair_kernel(const uint4* __restrict__ a, const uint4* __restrict__ b,
                                                              const uint4* __restrict__ eq, uint64_t half,
                                                              uint4* __restrict__ out) {
-  __shared__ uint4 tabs[16 * RD_WG];
+  __shared__ uint4 tabs[8 * RD_WG];
   __shared__ uint4 red[2][RD_WG / 64];
   U256 g1, ginf;
   u256_zero(g1);
   u256_zero(ginf);
   uint4* my = tabs + threadIdx.x;
+  table3_init(tabs, threadIdx.x);
   for (uint64_t x = (uint64_t)blockIdx.x * RD_WG + threadIdx.x; x < half; x += (uint64_t)gridDim.x * RD_WG) {
     const uint4 a0 = a[2 * x], a1 = a[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1], e = eq[x];
-    table_store_private(tabs, RD_WG, threadIdx.x, e);
-    const uint4 ea1 = comb_mul_rt(a1, my, RD_WG);
-    const uint4 eas = comb_mul_rt(f_add(a0, a1), my, RD_WG);
-    table_store_private(tabs, RD_WG, threadIdx.x, ea1);
-    comb_mul_acc_rt(b1, my, RD_WG, g1);
-    table_store_private(tabs, RD_WG, threadIdx.x, eas);
-    comb_mul_acc_rt(f_add(b0, b1), my, RD_WG, ginf);
+    table3_store_private(tabs, RD_WG, threadIdx.x, e);
+    const uint4 ea1 = comb3_mul_rt(a1, my, RD_WG);
+    const uint4 eas = comb3_mul_rt(f_add(a0, a1), my, RD_WG);
+    table3_store_private(tabs, RD_WG, threadIdx.x, ea1);
+    comb3_mul_acc_rt(b1, my, RD_WG, g1);
+    table3_store_private(tabs, RD_WG, threadIdx.x, eas);
+    comb3_mul_acc_rt(f_add(b0, b1), my, RD_WG, ginf);
   }
   wg_reduce_store(f_reduce(g1), red[0], out, 0);
   wg_reduce_store(f_reduce(ginf), red[1], out, 1);
@@ -478,14 +480,14 @@ enum ZfMode { ZF_PLAIN = 0, ZF_FOLD = 1, ZF_BITS = 2 };
 // not null) receives eq_hi with its lowest coordinate summed out, the next
 // round's eq_hi: eq_hi_next[y] = eq_hi[2y] + eq_hi[2y + 1].
 template <int MODE>
-__global__ void __launch_bounds__(ZF_WG) zc_fold_round_kernel(const uint4* __restrict__ a, const uint4* __restrict__ b,
+__global__ void __launch_bounds__(ZF_WG) __attribute__((amdgpu_waves_per_eu(4))) zc_fold_round_kernel(const uint4* __restrict__ a, const uint4* __restrict__ b,
                                                              uint64_t n_chunks, uint4 rho,
                                                              const uint4* __restrict__ w,
                                                              const uint4* __restrict__ e_lo,
                                                              const uint4* __restrict__ e_hi, uint4* __restrict__ a_out,
                                                              uint4* __restrict__ b_out, uint4* __restrict__ e_hi_next,
                                                              uint4* __restrict__ out) {
-  __shared__ uint4 tabs[16 * ZF_WG];  // per-lane tables of the data x data products
+  __shared__ uint4 tabs[8 * ZF_WG];  // per-lane tables of the data x data products (3-bit comb)
   __shared__ uint4 rtab[128], rnib[32];
   __shared__ uint4 htab[128], hnib[32];
   __shared__ uint4 wnib[MODE == ZF_BITS ? 16 * 16 : 1];  // 16 nibble positions x 16 subset sums of w
@@ -505,6 +507,7 @@ __global__ void __launch_bounds__(ZF_WG) zc_fold_round_kernel(const uint4* __res
   u256_zero(g1);
   u256_zero(gi);
   uint4* my = tabs + tid;
+  table3_init(tabs, tid);
   // The chunk's inputs are loaded one iteration ahead (u: a, v: b; FOLD 4 each,
   // PLAIN 2 each, BITS 1 each), so their latency overlaps the previous chunk's
   // products and this chunk's table build.
@@ -558,14 +561,14 @@ __global__ void __launch_bounds__(ZF_WG) zc_fold_round_kernel(const uint4* __res
     }
     const uint4 ea1 = combjr_mul_add(a1, f_zero(), htab);
     const uint4 eas = combjr_mul_add(f_add(a0, a1), f_zero(), htab);
-    table_store_private(tabs, ZF_WG, tid, ea1);
-    comb_mul_acc_rt<8>(b1, my, ZF_WG, g1);
-    table_store_private(tabs, ZF_WG, tid, eas);
-    comb_mul_acc_rt<8>(f_add(b0, b1), my, ZF_WG, gi);
+    table3_store_private(tabs, ZF_WG, tid, ea1);
+    comb3_mul_acc_rt(b1, my, ZF_WG, g1);
+    table3_store_private(tabs, ZF_WG, tid, eas);
+    comb3_mul_acc_rt(f_add(b0, b1), my, ZF_WG, gi);
   }
-  table_store_private(tabs, ZF_WG, tid, e_lo[tid]);  // the lane's own table: no barrier needed
-  const uint4 r1 = comb_mul_rt(f_reduce(g1), my, ZF_WG);
-  const uint4 ri = comb_mul_rt(f_reduce(gi), my, ZF_WG);
+  table3_store_private(tabs, ZF_WG, tid, e_lo[tid]);  // the lane's own table: no barrier needed
+  const uint4 r1 = comb3_mul_rt(f_reduce(g1), my, ZF_WG);
+  const uint4 ri = comb3_mul_rt(f_reduce(gi), my, ZF_WG);
   wg_reduce_store(r1, red[0], out, 0);
   wg_reduce_store(ri, red[1], out, 1);
 }
@@ -628,7 +631,7 @@ __global__ void __launch_bounds__(256) zc_scaled_weights_kernel(const uint4* __r
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(ZF_WG) zc_bits_round_kernel(const uint4* __restrict__ a, const uint4* __restrict__ b,
+__global__ void __launch_bounds__(ZF_WG) __attribute__((amdgpu_waves_per_eu(MODE == ZB_FOLD ? 3 : 2))) zc_bits_round_kernel(const uint4* __restrict__ a, const uint4* __restrict__ b,
                                                              uint64_t n_chunks, uint4 rho,
                                                              const uint4* __restrict__ w,
                                                              const uint4* __restrict__ ew,
@@ -636,7 +639,7 @@ __global__ void __launch_bounds__(ZF_WG) zc_bits_round_kernel(const uint4* __res
                                                              const uint4* __restrict__ e_hi, uint4* __restrict__ a_out,
                                                              uint4* __restrict__ b_out, uint4* __restrict__ e_hi_next,
                                                              uint4* __restrict__ out) {
-  __shared__ uint4 tabs[16 * ZF_WG];  // per-lane tables of the data x data products
+  __shared__ uint4 tabs[8 * ZF_WG];  // per-lane tables of the data x data products (3-bit comb)
   __shared__ uint4 wnib[256];         // N: nibble tables of w
   __shared__ uint4 xnib[256];         // ZB_MSG: of eq_hi[c] * w (per chunk); ZB_FOLD: of rho * w (N_rho)
   __shared__ uint4 htab[MODE == ZB_FOLD ? 128 : 1], hnib[MODE == ZB_FOLD ? 32 : 1];
@@ -656,6 +659,7 @@ __global__ void __launch_bounds__(ZF_WG) zc_bits_round_kernel(const uint4* __res
   u256_zero(g1);
   u256_zero(gi);
   uint4* my = tabs + tid;
+  table3_init(tabs, tid);
   // A pair's rows are loaded one chunk ahead: ZB_MSG rows 2x, 2x + 1 (one uint4 per
   // vector), ZB_FOLD rows 4y .. 4y + 3 (two).
   constexpr int NL = MODE == ZB_FOLD ? 2 : 1;
@@ -705,14 +709,14 @@ __global__ void __launch_bounds__(ZF_WG) zc_bits_round_kernel(const uint4* __res
       a1 = combjr_mul_add(ta1, f_zero(), htab);
       as = combjr_mul_add(f_add(a0, ta1), f_zero(), htab);
     }
-    table_store_private(tabs, ZF_WG, tid, a1);
-    comb_mul_acc_rt<8>(b1, my, ZF_WG, g1);
-    table_store_private(tabs, ZF_WG, tid, as);
-    comb_mul_acc_rt<8>(bs, my, ZF_WG, gi);
+    table3_store_private(tabs, ZF_WG, tid, a1);
+    comb3_mul_acc_rt(b1, my, ZF_WG, g1);
+    table3_store_private(tabs, ZF_WG, tid, as);
+    comb3_mul_acc_rt(bs, my, ZF_WG, gi);
   }
-  table_store_private(tabs, ZF_WG, tid, e_lo[tid]);  // the lane's own table: no barrier needed
-  const uint4 r1 = comb_mul_rt(f_reduce(g1), my, ZF_WG);
-  const uint4 ri = comb_mul_rt(f_reduce(gi), my, ZF_WG);
+  table3_store_private(tabs, ZF_WG, tid, e_lo[tid]);  // the lane's own table: no barrier needed
+  const uint4 r1 = comb3_mul_rt(f_reduce(g1), my, ZF_WG);
+  const uint4 ri = comb3_mul_rt(f_reduce(gi), my, ZF_WG);
   wg_reduce_store(r1, red[0], out, 0);
   wg_reduce_store(ri, red[1], out, 1);
 }

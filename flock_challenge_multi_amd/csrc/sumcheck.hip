@@ -51,7 +51,7 @@ __device__ __forceinline__ void wg_xor_reduce_store(uint4 v, uint4* red_lds, uin
 template <bool EVAL>
 __global__ void __launch_bounds__(SC_WG) round_msg_kernel(const uint4* __restrict__ f, const uint4* __restrict__ b,
                                                          uint64_t n_pairs, uint4* __restrict__ out) {
-  __shared__ uint4 tabs[16 * SC_WG];
+  __shared__ uint4 tabs[8 * SC_WG];
   __shared__ uint4 red[3][SC_WG / 64];
   const uint32_t tid = threadIdx.x;
   U256 a0, a2, a1;
@@ -59,15 +59,16 @@ __global__ void __launch_bounds__(SC_WG) round_msg_kernel(const uint4* __restric
   u256_zero(a2);
   u256_zero(a1);
   uint4* my = tabs + tid;
+  table3_init(tabs, tid);
   for (uint64_t j = (uint64_t)blockIdx.x * SC_WG + tid; j < n_pairs; j += (uint64_t)gridDim.x * SC_WG) {
     const uint4 f0 = f[2 * j], f1 = f[2 * j + 1], b0 = b[2 * j], b1 = b[2 * j + 1];
-    table_store_private(tabs, SC_WG, tid, f0);
-    comb_mul_acc_rt<2>(b0, my, SC_WG, a0);
-    table_store_private(tabs, SC_WG, tid, f_add(f0, f1));
-    comb_mul_acc_rt<2>(f_add(b0, b1), my, SC_WG, a2);
+    table3_store_private(tabs, SC_WG, tid, f0);
+    comb3_mul_acc_rt(b0, my, SC_WG, a0);
+    table3_store_private(tabs, SC_WG, tid, f_add(f0, f1));
+    comb3_mul_acc_rt(f_add(b0, b1), my, SC_WG, a2);
     if (EVAL) {
-      table_store_private(tabs, SC_WG, tid, f1);
-      comb_mul_acc_rt<2>(b1, my, SC_WG, a1);
+      table3_store_private(tabs, SC_WG, tid, f1);
+      comb3_mul_acc_rt(b1, my, SC_WG, a1);
     }
   }
   const uint4 u0 = f_reduce(a0), u2 = f_reduce(a2);
@@ -97,7 +98,7 @@ static unsigned reduce_grid(uint64_t items) {
 __global__ void __launch_bounds__(SC_WG) fold_and_msg_kernel(const uint4* __restrict__ f, const uint4* __restrict__ b,
                                                             uint64_t n_quads, uint4 r, uint4* __restrict__ f_out,
                                                             uint4* __restrict__ b_out, uint4* __restrict__ out) {
-  __shared__ uint4 tabs[16 * SC_WG];
+  __shared__ uint4 tabs[8 * SC_WG];
   __shared__ uint4 rtab[128];  // nibble-position tables of r
   __shared__ uint4 rnib[32];
   __shared__ uint4 red[2][SC_WG / 64];
@@ -107,6 +108,7 @@ __global__ void __launch_bounds__(SC_WG) fold_and_msg_kernel(const uint4* __rest
   u256_zero(a0);
   u256_zero(a2);
   uint4* my = tabs + tid;
+  table3_init(tabs, tid);
   for (uint64_t q = (uint64_t)blockIdx.x * SC_WG + tid; q < n_quads; q += (uint64_t)gridDim.x * SC_WG) {
     const uint4* fp = f + 4 * q;
     const uint4* bp = b + 4 * q;
@@ -120,10 +122,10 @@ __global__ void __launch_bounds__(SC_WG) fold_and_msg_kernel(const uint4* __rest
     f_out[2 * q + 1] = nf1;
     b_out[2 * q] = nb0;
     b_out[2 * q + 1] = nb1;
-    table_store_private(tabs, SC_WG, tid, nf0);
-    comb_mul_acc_rt<8>(nb0, my, SC_WG, a0);
-    table_store_private(tabs, SC_WG, tid, f_add(nf0, nf1));
-    comb_mul_acc_rt<8>(f_add(nb0, nb1), my, SC_WG, a2);
+    table3_store_private(tabs, SC_WG, tid, nf0);
+    comb3_mul_acc_rt(nb0, my, SC_WG, a0);
+    table3_store_private(tabs, SC_WG, tid, f_add(nf0, nf1));
+    comb3_mul_acc_rt(f_add(nb0, nb1), my, SC_WG, a2);
   }
   wg_xor_reduce_store(f_reduce(a0), red[0], out, 0);
   wg_xor_reduce_store(f_reduce(a2), red[1], out, 1);
