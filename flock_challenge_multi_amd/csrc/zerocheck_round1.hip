@@ -30,7 +30,8 @@
 //     lookups), stored to LDS;
 //   phase 2a, lane = (byte word w, J-group): Horner over K for the half's 2 J
 //     values of its group and 4 l at once (packed bytes); after both halves,
-//     phi8 by two nibble lookups, shifted by J, for its 4 J;
+//     x^J phi8 of each byte by two lookups in J's table of x^J phi8(n) and
+//     x^J phi8(n << 4) (reduced), summed over its 4 J;
 //   phase 2b, lane = l: the sum over the four J-groups, then one product by
 //     eq_outer with nibble-pair tables (comb4p_mul_add).
 // Every data-indexed 16-byte LDS table has 16 entries (conflict-free); the byte
@@ -54,7 +55,7 @@ struct Round1Tables {
   uint8_t mul_hi[256 * 16];    // [a][n] = a * (n << 4) in GF(2^8)
   uint8_t mul64[256 * 256];    // [a][b] = a * b in GF(2^8)
   uint8_t M[64 * 64];          // [s][l]: the LDE of the unit vector e_s at point l
-  uint4 phi_nib[32];           // phi8(n), then phi8(n << 4), n < 16
+  uint4 xj[16 * 32];           // [J][n] = x^J * phi8(n), [J][16 + n] = x^J * phi8(n << 4), reduced
   uint4 phi_pow[8];            // phi8(2^i)
 };
 
@@ -146,11 +147,14 @@ static void build_round1_tables(Round1Tables& T) {
   auto as_u4 = [](flk_f128 f) {
     return make_uint4((uint32_t)f.lo, (uint32_t)(f.lo >> 32), (uint32_t)f.hi, (uint32_t)(f.hi >> 32));
   };
-  for (int n = 0; n < 16; ++n) {
-    T.phi_nib[n] = as_u4(phi8_host((uint8_t)n));
-    T.phi_nib[16 + n] = as_u4(phi8_host((uint8_t)(n << 4)));
-  }
   for (int i = 0; i < 8; ++i) T.phi_pow[i] = as_u4(phi8_host((uint8_t)(1u << i)));
+  for (int J = 0; J < 16; ++J) {
+    const flk_f128 xJ{1ull << J, 0};
+    for (int n = 0; n < 16; ++n) {
+      T.xj[J * 32 + n] = as_u4(host_mul(xJ, phi8_host((uint8_t)n)));
+      T.xj[J * 32 + 16 + n] = as_u4(host_mul(xJ, phi8_host((uint8_t)(n << 4))));
+    }
+  }
 }
 
 // C_s / D with C_s = phi8(0x1C), D = prod_{i<4} (1 + x^(2^i)).
@@ -226,7 +230,7 @@ __device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcn
 // ypart[(q * 4 + j) * 64 + (lane ^ 16 j)] (the XOR keeps the phase-2b gathers of
 // the four j in distinct banks).
 constexpr uint32_t R1_RP = 17, R1_YP = 64;
-constexpr uint32_t R1_SCRATCH = 20 * R1_YP;  // words; >= 64 * R1_RP
+constexpr uint32_t R1_SCRATCH = 64 * R1_RP;  // words; >= 16 * R1_YP
 
 // Each wave takes blocks blockIdx.x * NW + wave, then strides by gridDim.x * NW;
 // each block follows the phases described at the top of the file.
@@ -237,7 +241,7 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
     const uint4* __restrict__ eq_outer, const Round1Tables* __restrict__ T, uint4* __restrict__ out) {
   constexpr int NW = r1_waves<FULL>(), R1_WG = NW * 64;
   static_assert(64 * R1_RP <= R1_SCRATCH, "a half block's rows must fit in the scratch");
-  static_assert(64 * 4 <= R1_SCRATCH, "a wave's sums must fit in its scratch");
+  static_assert(16 * R1_YP <= R1_SCRATCH, "a wave's sums must fit in its scratch");
   // LDE contributions, chunk-major: lde16[(p * 4 + q) * 16 + v] is bytes 16q..16q+15
   // of the 64-byte image of nibble value v at nibble position p.  For a fixed
   // (p, q) the 16 values sit in 16 distinct LDS bank groups, so the data-indexed
@@ -249,7 +253,10 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
     // mul[a * 16 + n] = a * n and mul[4096 + a * 16 + n] = a * (n << 4)
     uint8_t mul[FULL ? 65536 : 8192];
     uint4 lde16[16 * 4 * 16];
-    uint4 phin[32];  // phi8(n), phi8(n << 4): phi8(v) = phin[v & 15] ^ phin[16 + (v >> 4)]
+    // x^J * phi8(v) = xj[32 J + (v & 15)] ^ xj[32 J + 16 + (v >> 4)] (reduced): the
+    // 16 lanes of a quarter-wave share J (one J-group), so their lookups stay in
+    // one 16-entry table each
+    uint4 xj[16 * 32];
     uint4 etab[NW][32];  // eq_outer * n and eq_outer * x^4 * n (comb4p_mul_add)
     uint32_t scratch[NW][R1_SCRATCH];
   };
@@ -257,7 +264,7 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
   uint4* const lde16 = L.lde16;
   uint8_t* const mul_lo = L.mul;
   uint8_t* const mul_hi = L.mul + 4096;
-  uint4* const phin = L.phin;
+  uint4* const xj = L.xj;
   const uint32_t tid = threadIdx.x, wave = tid >> 6, t = tid & 63u;
   uint32_t* const rows = L.scratch[wave];
   uint32_t* const ypart = rows;
@@ -279,7 +286,7 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
         reinterpret_cast<uint32_t*>(mul_hi)[i] = mhi[i];
       }
     }
-    if (tid < 32) phin[tid] = T->phi_nib[tid];
+    for (uint32_t i = tid; i < 16 * 32; i += R1_WG) xj[i] = T->xj[i];
   }
   // phase-2a role: 16 words x 4 J-groups; the group jg takes J = 2jg, 2jg + 1 of
   // each half (J = 8 * half + 2 * jg + jj)
@@ -362,49 +369,40 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
       }
       wave_lds_sync();  // every lane has read the half's rows; the next writes reuse them
     }
-    uint32_t Y[4][5];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int q = 0; q < 5; ++q) Y[j][q] = 0u;
+    // Y[j] = sum over the lane's four J of x^J phi8(byte j of wj[i]), reduced
+    uint4 Y[4] = {f_zero(), f_zero(), f_zero(), f_zero()};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const uint32_t J = 8 * (i >> 1) + 2 * jg + (i & 1);  // shift phi8(w) by J < 16 bits
+      const uint4* xt = xj + 32 * (8 * (i >> 1) + 2 * jg + (i & 1));
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const uint32_t byte = (wj[i] >> (8 * j)) & 255u;
-        const uint4 f0 = phin[byte & 15u], f1 = phin[16 + (byte >> 4)];
-        const uint4 f = make_uint4(f0.x ^ f1.x, f0.y ^ f1.y, f0.z ^ f1.z, f0.w ^ f1.w);
-        if (J == 0) {
-          Y[j][0] ^= f.x; Y[j][1] ^= f.y; Y[j][2] ^= f.z; Y[j][3] ^= f.w;
-        } else {
-          Y[j][0] ^= f.x << J;
-          Y[j][1] ^= funnel(f.y, f.x, 32 - J);
-          Y[j][2] ^= funnel(f.z, f.y, 32 - J);
-          Y[j][3] ^= funnel(f.w, f.z, 32 - J);
-          Y[j][4] ^= f.w >> (32 - J);
-        }
+        const uint4 f0 = xt[byte & 15u], f1 = xt[16 + (byte >> 4)];
+        Y[j] = make_uint4(xor3(Y[j].x, f0.x, f1.x), xor3(Y[j].y, f0.y, f1.y), xor3(Y[j].z, f0.z, f1.z),
+                          xor3(Y[j].w, f0.w, f1.w));
       }
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int q = 0; q < 5; ++q) ypart[(q * 4 + j) * R1_YP + (t ^ (16u * j))] = Y[j][q];
+    for (int j = 0; j < 4; ++j) {
+      ypart[(0 * 4 + j) * R1_YP + (t ^ (16u * j))] = Y[j].x;
+      ypart[(1 * 4 + j) * R1_YP + (t ^ (16u * j))] = Y[j].y;
+      ypart[(2 * 4 + j) * R1_YP + (t ^ (16u * j))] = Y[j].z;
+      ypart[(3 * 4 + j) * R1_YP + (t ^ (16u * j))] = Y[j].w;
+    }
     wave_lds_sync();
     // ---- phase 2b: lane l: finish the J sum over the four J-groups, multiply by
     // eq_outer ----
     {
       const uint32_t l = t, ww = l >> 2, j = l & 3u;
-      uint32_t y[8];
+      uint32_t y[4];
 #pragma unroll
-      for (int q = 0; q < 5; ++q) {
+      for (int q = 0; q < 4; ++q) {
         // writer lane of J-group g: ww + 16 g, stored at (ww + 16 g) ^ 16 j
         const uint32_t* yp = ypart + (q * 4 + j) * R1_YP;
         y[q] = xor3(yp[ww ^ (16u * j)], yp[(ww + 16u) ^ (16u * j)], yp[(ww + 32u) ^ (16u * j)]) ^
                yp[(ww + 48u) ^ (16u * j)];
       }
-      y[5] = y[6] = y[7] = 0u;
-      acc = comb4p_mul_add(f_reduce_add(y, f_zero()), acc, etab);
+      acc = comb4p_mul_add(make_uint4(y[0], y[1], y[2], y[3]), acc, etab);
     }
     wave_lds_sync();  // the next block overwrites the scratch and etab
   }
