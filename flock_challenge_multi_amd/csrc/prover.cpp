@@ -100,6 +100,18 @@ void zerocheck_prove(Fs& ch, const uint8_t* a_bits, const uint8_t* b_bits, const
   DeviceVec<F128> r1(128, st);
   chk(flk::zc_round1_keep_s(a_bits, b_bits, c_bits, m, r_outer.data(), r1.data(), r1.data() + 64, s_c, st),
       "zc_round1");
+  // The eq tables of the device multilinear rounds depend on r alone, so they are
+  // enqueued behind round 1 instead of after its read-back.  eq_lo: each device
+  // round's first 8 eq coordinates; eq_hi (two buffers, alternating): the rest,
+  // handed from round to round by summing out the lowest coordinate.
+  const size_t d0 = n_mlv - 1;
+  const size_t n_dev = d0 >= 8 ? d0 - 7 : 0;  // rounds i with n_mlv - i - 1 >= 8
+  DeviceVec<F128> eq_lo(256 * (n_dev ? n_dev : 1), st),
+      eq_hi[2] = {DeviceVec<F128>(d0 >= 8 ? size_t(1) << (d0 - 8) : 1, st),
+                  DeviceVec<F128>(d0 >= 9 ? size_t(1) << (d0 - 9) : 1, st)};
+  for (size_t i = 0; i < n_dev; ++i)
+    chk(flk_build_eq(r.data() + K_SKIP + i + 1, 8, eq_lo.data() + 256 * i, st), "build_eq(lo)");
+  if (d0 >= 8) chk(flk_build_eq(r.data() + K_SKIP + 1 + 8, d0 - 8, eq_hi[0].data(), st), "build_eq(hi)");
   const std::vector<F128> h = r1.to_host();
   pf.round1_ab.assign(h.begin(), h.begin() + 64);
   pf.round1_c.assign(h.begin() + 64, h.end());
@@ -125,13 +137,7 @@ void zerocheck_prove(Fs& ch, const uint8_t* a_bits, const uint8_t* b_bits, const
   DeviceVec<F128> wd(ws, st);
   size_t n = size_t(1) << n_mlv;
   DeviceVec<F128> am, bm, msg(2, st);
-  const size_t d0 = n_mlv - 1;
-  // eq_lo: each device round's first 8 eq coordinates (all known before round 0,
-  // so the tables are built up front, off the rounds' critical path); eq_hi (two
-  // buffers, alternating): the rest, handed from round to round by summing out
-  // the lowest coordinate.
-  const size_t n_dev = d0 >= 8 ? d0 - 7 : 0;  // rounds i with n_mlv - i - 1 >= 8
-  // With two such rounds, rounds 0 and 1 both read the bits and the fold at z is
+  // With two device rounds, rounds 0 and 1 both read the bits and the fold at z is
   // never written (round 0 sends the message alone, round 1 writes the table
   // bound by rho_0); otherwise the fold at z is written in full.
   const bool from_bits = n_dev >= 2;
@@ -139,14 +145,7 @@ void zerocheck_prove(Fs& ch, const uint8_t* a_bits, const uint8_t* b_bits, const
     am = DeviceVec<F128>(n, st);
     bm = DeviceVec<F128>(n, st);
   }
-  DeviceVec<F128> eq_lo(256 * (n_dev ? n_dev : 1), st),
-      eq_hi[2] = {DeviceVec<F128>(d0 >= 8 ? size_t(1) << (d0 - 8) : 1, st),
-                  DeviceVec<F128>(d0 >= 9 ? size_t(1) << (d0 - 9) : 1, st)};
-  for (size_t i = 0; i < n_dev; ++i)
-    chk(flk_build_eq(r.data() + K_SKIP + i + 1, 8, eq_lo.data() + 256 * i, st), "build_eq(lo)");
-  if (d0 >= 8) {
-    chk(flk_build_eq(r.data() + K_SKIP + 1 + 8, d0 - 8, eq_hi[0].data(), st), "build_eq(hi)");
-  } else {
+  if (d0 < 8) {
     chk(flk_zc_fold_at_z(a_bits, m, K_SKIP, wd.data(), am.data(), st), "zc_fold_at_z(a)");
     chk(flk_zc_fold_at_z(b_bits, m, K_SKIP, wd.data(), bm.data(), st), "zc_fold_at_z(b)");
   }

@@ -10,8 +10,8 @@
 // with alpha = phi8(0x02), C_s = phi8(0x1C), D = prod_{i<4} (1 + x^(2^i)), so
 //     P[l] = (C_s / D) sum_{x_outer} eq_outer[x_outer] *
 //            sum_J x^J phi8( sum_K alpha^K v_l(K, J, x_outer) )
-// and the inner sums are GF(2^8) work (Horner in alpha = xtime on packed bytes)
-// plus bit shifts by J; one GF(2^128) product per (x_outer, l).
+// and the inner sums are GF(2^8) work (Horner in alpha) plus x^J phi8 of the
+// J-sums; one GF(2^128) product per (x_outer, l).
 //
 // P_C needs no per-row LDE: the LDE is GF(2^8)-linear and c is a bit, so
 //     P_C[l] = sum_s phi8(M[s][l]) * Q[s],   Q[s] = sum_x eq(x) c_x[s],
@@ -20,23 +20,15 @@
 // read as 2^(m-7) elements of 128 bits, weighted by eq over r[7..m)), and a
 // 64-thread epilogue binds r[6] and applies M.
 //
-// In the P_AB kernel one wave processes one 128-row block (one x_outer) at a
-// time; per block:
-//   phase 1, per 64-row half, lane t = row 64 * half + t (row r = (K, J) =
-//     (r & 7, r >> 3)): the 64-point LDE of the row's a and b bits (a fixed GF(2)-linear map 64 bits -> 64 bytes, applied
-//     with 16 nibble tables of 64 bytes), v_ab = a_l * b_l in GF(2^8) from the
-//     full 64 KiB product table (one lookup of a * 256 + b, the index formed by
-//     one byte permute; FLOCK_R1_FULLTAB=0 selects split-nibble tables, two
-//     lookups), stored to LDS;
-//   phase 2a, lane = (byte word w, J-group): Horner over K for the half's 2 J
-//     values of its group and 4 l at once (packed bytes); after both halves,
-//     x^J phi8 of each byte by two lookups in J's table of x^J phi8(n) and
-//     x^J phi8(n << 4) (reduced), summed over its 4 J;
-//   phase 2b, lane = l: the sum over the four J-groups, then one product by
-//     eq_outer with nibble-pair tables (comb4p_mul_add).
-// Every data-indexed 16-byte LDS table has 16 entries (conflict-free); the byte
-// product table takes one lookup per product where the split tables take two
-// and 2.3x the address arithmetic, which outweighs its bank conflicts.
+// The P_AB kernel keeps the GF(2^8) work on bit planes: the LDE (a fixed
+// GF(2)-linear map, 64 bits -> 64 bytes) is applied with nibble tables whose
+// entries are already transposed, so a row's 64 values arrive as 8 planes per
+// 32 points (plane p: bit p of each value).  A product of 32 value pairs is then
+// 64 AND-XORs plus the reduction, with no lookups (a 64 KiB byte-product table
+// costs a quarter of the CU's cycles in LDS bank conflicts), and the Horner step
+// in alpha moves planes.  A lane owns the 8 rows (K, J) of one J of a block, so
+// the sum over K never leaves it; the J-sums go through the wave's LDS as bytes.
+// Every data-indexed 16-byte LDS table has 16 entries (conflict-free).
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
@@ -50,13 +42,10 @@
 namespace flk {
 
 struct Round1Tables {
-  uint32_t lde[16 * 16 * 16];  // [nibble pos p][value v][word w]: bytes l = 4w..4w+3
-  uint8_t mul_lo[256 * 16];    // [a][n] = a * n        in GF(2^8)
-  uint8_t mul_hi[256 * 16];    // [a][n] = a * (n << 4) in GF(2^8)
-  uint8_t mul64[256 * 256];    // [a][b] = a * b in GF(2^8)
+  uint32_t lde[16 * 16 * 16];  // [nibble pos p][value v][word w]: bit planes of the LDE image (below)
   uint8_t M[64 * 64];          // [s][l]: the LDE of the unit vector e_s at point l
-  uint4 xj[16 * 32];           // [J][n] = x^J * phi8(n), [J][16 + n] = x^J * phi8(n << 4), reduced
   uint4 phi_pow[8];            // phi8(2^i)
+  uint4 xj[16 * 32];           // [J][n] = x^J * phi8(n), [J][16 + n] = x^J * phi8(n << 4), reduced
 };
 
 // ---- host GF(2^8) (AES polynomial) and its additive NTT (CF/ntt.rs:22-169) ----
@@ -134,15 +123,13 @@ static void build_round1_tables(Round1Tables& T) {
       for (int j = 0; j < 4; ++j)
         if ((val >> j) & 1)
           for (int l = 0; l < 64; ++l) bytes[l] ^= M[4 * p + j][l];
-      memcpy(&T.lde[(p * 16 + val) * 16], bytes, 64);
+      // bit planes: word w = 8h + p' holds bit p' of bytes l = 32h .. 32h + 31
+      for (int w = 0; w < 16; ++w) {
+        uint32_t word = 0;
+        for (int i = 0; i < 32; ++i) word |= (uint32_t)((bytes[32 * (w >> 3) + i] >> (w & 7)) & 1) << i;
+        T.lde[(p * 16 + val) * 16 + w] = word;
+      }
     }
-  for (int a = 0; a < 256; ++a)
-    for (int n = 0; n < 16; ++n) {
-      T.mul_lo[a * 16 + n] = g8mul((uint8_t)a, (uint8_t)n);
-      T.mul_hi[a * 16 + n] = g8mul((uint8_t)a, (uint8_t)(n << 4));
-    }
-  for (int a = 0; a < 256; ++a)
-    for (int b = 0; b < 256; ++b) T.mul64[a * 256 + b] = g8mul((uint8_t)a, (uint8_t)b);
   memcpy(T.M, M, sizeof(T.M));
   auto as_u4 = [](flk_f128 f) {
     return make_uint4((uint32_t)f.lo, (uint32_t)(f.lo >> 32), (uint32_t)f.hi, (uint32_t)(f.hi >> 32));
@@ -205,18 +192,77 @@ static const Round1Tables* round1_tables_device(int* status) {
   return d;
 }
 
-constexpr int R1_ROWS = 128;  // rows of one block (one x_outer)
-// Waves per work-group: each wave processes whole blocks on its own (no
-// work-group barrier inside the block loop), so the waves of a CU drift apart
-// and one wave's table lookups overlap another's arithmetic.  A wave's block
-// scratch holds one 64-row half of a block at a time (5.5 KiB), so with the
-// full 64 KiB product table one work-group of 12 waves fits the LDS (146 KiB);
-// with the split-nibble tables (8 KiB) two work-groups of 8 waves (69 KiB each).
-template <bool FULL>
-constexpr int r1_waves() { return FULL ? 12 : 8; }
+constexpr int R1_ROWS = 128;   // rows of one block (one x_outer)
+constexpr int R1_NW = 8;       // waves per work-group
+constexpr int R1_QUAD = 4;     // blocks per wave pass: lane = (block b, J)
+constexpr uint32_t R1_XS = 17; // words per (b, J) row of a wave's exchange (16 + 1: conflict-free stores)
 
-__device__ __forceinline__ uint32_t xtime4(uint32_t w) {
-  return ((w & 0x7F7F7F7Fu) << 1) ^ (((w >> 7) & 0x01010101u) * 0x1Bu);
+// GF(2^8) (x^8 + x^4 + x^3 + x + 1) products of 32 pairs at once on bit planes:
+// c[p] bit i = bit p of a_i * b_i, where a[p] bit i = bit p of a_i.  Schoolbook
+// (an AND-XOR per term in one V_BITOP3_B32, truth table 0x6A = (s0 & s1) ^ s2),
+// then x^k = x^(k-4) + x^(k-5) + x^(k-7) + x^(k-8) for k = 14 .. 8.
+__device__ __forceinline__ void gf8_mul_planes(const uint32_t* a, const uint32_t* b, uint32_t* c) {
+  uint32_t t[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) {
+    const int i0 = k < 8 ? 0 : k - 7, i1 = k < 8 ? k : 7;
+    t[k] = a[i0] & b[k - i0];
+#pragma unroll
+    for (int i = i0 + 1; i <= i1; ++i) t[k] = __builtin_amdgcn_bitop3_b32(a[i], b[k - i], t[k], 0x6A);
+  }
+#pragma unroll
+  for (int k = 14; k >= 8; --k) {
+    t[k - 8] ^= t[k];
+    t[k - 7] ^= t[k];
+    t[k - 5] ^= t[k];
+    t[k - 4] ^= t[k];
+  }
+#pragma unroll
+  for (int p = 0; p < 8; ++p) c[p] = t[p];
+}
+
+// u <- x * u + v on bit planes (x = alpha = 0x02): the planes move up one, plane
+// 7 folds into planes 0, 1, 3, 4.
+__device__ __forceinline__ void gf8_horner_planes(uint32_t* u, const uint32_t* v) {
+  const uint32_t u7 = u[7];
+  u[7] = u[6] ^ v[7];
+  u[6] = u[5] ^ v[6];
+  u[5] = u[4] ^ v[5];
+  u[4] = xor3(u[3], u7, v[4]);
+  u[3] = xor3(u[2], u7, v[3]);
+  u[2] = u[1] ^ v[2];
+  u[1] = xor3(u[0], u7, v[1]);
+  u[0] = u7 ^ v[0];
+}
+
+// Bit planes -> bytes: P[p] bit i = bit p of byte i (i < 32) becomes w[q] byte j =
+// byte 4q + j.  Three delta-swap stages transpose the 8 x 8 bit blocks of each
+// byte column (afterwards P[c] byte k = byte 8k + c), then byte permutes gather
+// w[2k + g] = (P[4g] .. P[4g + 3]) byte k.
+__device__ __forceinline__ void planes_to_bytes(uint32_t* P, uint32_t* w) {
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const int d = 1 << s;
+    const uint32_t mask = s == 0 ? 0x55555555u : (s == 1 ? 0x33333333u : 0x0F0F0F0Fu);
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      if (p & d) continue;
+      const uint32_t x = __builtin_amdgcn_bitop3_b32(P[p] >> d, P[p + d], mask, 0x28);  // (s0 ^ s1) & s2
+      P[p + d] ^= x;
+      P[p] ^= x << d;
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    // x01 = (P0.b0, P1.b0, P0.b2, P1.b2), y01 = (P0.b1, P1.b1, P0.b3, P1.b3), likewise x23, y23
+    const uint32_t* Q = P + 4 * g;
+    const uint32_t x01 = __builtin_amdgcn_perm(Q[1], Q[0], 0x06020400u), y01 = __builtin_amdgcn_perm(Q[1], Q[0], 0x07030501u);
+    const uint32_t x23 = __builtin_amdgcn_perm(Q[3], Q[2], 0x06020400u), y23 = __builtin_amdgcn_perm(Q[3], Q[2], 0x07030501u);
+    w[g] = __builtin_amdgcn_perm(x23, x01, 0x05040100u);      // byte column 0
+    w[4 + g] = __builtin_amdgcn_perm(x23, x01, 0x07060302u);  // byte column 2
+    w[2 + g] = __builtin_amdgcn_perm(y23, y01, 0x05040100u);  // byte column 1
+    w[6 + g] = __builtin_amdgcn_perm(y23, y01, 0x07060302u);  // byte column 3
+  }
 }
 
 // Orders the LDS accesses of one wave: the compiler moves no memory access
@@ -224,90 +270,76 @@ __device__ __forceinline__ uint32_t xtime4(uint32_t w) {
 // one lane stored is visible to every lane of the wave afterwards.
 __device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
-// Per-wave block scratch: row r (0..63) of the current half at r * 17 (16
-// product words; 17 * 16 = 16 mod 64, so the phase-2a column reads of the four
-// J-groups fall in distinct banks), later the phase-2a partial sums at
-// ypart[(q * 4 + j) * 64 + (lane ^ 16 j)] (the XOR keeps the phase-2b gathers of
-// the four j in distinct banks).
-constexpr uint32_t R1_RP = 17, R1_YP = 64;
-constexpr uint32_t R1_SCRATCH = 64 * R1_RP;  // words; >= 16 * R1_YP
-
-// Each wave takes blocks blockIdx.x * NW + wave, then strides by gridDim.x * NW;
-// each block follows the phases described at the top of the file.
+// One wave takes four blocks per pass (quads g = blockIdx.x * R1_NW + wave, then
+// strides by gridDim.x * R1_NW); lane (b, J) owns rows (K, J), K < 8, of block
+// 4g + b:
+//   phase 1, for K = 7 .. 0: the LDE of the row's a and b bits as bit planes
+//     (nibble tables, conflict-free), v = a * b on the planes (no lookups), and
+//     the Horner step u = alpha u + v, so u = sum_K alpha^K v_K without leaving
+//     the lane; then u to bytes, stored to the wave's exchange;
+//   phase 2, lane l, per block: W_l = sum_J x^J phi8(u_{J,l}) by two lookups per J
+//     in J's table, then acc_l += eq_outer[block] * W_l (comb4p_mul_add).
 // out: work-group partials, out[l * gridDim.x + blockIdx.x] for the 64 outputs l.
-template <bool FULL>
-__global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
-    const uint64_t* __restrict__ A, const uint64_t* __restrict__ B, uint64_t n_blocks,
-    const uint4* __restrict__ eq_outer, const Round1Tables* __restrict__ T, uint4* __restrict__ out) {
-  constexpr int NW = r1_waves<FULL>(), R1_WG = NW * 64;
-  static_assert(64 * R1_RP <= R1_SCRATCH, "a half block's rows must fit in the scratch");
-  static_assert(16 * R1_YP <= R1_SCRATCH, "a wave's sums must fit in its scratch");
-  // LDE contributions, chunk-major: lde16[(p * 4 + q) * 16 + v] is bytes 16q..16q+15
-  // of the 64-byte image of nibble value v at nibble position p.  For a fixed
+__global__ void __launch_bounds__(R1_NW * 64) zc_round1_kernel(const uint64_t* __restrict__ A,
+                                                              const uint64_t* __restrict__ B, uint64_t n_blocks,
+                                                              const uint4* __restrict__ eq_outer,
+                                                              const Round1Tables* __restrict__ T,
+                                                              uint4* __restrict__ out) {
+  constexpr int R1_WG = R1_NW * 64;
+  // LDE contributions, chunk-major: lde16[(p * 4 + q) * 16 + v] is words 4q..4q+3
+  // of the bit-plane image of nibble value v at nibble position p.  For a fixed
   // (p, q) the 16 values sit in 16 distinct LDS bank groups, so the data-indexed
   // lookups of a wave never conflict (16 KiB).
-  // One LDS block with the byte product tables first: their lookups then fit the
-  // 16-bit offset field of ds_read_u8 (no address add per lookup).
   struct alignas(16) Lds {
-    // FULL: mul[a * 256 + b] = a * b (one lookup per product); else
-    // mul[a * 16 + n] = a * n and mul[4096 + a * 16 + n] = a * (n << 4)
-    uint8_t mul[FULL ? 65536 : 8192];
     uint4 lde16[16 * 4 * 16];
-    // x^J * phi8(v) = xj[32 J + (v & 15)] ^ xj[32 J + 16 + (v >> 4)] (reduced): the
-    // 16 lanes of a quarter-wave share J (one J-group), so their lookups stay in
-    // one 16-entry table each
+    // x^J * phi8(v) = xj[32 J + (v & 15)] ^ xj[32 J + 16 + (v >> 4)] (reduced); J is
+    // wave-uniform in phase 2, so each lookup stays in one 16-entry table
     uint4 xj[16 * 32];
-    uint4 etab[NW][32];  // eq_outer * n and eq_outer * x^4 * n (comb4p_mul_add)
-    uint32_t scratch[NW][R1_SCRATCH];
+    uint4 etab[R1_NW][R1_QUAD][32];  // eq_outer * n and eq_outer * x^4 * n (comb4p_mul_add)
+    uint32_t xs[R1_NW][R1_QUAD * 16 * R1_XS];  // u bytes of (b, J) at (16 b + J) * R1_XS
   };
   __shared__ Lds L;
   uint4* const lde16 = L.lde16;
-  uint8_t* const mul_lo = L.mul;
-  uint8_t* const mul_hi = L.mul + 4096;
   uint4* const xj = L.xj;
   const uint32_t tid = threadIdx.x, wave = tid >> 6, t = tid & 63u;
-  uint32_t* const rows = L.scratch[wave];
-  uint32_t* const ypart = rows;
-  uint4* const etab = L.etab[wave];
+  const uint32_t b = t >> 4, J = t & 15u;
+  uint4(*const etab)[32] = L.etab[wave];
+  uint32_t* const xs = L.xs[wave];
   {
     const uint4* src = reinterpret_cast<const uint4*>(T->lde);  // [p][v][q]
     for (uint32_t i = tid; i < 16 * 4 * 16; i += R1_WG) {
       const uint32_t p = i >> 6, q = (i >> 4) & 3u, v = i & 15u;
       lde16[i] = src[(p * 16 + v) * 4 + q];
     }
-    if (FULL) {
-      const uint4* m64 = reinterpret_cast<const uint4*>(T->mul64);
-      for (uint32_t i = tid; i < 4096; i += R1_WG) reinterpret_cast<uint4*>(L.mul)[i] = m64[i];
-    } else {
-      const uint32_t* mlo = reinterpret_cast<const uint32_t*>(T->mul_lo);
-      const uint32_t* mhi = reinterpret_cast<const uint32_t*>(T->mul_hi);
-      for (uint32_t i = tid; i < 1024; i += R1_WG) {
-        reinterpret_cast<uint32_t*>(mul_lo)[i] = mlo[i];
-        reinterpret_cast<uint32_t*>(mul_hi)[i] = mhi[i];
-      }
-    }
     for (uint32_t i = tid; i < 16 * 32; i += R1_WG) xj[i] = T->xj[i];
   }
-  // phase-2a role: 16 words x 4 J-groups; the group jg takes J = 2jg, 2jg + 1 of
-  // each half (J = 8 * half + 2 * jg + jj)
-  const uint32_t w = t & 15u, jg = t >> 4;
   uint4 acc = f_zero();
+  const uint64_t n_quads = (n_blocks + R1_QUAD - 1) / R1_QUAD;
   __syncthreads();  // the tables are read-only from here; the waves run apart
 
-  for (uint64_t blk = (uint64_t)blockIdx.x * NW + wave; blk < n_blocks; blk += (uint64_t)gridDim.x * NW) {
-    // eq_outer[blk] nibble tables for comb4p_mul_add
-    if (t < 32) {
-      uint4 e = eq_outer[blk];
-      if (t >= 16) e = f_mul_x4(e);
-      etab[t] = comb_entry(e, t & 15u);
+  for (uint64_t g = (uint64_t)blockIdx.x * R1_NW + wave; g < n_quads; g += (uint64_t)gridDim.x * R1_NW) {
+    const uint64_t blk = R1_QUAD * g + b;
+    const bool live = blk < n_blocks;  // a quad past the last block (fewer than 4 blocks) adds zero
+    {
+      const uint4 e = live ? eq_outer[blk] : f_zero();
+      etab[b][J] = comb_entry(e, J);
+      etab[b][16 + J] = comb_entry(f_mul_x4(e), J);
     }
-    uint32_t wj[4];
+    // rows 8J + K, K = 7 .. 0, of the block (64 contiguous bytes of each bit
+    // vector), each loaded one step ahead
+    const uint64_t* const ap = A + blk * R1_ROWS + 8 * J;
+    const uint64_t* const bp = B + blk * R1_ROWS + 8 * J;
+    uint64_t a_nx = live ? ap[7] : 0ull, b_nx = live ? bp[7] : 0ull;
+    uint32_t u[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) u[q] = 0u;
 #pragma unroll 1
-    for (uint32_t half = 0; half < 2; ++half) {
-      // ---- phase 1: row t of the half, block row r = 64 * half + t = (K, J) =
-      // (r & 7, r >> 3) ----
-      const uint64_t row = blk * R1_ROWS + 64u * half + t;
-      const uint64_t a = A[row], b = B[row];
+    for (int K = 7; K >= 0; --K) {
+      const uint64_t a = a_nx, bb = b_nx;
+      if (live && K > 0) {
+        a_nx = ap[K - 1];
+        b_nx = bp[K - 1];
+      }
       uint32_t ah[16], bh[16];
 #pragma unroll
       for (int q = 0; q < 16; ++q) ah[q] = bh[q] = 0u;
@@ -315,7 +347,7 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
 #pragma unroll 1
       for (uint32_t p = 0; p < 16; p += 2) {
         const uint32_t na0 = (uint32_t)(a >> (4 * p)) & 15u, na1 = (uint32_t)(a >> (4 * p + 4)) & 15u;
-        const uint32_t nb0 = (uint32_t)(b >> (4 * p)) & 15u, nb1 = (uint32_t)(b >> (4 * p + 4)) & 15u;
+        const uint32_t nb0 = (uint32_t)(bb >> (4 * p)) & 15u, nb1 = (uint32_t)(bb >> (4 * p + 4)) & 15u;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const uint4* c0 = lde16 + (p * 4 + q) * 16;
@@ -331,89 +363,44 @@ __global__ void __launch_bounds__(r1_waves<FULL>() * 64) zc_round1_kernel(
           bh[4 * q + 3] = xor3(bh[4 * q + 3], y0.w, y1.w);
         }
       }
-      uint32_t* rd = rows + t * R1_RP;
+      // planes 8h .. 8h + 7 hold the points l = 32h .. 32h + 31
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        uint32_t v = 0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (FULL) {
-            // index (byte j of ah) * 256 + (byte j of bh) in one byte permute
-            const uint32_t idx = __builtin_amdgcn_perm(ah[q], bh[q], 0x0c0c0000u | ((4u + j) << 8) | (uint32_t)j);
-            v |= (uint32_t)L.mul[idx] << (8 * j);
-          } else {
-            const uint32_t x16 = ((ah[q] >> (8 * j)) & 255u) << 4, y = (bh[q] >> (8 * j)) & 255u;
-            v |= (uint32_t)(mul_lo[x16 | (y & 15u)] ^ mul_hi[x16 | (y >> 4)]) << (8 * j);
-          }
-        }
-        rd[q] = v;
-      }
-      wave_lds_sync();
-      // ---- phase 2a (this half): lane (w, jg): Horner over K in alpha = 0x02
-      // (xtime on packed bytes: 4 l per word) for J = 8 * half + 2 * jg + jj ----
-      uint32_t h[2];
-#pragma unroll
-      for (int jj = 0; jj < 2; ++jj) {
-        const uint32_t* col = rows + (8u * (2u * jg + jj)) * R1_RP + w;
-        h[jj] = 0;
-#pragma unroll
-        for (int K = 7; K >= 0; --K) h[jj] = xtime4(h[jj]) ^ col[R1_RP * K];
-      }
-      // constant indices on both sides of the (wave-uniform) branch keep wj in registers
-      if (half == 0) {
-        wj[0] = h[0];
-        wj[1] = h[1];
-      } else {
-        wj[2] = h[0];
-        wj[3] = h[1];
-      }
-      wave_lds_sync();  // every lane has read the half's rows; the next writes reuse them
-    }
-    // Y[j] = sum over the lane's four J of x^J phi8(byte j of wj[i]), reduced
-    uint4 Y[4] = {f_zero(), f_zero(), f_zero(), f_zero()};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const uint4* xt = xj + 32 * (8 * (i >> 1) + 2 * jg + (i & 1));
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t byte = (wj[i] >> (8 * j)) & 255u;
-        const uint4 f0 = xt[byte & 15u], f1 = xt[16 + (byte >> 4)];
-        Y[j] = make_uint4(xor3(Y[j].x, f0.x, f1.x), xor3(Y[j].y, f0.y, f1.y), xor3(Y[j].z, f0.z, f1.z),
-                          xor3(Y[j].w, f0.w, f1.w));
+      for (int h = 0; h < 2; ++h) {
+        uint32_t v[8];
+        gf8_mul_planes(ah + 8 * h, bh + 8 * h, v);
+        gf8_horner_planes(u + 8 * h, v);
       }
     }
+    uint32_t ub[16];
+    planes_to_bytes(u, ub);
+    planes_to_bytes(u + 8, ub + 8);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      ypart[(0 * 4 + j) * R1_YP + (t ^ (16u * j))] = Y[j].x;
-      ypart[(1 * 4 + j) * R1_YP + (t ^ (16u * j))] = Y[j].y;
-      ypart[(2 * 4 + j) * R1_YP + (t ^ (16u * j))] = Y[j].z;
-      ypart[(3 * 4 + j) * R1_YP + (t ^ (16u * j))] = Y[j].w;
-    }
+    for (int q = 0; q < 16; ++q) xs[(16 * b + J) * R1_XS + q] = ub[q];
     wave_lds_sync();
-    // ---- phase 2b: lane l: finish the J sum over the four J-groups, multiply by
-    // eq_outer ----
-    {
-      const uint32_t l = t, ww = l >> 2, j = l & 3u;
-      uint32_t y[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        // writer lane of J-group g: ww + 16 g, stored at (ww + 16 g) ^ 16 j
-        const uint32_t* yp = ypart + (q * 4 + j) * R1_YP;
-        y[q] = xor3(yp[ww ^ (16u * j)], yp[(ww + 16u) ^ (16u * j)], yp[(ww + 32u) ^ (16u * j)]) ^
-               yp[(ww + 48u) ^ (16u * j)];
+    // ---- phase 2: lane l ----
+    const uint32_t l = t, lw = l >> 2, lsh = 8 * (l & 3u);
+#pragma unroll 1
+    for (uint32_t b2 = 0; b2 < R1_QUAD; ++b2) {
+      uint4 W = f_zero();
+#pragma unroll 4
+      for (uint32_t j2 = 0; j2 < 16; ++j2) {
+        const uint32_t byte = (xs[(16 * b2 + j2) * R1_XS + lw] >> lsh) & 255u;
+        const uint4* xt = xj + 32 * j2;
+        const uint4 f0 = xt[byte & 15u], f1 = xt[16 + (byte >> 4)];
+        W = make_uint4(xor3(W.x, f0.x, f1.x), xor3(W.y, f0.y, f1.y), xor3(W.z, f0.z, f1.z), xor3(W.w, f0.w, f1.w));
       }
-      acc = comb4p_mul_add(make_uint4(y[0], y[1], y[2], y[3]), acc, etab);
+      acc = comb4p_mul_add(W, acc, etab[b2]);
     }
-    wave_lds_sync();  // the next block overwrites the scratch and etab
+    wave_lds_sync();  // the next pass overwrites the exchange and etab
   }
   // Sum the waves' accumulators for output l = t, then store the work-group's
   // partial at out[l * gridDim.x + blockIdx.x].
   __syncthreads();
-  reinterpret_cast<uint4*>(rows)[t] = acc;
+  reinterpret_cast<uint4*>(L.xs[wave])[t] = acc;
   __syncthreads();
   if (wave == 0) {
 #pragma unroll
-    for (int k = 1; k < NW; ++k) acc = f_add(acc, reinterpret_cast<const uint4*>(L.scratch[k])[t]);
+    for (int k = 1; k < R1_NW; ++k) acc = f_add(acc, reinterpret_cast<const uint4*>(L.xs[k])[t]);
     out[(size_t)t * gridDim.x + blockIdx.x] = acc;
   }
 }
@@ -485,8 +472,8 @@ int flk::zc_round1_keep_s(const uint8_t* a_bits, const uint8_t* b_bits, const ui
   if (!a_bits || !b_bits || !c_bits || !out_ab || !out_c || (m > 13 && !r_outer))
     return set_error(FLK_E_SHAPE, "zc_round1: null pointer");
   if (m < 13 || m > 40) return set_error(FLK_E_SHAPE, "zc_round1: m=%zu out of range (13..40)", m);
-  if (((uintptr_t)a_bits | (uintptr_t)b_bits) % 8 || (uintptr_t)c_bits % 16)
-    return set_error(FLK_E_SHAPE, "zc_round1: a_bits and b_bits must be 8-byte aligned, c_bits 16-byte aligned");
+  if (((uintptr_t)a_bits | (uintptr_t)b_bits | (uintptr_t)c_bits) % 16)
+    return set_error(FLK_E_SHAPE, "zc_round1: a_bits, b_bits and c_bits must be 16-byte aligned");
   if (out_c != out_ab + 64) return set_error(FLK_E_SHAPE, "zc_round1: out_c must follow out_ab (one 128-entry buffer)");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const Round1Tables* T = round1_tables_device(&rc);
@@ -531,28 +518,15 @@ int flk::zc_round1_keep_s(const uint8_t* a_bits, const uint8_t* b_bits, const ui
   uint4* out = reinterpret_cast<uint4*>(out_ab);
   hipLaunchKernelGGL(zc_round1_lift_c_kernel, dim3(1), dim3(256), 0, st, S, u4_of(fixed[0]), T, out + 64);
   FLK_LAUNCH_CHECK("zc_round1_lift_c_kernel");
-  // The full 64 KiB product table (one work-group of 12 waves per CU) unless
-  // FLOCK_R1_FULLTAB=0 selects the split-nibble tables (two work-groups of 8
-  // waves per CU).  Grids of 512 (full) or 1024 (split) work-groups: several
-  // blocks per wave, so the waves' uneven block counts even out.
-  static const bool full = [] {
-    const char* v = getenv("FLOCK_R1_FULLTAB");
-    return !(v && v[0] == '0');
-  }();
-  const int nw = full ? r1_waves<true>() : r1_waves<false>();
-  const uint64_t n_iter = (n_outer + nw - 1) / nw;
-  const uint64_t cap = full ? 512 : 1024;
-  const unsigned grid = (unsigned)(n_iter < cap ? n_iter : cap);
+  // Quads of blocks, one per wave pass; a grid of at most 512 work-groups (two
+  // per CU) gives every wave several quads.
+  const uint64_t n_quads = (n_outer + R1_QUAD - 1) / R1_QUAD;
+  const uint64_t n_iter = (n_quads + R1_NW - 1) / R1_NW;
+  const unsigned grid = (unsigned)(n_iter < 512 ? n_iter : 512);
   uint4* parts = nullptr;
   if ((rc = partials_alloc(&parts, grid, 64, out, st))) return fail(rc);
-  if (full)
-    hipLaunchKernelGGL(zc_round1_kernel<true>, dim3(grid), dim3(nw * 64), 0, st,
-                       reinterpret_cast<const uint64_t*>(a_bits), reinterpret_cast<const uint64_t*>(b_bits),
-                       (uint64_t)n_outer, eq, T, parts);
-  else
-    hipLaunchKernelGGL(zc_round1_kernel<false>, dim3(grid), dim3(nw * 64), 0, st,
-                       reinterpret_cast<const uint64_t*>(a_bits), reinterpret_cast<const uint64_t*>(b_bits),
-                       (uint64_t)n_outer, eq, T, parts);
+  hipLaunchKernelGGL(zc_round1_kernel, dim3(grid), dim3(R1_NW * 64), 0, st, reinterpret_cast<const uint64_t*>(a_bits),
+                     reinterpret_cast<const uint64_t*>(b_bits), (uint64_t)n_outer, eq, T, parts);
   FLK_LAUNCH_CHECK("zc_round1_kernel");
   if ((rc = partials_sum_free(parts, grid, 64, out, st))) return fail(rc);
   const flk_f128 sc = round1_scale();

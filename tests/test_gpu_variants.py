@@ -66,17 +66,3 @@ def test_witness_variant_matches_oracle(dev, env):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "MISMATCH" not in r.stdout
 
-
-# Zerocheck round 1's GF(2^8) products: split-nibble tables (two work-groups per CU)
-# or the full 64 KiB product table (one work-group per CU).
-ROUND1_VARIANTS = [{"FLOCK_R1_FULLTAB": v} for v in ("0", "1")]
-
-
-@pytest.mark.parametrize("env", ROUND1_VARIANTS, ids=lambda e: ",".join(f"{k}={v}" for k, v in e.items()))
-def test_round1_variant_matches_oracle(dev, env):
-    print(f"variant {env}", flush=True)
-    r = subprocess.run([sys.executable, CHECK, "round1"], env={**os.environ, **env}, capture_output=True,
-                       text=True, timeout=100)
-    print(r.stdout, r.stderr, flush=True)
-    assert r.returncode == 0, r.stdout + r.stderr
-    assert "MISMATCH" not in r.stdout
