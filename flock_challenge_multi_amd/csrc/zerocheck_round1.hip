@@ -210,15 +210,18 @@ __device__ __forceinline__ void gf8_mul_planes(const uint32_t* a, const uint32_t
 #pragma unroll
     for (int i = i0 + 1; i <= i1; ++i) t[k] = __builtin_amdgcn_bitop3_b32(a[i], b[k - i], t[k], 0x6A);
   }
-#pragma unroll
-  for (int k = 14; k >= 8; --k) {
-    t[k - 8] ^= t[k];
-    t[k - 7] ^= t[k];
-    t[k - 5] ^= t[k];
-    t[k - 4] ^= t[k];
-  }
-#pragma unroll
-  for (int p = 0; p < 8; ++p) c[p] = t[p];
+  // t12..t14 fold into t8..t10 first (x^12 = x^8 + x^7 + x^5 + x^4, ...), then
+  // t8..t14 into t0..t7, with three-input XORs
+  const uint32_t t8 = xor3(t[8], t[12], t[13]), t9 = xor3(t[9], t[13], t[14]), t10 = t[10] ^ t[14];
+  const uint32_t t11 = t[11], t12 = t[12], t13 = t[13], t14 = t[14];
+  c[0] = t[0] ^ t8;
+  c[1] = xor3(t[1], t8, t9);
+  c[2] = xor3(t[2], t9, t10);
+  c[3] = xor3(t[3], t8, t10) ^ t11;
+  c[4] = xor3(xor3(t[4], t8, t9), t11, t12);
+  c[5] = xor3(xor3(t[5], t9, t10), t12, t13);
+  c[6] = xor3(xor3(t[6], t10, t11), t13, t14);
+  c[7] = xor3(t[7], t11, t12) ^ t14;
 }
 
 // u <- x * u + v on bit planes (x = alpha = 0x02): the planes move up one, plane
@@ -262,6 +265,47 @@ __device__ __forceinline__ void planes_to_bytes(uint32_t* P, uint32_t* w) {
     w[4 + g] = __builtin_amdgcn_perm(x23, x01, 0x07060302u);  // byte column 2
     w[2 + g] = __builtin_amdgcn_perm(y23, y01, 0x05040100u);  // byte column 1
     w[6 + g] = __builtin_amdgcn_perm(y23, y01, 0x07060302u);  // byte column 3
+  }
+}
+
+// The bit-plane LDE images of the 64-bit rows a and b into ah[16], bh[16]: the
+// XOR over the 16 nibble positions P of the table entries (64 bytes each, four
+// 16-byte quarters q).  A table address is nibble * 16 plus an immediate: one
+// SDWA byte-select AND per nibble (the low nibbles from the row shifted left by
+// 4 once).  Unrolled over P; the loads of a position pair are issued together.
+__device__ __forceinline__ void lde_planes(uint64_t a, uint64_t b, const uint4* __restrict__ lde16, uint32_t* ah,
+                                           uint32_t* bh) {
+#pragma unroll
+  for (int q = 0; q < 16; ++q) ah[q] = bh[q] = 0u;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {  // nibbles 8h .. 8h + 7: word h of the rows
+    const uint32_t aw = (uint32_t)(a >> (32 * h)), bw = (uint32_t)(b >> (32 * h));
+    const uint32_t aw4 = aw << 4, bw4 = bw << 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {  // byte j: nibble positions P = 8h + 2j (low), P + 1 (high)
+      uint32_t oa0, oa1, ob0, ob1;
+      switch (j) {
+        case 0: oa0 = hinib_x16<0>(aw4); oa1 = hinib_x16<0>(aw); ob0 = hinib_x16<0>(bw4); ob1 = hinib_x16<0>(bw); break;
+        case 1: oa0 = hinib_x16<1>(aw4); oa1 = hinib_x16<1>(aw); ob0 = hinib_x16<1>(bw4); ob1 = hinib_x16<1>(bw); break;
+        case 2: oa0 = hinib_x16<2>(aw4); oa1 = hinib_x16<2>(aw); ob0 = hinib_x16<2>(bw4); ob1 = hinib_x16<2>(bw); break;
+        default: oa0 = hinib_x16<3>(aw4); oa1 = hinib_x16<3>(aw); ob0 = hinib_x16<3>(bw4); ob1 = hinib_x16<3>(bw); break;
+      }
+      const int P = 8 * h + 2 * j;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint4* c0 = lde16 + (P * 4 + q) * 16;
+        const uint4* c1 = c0 + 64;
+        const uint4 x0 = lds_entry(c0, oa0), x1 = lds_entry(c1, oa1), y0 = lds_entry(c0, ob0), y1 = lds_entry(c1, ob1);
+        ah[4 * q] = xor3(ah[4 * q], x0.x, x1.x);
+        ah[4 * q + 1] = xor3(ah[4 * q + 1], x0.y, x1.y);
+        ah[4 * q + 2] = xor3(ah[4 * q + 2], x0.z, x1.z);
+        ah[4 * q + 3] = xor3(ah[4 * q + 3], x0.w, x1.w);
+        bh[4 * q] = xor3(bh[4 * q], y0.x, y1.x);
+        bh[4 * q + 1] = xor3(bh[4 * q + 1], y0.y, y1.y);
+        bh[4 * q + 2] = xor3(bh[4 * q + 2], y0.z, y1.z);
+        bh[4 * q + 3] = xor3(bh[4 * q + 3], y0.w, y1.w);
+      }
+    }
   }
 }
 
@@ -341,28 +385,7 @@ __global__ void __launch_bounds__(R1_NW * 64) zc_round1_kernel(const uint64_t* _
         b_nx = bp[K - 1];
       }
       uint32_t ah[16], bh[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) ah[q] = bh[q] = 0u;
-      // Rolled over nibble-position pairs: at most 16 table loads in flight.
-#pragma unroll 1
-      for (uint32_t p = 0; p < 16; p += 2) {
-        const uint32_t na0 = (uint32_t)(a >> (4 * p)) & 15u, na1 = (uint32_t)(a >> (4 * p + 4)) & 15u;
-        const uint32_t nb0 = (uint32_t)(bb >> (4 * p)) & 15u, nb1 = (uint32_t)(bb >> (4 * p + 4)) & 15u;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const uint4* c0 = lde16 + (p * 4 + q) * 16;
-          const uint4* c1 = c0 + 64;
-          const uint4 x0 = c0[na0], x1 = c1[na1], y0 = c0[nb0], y1 = c1[nb1];
-          ah[4 * q] = xor3(ah[4 * q], x0.x, x1.x);
-          ah[4 * q + 1] = xor3(ah[4 * q + 1], x0.y, x1.y);
-          ah[4 * q + 2] = xor3(ah[4 * q + 2], x0.z, x1.z);
-          ah[4 * q + 3] = xor3(ah[4 * q + 3], x0.w, x1.w);
-          bh[4 * q] = xor3(bh[4 * q], y0.x, y1.x);
-          bh[4 * q + 1] = xor3(bh[4 * q + 1], y0.y, y1.y);
-          bh[4 * q + 2] = xor3(bh[4 * q + 2], y0.z, y1.z);
-          bh[4 * q + 3] = xor3(bh[4 * q + 3], y0.w, y1.w);
-        }
-      }
+      lde_planes(a, bb, lde16, ah, bh);
       // planes 8h .. 8h + 7 hold the points l = 32h .. 32h + 31
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -518,11 +541,11 @@ int flk::zc_round1_keep_s(const uint8_t* a_bits, const uint8_t* b_bits, const ui
   uint4* out = reinterpret_cast<uint4*>(out_ab);
   hipLaunchKernelGGL(zc_round1_lift_c_kernel, dim3(1), dim3(256), 0, st, S, u4_of(fixed[0]), T, out + 64);
   FLK_LAUNCH_CHECK("zc_round1_lift_c_kernel");
-  // Quads of blocks, one per wave pass; a grid of at most 512 work-groups (two
-  // per CU) gives every wave several quads.
+  // Quads of blocks, one per wave pass; at most 1024 work-groups (two resident
+  // per CU, four rounds of them), so every wave takes several quads.
   const uint64_t n_quads = (n_outer + R1_QUAD - 1) / R1_QUAD;
   const uint64_t n_iter = (n_quads + R1_NW - 1) / R1_NW;
-  const unsigned grid = (unsigned)(n_iter < 512 ? n_iter : 512);
+  const unsigned grid = (unsigned)(n_iter < 1024 ? n_iter : 1024);
   uint4* parts = nullptr;
   if ((rc = partials_alloc(&parts, grid, 64, out, st))) return fail(rc);
   hipLaunchKernelGGL(zc_round1_kernel, dim3(grid), dim3(R1_NW * 64), 0, st, reinterpret_cast<const uint64_t*>(a_bits),
