@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 # kernel, with their eq tables) from two rocprofv3 --pmc passes (FETCH_SIZE,
 # WRITE_SIZE) scaled by a 1 GiB device copy: read = 2.00 x FETCH_SIZE, write =
 # 1.00 x WRITE_SIZE (profiles/proof_m32.md)
-ZC_ROUND1_TRAFFIC_M32 = 1_655_464_456
+ZC_ROUND1_TRAFFIC_M32 = 1_652_789_616
 
 
 def parse():

@@ -709,10 +709,11 @@ __global__ void __launch_bounds__(ZF_WG) __attribute__((amdgpu_waves_per_eu(MODE
       a1 = combjr_mul_add(ta1, f_zero(), htab);
       as = combjr_mul_add(f_add(a0, ta1), f_zero(), htab);
     }
+    // four comb steps unrolled: more table loads in flight at two waves per SIMD
     table3_store_private(tabs, ZF_WG, tid, a1);
-    comb3_mul_acc_rt(b1, my, ZF_WG, g1);
+    comb3_mul_acc_rt<4>(b1, my, ZF_WG, g1);
     table3_store_private(tabs, ZF_WG, tid, as);
-    comb3_mul_acc_rt(bs, my, ZF_WG, gi);
+    comb3_mul_acc_rt<4>(bs, my, ZF_WG, gi);
   }
   table3_store_private(tabs, ZF_WG, tid, e_lo[tid]);  // the lane's own table: no barrier needed
   const uint4 r1 = comb3_mul_rt(f_reduce(g1), my, ZF_WG);
