@@ -130,6 +130,8 @@ SIGNATURES = {
     "flk_blake3_proof_bound": (_I, [_P, ctypes.POINTER(_SZ)]),
     "flk_inject_device_failures": (None, [ctypes.c_uint, ctypes.c_uint]),
     "flk_read_to_host": (_I, [_P, _P, _SZ, _P]),
+    "flk_read_arm": (_I, [_P, _SZ]),
+    "flk_read_armed": (_I, [_P, _SZ, _P]),
     "flk_zc_fold_round": (_I, [_P, _P, _SZ, _P, _P, _P, _SZ, _P, _P, _P, _P, _P]),
     "flk_zc_fold_at_z_round": (_I, [_P, _P, _SZ, _P, _P, _P, _P, _P, _P, _P, _P]),
     "flk_zc_fold_at_z_bind_round": (_I, [_P, _P, _SZ, _P, _P, _P, _P, _P, _P, _P, _P, _P]),

@@ -70,6 +70,22 @@ int partials_sum_free(uint4* parts, unsigned grid, int n_out, uint4* out, hipStr
 // parts (caller-owned scratch).
 int partials_sum(const uint4* parts, unsigned grid, int n_out, uint4* out, hipStream_t st);
 
+// Reductions whose result the calling thread reads back at once.  reduce_begin
+// takes the partials (partials_alloc) and, when the calling thread armed a
+// read-back of the n_out results (flk_read_arm), its mailbox; reduce_end sums
+// the partials on one launch that also stores the results into the mailbox and
+// sets its sequence word, so no publish kernel follows (without an arm it is
+// partials_sum_free).
+struct ReduceTail {
+  uint4* parts;   // partials, parts[k * grid + blockIdx.x]
+  uint4* out;     // the n_out results (device)
+  uint4* box;     // mapped host memory for the results; NULL: no arm taken
+  uint32_t* flag; // the mailbox's sequence word
+  uint32_t seq;
+};
+int reduce_begin(unsigned grid, int n_out, uint4* out, hipStream_t st, ReduceTail* rt);
+int reduce_end(const ReduceTail& rt, unsigned grid, int n_out, hipStream_t st);
+
 // fold_1b_rows against the split eq tensor eq_lo (2^lo_log) x eq_hi (2^hi_log),
 // lo_log >= 8: out128 (device) = sum_i bit_r(z_i) eq_lo[i mod 2^lo_log] eq_hi[i >> lo_log].
 int rs_fold_1b_rows_split(const flk_f128* z, const flk_f128* eq_lo, size_t lo_log, const flk_f128* eq_hi,

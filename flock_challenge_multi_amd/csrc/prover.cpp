@@ -165,27 +165,35 @@ void zerocheck_prove(Fs& ch, const uint8_t* a_bits, const uint8_t* b_bits, const
       DeviceVec<F128>& hi = eq_hi[i & 1];
       DeviceVec<F128>& hi_next = eq_hi[(i + 1) & 1];
       const F128* lo = eq_lo.data() + 256 * i;  // eq over pt[0 .. 8)
+      // each round's message comes back through an armed read-back: the round
+      // kernel's last work-group stores it into host memory
+      std::vector<F128> h;
       if (i == 0) {  // the fold at z and the first message in one pass over the bits
-        chk(flk_zc_fold_at_z_round(a_bits, b_bits, m, wd.data(), lo, hi.data(), from_bits ? nullptr : am.data(),
-                                   from_bits ? nullptr : bm.data(), hi_next.data(), msg.data(), st),
-            "zc_fold_at_z_round");
+        h = flock::read_reduction(msg, 2, st, [&] {
+          chk(flk_zc_fold_at_z_round(a_bits, b_bits, m, wd.data(), lo, hi.data(), from_bits ? nullptr : am.data(),
+                                     from_bits ? nullptr : bm.data(), hi_next.data(), msg.data(), st),
+              "zc_fold_at_z_round");
+        });
       } else if (i == 1 && from_bits) {  // the fold at z bound by rho_0, and the second message
         am = DeviceVec<F128>(n / 2, st);
         bm = DeviceVec<F128>(n / 2, st);
-        chk(flk_zc_fold_at_z_bind_round(a_bits, b_bits, m, wd.data(), &rhos.back(), lo, hi.data(), am.data(),
-                                        bm.data(), hi_next.data(), msg.data(), st),
-            "zc_fold_at_z_bind_round");
+        h = flock::read_reduction(msg, 2, st, [&] {
+          chk(flk_zc_fold_at_z_bind_round(a_bits, b_bits, m, wd.data(), &rhos.back(), lo, hi.data(), am.data(),
+                                          bm.data(), hi_next.data(), msg.data(), st),
+              "zc_fold_at_z_bind_round");
+        });
         n /= 2;
       } else {
         DeviceVec<F128> an(n / 2, st), bn(n / 2, st);
-        chk(flk_zc_fold_round(am.data(), bm.data(), n, &rhos.back(), lo, hi.data(), d - 8, an.data(),
-                              bn.data(), hi_next.data(), msg.data(), st),
-            "zc_fold_round");
+        h = flock::read_reduction(msg, 2, st, [&] {
+          chk(flk_zc_fold_round(am.data(), bm.data(), n, &rhos.back(), lo, hi.data(), d - 8, an.data(), bn.data(),
+                                hi_next.data(), msg.data(), st),
+              "zc_fold_round");
+        });
         am = std::move(an);
         bm = std::move(bn);
         n /= 2;
       }
-      const std::vector<F128> h = msg.to_host();
       g = {h[0], h[1]};
     } else {
       if (on_device) {  // the tables move to the host once they are small
@@ -264,8 +272,8 @@ void lincheck_prove(Fs& ch, const flk_r1cs* r1cs, const uint8_t* z_stripe, size_
   size_t n = k;
   std::vector<F128> rr;
   for (size_t t = 0; t < rest; ++t) {
-    chk(flk_lc_round_msg_msb(comb.data(), zv.data(), n, msg.data(), st), "lc_round_msg_msb");
-    const std::vector<F128> q = msg.to_host();
+    const std::vector<F128> q = flock::read_reduction(
+        msg, 2, st, [&] { chk(flk_lc_round_msg_msb(comb.data(), zv.data(), n, msg.data(), st), "lc_round_msg_msb"); });
     ch.f128(q[0]);
     ch.f128(q[1]);
     const F128 ri = ch.sample();

@@ -177,7 +177,7 @@ static unsigned rd_grid_cap() {
 
 __global__ void __launch_bounds__(RD_WG) lc_round_msg_msb_kernel(const uint4* __restrict__ comb,
                                                                 const uint4* __restrict__ z, uint64_t half,
-                                                                uint4* __restrict__ out) {
+                                                                ReduceTail rt) {
   __shared__ uint4 tabs[8 * RD_WG];
   __shared__ uint4 red[2][RD_WG / 64];
   U256 e1, einf;
@@ -192,8 +192,8 @@ __global__ void __launch_bounds__(RD_WG) lc_round_msg_msb_kernel(const uint4* __
     table3_store_private(tabs, RD_WG, threadIdx.x, f_add(hi, lo));
     comb3_mul_acc_rt(f_add(zh, zl), my, RD_WG, einf);
   }
-  wg_reduce_store(f_reduce(e1), red[0], out, 0);
-  wg_reduce_store(f_reduce(einf), red[1], out, 1);
+  wg_reduce_store(f_reduce(e1), red[0], rt.parts, 0);
+  wg_reduce_store(f_reduce(einf), red[1], rt.parts, 1);
 }
 
 __global__ void __launch_bounds__(256) lc_bind_top_kernel(uint4* __restrict__ v, uint64_t half, uint4 r) {
@@ -486,7 +486,7 @@ __global__ void __launch_bounds__(ZF_WG) __attribute__((amdgpu_waves_per_eu(4)))
                                                              const uint4* __restrict__ e_lo,
                                                              const uint4* __restrict__ e_hi, uint4* __restrict__ a_out,
                                                              uint4* __restrict__ b_out, uint4* __restrict__ e_hi_next,
-                                                             uint4* __restrict__ out) {
+                                                             ReduceTail rt) {
   __shared__ uint4 tabs[8 * ZF_WG];  // per-lane tables of the data x data products (3-bit comb)
   __shared__ uint4 rtab[128], rnib[32];
   __shared__ uint4 htab[128], hnib[32];
@@ -569,8 +569,8 @@ __global__ void __launch_bounds__(ZF_WG) __attribute__((amdgpu_waves_per_eu(4)))
   table3_store_private(tabs, ZF_WG, tid, e_lo[tid]);  // the lane's own table: no barrier needed
   const uint4 r1 = comb3_mul_rt(f_reduce(g1), my, ZF_WG);
   const uint4 ri = comb3_mul_rt(f_reduce(gi), my, ZF_WG);
-  wg_reduce_store(r1, red[0], out, 0);
-  wg_reduce_store(ri, red[1], out, 1);
+  wg_reduce_store(r1, red[0], rt.parts, 0);
+  wg_reduce_store(ri, red[1], rt.parts, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -638,7 +638,7 @@ __global__ void __launch_bounds__(ZF_WG) __attribute__((amdgpu_waves_per_eu(MODE
                                                              const uint4* __restrict__ e_lo,
                                                              const uint4* __restrict__ e_hi, uint4* __restrict__ a_out,
                                                              uint4* __restrict__ b_out, uint4* __restrict__ e_hi_next,
-                                                             uint4* __restrict__ out) {
+                                                             ReduceTail rt) {
   __shared__ uint4 tabs[8 * ZF_WG];  // per-lane tables of the data x data products (3-bit comb)
   __shared__ uint4 wnib[256];         // N: nibble tables of w
   __shared__ uint4 xnib[256];         // ZB_MSG: of eq_hi[c] * w (per chunk); ZB_FOLD: of rho * w (N_rho)
@@ -718,8 +718,8 @@ __global__ void __launch_bounds__(ZF_WG) __attribute__((amdgpu_waves_per_eu(MODE
   table3_store_private(tabs, ZF_WG, tid, e_lo[tid]);  // the lane's own table: no barrier needed
   const uint4 r1 = comb3_mul_rt(f_reduce(g1), my, ZF_WG);
   const uint4 ri = comb3_mul_rt(f_reduce(gi), my, ZF_WG);
-  wg_reduce_store(r1, red[0], out, 0);
-  wg_reduce_store(ri, red[1], out, 1);
+  wg_reduce_store(r1, red[0], rt.parts, 0);
+  wg_reduce_store(ri, red[1], rt.parts, 1);
 }
 
 static unsigned grid_for(uint64_t items, int wg, unsigned cap) {
@@ -790,12 +790,12 @@ int flk_lc_round_msg_msb(const flk_f128* comb, const flk_f128* z, size_t n, flk_
   hipStream_t st = static_cast<hipStream_t>(stream);
   uint4* o = reinterpret_cast<uint4*>(out2);
   const unsigned grid = grid_for(n / 2, RD_WG, RD_GRID_CAP);
-  uint4* parts = nullptr;
-  if ((rc = partials_alloc(&parts, grid, 2, o, st))) return rc;
+  ReduceTail rt;
+  if ((rc = reduce_begin(grid, 2, o, st, &rt))) return rc;
   hipLaunchKernelGGL(lc_round_msg_msb_kernel, dim3(grid), dim3(RD_WG), 0, st, reinterpret_cast<const uint4*>(comb),
-                     reinterpret_cast<const uint4*>(z), (uint64_t)(n / 2), parts);
+                     reinterpret_cast<const uint4*>(z), (uint64_t)(n / 2), rt);
   FLK_LAUNCH_CHECK("lc_round_msg_msb_kernel");
-  return partials_sum_free(parts, grid, 2, o, st);
+  return reduce_end(rt, grid, 2, st);
 }
 
 int flk_lc_bind_top(flk_f128* v, size_t n, flk_f128 r, void* stream) {
@@ -863,8 +863,8 @@ static int zc_fold_round_launch(int mode, const void* a, const void* b, uint64_t
   uint4* o = reinterpret_cast<uint4*>(out2);
   const uint64_t n_chunks = uint64_t(1) << hi_log;
   const unsigned grid = (unsigned)std::min<uint64_t>(n_chunks, 2048);
-  uint4* parts = nullptr;
-  int rc = partials_alloc(&parts, grid, 2, o, st);
+  ReduceTail rt;
+  int rc = reduce_begin(grid, 2, o, st, &rt);
   if (rc) return rc;
   const uint4 r = rho ? u4(*rho) : f_zero_host();
   auto* ka = reinterpret_cast<const uint4*>(a);
@@ -877,15 +877,15 @@ static int zc_fold_round_launch(int mode, const void* a, const void* b, uint64_t
   auto* hn = hi_log ? reinterpret_cast<uint4*>(eq_hi_next) : nullptr;
   if (mode == ZF_FOLD)
     hipLaunchKernelGGL(zc_fold_round_kernel<ZF_FOLD>, dim3(grid), dim3(ZF_WG), 0, st, ka, kb, n_chunks, r, kw, kl, kh,
-                       ao, bo, hn, parts);
+                       ao, bo, hn, rt);
   else if (mode == ZF_BITS)
     hipLaunchKernelGGL(zc_fold_round_kernel<ZF_BITS>, dim3(grid), dim3(ZF_WG), 0, st, ka, kb, n_chunks, r, kw, kl, kh,
-                       ao, bo, hn, parts);
+                       ao, bo, hn, rt);
   else
     hipLaunchKernelGGL(zc_fold_round_kernel<ZF_PLAIN>, dim3(grid), dim3(ZF_WG), 0, st, ka, kb, n_chunks, r, kw, kl,
-                       kh, ao, bo, hn, parts);
+                       kh, ao, bo, hn, rt);
   FLK_LAUNCH_CHECK("zc_fold_round_kernel");
-  return partials_sum_free(parts, grid, 2, o, st);
+  return reduce_end(rt, grid, 2, st);
 }
 
 // The two bits rounds (zc_bits_round_kernel): ZB_MSG takes the eq_hi-scaled
@@ -915,8 +915,8 @@ static int zc_bits_round_launch(int mode, const uint8_t* a_bits, const uint8_t* 
       return device_failure(le, "zc_scaled_weights_kernel");
     }
   }
-  uint4* parts = nullptr;
-  int rc = partials_alloc(&parts, grid, 2, o, st);
+  ReduceTail rt;
+  int rc = reduce_begin(grid, 2, o, st, &rt);
   if (rc) {
     if (ew) (void)stream_free(ew, st);
     return rc;
@@ -930,16 +930,16 @@ static int zc_bits_round_launch(int mode, const uint8_t* a_bits, const uint8_t* 
   auto* hn = hi_log ? reinterpret_cast<uint4*>(eq_hi_next) : nullptr;
   if (mode == ZB_MSG)
     hipLaunchKernelGGL(zc_bits_round_kernel<ZB_MSG>, dim3(grid), dim3(ZF_WG), 0, st, ka, kb, n_chunks, r, kw, ew, kl,
-                       kh, ao, bo, hn, parts);
+                       kh, ao, bo, hn, rt);
   else
     hipLaunchKernelGGL(zc_bits_round_kernel<ZB_FOLD>, dim3(grid), dim3(ZF_WG), 0, st, ka, kb, n_chunks, r, kw, ew, kl,
-                       kh, ao, bo, hn, parts);
+                       kh, ao, bo, hn, rt);
   hipError_t le = hipGetLastError();
   if (le != hipSuccess) {
     if (ew) (void)stream_free(ew, st);
     return device_failure(le, "zc_bits_round_kernel");
   }
-  rc = partials_sum_free(parts, grid, 2, o, st);
+  rc = reduce_end(rt, grid, 2, st);
   if (ew) {
     hipError_t fe = stream_free(ew, st);
     if (!rc && fe != hipSuccess) rc = device_failure(fe, "zc_fold_at_z_round: scratch free");

@@ -518,9 +518,11 @@ __global__ void __launch_bounds__(256) publish_kernel(const uint8_t* __restrict_
   } else {
     for (uint32_t i = threadIdx.x; i < bytes; i += blockDim.x) box[i] = src[i];
   }
-  __threadfence_system();  // every thread's stores reach host memory before the flag
+  // every thread's stores, ordered before thread 0's fence by the barrier, reach
+  // host memory before the flag (one system-scope fence, not one per thread)
   __syncthreads();
   if (threadIdx.x == 0) {
+    __threadfence_system();
     __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
@@ -627,38 +629,42 @@ static hipError_t read_bounce(void* dst, const void* src, size_t bytes, hipStrea
   return e;
 }
 
-hipError_t read_to_host(void* dst, const void* src, size_t bytes, hipStream_t st) {
-  if (bytes == 0) return hipSuccess;
-  static thread_local Mailbox mb;
+// The calling thread's mailbox on the current device, made on first use (one
+// per thread, for its first device; NULL when unavailable: a thread that moved
+// to another device, or no mapped pinned memory).
+static thread_local Mailbox t_mb;
+static Mailbox* mailbox_current() {
   int dev = 0;
-  if (bytes > MAILBOX_BYTES || hipGetDevice(&dev) != hipSuccess) return read_bounce(dst, src, bytes, st);
-  if (mb.device != dev && !mb.failed) {
-    // One mailbox per thread, for its first device (a thread that moves to
-    // another device falls back to the bounce buffer).
-    if (mb.device == -1) {
-      void* h = nullptr;
-      void* d = nullptr;
-      if (hipHostMalloc(&h, MAILBOX_BYTES + 256, hipHostMallocCoherent | hipHostMallocMapped) == hipSuccess &&
-          hipHostGetDevicePointer(&d, h, 0) == hipSuccess) {
-        memset(h, 0, MAILBOX_BYTES + 256);
-        mb.host = static_cast<uint8_t*>(h);
-        mb.dev = static_cast<uint8_t*>(d);
-        mb.device = dev;
-      } else {
-        (void)hipGetLastError();
-        if (h) (void)hipHostFree(h);
-        mb.failed = true;
-      }
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  Mailbox& mb = t_mb;
+  if (mb.device == -1 && !mb.failed) {
+    void* h = nullptr;
+    void* d = nullptr;
+    if (hipHostMalloc(&h, MAILBOX_BYTES + 256, hipHostMallocCoherent | hipHostMallocMapped) == hipSuccess &&
+        hipHostGetDevicePointer(&d, h, 0) == hipSuccess) {
+      memset(h, 0, MAILBOX_BYTES + 256);
+      mb.host = static_cast<uint8_t*>(h);
+      mb.dev = static_cast<uint8_t*>(d);
+      mb.device = dev;
+    } else {
+      (void)hipGetLastError();
+      if (h) (void)hipHostFree(h);
+      mb.failed = true;
     }
   }
-  if (mb.failed || mb.device != dev) return read_bounce(dst, src, bytes, st);
-  const uint32_t seq = ++mb.seq;
-  uint32_t* flag_dev = reinterpret_cast<uint32_t*>(mb.dev + MAILBOX_BYTES);
+  return (mb.failed || mb.device != dev) ? nullptr : &mb;
+}
+
+// Waits until the mailbox's sequence word reads seq (the stream is past the
+// store), synchronising the stream after MAILBOX_SPIN_S, then copies the bytes.
+// An armed reduction sets the word from inside a kernel that may be followed by
+// copies enqueued since, so only a publish kernel's word (or a synchronisation)
+// lets the upload ring recycle.
+// stream_done: the word was set by the last operation enqueued on st (the publish
+// kernel), so everything before it, the upload ring's copies included, is done.
+static hipError_t mailbox_wait(Mailbox& mb, uint32_t seq, void* dst, size_t bytes, hipStream_t st,
+                               bool stream_done) {
   const uint32_t* flag_host = reinterpret_cast<const uint32_t*>(mb.host + MAILBOX_BYTES);
-  hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(256), 0, st, static_cast<const uint8_t*>(src), mb.dev,
-                     (uint32_t)bytes, flag_dev, seq);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
   const auto t0 = std::chrono::steady_clock::now();
   bool seen = false;
   for (;;) {
@@ -670,13 +676,118 @@ hipError_t read_to_host(void* dst, const void* src, size_t bytes, hipStream_t st
     _mm_pause();
   }
   if (!seen) {
-    e = hipStreamSynchronize(st);
+    hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) return e;
     if (__atomic_load_n(flag_host, __ATOMIC_ACQUIRE) != seq) return hipErrorUnknown;
   }
   memcpy(dst, mb.host, bytes);
-  ring_passed(st);
+  if (stream_done || !seen) ring_passed(st);
   return hipSuccess;
+}
+
+hipError_t read_to_host(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return hipSuccess;
+  Mailbox* mb = bytes <= MAILBOX_BYTES ? mailbox_current() : nullptr;
+  if (!mb) return read_bounce(dst, src, bytes, st);
+  const uint32_t seq = ++mb->seq;
+  uint32_t* flag_dev = reinterpret_cast<uint32_t*>(mb->dev + MAILBOX_BYTES);
+  hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(256), 0, st, static_cast<const uint8_t*>(src), mb->dev,
+                     (uint32_t)bytes, flag_dev, seq);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return mailbox_wait(*mb, seq, dst, bytes, st, true);
+}
+
+// ---------------------------------------------------------------------------
+// Armed read-backs.  flk_read_arm(dst, bytes) hands the next reduction on this
+// thread whose result is `bytes` long the calling thread's mailbox: the launch
+// that sums its work-group partials (partials_sum_publish_kernel) also stores the
+// result there and sets the sequence word, so no publish kernel runs after it;
+// flk_read_armed then waits for the word and copies the bytes to dst.  When no
+// reduction took the arm, flk_read_armed reads the result the ordinary way.
+// ---------------------------------------------------------------------------
+struct ReadArm {
+  void* dst = nullptr;
+  size_t bytes = 0;
+  bool armed = false, taken = false;
+  uint32_t seq = 0;
+};
+static thread_local ReadArm t_arm;
+
+static bool take_read_arm(size_t bytes, uint4** box, uint32_t** flag, uint32_t* seq) {
+  ReadArm& a = t_arm;
+  if (!a.armed || a.taken || a.bytes != bytes) return false;
+  Mailbox* mb = mailbox_current();
+  if (!mb) return false;
+  a.taken = true;
+  a.seq = *seq = ++mb->seq;
+  *box = reinterpret_cast<uint4*>(mb->dev);
+  *flag = reinterpret_cast<uint32_t*>(mb->dev + MAILBOX_BYTES);
+  return true;
+}
+
+// out[k] = box[k] = XOR of parts[k * grid .. k * grid + grid) for k < n_out, one
+// work-group of 1024 lanes, then the mailbox's sequence word (after a
+// system-scope fence, as publish_kernel).
+__global__ void __launch_bounds__(1024) partials_sum_publish_kernel(const uint4* __restrict__ parts, uint32_t grid,
+                                                                   uint32_t n_out, uint4* __restrict__ out,
+                                                                   uint4* __restrict__ box, uint32_t* flag,
+                                                                   uint32_t seq) {
+  __shared__ uint4 red[4][16];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  for (uint32_t k = 0; k < n_out; ++k) {
+    const uint4* p = parts + (size_t)k * grid;
+    uint4 acc = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll 8
+    for (uint32_t i = threadIdx.x; i < grid; i += 1024) {
+      const uint4 v = p[i];
+      acc = make_uint4(acc.x ^ v.x, acc.y ^ v.y, acc.z ^ v.z, acc.w ^ v.w);
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1)
+      acc = make_uint4(acc.x ^ __shfl_xor(acc.x, m), acc.y ^ __shfl_xor(acc.y, m), acc.z ^ __shfl_xor(acc.z, m),
+                       acc.w ^ __shfl_xor(acc.w, m));
+    if (lane == 0) red[k][wave] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x < n_out) {
+    uint4 r = red[threadIdx.x][0];
+    for (uint32_t w = 1; w < 16; ++w) {
+      const uint4 v = red[threadIdx.x][w];
+      r = make_uint4(r.x ^ v.x, r.y ^ v.y, r.z ^ v.z, r.w ^ v.w);
+    }
+    out[threadIdx.x] = r;
+    box[threadIdx.x] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence_system();  // the results reach host memory before the sequence word
+    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+int reduce_begin(unsigned grid, int n_out, uint4* out, hipStream_t st, ReduceTail* rt) {
+  rt->out = out;
+  rt->box = nullptr;
+  rt->flag = nullptr;
+  rt->seq = 0;
+  int rc = partials_alloc(&rt->parts, grid, n_out, out, st);
+  if (rc) return rc;
+  if (n_out <= 4) take_read_arm((size_t)n_out * sizeof(uint4), &rt->box, &rt->flag, &rt->seq);
+  return FLK_OK;
+}
+
+int reduce_end(const ReduceTail& rt, unsigned grid, int n_out, hipStream_t st) {
+  if (!rt.box) return partials_sum_free(rt.parts, grid, n_out, rt.out, st);
+  // with grid == 1 the partials are out itself: the launch only publishes them
+  hipLaunchKernelGGL(partials_sum_publish_kernel, dim3(1), dim3(1024), 0, st, rt.parts, grid, (uint32_t)n_out, rt.out,
+                     rt.box, rt.flag, rt.seq);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return device_failure(e, "partials_sum_publish_kernel");
+  if (rt.parts == rt.out) return FLK_OK;
+  e = stream_free(rt.parts, st);
+  if (e != hipSuccess) return device_failure(e, "reduction scratch: free");
+  return FLK_OK;
 }
 
 int zero_n(uint4* p, uint64_t n, hipStream_t st) {
@@ -824,6 +935,35 @@ int flk_read_to_host(void* dst, const void* src, size_t bytes, void* stream) {
   if (!dst || !src) return flk::set_error(FLK_E_SHAPE, "flk_read_to_host: null pointer");
   hipError_t e = flk::read_to_host(dst, src, bytes, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return flk::device_failure(e, "flk_read_to_host");
+  return FLK_OK;
+}
+
+int flk_read_arm(void* dst, size_t bytes) {
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!dst || !bytes || bytes > flk::MAILBOX_BYTES)
+    return flk::set_error(FLK_E_SHAPE, "flk_read_arm: dst must be non-null and 1 <= bytes <= %u", flk::MAILBOX_BYTES);
+  flk::t_arm = flk::ReadArm{};
+  flk::t_arm.dst = dst;
+  flk::t_arm.bytes = bytes;
+  flk::t_arm.armed = true;
+  return FLK_OK;
+}
+
+int flk_read_armed(const void* src, size_t bytes, void* stream) {
+  const flk::ReadArm a = flk::t_arm;
+  flk::t_arm = flk::ReadArm{};
+  int rc = flk::check_enabled();
+  if (rc) return rc;
+  if (!a.armed || a.bytes != bytes || !src)
+    return flk::set_error(FLK_E_SHAPE, "flk_read_armed: no read-back of %zu bytes armed on this thread", bytes);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipError_t e;
+  if (a.taken)
+    e = flk::mailbox_wait(flk::t_mb, a.seq, a.dst, bytes, st, false);
+  else
+    e = flk::read_to_host(a.dst, src, bytes, st);
+  if (e != hipSuccess) return flk::device_failure(e, "flk_read_armed");
   return FLK_OK;
 }
 

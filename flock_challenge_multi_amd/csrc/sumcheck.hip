@@ -50,7 +50,7 @@ __device__ __forceinline__ void wg_xor_reduce_store(uint4 v, uint4* red_lds, uin
 // out: work-group partials (2 or 3 outputs x gridDim.x), summed afterwards.
 template <bool EVAL>
 __global__ void __launch_bounds__(SC_WG) round_msg_kernel(const uint4* __restrict__ f, const uint4* __restrict__ b,
-                                                         uint64_t n_pairs, uint4* __restrict__ out) {
+                                                         uint64_t n_pairs, ReduceTail rt) {
   __shared__ uint4 tabs[8 * SC_WG];
   __shared__ uint4 red[3][SC_WG / 64];
   const uint32_t tid = threadIdx.x;
@@ -72,9 +72,9 @@ __global__ void __launch_bounds__(SC_WG) round_msg_kernel(const uint4* __restric
     }
   }
   const uint4 u0 = f_reduce(a0), u2 = f_reduce(a2);
-  wg_xor_reduce_store(u0, red[0], out, 0);
-  wg_xor_reduce_store(u2, red[1], out, 1);
-  if (EVAL) wg_xor_reduce_store(f_add(u0, f_reduce(a1)), red[2], out, 2);
+  wg_xor_reduce_store(u0, red[0], rt.parts, 0);
+  wg_xor_reduce_store(u2, red[1], rt.parts, 1);
+  if (EVAL) wg_xor_reduce_store(f_add(u0, f_reduce(a1)), red[2], rt.parts, 2);
 }
 
 // Work-groups of the product-sum kernels (sums go through partials, so the
@@ -97,7 +97,7 @@ static unsigned reduce_grid(uint64_t items) {
 // out: work-group partials (2 outputs x gridDim.x), summed afterwards.
 __global__ void __launch_bounds__(SC_WG) fold_and_msg_kernel(const uint4* __restrict__ f, const uint4* __restrict__ b,
                                                             uint64_t n_quads, uint4 r, uint4* __restrict__ f_out,
-                                                            uint4* __restrict__ b_out, uint4* __restrict__ out) {
+                                                            uint4* __restrict__ b_out, ReduceTail rt) {
   __shared__ uint4 tabs[8 * SC_WG];
   __shared__ uint4 rtab[128];  // nibble-position tables of r
   __shared__ uint4 rnib[32];
@@ -127,8 +127,8 @@ __global__ void __launch_bounds__(SC_WG) fold_and_msg_kernel(const uint4* __rest
     table3_store_private(tabs, SC_WG, tid, f_add(nf0, nf1));
     comb3_mul_acc_rt(f_add(nb0, nb1), my, SC_WG, a2);
   }
-  wg_xor_reduce_store(f_reduce(a0), red[0], out, 0);
-  wg_xor_reduce_store(f_reduce(a2), red[1], out, 1);
+  wg_xor_reduce_store(f_reduce(a0), red[0], rt.parts, 0);
+  wg_xor_reduce_store(f_reduce(a2), red[1], rt.parts, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -271,12 +271,12 @@ int flk_sc_round_msg(const flk_f128* f, const flk_f128* b, size_t n, flk_f128* o
   uint4* o = reinterpret_cast<uint4*>(out2);
   const uint64_t pairs = n / 2;
   const unsigned grid = flk::reduce_grid(pairs);
-  uint4* parts = nullptr;
-  if ((rc = flk::partials_alloc(&parts, grid, 2, o, st))) return rc;
+  flk::ReduceTail rt;
+  if ((rc = flk::reduce_begin(grid, 2, o, st, &rt))) return rc;
   hipLaunchKernelGGL(flk::round_msg_kernel<false>, dim3(grid), dim3(SC_WG), 0, st,
-                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, parts);
+                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, rt);
   FLK_LAUNCH_CHECK("round_msg_kernel");
-  return flk::partials_sum_free(parts, grid, 2, o, st);
+  return flk::reduce_end(rt, grid, 2, st);
 }
 
 int flk_sc_round_msg_noalloc(const flk_f128* f, const flk_f128* b, size_t n, flk_f128* out2, void* stream) {
@@ -298,8 +298,9 @@ int flk_sc_round_msg_noalloc(const flk_f128* f, const flk_f128* b, size_t n, flk
     parts = flk::noalloc_scratch(&rc);
     if (!parts) return rc;
   }
+  const flk::ReduceTail rt{parts, o, nullptr, nullptr, 0u};
   hipLaunchKernelGGL(flk::round_msg_kernel<false>, dim3(grid), dim3(SC_WG), 0, st,
-                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, parts);
+                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, rt);
   FLK_LAUNCH_CHECK("round_msg_kernel(noalloc)");
   if (grid > 1) {
     // every partial [k * grid, k * grid + grid) was just stored by its work-group,
@@ -319,12 +320,12 @@ int flk_sc_round_msg_and_eval(const flk_f128* f, const flk_f128* b, size_t n, fl
   uint4* o = reinterpret_cast<uint4*>(out3);
   const uint64_t pairs = n / 2;
   const unsigned grid = flk::reduce_grid(pairs);
-  uint4* parts = nullptr;
-  if ((rc = flk::partials_alloc(&parts, grid, 3, o, st))) return rc;
+  flk::ReduceTail rt;
+  if ((rc = flk::reduce_begin(grid, 3, o, st, &rt))) return rc;
   hipLaunchKernelGGL(flk::round_msg_kernel<true>, dim3(grid), dim3(SC_WG), 0, st,
-                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, parts);
+                     reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), pairs, rt);
   FLK_LAUNCH_CHECK("round_msg_kernel<eval>");
-  return flk::partials_sum_free(parts, grid, 3, o, st);
+  return flk::reduce_end(rt, grid, 3, st);
 }
 
 int flk_sc_fold_and_msg(const flk_f128* f, const flk_f128* b, size_t n, flk_f128 r, flk_f128* f_out, flk_f128* b_out,
@@ -337,13 +338,13 @@ int flk_sc_fold_and_msg(const flk_f128* f, const flk_f128* b, size_t n, flk_f128
   uint4* o = reinterpret_cast<uint4*>(out2);
   const uint64_t quads = n / 4;
   const unsigned grid = flk::reduce_grid(quads);
-  uint4* parts = nullptr;
-  if ((rc = flk::partials_alloc(&parts, grid, 2, o, st))) return rc;
+  flk::ReduceTail rt;
+  if ((rc = flk::reduce_begin(grid, 2, o, st, &rt))) return rc;
   hipLaunchKernelGGL(flk::fold_and_msg_kernel, dim3(grid), dim3(SC_WG), 0, st,
                      reinterpret_cast<const uint4*>(f), reinterpret_cast<const uint4*>(b), quads, flk::to_u4(r),
-                     reinterpret_cast<uint4*>(f_out), reinterpret_cast<uint4*>(b_out), parts);
+                     reinterpret_cast<uint4*>(f_out), reinterpret_cast<uint4*>(b_out), rt);
   FLK_LAUNCH_CHECK("fold_and_msg_kernel");
-  return flk::partials_sum_free(parts, grid, 2, o, st);
+  return flk::reduce_end(rt, grid, 2, st);
 }
 
 int flk_sc_glue(flk_f128* b, const flk_f128* b_new, size_t n, flk_f128 alpha, void* stream) {

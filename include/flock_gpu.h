@@ -116,6 +116,22 @@ int flk_copy_to_host(void *dst, const void *src, size_t bytes, void *stream);
 /* Copy device -> host and wait for `stream` (dst is HOST memory; small reads go
  * through a per-thread pinned buffer, which halves the latency of a read-back). */
 int flk_read_to_host(void *dst, const void *src, size_t bytes, void *stream);
+/* Armed read-back of a reduction's result.  flk_read_arm(dst, bytes) marks the
+ * next reduction entry point called on this thread whose result is `bytes` long
+ * (a round message: flk_zc_fold_round, flk_zc_fold_at_z_round,
+ * flk_zc_fold_at_z_bind_round, flk_sc_round_msg, flk_sc_round_msg_and_eval,
+ * flk_sc_fold_and_msg, flk_lc_round_msg_msb): the launch that sums its
+ * work-group partials also stores the result into the calling thread's pinned
+ * mailbox (besides its device output) and sets the mailbox's sequence word, so
+ * no publish launch follows it; flk_read_armed(src, bytes, stream) then waits
+ * for the word and copies the result to dst.  When no reduction took the arm
+ * (another size, an entry point without the fused sum), flk_read_armed reads
+ * `src` (the device output) the ordinary way, so the pair is always equivalent
+ * to flk_read_to_host(dst, src, bytes).  An arm not followed by flk_read_armed
+ * stays on the thread until the next flk_read_arm or flk_read_armed.  No
+ * reference counterpart (a latency optimisation of the round read-backs). */
+int flk_read_arm(void *dst, size_t bytes);
+int flk_read_armed(const void *src, size_t bytes, void *stream);
 /* Enqueue a device -> device copy on `stream`. */
 int flk_copy_device_to_device(void *dst, const void *src, size_t bytes, void *stream);
 /* Wait for all work enqueued on `stream`. */

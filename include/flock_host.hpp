@@ -99,6 +99,19 @@ class DeviceVec {
   void* stream_ = nullptr;
 };
 
+// The n results of the reduction that `launch` enqueues (a round message), read
+// back through an armed read-back (flk_read_arm / flk_read_armed): the
+// reduction's last work-group stores them straight into host memory, so no sum
+// or publish kernel runs between the reduction and the host.
+template <class T, class Launch>
+inline std::vector<T> read_reduction(const DeviceVec<T>& out, size_t n, void* stream, Launch&& launch) {
+  std::vector<T> h(n);
+  check(flk_read_arm(h.data(), n * sizeof(T)), "flk_read_arm");
+  launch();
+  check(flk_read_armed(out.data(), n * sizeof(T), stream), "flk_read_armed");
+  return h;
+}
+
 inline bool is_pow2(size_t n) { return n && !(n & (n - 1)); }
 inline size_t log2_exact(size_t n) {
   if (!is_pow2(n)) throw std::invalid_argument("length must be a power of two");
@@ -235,11 +248,13 @@ class SumcheckProver {
     if (pending_) throw std::logic_error("fold before ordinary glue");
     const size_t n = f_.len();
     DeviceVec<F128> nf(n / 2, stream_), nb(n / 2, stream_);
-    check(flk_sc_fold_and_msg(f_.data(), basis_.data(), n, r, nf.data(), nb.data(), msg_.data(), stream_),
-          "fold_and_msg");
+    const std::vector<F128> h = read_reduction(msg_, 2, stream_, [&] {
+      check(flk_sc_fold_and_msg(f_.data(), basis_.data(), n, r, nf.data(), nb.data(), msg_.data(), stream_),
+            "fold_and_msg");
+    });
     f_ = std::move(nf);
     basis_ = std::move(nb);
-    SumcheckMessage m = read_msg();
+    SumcheckMessage m{h[0], h[1]};
     transcript_.push_back(m);
     return m;
   }
@@ -284,11 +299,8 @@ class SumcheckProver {
   SumcheckProver(DeviceVec<F128> f, DeviceVec<F128> b, F128 h, void* stream)
       : f_(std::move(f)), basis_(std::move(b)), msg_(2, stream), t_r_(h), stream_(stream) {}
   SumcheckMessage message_of(const DeviceVec<F128>& f, const DeviceVec<F128>& b) {
-    check(flk_sc_round_msg(f.data(), b.data(), f.len(), msg_.data(), stream_), "round_msg");
-    return read_msg();
-  }
-  SumcheckMessage read_msg() {
-    std::vector<F128> h = msg_.to_host();
+    const std::vector<F128> h = read_reduction(
+        msg_, 2, stream_, [&] { check(flk_sc_round_msg(f.data(), b.data(), f.len(), msg_.data(), stream_), "round_msg"); });
     return SumcheckMessage{h[0], h[1]};
   }
   DeviceVec<F128> f_, basis_, msg_, b_new_;

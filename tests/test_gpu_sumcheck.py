@@ -123,3 +123,67 @@ def test_induce_sumcheck_poly_matches_oracle(orc, dev, log_cols, n_v, nq, n_alph
     torch.cuda.synchronize()
     assert same(from_dev(basis), want_basis)
     assert f128_to_ints(from_dev(s))[0] == want_sum
+
+
+def _armed(fn, out, nbytes):
+    """Run fn (which enqueues a reduction writing `out`) with an armed read-back
+    of nbytes; returns the host bytes the arm delivered."""
+    import ctypes
+    from flock_challenge_multi_amd import _native as nat
+    host = (ctypes.c_uint8 * nbytes)()
+    nat.call("flk_read_arm", ctypes.cast(host, ctypes.c_void_p), nbytes)
+    fn()
+    nat.call("flk_read_armed", ctypes.c_void_p(out.data_ptr()), nbytes, hp._stream())
+    return np.frombuffer(bytes(host), dtype=np.int64).reshape(-1, 2)
+
+
+@pytest.mark.parametrize("logn", [2, 9, 16, 22])
+def test_armed_read_back_equals_plain_read(orc, dev, logn):
+    """The round messages read back through flk_read_arm/flk_read_armed (the
+    reduction's last work-group stores them into host memory) equal the device
+    outputs and the oracle, for one work-group and for thousands; the same
+    calls, repeated, leave the stream's completion counter usable."""
+    _log(f"armed read-back logn={logn}")
+    rng = random.Random(7 * logn)
+    f, b = orc.fill(500 + logn, 1 << logn), orc.fill(600 + logn, 1 << logn)
+    df, db = to_dev(f), to_dev(b)
+    for rep in range(3):
+        out2 = torch.full((2, 2), -1, dtype=torch.int64, device="cuda")
+        h = _armed(lambda: hp.round_msg(df, db, out=out2), out2, 32)
+        want = orc.round_msg(f, b)
+        assert same(from_dev(torch.from_numpy(h.copy()).cuda()), want), rep
+        assert same(from_dev(out2), want), rep
+        out3 = torch.full((3, 2), -1, dtype=torch.int64, device="cuda")
+        h3 = _armed(lambda: hp.round_msg_and_eval(df, db, out=out3), out3, 48)
+        assert same(from_dev(torch.from_numpy(h3.copy()).cuda()), orc.round_msg_and_eval(f, b)), rep
+        if logn >= 2:
+            r = rng.getrandbits(128)
+            fo, bo, msg = orc.fold_and_msg(f, b, r)
+            gm = torch.full((2, 2), -1, dtype=torch.int64, device="cuda")
+            nf = torch.empty(((1 << logn) // 2, 2), dtype=torch.int64, device="cuda")
+            nb = torch.empty_like(nf)
+            hm = _armed(lambda: hp.fold_and_msg(df, db, r, f_out=nf, b_out=nb, out=gm), gm, 32)
+            assert same(from_dev(torch.from_numpy(hm.copy()).cuda()), msg), rep
+            assert same(from_dev(nf), fo) and same(from_dev(nb), bo), rep
+
+
+def test_armed_read_back_falls_back_when_not_taken(orc, dev):
+    """An arm whose size no reduction matches is not taken: flk_read_armed reads
+    the device source the ordinary way.  Without an arm it is refused."""
+    import ctypes
+    from flock_challenge_multi_amd import _native as nat
+    f, b = orc.fill(701, 1 << 10), orc.fill(702, 1 << 10)
+    out2 = torch.full((2, 2), -1, dtype=torch.int64, device="cuda")
+    hp.round_msg(to_dev(f), to_dev(b), out=out2)
+    torch.cuda.synchronize()
+    # armed for 3 results, the round message has 2: read through the fallback
+    host = (ctypes.c_uint8 * 48)()
+    src = torch.cat([out2, torch.zeros((1, 2), dtype=torch.int64, device="cuda")])
+    nat.call("flk_read_arm", ctypes.cast(host, ctypes.c_void_p), 48)
+    hp.round_msg(to_dev(f), to_dev(b), out=out2)
+    torch.cuda.synchronize()
+    nat.call("flk_read_armed", ctypes.c_void_p(src.data_ptr()), 48, hp._stream())
+    got = np.frombuffer(bytes(host), dtype=np.int64).reshape(-1, 2)
+    assert (got == src.cpu().numpy()).all()
+    rc = nat.load().flk_read_armed(ctypes.c_void_p(src.data_ptr()), 48, hp._stream())
+    assert rc == nat.FLK_E_SHAPE
