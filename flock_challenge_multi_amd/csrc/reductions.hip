@@ -211,8 +211,8 @@ __global__ void __launch_bounds__(256) lc_bind_top_kernel(uint4* __restrict__ v,
 // A tile of 256 elements is staged in LDS; every group of 4 elements gets a
 // 16-entry subset-sum table of its eq values.  The tile's bits are transposed
 // once into nibbles nib[g][r] = (bit r of the group's 4 elements), packed
-// eight to a word, so lane (r, half) reads one word per group and looks its
-// nibble up.  out: work-group partials, out[r * gridDim.x + blockIdx.x].
+// eight to a word, so lane (r, half) reads the words of four groups with one
+// 16-byte read and looks each group's nibble up.  out: work-group partials, out[r * gridDim.x + blockIdx.x].
 // ---------------------------------------------------------------------------
 constexpr int F1_WG = 256;
 // Entries between the 16-entry subset-sum tables of consecutive element groups:
@@ -224,6 +224,10 @@ constexpr int F1_TS = 17;
 // reads (lane (g, w) reads word w of the group's four elements) fall in 64
 // distinct banks.
 constexpr int F1_ZS = 5;
+// Words per row of the nibble words, stored [r / 8][group]: 64 groups plus 4 of
+// padding, so a wave's 16-byte reads of 4 consecutive groups at its 8 rows fall
+// in distinct banks, and so do the transpose's stores (row 4w + j, group g).
+constexpr int F1_NP = 68;
 
 // Lane K of the caller's aligned lane quad, broadcast to the quad (DPP).
 template <int K>
@@ -282,7 +286,7 @@ __device__ __forceinline__ void f1_tile(uint4 zv, uint4 ev, uint32_t t, uint32_t
   // i.e. byte j of word w of the group's four elements, bit b of element k at
   // bit 4b + k.  Lane t = 4g + w gathers byte j of its four words into one word
   // (byte k = element k) with byte permutes, transposes it in registers, and
-  // stores the four words of (g, w) with one 16-byte store.
+  // stores the four words at rows 4w .. 4w+3 of column g.
   {
     const uint32_t g = t >> 2, w = t & 3u;
     const uint32_t* zg = zt + F1_ZS * 4 * g + w;
@@ -294,16 +298,25 @@ __device__ __forceinline__ void f1_tile(uint4 zv, uint4 ev, uint32_t t, uint32_t
     v.y = transpose_4x8(__builtin_amdgcn_perm(lo23, lo01, 0x07060302u));
     v.z = transpose_4x8(__builtin_amdgcn_perm(hi23, hi01, 0x05040100u));
     v.w = transpose_4x8(__builtin_amdgcn_perm(hi23, hi01, 0x07060302u));
-    reinterpret_cast<uint4*>(nibw)[t] = v;
+    nibw[(4 * w + 0) * F1_NP + g] = v.x;
+    nibw[(4 * w + 1) * F1_NP + g] = v.y;
+    nibw[(4 * w + 2) * F1_NP + g] = v.z;
+    nibw[(4 * w + 3) * F1_NP + g] = v.w;
   }
   __syncthreads();
   const uint32_t qr = r >> 3, shr = 4u * (r & 7u);
-#pragma unroll 4
-  for (uint32_t g = halfsel * 32; g < halfsel * 32 + 32; g += 2) {
-    const uint4 x0 = tabs[g * F1_TS + ((nibw[g * 16 + qr] >> shr) & 15u)];
-    const uint4 x1 = tabs[(g + 1) * F1_TS + ((nibw[(g + 1) * 16 + qr] >> shr) & 15u)];
+  const uint32_t* nrow = nibw + qr * F1_NP;
+#pragma unroll 2
+  for (uint32_t g = halfsel * 32; g < halfsel * 32 + 32; g += 4) {
+    const uint4 nw = *reinterpret_cast<const uint4*>(nrow + g);  // nibble words of groups g .. g+3
+    const uint4 x0 = tabs[g * F1_TS + ((nw.x >> shr) & 15u)];
+    const uint4 x1 = tabs[(g + 1) * F1_TS + ((nw.y >> shr) & 15u)];
+    const uint4 x2 = tabs[(g + 2) * F1_TS + ((nw.z >> shr) & 15u)];
+    const uint4 x3 = tabs[(g + 3) * F1_TS + ((nw.w >> shr) & 15u)];
     acc = make_uint4(xor3(acc.x, x0.x, x1.x), xor3(acc.y, x0.y, x1.y), xor3(acc.z, x0.z, x1.z),
                      xor3(acc.w, x0.w, x1.w));
+    acc = make_uint4(xor3(acc.x, x2.x, x3.x), xor3(acc.y, x2.y, x3.y), xor3(acc.z, x2.z, x3.z),
+                     xor3(acc.w, x2.w, x3.w));
   }
   __syncthreads();
 }
@@ -312,7 +325,7 @@ __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_kernel(const uint4* __r
                                                                const uint4* __restrict__ eq, uint64_t n,
                                                                uint4* __restrict__ out) {
   __shared__ __align__(16) uint32_t zt[F1_WG * F1_ZS];
-  __shared__ __align__(16) uint32_t nibw[(F1_WG / 4) * 16];  // [group][r / 8]: nibbles of r = 8q .. 8q+7
+  __shared__ __align__(16) uint32_t nibw[16 * F1_NP];  // [r / 8][group]: nibbles of r = 8q .. 8q+7
   __shared__ uint4 tabs[(F1_WG / 4) * F1_TS];
   __shared__ uint4 red[128];
   const uint32_t t = threadIdx.x;
@@ -346,7 +359,7 @@ __global__ void __launch_bounds__(F1_WG) rs_fold_1b_rows_split_kernel(const uint
                                                                      const uint4* __restrict__ eq_hi,
                                                                      uint64_t n_hi, uint4* __restrict__ out) {
   __shared__ __align__(16) uint32_t zt[F1_WG * F1_ZS];
-  __shared__ __align__(16) uint32_t nibw[(F1_WG / 4) * 16];
+  __shared__ __align__(16) uint32_t nibw[16 * F1_NP];
   __shared__ uint4 tabs[(F1_WG / 4) * F1_TS];
   __shared__ uint4 red[128];
   __shared__ uint4 etab[16];
